@@ -1,0 +1,442 @@
+/*
+ * or_step.c -- grid-level restatement of one RK2 step of
+ * pseudotensor/harm_finalwham (TIMEORDER 2, NOENOFLUX, FLUXCTTOTH):
+ * ghost fill, reconstruction, Riemann fluxes, flux-CT, conserved update,
+ * Noble 2D inversion and fixup1zone.  TEST INFRASTRUCTURE ONLY.
+ */
+#include "or_internal.h"
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define BIG (1.e+50) /* global.h:1040 */
+
+static double rmax(double a, double b) { if (a > b) return a; else return b; } /* phys.c:1797 */
+static double rmin(double a, double b) { if (a > b) return b; else return a; } /* phys.c:1804 */
+
+/* global.h:361-362 */
+#define MINMODB(a, b) ((fabs(a) < fabs(b)) ? (a) : (b))
+#define MINMOD(a, b) (((a) * (b) <= 0) ? 0.0 : MINMODB(a, b))
+
+static int dir_active(const or_dims *d, int dir)
+{
+  return (dir == 1) ? (d->n1 > 1) : (dir == 2) ? (d->n2 > 1) : (d->n3 > 1);
+}
+
+/* ------------------------------------------------------------------ */
+/* bound_prim: bounds.fishmon.c:29 bound_prim_user (EXTRAP 0, POLEDEATH 1) */
+/* ------------------------------------------------------------------ */
+int or_bound_prim(const or_grid *G, double *p)
+{
+  or_dims d = or_get_dims(G);
+  int i, j, k, pl;
+  or_geom g;
+  if (d.n1 > 1) {
+    /* inner radial boundary: copy i=0, then inflow check and gamma limit */
+    for (j = 0; j < d.n2; j++)
+      for (k = 0; k < d.n3; k++) {
+        for (i = -d.nb1; i <= -1; i++)
+          for (pl = 0; pl < OR_NPR; pl++)
+            p[or_zidx(&d, i, j, k) * 8 + pl] = p[or_zidx(&d, 0, j, k) * 8 + pl];
+        for (i = -d.nb1; i <= -1; i++) {
+          or_load_geom(G, &d, i, j, OL_CENT, &g);
+          if (or_inflow_check_r(&p[or_zidx(&d, i, j, k) * 8], &g, -1, G->gammamax) > 0) return 1;
+          if (or_limit_gamma(G->gammamax, &p[or_zidx(&d, i, j, k) * 8], &g) > 0) return 1;
+        }
+      }
+    /* outer radial boundary */
+    for (j = 0; j < d.n2; j++)
+      for (k = 0; k < d.n3; k++) {
+        for (i = d.n1; i < d.n1 + d.nb1; i++)
+          for (pl = 0; pl < OR_NPR; pl++)
+            p[or_zidx(&d, i, j, k) * 8 + pl] = p[or_zidx(&d, d.n1 - 1, j, k) * 8 + pl];
+        for (i = d.n1; i < d.n1 + d.nb1; i++) {
+          or_load_geom(G, &d, i, j, OL_CENT, &g);
+          if (or_inflow_check_r(&p[or_zidx(&d, i, j, k) * 8], &g, +1, G->gammamax) > 0) return 1;
+          if (or_limit_gamma(G->gammamax, &p[or_zidx(&d, i, j, k) * 8], &g) > 0) return 1;
+        }
+      }
+  }
+  if (d.n2 > 1) {
+    /* polar axis: mirror copy (POSDEFMETRIC 0 leaves signs) then POLEDEATH */
+    for (i = -d.nb1; i < d.n1 + d.nb1; i++)
+      for (k = 0; k < d.n3; k++)
+        for (j = -d.nb2; j <= -1; j++)
+          for (pl = 0; pl < OR_NPR; pl++)
+            p[or_zidx(&d, i, j, k) * 8 + pl] = p[or_zidx(&d, i, -j - 1, k) * 8 + pl];
+    for (i = -d.nb1; i < d.n1 + d.nb1; i++)
+      for (k = 0; k < d.n3; k++)
+        for (j = -d.nb2; j < 0 + 1; j++) {
+          p[or_zidx(&d, i, j, k) * 8 + OR_U2] *= 0;
+          p[or_zidx(&d, i, j, k) * 8 + OR_B2] *= 0.;
+        }
+    for (i = -d.nb1; i < d.n1 + d.nb1; i++)
+      for (k = 0; k < d.n3; k++)
+        for (j = d.n2; j < d.n2 + d.nb2; j++)
+          for (pl = 0; pl < OR_NPR; pl++)
+            p[or_zidx(&d, i, j, k) * 8 + pl] = p[or_zidx(&d, i, 2 * d.n2 - 1 - j, k) * 8 + pl];
+    for (i = -d.nb1; i < d.n1 + d.nb1; i++)
+      for (k = 0; k < d.n3; k++)
+        for (j = d.n2 - 1; j <= d.n2 - 1 + d.nb2; j++) {
+          p[or_zidx(&d, i, j, k) * 8 + OR_U2] *= 0;
+          p[or_zidx(&d, i, j, k) * 8 + OR_B2] *= 0.;
+        }
+  }
+  if (d.n3 > 1) {
+    /* periodic phi */
+    for (i = -d.nb1; i < d.n1 + d.nb1; i++)
+      for (j = -d.nb2; j < d.n2 + d.nb2; j++) {
+        for (k = -d.nb3; k <= -1; k++)
+          for (pl = 0; pl < OR_NPR; pl++)
+            p[or_zidx(&d, i, j, k) * 8 + pl] = p[or_zidx(&d, i, j, d.n3 + k) * 8 + pl];
+        for (k = d.n3; k < d.n3 + d.nb3; k++)
+          for (pl = 0; pl < OR_NPR; pl++)
+            p[or_zidx(&d, i, j, k) * 8 + pl] = p[or_zidx(&d, i, j, k - d.n3) * 8 + pl];
+      }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* reconstruction                                                      */
+/* ------------------------------------------------------------------ */
+
+/* interppoint.c:24 slope_lim_pointtype + interppoint.c:387 slope_lim_3points (MC);
+   dq over the range of set_interppoint_loop (interppoint.c:159) */
+static void mc_slopes(const or_dims *d, const double *p, int dir, double *dq)
+{
+  int i, j, k, pl;
+  int is = -d->nb1, ie = d->n1 + d->nb1 - 1, js = -d->nb2, je = d->n2 + d->nb2 - 1;
+  int ks = -d->nb3, ke = d->n3 + d->nb3 - 1;
+  int di = (dir == 1), dj = (dir == 2), dk = (dir == 3);
+  if (dir == 1) { is = -1; ie = d->n1; }
+  if (dir == 2) { js = -1; je = d->n2; }
+  if (dir == 3) { ks = -1; ke = d->n3; }
+  for (i = is; i <= ie; i++)
+    for (j = js; j <= je; j++)
+      for (k = ks; k <= ke; k++)
+        for (pl = 0; pl < OR_NPR; pl++) {
+          double yl = p[or_zidx(d, i - di, j - dj, k - dk) * 8 + pl];
+          double yc = p[or_zidx(d, i, j, k) * 8 + pl];
+          double yr = p[or_zidx(d, i + di, j + dj, k + dk) * 8 + pl];
+          double Dqm = 2.0 * (yc - yl);
+          double Dqp = 2.0 * (yr - yc);
+          double Dqc = 0.5 * (yr - yl);
+          double inner = MINMOD(Dqm, Dqp);
+          dq[or_zidx(d, i, j, k) * 8 + pl] = MINMOD(Dqc, inner);
+        }
+}
+
+int or_reconstruct(const or_grid *G, const double *p, int dir, double *pleft, double *pright)
+{
+  or_dims d = or_get_dims(G);
+  if (G->lim == 0) {
+    mc_slopes(&d, p, dir, pleft);
+    return 0;
+  }
+  /* WENO5BND: one call per pencil, pencils over the full transverse extent
+     (set_interp_loop, interpline.c:1205) */
+  {
+    int i, j, k;
+    if (dir == 1) {
+      for (j = -d.nb2; j < d.n2 + d.nb2; j++)
+        for (k = -d.nb3; k < d.n3 + d.nb3; k++) or_weno5bnd_line(G, &d, p, 1, 0, j, k, pleft, pright);
+    } else if (dir == 2) {
+      for (i = -d.nb1; i < d.n1 + d.nb1; i++)
+        for (k = -d.nb3; k < d.n3 + d.nb3; k++) or_weno5bnd_line(G, &d, p, 2, i, 0, k, pleft, pright);
+    } else {
+      for (i = -d.nb1; i < d.n1 + d.nb1; i++)
+        for (j = -d.nb2; j < d.n2 + d.nb2; j++) or_weno5bnd_line(G, &d, p, 3, i, j, 0, pleft, pright);
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* fluxes                                                              */
+/* ------------------------------------------------------------------ */
+
+/* fluxcompute.c:12 flux_compute_general (p2SFUevolve, get_wavespeeds, HLL/LaxF) */
+static int flux_compute_general(const or_grid *G, int dir, const or_geom *g, const double *p_l,
+                                const double *p_r, double *F, double *ctop)
+{
+  or_state ql, qr;
+  double F_l[OR_NPR], F_r[OR_NPR], U_l[OR_NPR], U_r[OR_NPR];
+  double cmax_l, cmin_l, cmax_r, cmin_r, lmin, lmax, cmin, cmax;
+  int pl;
+  if (or_get_state(p_l, g, &ql)) return 1;
+  or_primtoflux(p_l, &ql, dir, g, G->gam, F_l);
+  or_primtoflux(p_l, &ql, 0, g, G->gam, U_l);
+  if (or_get_state(p_r, g, &qr)) return 1;
+  or_primtoflux(p_r, &qr, dir, g, G->gam, F_r);
+  or_primtoflux(p_r, &qr, 0, g, G->gam, U_r);
+  /* wavespeeds.c:12 get_wavespeeds -> vchar.c:37, wavespeeds.c:325 cminmax_calc */
+  if (or_vchar(p_l, &ql, dir, g, G->gam, &cmax_l, &cmin_l)) return 1;
+  if (or_vchar(p_r, &qr, dir, g, G->gam, &cmax_r, &cmin_r)) return 1;
+  lmin = rmin(cmin_l, cmin_r);
+  lmax = rmax(cmax_l, cmax_r);
+  lmin = fabs(rmax(0., -lmin));
+  lmax = fabs(rmax(0., lmax));
+  cmin = lmin;
+  cmax = lmax;
+  *ctop = rmax(lmax, lmin);
+  if (G->flux == 0) {
+    /* fluxcompute.c:78 LAXFCOMPUTE */
+    for (pl = 0; pl < OR_NPR; pl++) F[pl] = 0.5 * ((F_l[pl] + F_r[pl]) - *ctop * (U_r[pl] - U_l[pl]));
+  } else {
+    /* fluxcompute.c:223 hllflux_compute */
+    if (cmax + cmin != 0.0) {
+      for (pl = 0; pl < OR_NPR; pl++)
+        F[pl] = ((cmax * F_l[pl] + cmin * F_r[pl]) - (cmax * cmin) * (U_r[pl] - U_l[pl])) / (cmax + cmin);
+    } else {
+      for (pl = 0; pl < OR_NPR; pl++)
+        F[pl] = 0.5 * ((F_l[pl] + F_r[pl]) - *ctop * (U_r[pl] - U_l[pl]));
+    }
+  }
+  return 0;
+}
+
+/* flux.c:469 fluxcalc_standard for one direction (faces of interp_loop_set,
+   initbase.c:1027-1054) */
+static int fluxcalc_dir(const or_grid *G, const or_dims *d, const double *p, int dir, double *F,
+                        double *ndt, double *pl_buf, double *pr_buf)
+{
+  int i, j, k, pl;
+  int di = (dir == 1), dj = (dir == 2), dk = (dir == 3);
+  int s1 = (d->n1 > 1), s2 = (d->n2 > 1), s3 = (d->n3 > 1);
+  int is = -s1, ie = d->n1 - 1 + s1, js = -s2, je = d->n2 - 1 + s2, ks = -s3, ke = d->n3 - 1 + s3;
+  int loc = (dir == 1) ? OL_FACE1 : (dir == 2) ? OL_FACE2 : OL_FACE3;
+  or_geom g;
+  double p_l[OR_NPR], p_r[OR_NPR], ctop, dtij;
+
+  if (dir == 1) { is = 0; ie = d->n1 - 1 + s1; }
+  if (dir == 2) { js = 0; je = d->n2 - 1 + s2; }
+  if (dir == 3) { ks = 0; ke = d->n3 - 1 + s3; }
+  *ndt = BIG;
+  or_reconstruct(G, p, dir, pl_buf, pr_buf);
+  for (i = is; i <= ie; i++)
+    for (j = js; j <= je; j++)
+      for (k = ks; k <= ke; k++) {
+        ptrdiff_t zm = or_zidx(d, i - di, j - dj, k - dk), z = or_zidx(d, i, j, k);
+        /* flux.c:753 getp2interplr */
+        for (pl = 0; pl < OR_NPR; pl++) {
+          if (G->lim == 0) {
+            p_l[pl] = p[zm * 8 + pl] + 0.5 * pl_buf[zm * 8 + pl];
+            p_r[pl] = p[z * 8 + pl] - 0.5 * pl_buf[z * 8 + pl];
+          } else {
+            p_l[pl] = pr_buf[zm * 8 + pl];
+            p_r[pl] = pl_buf[z * 8 + pl];
+          }
+        }
+        or_load_geom(G, d, i, j, loc, &g);
+        /* flux.c:629 BDIRCONT */
+        p_l[OR_B1 + dir - 1] = p_r[OR_B1 + dir - 1] = 0.5 * (p_l[OR_B1 + dir - 1] + p_r[OR_B1 + dir - 1]);
+        if (flux_compute_general(G, dir, &g, p_l, p_r, &F[z * 8], &ctop)) return 1;
+        dtij = G->cour * G->dx[dir] / ctop;
+        if (dtij < *ndt) *ndt = dtij;
+      }
+  return 0;
+}
+
+/* fluxct.c:10 flux_ct, FLUXCTTOTH with CORNGDETVERSION 1 */
+static void flux_ct(const or_grid *G, const or_dims *d, double *F1, double *F2, double *F3)
+{
+  int i, j, k, s1 = (d->n1 > 1), s2 = (d->n2 > 1), s3 = (d->n3 > 1);
+  size_t nz = (size_t)d->m1 * d->m2 * d->m3;
+  double *emf1 = calloc(nz, sizeof(double)), *emf2 = calloc(nz, sizeof(double)),
+         *emf3 = calloc(nz, sizeof(double));
+  double c1 = (s2 && s3) ? 0.25 : 0.5, c2 = (s1 && s3) ? 0.25 : 0.5, c3 = (s1 && s2) ? 0.25 : 0.5;
+#define ZI(a, b, c) or_zidx(d, (a), (b), (c))
+#define KM1(k) (s3 ? (k)-1 : (k))
+#define JM1(j) (s2 ? (j)-1 : (j))
+#define IM1(i) (s1 ? (i)-1 : (i))
+#define KP1(k) (s3 ? (k) + 1 : (k))
+#define JP1(j) (s2 ? (j) + 1 : (j))
+#define IP1(i) (s1 ? (i) + 1 : (i))
+  /* strip gdet from the field fluxes at the faces, fluxct.c:74-96 (only where
+     the corner EMFs below read them) */
+  for (i = -s1; i <= d->n1 - 1 + s1; i++)
+    for (j = -s2; j <= d->n2 - 1 + s2; j++)
+      for (k = -s3; k <= d->n3 - 1 + s3; k++) {
+        ptrdiff_t z = ZI(i, j, k);
+        int b;
+        for (b = OR_B1; b <= OR_B3; b++) {
+          if (s1) F1[z * 8 + b] /= or_gdet(G, d, i, j, OL_FACE1);
+          if (s2) F2[z * 8 + b] /= or_gdet(G, d, i, j, OL_FACE2);
+          if (s3) F3[z * 8 + b] /= or_gdet(G, d, i, j, OL_FACE3);
+        }
+      }
+  /* corner EMFs (fluxct.c:148-198) times gdet at the corner (fluxct.c:202-218),
+     over the corners the active-zone fluxes read */
+  for (i = 0; i <= d->n1 - 1 + s1; i++)
+    for (j = 0; j <= d->n2 - 1 + s2; j++)
+      for (k = 0; k <= d->n3 - 1 + s3; k++) {
+        ptrdiff_t z = ZI(i, j, k);
+        double e1 = 0.0, e2 = 0.0, e3 = 0.0;
+        if (s2 || s3) {
+          if (s2 && s3)
+            e1 = c1 * (+F2[z * 8 + OR_B3] + F2[ZI(i, j, KM1(k)) * 8 + OR_B3] - F3[z * 8 + OR_B2] -
+                       F3[ZI(i, JM1(j), k) * 8 + OR_B2]);
+          else if (s2)
+            e1 = c1 * (+F2[z * 8 + OR_B3] + F2[ZI(i, j, KM1(k)) * 8 + OR_B3]);
+          else
+            e1 = c1 * (-F3[z * 8 + OR_B2] - F3[ZI(i, JM1(j), k) * 8 + OR_B2]);
+        }
+        if (s1 || s3) {
+          if (s1 && s3)
+            e2 = c2 * (+F3[z * 8 + OR_B1] + F3[ZI(IM1(i), j, k) * 8 + OR_B1] - F1[z * 8 + OR_B3] -
+                       F1[ZI(i, j, KM1(k)) * 8 + OR_B3]);
+          else if (s3)
+            e2 = c2 * (+F3[z * 8 + OR_B1] + F3[ZI(IM1(i), j, k) * 8 + OR_B1]);
+          else
+            e2 = c2 * (-F1[z * 8 + OR_B3] - F1[ZI(i, j, KM1(k)) * 8 + OR_B3]);
+        }
+        if (s1 || s2) {
+          if (s1 && s2)
+            e3 = c3 * (+F1[z * 8 + OR_B2] + F1[ZI(i, JM1(j), k) * 8 + OR_B2] - F2[z * 8 + OR_B1] -
+                       F2[ZI(IM1(i), j, k) * 8 + OR_B1]);
+          else if (s1)
+            e3 = c3 * (+F1[z * 8 + OR_B2] + F1[ZI(i, JM1(j), k) * 8 + OR_B2]);
+          else
+            e3 = c3 * (-F2[z * 8 + OR_B1] - F2[ZI(IM1(i), j, k) * 8 + OR_B1]);
+        }
+        emf1[z] = e1 * or_gdet(G, d, i, j, OL_CORN1);
+        emf2[z] = e2 * or_gdet(G, d, i, j, OL_CORN2);
+        emf3[z] = e3 * or_gdet(G, d, i, j, OL_CORN3);
+      }
+  /* rewrite the field fluxes from the EMFs, fluxct.c:720-781 */
+  for (i = 0; i <= d->n1 - 1 + s1; i++)
+    for (j = 0; j <= d->n2 - 1 + s2; j++)
+      for (k = 0; k <= d->n3 - 1 + s3; k++) {
+        ptrdiff_t z = ZI(i, j, k);
+        if (s1 && j < d->n2 && k < d->n3) {
+          F1[z * 8 + OR_B1] = 0.;
+          F1[z * 8 + OR_B2] = 0.5 * (emf3[z] + emf3[ZI(i, JP1(j), k)]);
+          F1[z * 8 + OR_B3] = -0.5 * (emf2[z] + emf2[ZI(i, j, KP1(k))]);
+        }
+        if (s2 && i < d->n1 && k < d->n3) {
+          F2[z * 8 + OR_B1] = -0.5 * (emf3[z] + emf3[ZI(IP1(i), j, k)]);
+          F2[z * 8 + OR_B2] = 0.;
+          F2[z * 8 + OR_B3] = 0.5 * (emf1[z] + emf1[ZI(i, j, KP1(k))]);
+        }
+        if (s3 && i < d->n1 && j < d->n2) {
+          F3[z * 8 + OR_B1] = 0.5 * (emf2[z] + emf2[ZI(IP1(i), j, k)]);
+          F3[z * 8 + OR_B2] = -0.5 * (emf1[z] + emf1[ZI(i, JP1(j), k)]);
+          F3[z * 8 + OR_B3] = 0.;
+        }
+      }
+  free(emf1);
+  free(emf2);
+  free(emf3);
+}
+
+int or_fluxcalc(const or_grid *G, const double *p, double *F1, double *F2, double *F3, double *ndt)
+{
+  or_dims d = or_get_dims(G);
+  size_t nz = (size_t)d.m1 * d.m2 * d.m3;
+  double *bl = calloc(nz * 8, sizeof(double)), *br = calloc(nz * 8, sizeof(double));
+  int rc = 0, dir;
+  double *Fd[4] = {NULL, F1, F2, F3};
+  for (dir = 1; dir <= 3; dir++) {
+    ndt[dir - 1] = BIG;
+    if (!dir_active(&d, dir)) continue;
+    if (fluxcalc_dir(G, &d, p, dir, Fd[dir], &ndt[dir - 1], bl, br)) { rc = 1; break; }
+  }
+  if (!rc) flux_ct(G, &d, F1, F2, F3);
+  free(bl);
+  free(br);
+  return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* advance.c:58 advance_standard + step_ch.c:35 step_ch_simplempi       */
+/* ------------------------------------------------------------------ */
+static int advance(const or_grid *G, const double *pi, const double *pb, double *pf, double *uf,
+                   double *ucum, const double CUf[4], const double Cunew[3], int timeorder,
+                   double dt, double *ndt_out, int64_t *nfail, int64_t *iters)
+{
+  or_dims d = or_get_dims(G);
+  size_t nz = (size_t)d.m1 * d.m2 * d.m3;
+  double *F1 = calloc(nz * 8, sizeof(double)), *F2 = calloc(nz * 8, sizeof(double)),
+         *F3 = calloc(nz * 8, sizeof(double));
+  double ndt[3], idx1 = 1.0 / G->dx[1], idx2 = 1.0 / G->dx[2], idx3 = 1.0 / G->dx[3];
+  int i, j, k, pl, rc = 0;
+  int finalstep = (timeorder == 1);
+
+  if (or_fluxcalc(G, pb, F1, F2, F3, ndt)) { rc = 1; goto done; }
+  for (i = 0; i < d.n1; i++)
+    for (j = 0; j < d.n2; j++)
+      for (k = 0; k < d.n3; k++) {
+        ptrdiff_t z = or_zidx(&d, i, j, k);
+        or_geom g;
+        double Ui[OR_NPR], dUgeom[OR_NPR], dU1[OR_NPR], dU2[OR_NPR], dU3[OR_NPR], dUr[OR_NPR];
+        double Uf[OR_NPR], conn[4][4][4];
+        int fl, it;
+        if (timeorder == 0)
+          for (pl = 0; pl < OR_NPR; pl++) uf[z * 8 + pl] = ucum[z * 8 + pl] = 0.0;
+        or_load_geom(G, &d, i, j, OL_CENT, &g);
+        or_primtoU(&pi[z * 8], &g, G->gam, Ui);
+        memcpy(conn, &G->conn[((ptrdiff_t)(i + d.nb1) * d.m2 + (j + d.nb2)) * 64], sizeof(conn));
+        or_source(&pb[z * 8], &g, (const double(*)[4][4])conn, G->gam, dUgeom);
+        /* advance.c:741 flux2dUavg (FLUXB != FLUXCD) */
+        for (pl = 0; pl < OR_NPR; pl++) {
+          dU1[pl] = 0;
+          dU2[pl] = 0;
+          dU3[pl] = 0;
+          if (d.n1 > 1) dU1[pl] = (-(F1[or_zidx(&d, i + 1, j, k) * 8 + pl] - F1[z * 8 + pl]) * idx1);
+          if (d.n2 > 1) dU2[pl] = (-(F2[or_zidx(&d, i, j + 1, k) * 8 + pl] - F2[z * 8 + pl]) * idx2);
+          if (d.n3 > 1) dU3[pl] = (-(F3[or_zidx(&d, i, j, k + 1) * 8 + pl] - F3[z * 8 + pl]) * idx3);
+          dUr[pl] = dU1[pl] + dU2[pl] + dU3[pl];
+        }
+        /* advance.c:922 dUtoU */
+        for (pl = 0; pl < OR_NPR; pl++)
+          uf[z * 8 + pl] = Uf[pl] =
+              CUf[0] * Ui[pl] + CUf[1] * uf[z * 8 + pl] + CUf[2] * dt * (dUr[pl] + dUgeom[pl]);
+        for (pl = 0; pl < OR_NPR; pl++)
+          ucum[z * 8 + pl] += Cunew[0] * Ui[pl] + Cunew[1] * dt * (dUr[pl] + dUgeom[pl]) + Cunew[2] * Uf[pl];
+        for (pl = 0; pl < OR_NPR; pl++) pf[z * 8 + pl] = pb[z * 8 + pl];
+        if (finalstep) or_utoprimgen(&ucum[z * 8], &g, G->gam, &pf[z * 8], &fl, &it);
+        else or_utoprimgen(Uf, &g, G->gam, &pf[z * 8], &fl, &it);
+        if (fl) (*nfail)++;
+        *iters += it;
+        if (finalstep) {
+          const double *fz = &G->floors[((ptrdiff_t)(i + d.nb1) * d.m2 + (j + d.nb2)) * 2];
+          or_fixup1zone(&pf[z * 8], &g, fz[0], fz[1], G->gammamax);
+        }
+      }
+  *ndt_out = G->defcon * 1. / (1. / ndt[0] + 1. / ndt[1] + 1. / ndt[2]);
+done:
+  free(F1);
+  free(F2);
+  free(F3);
+  return rc;
+}
+
+int or_step(const or_grid *G, double *p, double *dt_io, int64_t *nfail, int64_t *iters)
+{
+  or_dims d = or_get_dims(G);
+  size_t nz = (size_t)d.m1 * d.m2 * d.m3;
+  double *pk = malloc(nz * 8 * sizeof(double));
+  double *uf = calloc(nz * 8, sizeof(double)), *ucum = calloc(nz * 8, sizeof(double));
+  /* step_ch.c:506-520, TIMEORDER 2 midpoint */
+  const double CUf[2][4] = {{1.0, 0.0, 0.5, 0.0}, {1.0, 0.0, 1.0, 0.0}};
+  const double Cunew[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}};
+  double dt = *dt_io, ndt = BIG;
+  int rc = 0;
+  memcpy(pk, p, nz * 8 * sizeof(double));
+  /* substep 0: pi=p pb=p pf=pk */
+  if (advance(G, p, p, pk, uf, ucum, CUf[0], Cunew[0], 0, dt, &ndt, nfail, iters)) { rc = 1; goto done; }
+  if (or_bound_prim(G, pk) || or_bound_prim(G, pk)) { rc = 1; goto done; }
+  /* substep 1: pi=p pb=pk pf=p */
+  if (advance(G, p, pk, p, uf, ucum, CUf[1], Cunew[1], 1, dt, &ndt, nfail, iters)) { rc = 1; goto done; }
+  if (or_bound_prim(G, p) || or_bound_prim(G, p)) { rc = 1; goto done; }
+  /* step_ch.c:243-247 */
+  if (ndt > G->safe * dt) ndt = G->safe * dt;
+  *dt_io = ndt;
+done:
+  free(pk);
+  free(uf);
+  free(ucum);
+  return rc;
+}

@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures under tests/golden/ from the reference build.
+
+The reference (pseudotensor/harm_finalwham) is compiled from its own C
+sources by oracle/refbuild/build_ref.sh into oracle/_ref/harness_<tag>; this
+script runs those executables and stores their inputs and outputs as .npz
+files.  It needs /root/reference (only in the build container); the GPU box
+uses the committed .npz files.
+
+Fixtures
+  c2p_g43.npz / c2p_g53.npz   batched Noble-2D inversions (Gamma 4/3, 5/3):
+        metric, true primitives, guess -> U (reference primtoU), inverted
+        primitives, failure code, Newton iterations (utoprimgen.c:23)
+  t2d_mc.npz     2-D torus 32x32, MC + LaxF + flux-CT, 5 RK2 steps
+  t2d_weno.npz   2-D torus 32x32, WENO5BND + HLL + flux-CT, 5 RK2 steps
+  t3d_weno.npz   3-D torus 16x16x8, WENO5BND + HLL + flux-CT, 1 RK2 step
+Each torus fixture carries the reference geometry tables, the initial
+primitives, the reconstruction states (or MC slopes) per direction, the
+fluxes after one fluxcalc()+flux_ct() and the primitives after steps.
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.abspath(os.path.join(HERE, "..", ".."))
+BUILD = os.path.join(REPO, "oracle", "refbuild", "build_ref.sh")
+REFDIR = os.path.join(REPO, "oracle", "_ref")
+
+C2P_SEED = 20240611
+LOCS = [0, 1, 2, 3, 4, 5, 7]  # FACE1..3, CORN1..3, CENT (global.h:946-953); CORNT unused
+
+
+def ks_metric(r, th, a):
+    """Kerr-Schild metric in (t, r, theta, phi) at BL radius r, theta
+    (metric.c:697 set_gcov_ksmetric) and its closed-form inverse."""
+    n = r.shape[0]
+    s, c = np.sin(th), np.cos(th)
+    s2 = s * s
+    rho2 = r * r + a * a * c * c
+    gcov = np.zeros((n, 4, 4))
+    gcov[:, 0, 0] = -1.0 + 2.0 * r / rho2
+    gcov[:, 0, 1] = gcov[:, 1, 0] = 2.0 * r / rho2
+    gcov[:, 0, 3] = gcov[:, 3, 0] = -2.0 * a * r * s2 / rho2
+    gcov[:, 1, 1] = 1.0 + 2.0 * r / rho2
+    gcov[:, 1, 3] = gcov[:, 3, 1] = -a * s2 * (1.0 + 2.0 * r / rho2)
+    gcov[:, 2, 2] = rho2
+    gcov[:, 3, 3] = s2 * (rho2 + a * a * s2 * (1.0 + 2.0 * r / rho2))
+    delta = r * r - 2.0 * r + a * a
+    gcon = np.zeros((n, 4, 4))
+    gcon[:, 0, 0] = -(1.0 + 2.0 * r / rho2)
+    gcon[:, 0, 1] = gcon[:, 1, 0] = 2.0 * r / rho2
+    gcon[:, 1, 1] = delta / rho2
+    gcon[:, 1, 3] = gcon[:, 3, 1] = a / rho2
+    gcon[:, 2, 2] = 1.0 / rho2
+    gcon[:, 3, 3] = 1.0 / (rho2 * s2)
+    gdet = rho2 * s
+    gpert = np.zeros((n, 4))
+    gpert[:, 0] = 2.0 * r / rho2
+    gpert[:, 1] = 2.0 * r / rho2
+    gpert[:, 2] = gcov[:, 2, 2] - 1.0
+    gpert[:, 3] = gcov[:, 3, 3] - 1.0
+    return gcov, gcon, gdet, gpert
+
+
+def sample_c2p(n, rng, a=0.9375):
+    """The batched-inversion distribution: rho log-U[1e-6,1], u log-U[1e-8,1e2],
+    Lorentz factor log-U[1,50] with isotropic direction, b^2/rho log-U[1e-4,1e3]
+    with isotropic direction, KS metric at r U[1.3,40], theta U(0.01,pi-0.01)."""
+    r = rng.uniform(1.3, 40.0, n)
+    th = rng.uniform(0.01, np.pi - 0.01, n)
+    gcov, gcon, gdet, gpert = ks_metric(r, th, a)
+    rho = 10.0 ** rng.uniform(-6, 0, n)
+    u = 10.0 ** rng.uniform(-8, 2, n)
+    lor = 10.0 ** rng.uniform(0, np.log10(50.0), n)
+    sig = 10.0 ** rng.uniform(-4, 3, n)
+    gs = gcov[:, 1:, 1:]
+
+    def unit(v):
+        nrm = np.sqrt(np.einsum("ni,nij,nj->n", v, gs, v))
+        return v / nrm[:, None]
+
+    ev = unit(rng.normal(size=(n, 3)))
+    ut = np.sqrt(lor * lor - 1.0)[:, None] * ev
+    eb = unit(rng.normal(size=(n, 3)))
+    # b^2 for unit-amplitude B (phys.c:849 get_state), then rescale to b^2/rho = sig
+    alpha = 1.0 / np.sqrt(-gcon[:, 0, 0])
+    beta = gcon[:, 0, 1:] * (alpha * alpha)[:, None]
+    gam_n = np.sqrt(1.0 + np.einsum("ni,nij,nj->n", ut, gs, ut))
+    ucon = np.zeros((n, 4))
+    ucon[:, 0] = gam_n / alpha
+    ucon[:, 1:] = ut - ucon[:, :1] * beta
+    ucov = np.einsum("nij,nj->ni", gcov, ucon)
+    bt = np.einsum("ni,ni->n", eb, ucov[:, 1:])
+    bcon = np.zeros((n, 4))
+    bcon[:, 0] = bt
+    bcon[:, 1:] = (eb + bt[:, None] * ucon[:, 1:]) / ucon[:, :1]
+    bsq1 = np.einsum("ni,nij,nj->n", bcon, gcov, bcon)
+    amp = np.sqrt(sig * rho / bsq1)
+    p = np.zeros((n, 8))
+    p[:, 0] = rho
+    p[:, 1] = u
+    p[:, 2:5] = ut
+    p[:, 5:8] = amp[:, None] * eb
+    guess = p.copy()
+    guess[:, 1] *= 1.0 + 1e-2 * rng.normal(size=n)
+    guess[:, 2:5] *= 1.0 + 1e-2 * rng.normal(size=(n, 3))
+    return gcov, gcon, gdet, gpert, p, guess
+
+
+def edge_cases(rng):
+    """Hand-made records that exercise the failure branches of
+    Utoprim_new_body (utoprim_2d_final.c:218-431)."""
+    gcov, gcon, gdet, gpert, p, guess = sample_c2p(8, rng)
+    guess[0, 2:5] *= 1e4      # utsq of the guess beyond UTSQ_TOO_BIG -> code 2
+    p[1, 1] = 0.0             # cold gas
+    guess[2] = p[2]           # exact guess
+    p[3, 5:8] = 0.0           # unmagnetised
+    guess[3] = p[3]
+    p[4, 2:5] = 0.0           # fluid at rest in the normal frame's coordinate basis
+    p[5, 1] = 1e-12           # near-cold, highly magnetised
+    return gcov, gcon, gdet, gpert, p, guess
+
+
+def run_c2p(harness, gam, n, seed):
+    rng = np.random.default_rng(seed)
+    parts = [sample_c2p(n - 8, rng), edge_cases(rng)]
+    gcov, gcon, gdet, gpert, p, guess = [np.concatenate(x) for x in zip(*parts)]
+    m = p.shape[0]
+    rec = np.concatenate([gcov.reshape(m, 16), gcon.reshape(m, 16), gdet[:, None], gpert, p, guess],
+                         axis=1)
+    assert rec.shape[1] == 53
+    with tempfile.TemporaryDirectory() as tmp:
+        fin = os.path.join(tmp, "in.bin")
+        fout = os.path.join(tmp, "out.bin")
+        rec.astype("<f8").tofile(fin)
+        subprocess.run([harness, "c2p", fin, fout, repr(gam)], cwd=tmp, check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        out = np.fromfile(fout, dtype="<f8").reshape(m, 18)
+    return dict(gcov=gcov, gcon=gcon, gdet=gdet, gcovpert=gpert, p_true=p, p_guess=guess,
+                U=out[:, :8], p_out=out[:, 8:16], pflag=out[:, 16].astype(np.int32),
+                iters=out[:, 17].astype(np.int32), gam=np.float64(gam), seed=np.int64(seed))
+
+
+def run_torus(harness, weno, nsteps):
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "out")
+        subprocess.run([harness, "torus", out, str(weno), str(nsteps)], cwd=tmp, check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        d = {f[:-4]: np.load(os.path.join(out, f)) for f in os.listdir(out) if f.endswith(".npy")}
+    # keep the geometry locations the hot path reads
+    for k in ("gcov", "gcon", "gcovpert", "gdet"):
+        d[k] = np.ascontiguousarray(d[k][:, :, LOCS])
+    d["locs"] = np.array(LOCS, dtype=np.int32)
+    d["weno"] = np.int32(weno)
+    d["nsteps"] = np.int32(nsteps)
+    return d
+
+
+def main():
+    builds = {"t2d": (32, 32, 1), "t3d": (16, 16, 8)}
+    for tag, (n1, n2, n3) in builds.items():
+        subprocess.run([BUILD, tag, str(n1), str(n2), str(n3)], check=True, stdout=subprocess.DEVNULL)
+    h2 = os.path.join(REFDIR, "harness_t2d")
+    h3 = os.path.join(REFDIR, "harness_t3d")
+    np.savez_compressed(os.path.join(HERE, "c2p_g43.npz"), **run_c2p(h2, 4.0 / 3.0, 1024, C2P_SEED))
+    np.savez_compressed(os.path.join(HERE, "c2p_g53.npz"), **run_c2p(h2, 5.0 / 3.0, 1024, C2P_SEED + 1))
+    np.savez_compressed(os.path.join(HERE, "t2d_mc.npz"), **run_torus(h2, 0, 5))
+    np.savez_compressed(os.path.join(HERE, "t2d_weno.npz"), **run_torus(h2, 1, 5))
+    np.savez_compressed(os.path.join(HERE, "t3d_weno.npz"), **run_torus(h3, 1, 1))
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    sys.exit(main())
