@@ -1,0 +1,397 @@
+// Host side of the C ABI (include/harmgpu.h): context, device memory,
+// layout conversion and the RK2 driver that plays the role of step_ch.c.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "harmgpu.h"
+#include "hg_grid.h"
+#include "hg_host.h"
+#include "hg_kernels.h"
+
+using namespace hg;
+
+static thread_local std::string g_last_error;
+
+void hg::set_error(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+extern "C" const char *hg_last_error(void) { return g_last_error.c_str(); }
+
+#define HIPCHECK(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      hg::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return 1;                                                                     \
+    }                                                                               \
+  } while (0)
+
+static int dalloc(double **p, size_t n) {
+  HIPCHECK(hipMalloc((void **)p, n * sizeof(double)));
+  HIPCHECK(hipMemset(*p, 0, n * sizeof(double)));
+  return 0;
+}
+
+extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
+  if (!cfg || !out) { set_error("hg_ctx_create: null argument"); return 1; }
+  *out = nullptr;
+  if (cfg->n1 < 1 || cfg->n2 < 1 || cfg->n3 < 1) { set_error("hg_ctx_create: grid sizes must be >= 1"); return 1; }
+  if (cfg->lim != HG_LIM_MC && cfg->lim != HG_LIM_WENO5BND) { set_error("hg_ctx_create: unsupported lim %d", cfg->lim); return 1; }
+  if (cfg->fluxmethod != HG_FLUX_LAXF && cfg->fluxmethod != HG_FLUX_HLL) {
+    set_error("hg_ctx_create: unsupported fluxmethod %d", cfg->fluxmethod);
+    return 1;
+  }
+  if (cfg->lim == HG_LIM_WENO5BND) { set_error("hg_ctx_create: WENO5BND not available in this build"); return 1; }
+  int need = (cfg->lim == HG_LIM_WENO5BND) ? 6 : 3;
+  if (cfg->nb < need) { set_error("hg_ctx_create: nb=%d < %d ghost layers needed", cfg->nb, need); return 1; }
+  if (cfg->n3 > 1 && cfg->n3 < cfg->nb) { set_error("hg_ctx_create: periodic n3=%d < nb=%d", cfg->n3, cfg->nb); return 1; }
+  if (cfg->n2 > 1 && cfg->n2 < cfg->nb) { set_error("hg_ctx_create: n2=%d < nb=%d", cfg->n2, cfg->nb); return 1; }
+  hg_ctx *c = new hg_ctx();
+  c->cfg = *cfg;
+  c->d = make_dims(cfg->n1, cfg->n2, cfg->n3, cfg->nb);
+  const Dims &d = c->d;
+  size_t ns = (size_t)d.spl * NPR, nz = (size_t)d.spl;
+  int rc = 0;
+  rc |= dalloc(&c->p, ns);
+  rc |= dalloc(&c->pk, ns);
+  for (int k = 0; k < 3; k++) rc |= dalloc(&c->F[k], ns);
+  for (int k = 0; k < 3; k++) rc |= dalloc(&c->emf[k], nz);
+  if (cfg->lim == HG_LIM_WENO5BND) {
+    rc |= dalloc(&c->pl, ns);
+    rc |= dalloc(&c->pr, ns);
+  }
+  rc |= dalloc(&c->geo, (size_t)4 * GCOMP * d.gplane);
+  rc |= dalloc(&c->corn, (size_t)3 * d.gplane);
+  rc |= dalloc(&c->conn, (size_t)64 * d.cplane);
+  rc |= dalloc(&c->floors, (size_t)2 * d.cplane);
+  rc |= dalloc(&c->dt, 1);
+  if (!rc) {
+    hipError_t e = hipMalloc((void **)&c->ndt_bits, 3 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->counters, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->err, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->counters, 0, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->err, 0, sizeof(int));
+    if (e != hipSuccess) { set_error("hg_ctx_create: %s", hipGetErrorString(e)); rc = 1; }
+  }
+  if (rc) { hg_ctx_destroy(c); return 1; }
+  *out = c;
+  return 0;
+}
+
+extern "C" int hg_ctx_destroy(hg_ctx *c) {
+  if (!c) return 0;
+  double *bufs[] = {c->p, c->pk, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2],
+                    c->pl, c->pr, c->geo, c->corn, c->conn, c->floors, c->dt};
+  for (double *b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->ndt_bits) (void)hipFree(c->ndt_bits);
+  if (c->counters) (void)hipFree(c->counters);
+  if (c->err) (void)hipFree(c->err);
+  delete c;
+  return 0;
+}
+
+// geometry in the host layout of harmgpu.h -> device tables of hg_grid.h
+int hg::upload_geometry_tables(hg_ctx *c, const double *gcov, const double *gcon, const double *gcovpert,
+                               const double *gdet, const double *conn, const double *r_cent) {
+  const Dims &d = c->d;
+  // host layout: [i][j][loc7] with loc order FACE1 FACE2 FACE3 CORN1 CORN2 CORN3 CENT
+  static const int full_src[4] = {6, 0, 1, 2};  // CENT, FACE1, FACE2, FACE3
+  std::vector<double> geo((size_t)4 * GCOMP * d.gplane), corn((size_t)3 * d.gplane);
+  for (int ii = 0; ii < d.g1; ii++)
+    for (int jj = 0; jj < d.g2; jj++) {
+      size_t h = ((size_t)ii * d.g2 + jj) * 7;
+      size_t g = (size_t)ii + (size_t)jj * d.g1;
+      for (int l = 0; l < 4; l++) {
+        size_t hs = h + full_src[l];
+        double *b = &geo[(size_t)l * GCOMP * d.gplane + g];
+        for (int a = 0; a < 16; a++) {
+          b[a * d.gplane] = gcov[hs * 16 + a];
+          b[(16 + a) * d.gplane] = gcon[hs * 16 + a];
+        }
+        b[32 * d.gplane] = gcovpert[hs * 4 + 0];
+        b[33 * d.gplane] = gdet[hs];
+      }
+      for (int l = 0; l < 3; l++) corn[(size_t)l * d.gplane + g] = gdet[h + 3 + l];
+    }
+  std::vector<double> cn((size_t)64 * d.cplane), fl((size_t)2 * d.cplane);
+  for (int ii = 0; ii < d.m1; ii++)
+    for (int jj = 0; jj < d.m2; jj++) {
+      size_t h = (size_t)ii * d.m2 + jj;
+      size_t g = (size_t)ii + (size_t)jj * d.m1;
+      for (int a = 0; a < 64; a++) cn[(size_t)a * d.cplane + g] = conn[h * 64 + a];
+      double r = r_cent[h];
+      // fixup.c:1562-1568 set_density_floors, rescaletype 0
+      fl[g] = c->cfg.rhomin * pow(r, -1.5);
+      fl[d.cplane + g] = c->cfg.uumin * pow(r, -2.5);
+    }
+  HIPCHECK(hipMemcpy(c->geo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->corn, corn.data(), corn.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->conn, cn.data(), cn.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->floors, fl.data(), fl.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->have_geometry = true;
+  return 0;
+}
+
+extern "C" int hg_upload_geometry(hg_ctx *c, const double *gcov, const double *gcon, const double *gcovpert,
+                                  const double *gdet, const double *conn, const double *r_cent) {
+  if (!c || !gcov || !gcon || !gcovpert || !gdet || !conn || !r_cent) {
+    set_error("hg_upload_geometry: null argument");
+    return 1;
+  }
+  return upload_geometry_tables(c, gcov, gcon, gcovpert, gdet, conn, r_cent);
+}
+
+// AoS host [i][j][k][pl] <-> SoA device [pl][k][j][i]
+int hg::copy_prims(hg_ctx *c, double *dev, const double *host) {
+  const Dims &d = c->d;
+  std::vector<double> t((size_t)d.spl * NPR);
+  for (int ii = 0; ii < d.m1; ii++)
+    for (int jj = 0; jj < d.m2; jj++)
+      for (int kk = 0; kk < d.m3; kk++) {
+        size_t h = (((size_t)ii * d.m2 + jj) * d.m3 + kk) * NPR;
+        size_t g = (size_t)ii + (size_t)jj * d.sj + (size_t)kk * d.sk;
+        for (int pl = 0; pl < NPR; pl++) t[g + pl * d.spl] = host[h + pl];
+      }
+  HIPCHECK(hipMemcpy(dev, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int hg::fetch_prims(hg_ctx *c, const double *dev, double *host) {
+  const Dims &d = c->d;
+  std::vector<double> t((size_t)d.spl * NPR);
+  HIPCHECK(hipMemcpy(t.data(), dev, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int ii = 0; ii < d.m1; ii++)
+    for (int jj = 0; jj < d.m2; jj++)
+      for (int kk = 0; kk < d.m3; kk++) {
+        size_t h = (((size_t)ii * d.m2 + jj) * d.m3 + kk) * NPR;
+        size_t g = (size_t)ii + (size_t)jj * d.sj + (size_t)kk * d.sk;
+        for (int pl = 0; pl < NPR; pl++) host[h + pl] = t[g + pl * d.spl];
+      }
+  return 0;
+}
+
+extern "C" int hg_set_prims(hg_ctx *c, const double *p) {
+  if (!c || !p) { set_error("hg_set_prims: null argument"); return 1; }
+  return copy_prims(c, c->p, p);
+}
+
+extern "C" int hg_get_prims(hg_ctx *c, double *p) {
+  if (!c || !p) { set_error("hg_get_prims: null argument"); return 1; }
+  HIPCHECK(hipDeviceSynchronize());
+  return fetch_prims(c, c->p, p);
+}
+
+static int check_err(hg_ctx *c) {
+  int e = 0;
+  HIPCHECK(hipMemcpy(&e, c->err, sizeof(int), hipMemcpyDeviceToHost));
+  if (e) {
+    HIPCHECK(hipMemset(c->err, 0, sizeof(int)));
+    set_error("device reported failure mask %d (1: ghost fill, 2: face state, 4: zone update)", e);
+    return 1;
+  }
+  return 0;
+}
+
+static int need_geometry(hg_ctx *c, const char *who) {
+  if (!c) { set_error("%s: null context", who); return 1; }
+  if (!c->have_geometry) { set_error("%s: geometry not set", who); return 1; }
+  return 0;
+}
+
+extern "C" int hg_bound_prim(hg_ctx *c, void *stream) {
+  if (need_geometry(c, "hg_bound_prim")) return 1;
+  hipStream_t s = (hipStream_t)stream;
+  launch_bound(c->p, c->geo, c->d, c->cfg.gammamax, c->err, s);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s));
+  return check_err(c);
+}
+
+// fluxcalc (flux.c:7) on primitives pb: per-direction fluxes and ndt bits,
+// then the flux-CT EMFs (fluxct.c:10)
+static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
+  const Dims &d = c->d;
+  const hg_config &f = c->cfg;
+  int nn[4] = {0, d.n1, d.n2, d.n3};
+  launch_reset_ndt(c->ndt_bits, s);
+  for (int dir = 1; dir <= 3; dir++) {
+    if (nn[dir] <= 1) continue;
+    if (f.lim == HG_LIM_MC) {
+      launch_flux_mc(dir, pb, c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,
+                     c->ndt_bits + dir - 1, c->err, s);
+    } else {
+      launch_weno(dir, pb, c->geo, c->pl, c->pr, d, f.gam, s);
+      launch_flux_lr(dir, c->pl, c->pr, c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,
+                     c->ndt_bits + dir - 1, c->err, s);
+    }
+  }
+  launch_emf(c->F[0], c->F[1], c->F[2], c->geo, c->corn, c->emf[0], c->emf[1], c->emf[2], d, s);
+}
+
+// the field fluxes rewritten from the EMFs (fluxct.c:720-781), on the host,
+// for hg_fluxcalc's diagnostic output only
+static void rewrite_field_fluxes(const Dims &d, double *F1, double *F2, double *F3, const double *e1,
+                                 const double *e2, const double *e3) {
+  int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  auto Z = [&](int i, int j, int k) { return (size_t)d.idx(i, j, k); };
+  for (int i = 0; i <= d.n1 - 1 + s1; i++)
+    for (int j = 0; j <= d.n2 - 1 + s2; j++)
+      for (int k = 0; k <= d.n3 - 1 + s3; k++) {
+        size_t z = Z(i, j, k);
+        int ip = s1 ? i + 1 : i, jp = s2 ? j + 1 : j, kp = s3 ? k + 1 : k;
+        size_t P = (size_t)d.spl;
+        if (s1 && j < d.n2 && k < d.n3) {
+          F1[z + B1 * P] = 0.;
+          F1[z + B2 * P] = 0.5 * (e3[z] + e3[Z(i, jp, k)]);
+          F1[z + B3 * P] = -0.5 * (e2[z] + e2[Z(i, j, kp)]);
+        }
+        if (s2 && i < d.n1 && k < d.n3) {
+          F2[z + B1 * P] = -0.5 * (e3[z] + e3[Z(ip, j, k)]);
+          F2[z + B2 * P] = 0.;
+          F2[z + B3 * P] = 0.5 * (e1[z] + e1[Z(i, j, kp)]);
+        }
+        if (s3 && i < d.n1 && j < d.n2) {
+          F3[z + B1 * P] = 0.5 * (e2[z] + e2[Z(ip, j, k)]);
+          F3[z + B2 * P] = -0.5 * (e1[z] + e1[Z(i, jp, k)]);
+          F3[z + B3 * P] = 0.;
+        }
+      }
+}
+
+extern "C" int hg_fluxcalc(hg_ctx *c, double *F1, double *F2, double *F3, double ndt[3], void *stream) {
+  if (need_geometry(c, "hg_fluxcalc")) return 1;
+  hipStream_t s = (hipStream_t)stream;
+  enqueue_fluxcalc(c, c->p, s);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s));
+  if (check_err(c)) return 1;
+  if (ndt) {
+    unsigned long long b[3];
+    HIPCHECK(hipMemcpy(b, c->ndt_bits, sizeof b, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; k++) memcpy(&ndt[k], &b[k], sizeof(double));
+  }
+  const Dims &d = c->d;
+  std::vector<double> h[3], e[3];
+  for (int k = 0; k < 3; k++) {
+    h[k].resize((size_t)d.spl * NPR);
+    e[k].resize((size_t)d.spl);
+    HIPCHECK(hipMemcpy(h[k].data(), c->F[k], h[k].size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(e[k].data(), c->emf[k], e[k].size() * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  rewrite_field_fluxes(d, h[0].data(), h[1].data(), h[2].data(), e[0].data(), e[1].data(), e[2].data());
+  double *outs[3] = {F1, F2, F3};
+  for (int k = 0; k < 3; k++) {
+    if (!outs[k]) continue;
+    for (int ii = 0; ii < d.m1; ii++)
+      for (int jj = 0; jj < d.m2; jj++)
+        for (int kk = 0; kk < d.m3; kk++) {
+          size_t hh = (((size_t)ii * d.m2 + jj) * d.m3 + kk) * NPR;
+          size_t g = (size_t)ii + (size_t)jj * d.sj + (size_t)kk * d.sk;
+          for (int pl = 0; pl < NPR; pl++) outs[k][hh + pl] = h[k][g + pl * d.spl];
+        }
+  }
+  return 0;
+}
+
+// one substep of step_ch.c:94-166 (advance + bound_prim; post_fixup is a
+// no-op while no zone fails its inversion)
+static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, double cuf2, int finalstep,
+                            hipStream_t s) {
+  const hg_config &f = c->cfg;
+  enqueue_fluxcalc(c, pb, s);
+  launch_update(pi, pb, pf, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2], c->geo, c->conn,
+                c->floors, c->dt, cuf2, finalstep, c->d, f.gam, f.dx, f.gammamax, c->counters, c->err, s);
+  if (finalstep) launch_dt(c->ndt_bits, c->dt, f.defcon, f.safe, s);
+  // step_ch.c:127 and :166 call bound_prim twice per substep; the second call
+  // is not a repeat of the first: POLEDEATH has zeroed U2 in the rows next to
+  // the axis, which the radial inflow check then sees in the corner ghosts
+  launch_bound(pf, c->geo, c->d, f.gammamax, c->err, s);
+  launch_bound(pf, c->geo, c->d, f.gammamax, c->err, s);
+}
+
+// step_ch.c:442 setup_rktimestep, TIMEORDER 2: CUf[0][2]=0.5, CUf[1][2]=1
+static void enqueue_step(hg_ctx *c, hipStream_t s) {
+  enqueue_substep(c, c->p, c->p, c->pk, 0.5, 0, s);
+  enqueue_substep(c, c->p, c->pk, c->p, 1.0, 1, s);
+}
+
+extern "C" int hg_set_dt(hg_ctx *c, double dt) {
+  if (!c) { set_error("hg_set_dt: null context"); return 1; }
+  HIPCHECK(hipMemcpy(c->dt, &dt, sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int hg_get_dt(hg_ctx *c, double *dt) {
+  if (!c || !dt) { set_error("hg_get_dt: null argument"); return 1; }
+  HIPCHECK(hipMemcpy(dt, c->dt, sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int hg_get_counters(hg_ctx *c, int64_t *nfail, int64_t *iters) {
+  if (!c) { set_error("hg_get_counters: null context"); return 1; }
+  unsigned long long h[2];
+  HIPCHECK(hipMemcpy(h, c->counters, sizeof h, hipMemcpyDeviceToHost));
+  if (nfail) *nfail = (int64_t)h[0];
+  if (iters) *iters = (int64_t)h[1];
+  HIPCHECK(hipMemset(c->counters, 0, sizeof h));
+  return 0;
+}
+
+extern "C" int hg_step_async(hg_ctx *c, int nsteps, void *stream) {
+  if (need_geometry(c, "hg_step_async")) return 1;
+  if (nsteps < 0) { set_error("hg_step_async: nsteps < 0"); return 1; }
+  hipStream_t s = (hipStream_t)stream;
+  for (int n = 0; n < nsteps; n++) enqueue_step(c, s);
+  HIPCHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int hg_step(hg_ctx *c, double *dt, int64_t *nfail, int64_t *iters, void *stream) {
+  if (need_geometry(c, "hg_step")) return 1;
+  if (!dt) { set_error("hg_step: null dt"); return 1; }
+  hipStream_t s = (hipStream_t)stream;
+  if (hg_set_dt(c, *dt)) return 1;
+  HIPCHECK(hipMemset(c->counters, 0, 2 * sizeof(unsigned long long)));
+  enqueue_step(c, s);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s));
+  if (check_err(c)) return 1;
+  if (hg_get_dt(c, dt)) return 1;
+  return hg_get_counters(c, nfail, iters);
+}
+
+extern "C" int hg_utoprim_batch(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
+                                const double *gdet, const double *gcovpert, double *pr, int32_t *pflag,
+                                int32_t *iters, int mode, void *stream) {
+  if (n < 0) { set_error("hg_utoprim_batch: n < 0"); return 1; }
+  if (n == 0) return 0;
+  if (!U || !gcov || !gcon || !gdet || !gcovpert || !pr || !pflag) {
+    set_error("hg_utoprim_batch: null argument");
+    return 1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (mode == HG_C2P_FP64) {
+    launch_c2p_batch(n, gam, U, gcov, gcon, gdet, gcovpert, pr, pflag, iters, s);
+  } else if (mode == HG_C2P_MIXED) {
+    set_error("hg_utoprim_batch: mixed-precision mode not available in this build");
+    return 1;
+  } else {
+    set_error("hg_utoprim_batch: unknown mode %d", mode);
+    return 1;
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s));
+  return 0;
+}

@@ -1,0 +1,31 @@
+// Internal host-side definitions shared by the C-ABI translation units.
+#pragma once
+#include <cstdarg>
+
+#include "harmgpu.h"
+#include "hg_grid.h"
+
+struct hg_ctx {
+  hg_config cfg;
+  hg::Dims d;
+  double *p = nullptr, *pk = nullptr;        // primitives and the RK2 midpoint state
+  double *F[3] = {nullptr, nullptr, nullptr};
+  double *emf[3] = {nullptr, nullptr, nullptr};
+  double *pl = nullptr, *pr = nullptr;       // WENO5BND interface states (per direction, reused)
+  double *geo = nullptr, *corn = nullptr, *conn = nullptr, *floors = nullptr;
+  double *dt = nullptr;
+  unsigned long long *ndt_bits = nullptr, *counters = nullptr;
+  int *err = nullptr;
+  bool have_geometry = false;
+};
+
+namespace hg {
+void set_error(const char *fmt, ...);
+int upload_geometry_tables(hg_ctx *c, const double *gcov, const double *gcon, const double *gcovpert,
+                           const double *gdet, const double *conn, const double *r_cent);
+int copy_prims(hg_ctx *c, double *dev, const double *host);
+int fetch_prims(hg_ctx *c, const double *dev, double *host);
+void launch_c2p_batch_mixed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
+                            const double *gdet, const double *gcovpert, double *pr, int *pflag, int *iters,
+                            hipStream_t s);
+}  // namespace hg

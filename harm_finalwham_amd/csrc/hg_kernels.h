@@ -1,0 +1,34 @@
+// Launchers of the device kernels (host side of hg_kernels.hip / hg_weno.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hg_grid.h"
+
+namespace hg {
+
+void launch_bound(double *p, const double *geo, const Dims &d, double gammamax, int *err, hipStream_t s);
+void launch_flux_mc(int dir, const double *p, const double *geo, double *F, const Dims &d, double gam,
+                    double cour, double dxdir, int fluxmethod, unsigned long long *ndt_bits, int *err,
+                    hipStream_t s);
+void launch_flux_lr(int dir, const double *pleft, const double *pright, const double *geo, double *F,
+                    const Dims &d, double gam, double cour, double dxdir, int fluxmethod,
+                    unsigned long long *ndt_bits, int *err, hipStream_t s);
+void launch_emf(const double *F1, const double *F2, const double *F3, const double *geo, const double *corn,
+                double *e1, double *e2, double *e3, const Dims &d, hipStream_t s);
+void launch_update(const double *pi, const double *pb, double *pf, const double *F1, const double *F2,
+                   const double *F3, const double *e1, const double *e2, const double *e3, const double *geo,
+                   const double *conn, const double *floors, const double *dtp, double cuf2, int finalstep,
+                   const Dims &d, double gam, const double *dx, double gammamax, unsigned long long *counters,
+                   int *err, hipStream_t s);
+void launch_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe, hipStream_t s);
+void launch_reset_ndt(unsigned long long *ndt_bits, hipStream_t s);
+void launch_c2p_batch(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
+                      const double *gdet, const double *gcovpert, double *pr, int *pflag, int *iters,
+                      hipStream_t s);
+
+// WENO5BND interface states along dir for every zone (hg_weno.hip); pleft/pright
+// have the layout of the primitives
+void launch_weno(int dir, const double *p, const double *geo, double *pleft, double *pright, const Dims &d,
+                 double gam, hipStream_t s);
+
+}  // namespace hg

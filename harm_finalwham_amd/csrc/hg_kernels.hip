@@ -1,0 +1,612 @@
+// HIP kernels of the per-step GRMHD hot path for gfx950 (FP64 vector ALU).
+//
+//   k_bound_x1/x2/x3   ghost fill (bounds.fishmon.c:29)
+//   k_flux_mc<dir>     MC reconstruction fused with the Riemann flux (flux.c:469)
+//   k_flux_lr<dir>     Riemann flux from precomputed interface states (WENO5BND)
+//   k_emf              flux-CT corner EMFs (fluxct.c:10)
+//   k_update           conserved update + source + Noble 2D + fixup1zone (advance.c:58)
+//   k_dt               time-step update (advance.c:231, step_ch.c:243)
+//   k_c2p_batch        batched inversion (utoprimgen.c:23)
+#include <hip/hip_runtime.h>
+
+#include "hg_c2p.h"
+#include "hg_grid.h"
+#include "hg_kernels.h"
+
+namespace hg {
+
+__device__ __forceinline__ void load_geom(const double *__restrict__ full, int loc, int64_t gplane, int64_t gi,
+                                          Geom &G) {
+  const double *b = full + (int64_t)loc * GCOMP * gplane + gi;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      G.gcov[a][c] = b[(a * 4 + c) * gplane];
+      G.gcon[a][c] = b[(16 + a * 4 + c) * gplane];
+    }
+  G.gpert0 = b[32 * gplane];
+  G.g = b[33 * gplane];
+}
+
+// block-wide min of a positive double, folded into *out with an integer
+// atomic (positive IEEE doubles order like their bit patterns)
+__device__ __forceinline__ void block_min_to(double v, unsigned long long *out) {
+  __shared__ double red[32];
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_down(v, off, 64);
+    v = (o < v) ? o : v;
+  }
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+    double m = red[0];
+    for (int w = 1; w < nw; w++) m = (red[w] < m) ? red[w] : m;
+    atomicMin(out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+__device__ __forceinline__ void block_sum_to(unsigned long long a, unsigned long long b,
+                                             unsigned long long *out_a, unsigned long long *out_b) {
+  __shared__ unsigned long long ra[32], rb[32];
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { ra[wid] = a; rb[wid] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+    unsigned long long sa = 0, sb = 0;
+    for (int w = 0; w < nw; w++) { sa += ra[w]; sb += rb[w]; }
+    if (sa) atomicAdd(out_a, sa);
+    if (sb) atomicAdd(out_b, sb);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ghost fill, bounds.fishmon.c:29 (EXTRAP 0, POLEDEATH 1, POSDEFMETRIC 0)
+// ---------------------------------------------------------------------------
+__global__ void k_bound_x1(double *__restrict__ p, const double *__restrict__ geo, Dims d, double gammamax,
+                           int *__restrict__ err) {
+  // threads: ((side*nb1 + g) * n3 + k) * n2 + j
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t total = 2LL * d.nb1 * d.n2 * d.n3;
+  if (t >= total) return;
+  int j = t % d.n2;
+  int64_t r = t / d.n2;
+  int k = r % d.n3;
+  r /= d.n3;
+  int g = r % d.nb1;
+  int side = (int)(r / d.nb1);
+  int i = side ? d.n1 + g : -1 - g;
+  int isrc = side ? d.n1 - 1 : 0;
+  double pr[NPR];
+  int64_t src = d.idx(isrc, j, k), dst = d.idx(i, j, k);
+#pragma unroll
+  for (int pl = 0; pl < NPR; pl++) pr[pl] = p[src + pl * d.spl];
+  Geom G;
+  load_geom(geo, GL_CENT, d.gplane, d.gidx(i, j), G);
+  if (inflow_check_r(pr, G, side ? +1 : -1, gammamax) > 0) atomicOr(err, 1);
+  if (limit_gamma(gammamax, pr, G) > 0) atomicOr(err, 1);
+#pragma unroll
+  for (int pl = 0; pl < NPR; pl++) p[dst + pl * d.spl] = pr[pl];
+}
+
+__global__ void k_bound_x2(double *__restrict__ p, Dims d) {
+  // threads: ((slot * n3 + k) * m1 + ii), slot in [0, 2*nb2)
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t total = 2LL * d.nb2 * d.n3 * d.m1;
+  if (t >= total) return;
+  int ii = t % d.m1;
+  int64_t r = t / d.m1;
+  int k = r % d.n3;
+  int slot = (int)(r / d.n3);
+  int i = ii - d.nb1;
+  int j, jsrc, jact = -1;
+  if (slot < d.nb2) {  // inner: j = -1-slot mirrors row slot
+    j = -1 - slot;
+    jsrc = slot;
+    if (slot == 0) jact = 0;
+  } else {
+    int s = slot - d.nb2;  // outer: j = n2+s mirrors row n2-1-s
+    j = d.n2 + s;
+    jsrc = d.n2 - 1 - s;
+    if (s == 0) jact = d.n2 - 1;
+  }
+  int64_t src = d.idx(i, jsrc, k), dst = d.idx(i, j, k);
+  double pr[NPR];
+#pragma unroll
+  for (int pl = 0; pl < NPR; pl++) pr[pl] = p[src + pl * d.spl];
+  pr[U2] *= 0;
+  pr[B2] *= 0.;
+#pragma unroll
+  for (int pl = 0; pl < NPR; pl++) p[dst + pl * d.spl] = pr[pl];
+  if (jact >= 0) {  // POLEDEATH also clears the first active row next to the axis
+    int64_t a = d.idx(i, jact, k);
+    p[a + U2 * d.spl] *= 0;
+    p[a + B2 * d.spl] *= 0.;
+  }
+}
+
+__global__ void k_bound_x3(double *__restrict__ p, Dims d) {
+  // threads: ((slot * m2 + jj) * m1 + ii), slot in [0, 2*nb3)
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t total = 2LL * d.nb3 * d.m2 * d.m1;
+  if (t >= total) return;
+  int ii = t % d.m1;
+  int64_t r = t / d.m1;
+  int jj = r % d.m2;
+  int slot = (int)(r / d.m2);
+  int i = ii - d.nb1, j = jj - d.nb2;
+  int k, ksrc;
+  if (slot < d.nb3) { k = -1 - slot; ksrc = d.n3 + k; }
+  else { k = d.n3 + (slot - d.nb3); ksrc = k - d.n3; }
+  int64_t src = d.idx(i, j, ksrc), dst = d.idx(i, j, k);
+#pragma unroll
+  for (int pl = 0; pl < NPR; pl++) p[dst + pl * d.spl] = p[src + pl * d.spl];
+}
+
+// ---------------------------------------------------------------------------
+// Riemann flux at one face from left/right states, fluxcompute.c:12
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool face_flux(int dir, const Geom &G, double *p_l, double *p_r, double gam,
+                                          int fluxmethod, double *F, double &ctop) {
+  // flux.c:629 BDIRCONT
+  double bav = 0.5 * (p_l[B1 + dir - 1] + p_r[B1 + dir - 1]);
+  p_l[B1 + dir - 1] = bav;
+  p_r[B1 + dir - 1] = bav;
+  State ql, qr;
+  double F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
+  if (get_state(p_l, G, ql)) return true;
+  {
+    Maxwell M;
+    maxwell(p_l, ql, G, M);
+    double p1 = one_plus_ud0(ql, G);
+    primtoflux(p_l, ql, M, p1, dir, G, gam, F_l);
+    primtoflux(p_l, ql, M, p1, 0, G, gam, U_l);
+  }
+  if (get_state(p_r, G, qr)) return true;
+  {
+    Maxwell M;
+    maxwell(p_r, qr, G, M);
+    double p1 = one_plus_ud0(qr, G);
+    primtoflux(p_r, qr, M, p1, dir, G, gam, F_r);
+    primtoflux(p_r, qr, M, p1, 0, G, gam, U_r);
+  }
+  double cmax_l, cmin_l, cmax_r, cmin_r;
+  if (vchar(p_l, ql, dir, G, gam, cmax_l, cmin_l)) return true;
+  if (vchar(p_r, qr, dir, G, gam, cmax_r, cmin_r)) return true;
+  // wavespeeds.c:325 cminmax_calc
+  double lmin = rmin(cmin_l, cmin_r);
+  double lmax = rmax(cmax_l, cmax_r);
+  lmin = fabs(rmax(0., -lmin));
+  lmax = fabs(rmax(0., lmax));
+  ctop = rmax(lmax, lmin);
+  if (fluxmethod == 1 && (lmax + lmin != 0.0)) {
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++)
+      F[pl] = ((lmax * F_l[pl] + lmin * F_r[pl]) - (lmax * lmin) * (U_r[pl] - U_l[pl])) / (lmax + lmin);
+  } else {
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) F[pl] = 0.5 * ((F_l[pl] + F_r[pl]) - ctop * (U_r[pl] - U_l[pl]));
+  }
+  return false;
+}
+
+// face range of interp_loop_set (initbase.c:1027-1054): faces 0..n along dir,
+// one extra layer on each side transverse to it
+struct FaceRange {
+  int is, js, ks, ni, nj, nk;
+};
+__host__ __device__ inline FaceRange face_range(const Dims &d, int dir) {
+  int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  FaceRange f;
+  f.is = -s1; f.ni = d.n1 + 2 * s1;
+  f.js = -s2; f.nj = d.n2 + 2 * s2;
+  f.ks = -s3; f.nk = d.n3 + 2 * s3;
+  if (dir == 1) { f.is = 0; f.ni = d.n1 + s1; }
+  if (dir == 2) { f.js = 0; f.nj = d.n2 + s2; }
+  if (dir == 3) { f.ks = 0; f.nk = d.n3 + s3; }
+  return f;
+}
+
+// global.h:361 MINMOD
+__device__ __forceinline__ double minmod(double a, double b) {
+  return ((a) * (b) <= 0) ? 0.0 : ((fabs(a) < fabs(b)) ? (a) : (b));
+}
+// interppoint.c:387 slope_lim_3points, MC
+__device__ __forceinline__ double mc_slope(double yl, double yc, double yr) {
+  double Dqm = 2.0 * (yc - yl);
+  double Dqp = 2.0 * (yr - yc);
+  double Dqc = 0.5 * (yr - yl);
+  double inner = minmod(Dqm, Dqp);
+  return minmod(Dqc, inner);
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) k_flux_mc(const double *__restrict__ p, const double *__restrict__ geo,
+                                                 double *__restrict__ F, Dims d, double gam, double cour,
+                                                 double dxdir, int fluxmethod,
+                                                 unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
+  FaceRange fr = face_range(d, DIR);
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)fr.ni * fr.nj * fr.nk;
+  double dtij = 1.e+50;
+  if (t < total) {
+    int i = fr.is + (int)(t % fr.ni);
+    int64_t r = t / fr.ni;
+    int j = fr.js + (int)(r % fr.nj);
+    int k = fr.ks + (int)(r / fr.nj);
+    int64_t z = d.idx(i, j, k);
+    int64_t s = (DIR == 1) ? 1 : (DIR == 2) ? d.sj : d.sk;
+    double p_l[NPR], p_r[NPR];
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) {
+      const double *c = p + z + pl * d.spl;
+      double ym2 = c[-2 * s], ym1 = c[-s], y0 = c[0], yp1 = c[s];
+      // flux.c:800-801 getp2interplr
+      p_l[pl] = ym1 + 0.5 * mc_slope(ym2, ym1, y0);
+      p_r[pl] = y0 - 0.5 * mc_slope(ym1, y0, yp1);
+    }
+    Geom G;
+    load_geom(geo, DIR, d.gplane, d.gidx(i, j), G);
+    double Fz[NPR], ctop;
+    if (face_flux(DIR, G, p_l, p_r, gam, fluxmethod, Fz, ctop)) {
+      atomicOr(err, 2);
+    } else {
+#pragma unroll
+      for (int pl = 0; pl < NPR; pl++) F[z + pl * d.spl] = Fz[pl];
+      dtij = cour * dxdir / ctop;
+    }
+  }
+  block_min_to(dtij, ndt_bits);
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) k_flux_lr(const double *__restrict__ pleft,
+                                                 const double *__restrict__ pright,
+                                                 const double *__restrict__ geo, double *__restrict__ F, Dims d,
+                                                 double gam, double cour, double dxdir, int fluxmethod,
+                                                 unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
+  FaceRange fr = face_range(d, DIR);
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)fr.ni * fr.nj * fr.nk;
+  double dtij = 1.e+50;
+  if (t < total) {
+    int i = fr.is + (int)(t % fr.ni);
+    int64_t r = t / fr.ni;
+    int j = fr.js + (int)(r % fr.nj);
+    int k = fr.ks + (int)(r / fr.nj);
+    int64_t z = d.idx(i, j, k);
+    int64_t s = (DIR == 1) ? 1 : (DIR == 2) ? d.sj : d.sk;
+    double p_l[NPR], p_r[NPR];
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) {
+      p_l[pl] = pright[z - s + pl * d.spl];
+      p_r[pl] = pleft[z + pl * d.spl];
+    }
+    Geom G;
+    load_geom(geo, DIR, d.gplane, d.gidx(i, j), G);
+    double Fz[NPR], ctop;
+    if (face_flux(DIR, G, p_l, p_r, gam, fluxmethod, Fz, ctop)) {
+      atomicOr(err, 2);
+    } else {
+#pragma unroll
+      for (int pl = 0; pl < NPR; pl++) F[z + pl * d.spl] = Fz[pl];
+      dtij = cour * dxdir / ctop;
+    }
+  }
+  block_min_to(dtij, ndt_bits);
+}
+
+// ---------------------------------------------------------------------------
+// flux-CT corner EMFs, fluxct.c:74-218 (FLUXCTTOTH, CORNGDETVERSION 1):
+// field fluxes divided by gdet at their faces, averaged to the corner,
+// multiplied by gdet at the corner.
+// ---------------------------------------------------------------------------
+__global__ void k_emf(const double *__restrict__ F1, const double *__restrict__ F2, const double *__restrict__ F3,
+                      const double *__restrict__ geo, const double *__restrict__ corn, double *__restrict__ emf1,
+                      double *__restrict__ emf2, double *__restrict__ emf3, Dims d) {
+  int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  int ni = d.n1 + s1, nj = d.n2 + s2, nk = d.n3 + s3;
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ni * nj * nk) return;
+  int i = (int)(t % ni);
+  int64_t r = t / ni;
+  int j = (int)(r % nj);
+  int k = (int)(r / nj);
+  int64_t z = d.idx(i, j, k);
+  int64_t zim = s1 ? z - 1 : z, zjm = s2 ? z - d.sj : z, zkm = s3 ? z - d.sk : z;
+  const int64_t P = d.spl;
+  int64_t g0 = d.gidx(i, j);
+  int64_t gim = s1 ? d.gidx(i - 1, j) : g0, gjm = s2 ? d.gidx(i, j - 1) : g0;
+  // gdet at FACE1/2/3 of (i,j) and of the neighbours the averages read
+  const double *gF1 = geo + ((int64_t)GL_FACE1 * GCOMP + 33) * d.gplane;
+  const double *gF2 = geo + ((int64_t)GL_FACE2 * GCOMP + 33) * d.gplane;
+  const double *gF3 = geo + ((int64_t)GL_FACE3 * GCOMP + 33) * d.gplane;
+  double c1 = (s2 && s3) ? 0.25 : 0.5, c2 = (s1 && s3) ? 0.25 : 0.5, c3 = (s1 && s2) ? 0.25 : 0.5;
+  double e1 = 0.0, e2 = 0.0, e3 = 0.0;
+  if (s2 && s3)
+    e1 = c1 * (+F2[z + B3 * P] / gF2[g0] + F2[zkm + B3 * P] / gF2[g0] - F3[z + B2 * P] / gF3[g0] -
+               F3[zjm + B2 * P] / gF3[gjm]);
+  else if (s2)
+    e1 = c1 * (+F2[z + B3 * P] / gF2[g0] + F2[zkm + B3 * P] / gF2[g0]);
+  else if (s3)
+    e1 = c1 * (-F3[z + B2 * P] / gF3[g0] - F3[zjm + B2 * P] / gF3[gjm]);
+  if (s1 && s3)
+    e2 = c2 * (+F3[z + B1 * P] / gF3[g0] + F3[zim + B1 * P] / gF3[gim] - F1[z + B3 * P] / gF1[g0] -
+               F1[zkm + B3 * P] / gF1[g0]);
+  else if (s3)
+    e2 = c2 * (+F3[z + B1 * P] / gF3[g0] + F3[zim + B1 * P] / gF3[gim]);
+  else if (s1)
+    e2 = c2 * (-F1[z + B3 * P] / gF1[g0] - F1[zkm + B3 * P] / gF1[g0]);
+  if (s1 && s2)
+    e3 = c3 * (+F1[z + B2 * P] / gF1[g0] + F1[zjm + B2 * P] / gF1[gjm] - F2[z + B1 * P] / gF2[g0] -
+               F2[zim + B1 * P] / gF2[gim]);
+  else if (s1)
+    e3 = c3 * (+F1[z + B2 * P] / gF1[g0] + F1[zjm + B2 * P] / gF1[gjm]);
+  else if (s2)
+    e3 = c3 * (-F2[z + B1 * P] / gF2[g0] - F2[zim + B1 * P] / gF2[gim]);
+  emf1[z] = e1 * corn[0 * d.gplane + g0];
+  emf2[z] = e2 * corn[1 * d.gplane + g0];
+  emf3[z] = e3 * corn[2 * d.gplane + g0];
+}
+
+// ---------------------------------------------------------------------------
+// per-zone update, advance.c:143-213: U(pi) + dt * (flux divergence + source),
+// Noble 2D inversion with guess pb, fixup1zone on the final substep
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, const double *__restrict__ pb,
+                                                double *__restrict__ pf, const double *__restrict__ F1,
+                                                const double *__restrict__ F2, const double *__restrict__ F3,
+                                                const double *__restrict__ emf1, const double *__restrict__ emf2,
+                                                const double *__restrict__ emf3, const double *__restrict__ geo,
+                                                const double *__restrict__ conn, const double *__restrict__ floors,
+                                                const double *__restrict__ dtp, double cuf2, int finalstep, Dims d,
+                                                double gam, double idx1, double idx2, double idx3, double gammamax,
+                                                unsigned long long *__restrict__ counters, int *__restrict__ err) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
+  unsigned long long nfail = 0, nit = 0;
+  if (t < total) {
+    int i = (int)(t % d.n1);
+    int64_t r = t / d.n1;
+    int j = (int)(r % d.n2);
+    int k = (int)(r / d.n2);
+    int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+    int64_t z = d.idx(i, j, k);
+    const int64_t P = d.spl;
+    Geom G;
+    load_geom(geo, GL_CENT, d.gplane, d.gidx(i, j), G);
+    double pri[NPR], prb[NPR];
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) {
+      pri[pl] = pi[z + pl * P];
+      prb[pl] = pb[z + pl * P];
+    }
+    double Ui[NPR], dUg[NPR];
+    bool bad = primtoU(pri, G, gam, Ui);
+    // phys.c:578 source -> phys.c:623 source_conn
+    {
+      State q;
+      bad |= get_state(prb, G, q);
+      Maxwell M;
+      maxwell(prb, q, G, M);
+      double p1 = one_plus_ud0(q, G);
+      double mhd[4][4];
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) mhd_row(prb, jj, q, M, p1, gam, mhd[jj]);
+      const double *cn = conn + d.cidx(i, j);
+      double dUc[4] = {0., 0., 0., 0.};
+#pragma unroll
+      for (int l = 0; l < 4; l++)
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++)
+#pragma unroll
+          for (int kk = 0; kk < 4; kk++) dUc[l] += mhd[jj][kk] * cn[(kk * 16 + l * 4 + jj) * d.cplane];
+#pragma unroll
+      for (int l = 0; l < 4; l++)
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) dUc[l] += -prb[RHO] * q.ucon[jj] * cn[(0 * 16 + l * 4 + jj) * d.cplane];
+      dUg[RHO] = 0.0 + G.g * 0;
+#pragma unroll
+      for (int l = 0; l < 4; l++) dUg[UU + l] = 0.0 + G.g * dUc[l];
+      dUg[B1] = 0.0 + G.g * 0;
+      dUg[B2] = 0.0 + G.g * 0;
+      dUg[B3] = 0.0 + G.g * 0;
+    }
+    // advance.c:741 flux2dUavg, field components from the flux-CT EMFs (fluxct.c:720-781)
+    double Uf[NPR];
+    const double dt = dtp[0];
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) {
+      double dU1 = 0, dU2 = 0, dU3 = 0;
+      if (pl < B1) {
+        if (s1) dU1 = (-(F1[z + 1 + pl * P] - F1[z + pl * P]) * idx1);
+        if (s2) dU2 = (-(F2[z + d.sj + pl * P] - F2[z + pl * P]) * idx2);
+        if (s3) dU3 = (-(F3[z + d.sk + pl * P] - F3[z + pl * P]) * idx3);
+      } else {
+        // F1[B1]=0, F1[B2]=0.5(e3+e3(j+1)), F1[B3]=-0.5(e2+e2(k+1)); likewise F2, F3
+        int64_t jp = s2 ? d.sj : 0, kp = s3 ? d.sk : 0, ip = s1 ? 1 : 0;
+        if (s1) {
+          double a, b;
+          if (pl == B1) { a = 0.; b = 0.; }
+          else if (pl == B2) { a = 0.5 * (emf3[z + 1] + emf3[z + 1 + jp]); b = 0.5 * (emf3[z] + emf3[z + jp]); }
+          else { a = -0.5 * (emf2[z + 1] + emf2[z + 1 + kp]); b = -0.5 * (emf2[z] + emf2[z + kp]); }
+          dU1 = (-(a - b) * idx1);
+        }
+        if (s2) {
+          double a, b;
+          if (pl == B1) { a = -0.5 * (emf3[z + d.sj] + emf3[z + d.sj + ip]); b = -0.5 * (emf3[z] + emf3[z + ip]); }
+          else if (pl == B2) { a = 0.; b = 0.; }
+          else { a = 0.5 * (emf1[z + d.sj] + emf1[z + d.sj + kp]); b = 0.5 * (emf1[z] + emf1[z + kp]); }
+          dU2 = (-(a - b) * idx2);
+        }
+        if (s3) {
+          double a, b;
+          if (pl == B1) { a = 0.5 * (emf2[z + d.sk] + emf2[z + d.sk + ip]); b = 0.5 * (emf2[z] + emf2[z + ip]); }
+          else if (pl == B2) { a = -0.5 * (emf1[z + d.sk] + emf1[z + d.sk + jp]); b = -0.5 * (emf1[z] + emf1[z + jp]); }
+          else { a = 0.; b = 0.; }
+          dU3 = (-(a - b) * idx3);
+        }
+      }
+      double dUr = dU1 + dU2 + dU3;
+      // advance.c:922 dUtoU with CUf = {1, 0, cuf2}; uf carries no memory
+      // across substeps in the midpoint scheme (CUf[1] = 0) and ucum equals
+      // Uf on the final substep (Cunew = {0,0,1})
+      Uf[pl] = 1.0 * Ui[pl] + 0.0 * 0.0 + cuf2 * dt * (dUr + dUg[pl]);
+    }
+    double pr[NPR];
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) pr[pl] = prb[pl];
+    int iters = 0;
+    int flag = utoprimgen_fp64(Uf, G, gam, pr, iters);
+    nit = (unsigned long long)iters;
+    nfail = flag ? 1ull : 0ull;
+    if (finalstep) {
+      // fixup.c:341 fixup1zone with set_density_floors rescaletype 0
+      double f0 = floors[d.cidx(i, j)], f1 = floors[d.cplane + d.cidx(i, j)];
+      if (f0 < 1.e-20) f0 = 1.e-20;
+      if (f1 < 1.e-20) f1 = 1.e-20;
+      if (f0 > pr[RHO]) pr[RHO] = f0;
+      if (f1 > pr[UU]) pr[UU] = f1;
+      if (limit_gamma(gammamax, pr, G) > 0) bad = true;
+    }
+    if (bad) atomicOr(err, 4);
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) pf[z + pl * P] = pr[pl];
+  }
+  block_sum_to(nfail, nit, &counters[0], &counters[1]);
+}
+
+// advance.c:231 ndt = defcon / (1/ndt1 + 1/ndt2 + 1/ndt3); on the final
+// substep step_ch.c:243-247 limits the growth and stores the next dt
+__global__ void k_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe) {
+  double n1 = __longlong_as_double((long long)ndt_bits[0]);
+  double n2 = __longlong_as_double((long long)ndt_bits[1]);
+  double n3 = __longlong_as_double((long long)ndt_bits[2]);
+  double ndt = defcon * 1. / (1. / n1 + 1. / n2 + 1. / n3);
+  double dt = dtp[0];
+  if (ndt > safe * dt) ndt = safe * dt;
+  dtp[0] = ndt;
+}
+
+__global__ void k_reset_ndt(unsigned long long *ndt_bits) {
+  if (threadIdx.x < 3) ndt_bits[threadIdx.x] = (unsigned long long)__double_as_longlong(1.e+50);
+}
+
+// ---------------------------------------------------------------------------
+// batched inversion, structure-of-arrays over n zones
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_c2p_batch(int64_t n, double gam, const double *__restrict__ U,
+                                                   const double *__restrict__ gcov, const double *__restrict__ gcon,
+                                                   const double *__restrict__ gdet,
+                                                   const double *__restrict__ gcovpert, double *__restrict__ pr,
+                                                   int *__restrict__ pflag, int *__restrict__ iters) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  Geom G;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      G.gcov[a][c] = gcov[(a * 4 + c) * n + t];
+      G.gcon[a][c] = gcon[(a * 4 + c) * n + t];
+    }
+  G.gpert0 = gcovpert[t];
+  G.g = gdet[t];
+  double u[NPR], p[NPR];
+#pragma unroll
+  for (int pl = 0; pl < NPR; pl++) {
+    u[pl] = U[pl * n + t];
+    p[pl] = pr[pl * n + t];
+  }
+  int it = 0;
+  int flag = utoprimgen_fp64(u, G, gam, p, it);
+#pragma unroll
+  for (int pl = 0; pl < NPR; pl++) pr[pl * n + t] = p[pl];
+  pflag[t] = flag;
+  if (iters) iters[t] = it;
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+static inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_bound(double *p, const double *geo, const Dims &d, double gammamax, int *err, hipStream_t s) {
+  if (d.n1 > 1) {
+    int64_t n = 2LL * d.nb1 * d.n2 * d.n3;
+    hipLaunchKernelGGL(k_bound_x1, dim3(nblocks(n, 128)), dim3(128), 0, s, p, geo, d, gammamax, err);
+  }
+  if (d.n2 > 1) {
+    int64_t n = 2LL * d.nb2 * d.n3 * d.m1;
+    hipLaunchKernelGGL(k_bound_x2, dim3(nblocks(n, 256)), dim3(256), 0, s, p, d);
+  }
+  if (d.n3 > 1) {
+    int64_t n = 2LL * d.nb3 * d.m2 * d.m1;
+    hipLaunchKernelGGL(k_bound_x3, dim3(nblocks(n, 256)), dim3(256), 0, s, p, d);
+  }
+}
+
+void launch_flux_mc(int dir, const double *p, const double *geo, double *F, const Dims &d, double gam,
+                    double cour, double dxdir, int fluxmethod, unsigned long long *ndt_bits, int *err,
+                    hipStream_t s) {
+  FaceRange fr = face_range(d, dir);
+  int64_t n = (int64_t)fr.ni * fr.nj * fr.nk;
+  dim3 g(nblocks(n, 256)), b(256);
+  if (dir == 1) hipLaunchKernelGGL(k_flux_mc<1>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 2) hipLaunchKernelGGL(k_flux_mc<2>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 3) hipLaunchKernelGGL(k_flux_mc<3>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+}
+
+void launch_flux_lr(int dir, const double *pleft, const double *pright, const double *geo, double *F,
+                    const Dims &d, double gam, double cour, double dxdir, int fluxmethod,
+                    unsigned long long *ndt_bits, int *err, hipStream_t s) {
+  FaceRange fr = face_range(d, dir);
+  int64_t n = (int64_t)fr.ni * fr.nj * fr.nk;
+  dim3 g(nblocks(n, 256)), b(256);
+  if (dir == 1) hipLaunchKernelGGL(k_flux_lr<1>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 2) hipLaunchKernelGGL(k_flux_lr<2>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 3) hipLaunchKernelGGL(k_flux_lr<3>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+}
+
+void launch_emf(const double *F1, const double *F2, const double *F3, const double *geo, const double *corn,
+                double *e1, double *e2, double *e3, const Dims &d, hipStream_t s) {
+  int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  int64_t n = (int64_t)(d.n1 + s1) * (d.n2 + s2) * (d.n3 + s3);
+  hipLaunchKernelGGL(k_emf, dim3(nblocks(n, 256)), dim3(256), 0, s, F1, F2, F3, geo, corn, e1, e2, e3, d);
+}
+
+void launch_update(const double *pi, const double *pb, double *pf, const double *F1, const double *F2,
+                   const double *F3, const double *e1, const double *e2, const double *e3, const double *geo,
+                   const double *conn, const double *floors, const double *dtp, double cuf2, int finalstep,
+                   const Dims &d, double gam, const double *dx, double gammamax, unsigned long long *counters,
+                   int *err, hipStream_t s) {
+  int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
+  hipLaunchKernelGGL(k_update, dim3(nblocks(n, 256)), dim3(256), 0, s, pi, pb, pf, F1, F2, F3, e1, e2, e3, geo,
+                     conn, floors, dtp, cuf2, finalstep, d, gam, 1.0 / dx[1], 1.0 / dx[2], 1.0 / dx[3], gammamax,
+                     counters, err);
+}
+
+void launch_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe, hipStream_t s) {
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, s, ndt_bits, dtp, defcon, safe);
+}
+
+void launch_reset_ndt(unsigned long long *ndt_bits, hipStream_t s) {
+  hipLaunchKernelGGL(k_reset_ndt, dim3(1), dim3(64), 0, s, ndt_bits);
+}
+
+void launch_c2p_batch(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
+                      const double *gdet, const double *gcovpert, double *pr, int *pflag, int *iters,
+                      hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_c2p_batch, dim3(nblocks(n, 256)), dim3(256), 0, s, n, gam, U, gcov, gcon, gdet, gcovpert,
+                     pr, pflag, iters);
+}
+
+}  // namespace hg

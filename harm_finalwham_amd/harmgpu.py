@@ -1,0 +1,174 @@
+"""ctypes binding of libharmgpu.so (include/harmgpu.h).
+
+The library must have been built in-tree (harm_finalwham_amd/build.py);
+loading fails loudly otherwise: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libharmgpu.so")
+
+HG_LIM_MC, HG_LIM_WENO5BND = 3, 17
+HG_FLUX_LAXF, HG_FLUX_HLL = 0, 1
+HG_C2P_FP64, HG_C2P_MIXED = 0, 1
+
+# every symbol include/harmgpu.h declares
+EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_build_mks_geometry",
+           "hg_init_torus", "hg_set_prims", "hg_get_prims", "hg_bound_prim", "hg_fluxcalc",
+           "hg_step", "hg_step_async", "hg_set_dt", "hg_get_dt", "hg_get_counters",
+           "hg_utoprim_batch", "hg_make_c2p_batch", "hg_last_error"]
+
+
+class HarmError(RuntimeError):
+    pass
+
+
+class HgConfig(C.Structure):
+    _fields_ = [("n1", C.c_int), ("n2", C.c_int), ("n3", C.c_int), ("nb", C.c_int),
+                ("lim", C.c_int), ("fluxmethod", C.c_int),
+                ("gam", C.c_double), ("cour", C.c_double), ("safe", C.c_double),
+                ("defcon", C.c_double), ("dx", C.c_double * 4),
+                ("rhomin", C.c_double), ("uumin", C.c_double), ("gammamax", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libharmgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise HarmError("libharmgpu.so not built: run python -m harm_finalwham_amd.build")
+    L = C.CDLL(LIBPATH)
+    d = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    i64p = C.POINTER(C.c_int64)
+    L.hg_ctx_create.argtypes = [C.POINTER(HgConfig), C.POINTER(vp)]
+    L.hg_ctx_destroy.argtypes = [vp]
+    L.hg_upload_geometry.argtypes = [vp, d, d, d, d, d, d]
+    L.hg_build_mks_geometry.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.hg_init_torus.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                C.c_uint64, d]
+    L.hg_set_prims.argtypes = [vp, d]
+    L.hg_get_prims.argtypes = [vp, d]
+    L.hg_bound_prim.argtypes = [vp, vp]
+    L.hg_fluxcalc.argtypes = [vp, d, d, d, d, vp]
+    L.hg_step.argtypes = [vp, d, i64p, i64p, vp]
+    L.hg_step_async.argtypes = [vp, C.c_int, vp]
+    L.hg_set_dt.argtypes = [vp, C.c_double]
+    L.hg_get_dt.argtypes = [vp, d]
+    L.hg_get_counters.argtypes = [vp, i64p, i64p]
+    L.hg_utoprim_batch.argtypes = [C.c_int64, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, vp]
+    L.hg_make_c2p_batch.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_uint64, vp, vp, vp, vp, vp,
+                                    vp, vp, vp]
+    L.hg_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise HarmError(lib().hg_last_error().decode())
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Grid:
+    """One grid on the GPU (hg_ctx).  Host arrays use the reference layout
+    p[i][j][k][pl] with ghost zones."""
+
+    def __init__(self, n1, n2, n3, nb=6, lim=HG_LIM_WENO5BND, fluxmethod=HG_FLUX_HLL, gam=4.0 / 3.0,
+                 cour=0.8, dx=(0.0, 1.0, 1.0, 1.0), safe=1.3, defcon=1.0, rhomin=1e-4, uumin=1e-6,
+                 gammamax=25.0):
+        cfg = HgConfig()
+        cfg.n1, cfg.n2, cfg.n3, cfg.nb, cfg.lim, cfg.fluxmethod = n1, n2, n3, nb, lim, fluxmethod
+        cfg.gam, cfg.cour, cfg.safe, cfg.defcon = gam, cour, safe, defcon
+        for k in range(4):
+            cfg.dx[k] = dx[k]
+        cfg.rhomin, cfg.uumin, cfg.gammamax = rhomin, uumin, gammamax
+        self.cfg = cfg
+        h = C.c_void_p()
+        _check(lib().hg_ctx_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        nb1, nb2, nb3 = (nb if n > 1 else 0 for n in (n1, n2, n3))
+        self.shape = (n1 + 2 * nb1, n2 + 2 * nb2, n3 + 2 * nb3, 8)
+
+    def close(self):
+        if self.h:
+            lib().hg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_geometry(self, gcov, gcon, gcovpert, gdet, conn, r_cent):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (gcov, gcon, gcovpert, gdet, conn, r_cent)]
+        _check(lib().hg_upload_geometry(self.h, *[_d(a) for a in arrs]))
+
+    def build_mks_geometry(self, a=0.9375, Rin=1.3, Rout=40.0, R0=0.0, hslope=0.3):
+        _check(lib().hg_build_mks_geometry(self.h, a, Rin, Rout, R0, hslope))
+
+    def init_torus(self, rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0, randfact=4e-2, seed=7):
+        dt = np.zeros(1)
+        _check(lib().hg_init_torus(self.h, rin, rmax, kappa, beta, randfact, seed, _d(dt)))
+        return float(dt[0])
+
+    def set_prims(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        assert p.shape == self.shape, (p.shape, self.shape)
+        _check(lib().hg_set_prims(self.h, _d(p)))
+
+    def get_prims(self):
+        p = np.zeros(self.shape)
+        _check(lib().hg_get_prims(self.h, _d(p)))
+        return p
+
+    def bound_prim(self):
+        _check(lib().hg_bound_prim(self.h, None))
+
+    def fluxcalc(self):
+        F = [np.zeros(self.shape) for _ in range(3)]
+        ndt = np.zeros(3)
+        _check(lib().hg_fluxcalc(self.h, _d(F[0]), _d(F[1]), _d(F[2]), _d(ndt), None))
+        return F, ndt
+
+    def step_ch(self, dt):
+        """One RK2 step; returns (next dt, failed inversions, Newton iterations)."""
+        dtv = np.array([dt], dtype=np.float64)
+        nf = C.c_int64()
+        it = C.c_int64()
+        _check(lib().hg_step(self.h, _d(dtv), C.byref(nf), C.byref(it), None))
+        return float(dtv[0]), nf.value, it.value
+
+    def step_async(self, nsteps, stream=None):
+        _check(lib().hg_step_async(self.h, nsteps, stream))
+
+    def set_dt(self, dt):
+        _check(lib().hg_set_dt(self.h, dt))
+
+    def get_dt(self):
+        v = np.zeros(1)
+        _check(lib().hg_get_dt(self.h, _d(v)))
+        return float(v[0])
+
+
+def utoprim_batch(U, gcov, gcon, gdet, gcovpert, pr, pflag, iters=None, gam=4.0 / 3.0, mode=HG_C2P_FP64,
+                  stream=None):
+    """Utoprimgen over a batch (utoprimgen.c:23) on device tensors in
+    structure-of-arrays layout: U[8,n], gcov[16,n], gcon[16,n], gdet[n],
+    gcovpert[n] (tt component), pr[8,n] guess in/solution out, pflag[n] int32,
+    iters[n] int32 or None.  Arguments are torch CUDA tensors."""
+    n = U.shape[1]
+    ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+    _check(lib().hg_utoprim_batch(n, gam, ptr(U), ptr(gcov), ptr(gcon), ptr(gdet), ptr(gcovpert), ptr(pr),
+                                  ptr(pflag), ptr(iters), mode, stream))

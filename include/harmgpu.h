@@ -1,0 +1,141 @@
+/*
+ * harmgpu.h -- C ABI of the MI355X implementation of the per-step GRMHD
+ * hot path of pseudotensor/harm_finalwham.
+ *
+ * The entry points replace, on device-resident state, the functions that
+ * advance.c / step_ch.c call in the reference:
+ *
+ *   hg_bound_prim     bound_prim()        step_ch.c:568 -> bounds.fishmon.c:29
+ *   hg_fluxcalc       fluxcalc()          flux.c:7 (which ends in flux_ct(), fluxct.c:10)
+ *   hg_advance        advance()           advance.c:16 -> advance_standard advance.c:58
+ *                     (flux update + source + Utoprimgen + fixup1zone per zone)
+ *   hg_step           step_ch()           step_ch.c:16 -> step_ch_simplempi step_ch.c:35
+ *                     (RK2 midpoint, two advance() + bound_prim() calls, dt update)
+ *   hg_utoprim_batch  Utoprimgen()        utoprimgen.c:23 with UTOPRIM2DFINAL
+ *                     (utoprim_2d_final.c:34), on a batch of independent zones
+ *
+ * Conventions follow the reference: NPR=8 primitives
+ * [rho, u, U1, U2, U3, B1, B2, B3] (relative 4-velocity, WHICHVEL VELREL4),
+ * conserved U = gdet*[rho u^t, T^t_t + rho u^t, T^t_i, B^i]
+ * (REMOVERESTMASSFROMUU 2), failure codes of global.h:782-799.
+ *
+ * Every function returns 0 on success and >=1 on error (the MYFUN
+ * convention of global.h:1700); hg_last_error() describes the last error.
+ * Soft per-zone inversion failures are not errors: they are reported as
+ * failure codes (hg_utoprim_batch) or counted (hg_step, hg_advance).
+ *
+ * Host arrays use the reference's layouts:
+ *   primitives  p[i][j][k][pl], i in [0,n1+2nb), ... (ghost zones included,
+ *               nb ghost layers on each side of every direction of size > 1)
+ *   geometry    per (i,j) with i in [0,n1+2nb+1), j in [0,n2+2nb+1) and
+ *               7 locations (FACE1, FACE2, FACE3, CORN1, CORN2, CORN3, CENT):
+ *               gcov[..][7][4][4], gcon[..][7][4][4], gcovpert[..][7][4], gdet[..][7]
+ *               conn[i][j][4][4][4] over (n1+2nb) x (n2+2nb), r_cent likewise.
+ * Device arrays passed to hg_utoprim_batch are structure-of-arrays (see there).
+ * Streams are hipStream_t passed as void* (NULL = default stream).
+ */
+#ifndef HARMGPU_H
+#define HARMGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HG_NPR 8
+
+/* reconstruction and flux choices (reference values of lim/fluxmethod, global.h:247-314) */
+#define HG_LIM_MC 3
+#define HG_LIM_WENO5BND 17
+#define HG_FLUX_LAXF 0
+#define HG_FLUX_HLL 1
+
+typedef struct hg_ctx hg_ctx;
+
+typedef struct {
+  int n1, n2, n3;      /* active zones */
+  int nb;              /* ghost layers per side (>= 6 for WENO5BND, initbase.c:792) */
+  int lim;             /* HG_LIM_MC or HG_LIM_WENO5BND */
+  int fluxmethod;      /* HG_FLUX_LAXF or HG_FLUX_HLL */
+  double gam;          /* adiabatic index */
+  double cour;         /* Courant factor */
+  double safe;         /* max dt growth per step (SAFE, step_ch.c:245) */
+  double defcon;       /* dt prefactor (advance.c:231) */
+  double dx[4];        /* coordinate spacings dx[1..3] */
+  double rhomin, uumin;/* floor coefficients RHOMIN, UUMIN (initbase.c:459-460) */
+  double gammamax;     /* GAMMAMAX (initbase.c:471) */
+} hg_config;
+
+/* Context with device-resident state for one grid.  The caller owns the
+   context; nothing in the step allocates. */
+int hg_ctx_create(const hg_config *cfg, hg_ctx **out);
+int hg_ctx_destroy(hg_ctx *ctx);
+
+/* Geometry tables in the host layout described above (reference set_grid.c:7
+   output restricted to the locations the hot path reads). */
+int hg_upload_geometry(hg_ctx *ctx, const double *gcov, const double *gcon,
+                       const double *gcovpert, const double *gdet, const double *conn,
+                       const double *r_cent);
+
+/* Builds the geometry of the modified Kerr-Schild grid (coord.c:389 LOGRSINTH,
+   metric.c:697 KS metric) on the host and uploads it: spin a, radial range
+   [Rin, Rout] with R0, theta concentration hslope. */
+int hg_build_mks_geometry(hg_ctx *ctx, double a, double Rin, double Rout, double R0,
+                          double hslope);
+
+/* Fishbone-Moncrief torus initial data (init.fishmon.c:237 init_primitives and
+   initbase.c:5 init) on the grid built by hg_build_mks_geometry: r_in, r_pmax,
+   kappa, beta_min, relative amplitude of the uniform random perturbation of u
+   and its seed.  Returns the initial dt (advance.c:957 set_dt) in *dt. */
+int hg_init_torus(hg_ctx *ctx, double rin, double rmax, double kappa, double beta,
+                  double randfact, uint64_t seed, double *dt);
+
+int hg_set_prims(hg_ctx *ctx, const double *p_host);
+int hg_get_prims(hg_ctx *ctx, double *p_host);
+
+/* bound_prim (step_ch.c:568) on the current primitives. */
+int hg_bound_prim(hg_ctx *ctx, void *stream);
+
+/* fluxcalc + flux_ct (flux.c:7, fluxct.c:10) on the current primitives;
+   F1..F3 (host, may be NULL) receive the fluxes in the p layout, ndt[3] the
+   per-direction time-step limits. */
+int hg_fluxcalc(hg_ctx *ctx, double *F1, double *F2, double *F3, double ndt[3], void *stream);
+
+/* One full RK2 step (step_ch.c:35 with TIMEORDER 2).  *dt in: step size,
+   out: the next step size.  nfail/iters (may be NULL) receive the number of
+   failed inversions and the total Newton iterations of the step. */
+int hg_step(hg_ctx *ctx, double *dt, int64_t *nfail, int64_t *iters, void *stream);
+
+/* Same as hg_step but leaves dt on the device and does not synchronise:
+   n steps are enqueued on `stream`; hg_get_dt reads the device dt. */
+int hg_step_async(hg_ctx *ctx, int nsteps, void *stream);
+int hg_set_dt(hg_ctx *ctx, double dt);
+int hg_get_dt(hg_ctx *ctx, double *dt);
+int hg_get_counters(hg_ctx *ctx, int64_t *nfail, int64_t *iters);
+
+/* Batched Noble-2D inversion on device memory (utoprimgen.c:23).  All
+   arrays are structure-of-arrays over n zones:
+     U[8][n] evolved conserved variables (with gdet), gcov[16][n], gcon[16][n],
+     gdet[n], gcovpert[n] (the tt component), pr[8][n] guess in / solution out,
+     pflag[n] failure code out, iters[n] Newton iterations out (may be NULL).
+   mode: HG_C2P_FP64 (reference arithmetic) or HG_C2P_MIXED (FP32 bulk Newton,
+   FP64 residual and final polish). */
+#define HG_C2P_FP64 0
+#define HG_C2P_MIXED 1
+int hg_utoprim_batch(int64_t n, double gam, const double *U, const double *gcov,
+                     const double *gcon, const double *gdet, const double *gcovpert,
+                     double *pr, int32_t *pflag, int32_t *iters, int mode, void *stream);
+
+/* Device-side generator of the batched-inversion benchmark distribution
+   (see hg_utoprim_batch for the layouts): KS metric of spin a at r, theta,
+   primitives, their conserved variables and a perturbed guess. */
+int hg_make_c2p_batch(int64_t n, double gam, double a, uint64_t seed, double *U, double *gcov,
+                      double *gcon, double *gdet, double *gcovpert, double *p_true,
+                      double *p_guess, void *stream);
+
+const char *hg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
