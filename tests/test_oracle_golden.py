@@ -27,7 +27,7 @@ def test_c2p_failure_codes_present():
     assert set(np.unique(d["pflag"])) - {0}
 
 
-@pytest.mark.parametrize("name", ["t2d_mc"])
+@pytest.mark.parametrize("name", ["t2d_mc", "t2d_weno", "t3d_weno"])
 def test_torus_steps_match_reference(name):
     d = load(name)
     G = po.Grid.from_fixture(d)
@@ -42,7 +42,7 @@ def test_torus_steps_match_reference(name):
     assert np.array_equal(p, d["p_final"])
 
 
-@pytest.mark.parametrize("name", ["t2d_mc"])
+@pytest.mark.parametrize("name", ["t2d_mc", "t2d_weno", "t3d_weno"])
 def test_fluxcalc_matches_reference(name):
     d = load(name)
     G = po.Grid.from_fixture(d)
@@ -59,3 +59,20 @@ def test_ghost_fill_is_identity_on_reference_state():
     d = load("t2d_mc")
     G = po.Grid.from_fixture(d)
     assert np.array_equal(G.bound(d["p_step1"]), d["p_step1"])
+
+
+@pytest.mark.parametrize("name", ["t2d_weno", "t3d_weno"])
+def test_weno5bnd_interface_states_match_reference(name):
+    """slope_lim_linetype_c2e (interpline.c:517) states of cells -1..n along each direction."""
+    d = load(name)
+    G = po.Grid.from_fixture(d)
+    nb = int(d["params"][3])
+    for dr in (1, 2, 3):
+        if "pleft_dir%d" % dr not in d:
+            continue
+        pl, pr = G.reconstruct(d["p_init"], dr)
+        sl = [slice(None)] * 3
+        sl[dr - 1] = slice(nb - 1, -nb + 1)
+        sl = tuple(sl)
+        assert np.array_equal(pl[sl], d["pleft_dir%d" % dr][sl])
+        assert np.array_equal(pr[sl], d["pright_dir%d" % dr][sl])
