@@ -49,9 +49,14 @@ def test_fluxcalc_matches_reference(name):
     nb = int(d["params"][3])
     F, ndt = G.fluxcalc(d["p_init"])
     assert np.array_equal(ndt, d["ndt_init"])
-    # faces read by the active-zone update: i=0..n1 (dir 1), j=0..n2 (dir 2)
-    assert np.array_equal(F[0][nb:-nb + 1, nb:-nb], d["F1_init"][nb:-nb + 1, nb:-nb])
-    assert np.array_equal(F[1][nb:-nb, nb:-nb + 1], d["F2_init"][nb:-nb, nb:-nb + 1])
+    # faces read by the active-zone update: i=0..n1 (dir 1), j=0..n2 (dir 2), k=0..n3 (dir 3)
+    n3 = int(d["params"][2])
+    k = slice(nb, -nb) if n3 > 1 else slice(None)
+    kf = slice(nb, -nb + 1) if n3 > 1 else slice(None)
+    assert np.array_equal(F[0][nb:-nb + 1, nb:-nb, k], d["F1_init"][nb:-nb + 1, nb:-nb, k])
+    assert np.array_equal(F[1][nb:-nb, nb:-nb + 1, k], d["F2_init"][nb:-nb, nb:-nb + 1, k])
+    if n3 > 1:
+        assert np.array_equal(F[2][nb:-nb, nb:-nb, kf], d["F3_init"][nb:-nb, nb:-nb, kf])
 
 
 def test_ghost_fill_is_identity_on_reference_state():
