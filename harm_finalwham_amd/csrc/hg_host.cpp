@@ -52,7 +52,6 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
     set_error("hg_ctx_create: unsupported fluxmethod %d", cfg->fluxmethod);
     return 1;
   }
-  if (cfg->lim == HG_LIM_WENO5BND) { set_error("hg_ctx_create: WENO5BND not available in this build"); return 1; }
   int need = (cfg->lim == HG_LIM_WENO5BND) ? 6 : 3;
   if (cfg->nb < need) { set_error("hg_ctx_create: nb=%d < %d ghost layers needed", cfg->nb, need); return 1; }
   if (cfg->n3 > 1 && cfg->n3 < cfg->nb) { set_error("hg_ctx_create: periodic n3=%d < nb=%d", cfg->n3, cfg->nb); return 1; }

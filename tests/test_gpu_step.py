@@ -21,7 +21,7 @@ def make_grid(d):
     return g
 
 
-@pytest.mark.parametrize("name", ["t2d_mc"])
+@pytest.mark.parametrize("name", ["t2d_mc", "t3d_weno"])
 def test_bound_prim_reproduces_reference(name):
     d = load(name)
     g = make_grid(d)
@@ -30,7 +30,7 @@ def test_bound_prim_reproduces_reference(name):
     assert np.array_equal(g.get_prims(), d["p_step1"])
 
 
-@pytest.mark.parametrize("name", ["t2d_mc"])
+@pytest.mark.parametrize("name", ["t2d_mc", "t2d_weno", "t3d_weno"])
 def test_fluxcalc_matches_reference(name):
     d = load(name)
     g = make_grid(d)
@@ -38,11 +38,16 @@ def test_fluxcalc_matches_reference(name):
     g.set_prims(d["p_init"])
     F, ndt = g.fluxcalc()
     assert np.array_equal(ndt, d["ndt_init"])
-    assert np.array_equal(F[0][nb:-nb + 1, nb:-nb], d["F1_init"][nb:-nb + 1, nb:-nb])
-    assert np.array_equal(F[1][nb:-nb, nb:-nb + 1], d["F2_init"][nb:-nb, nb:-nb + 1])
+    n3 = int(d["params"][2])
+    k = slice(nb, -nb) if n3 > 1 else slice(None)
+    kf = slice(nb, -nb + 1) if n3 > 1 else slice(None)
+    assert np.array_equal(F[0][nb:-nb + 1, nb:-nb, k], d["F1_init"][nb:-nb + 1, nb:-nb, k])
+    assert np.array_equal(F[1][nb:-nb, nb:-nb + 1, k], d["F2_init"][nb:-nb, nb:-nb + 1, k])
+    if n3 > 1:
+        assert np.array_equal(F[2][nb:-nb, nb:-nb, kf], d["F3_init"][nb:-nb, nb:-nb, kf])
 
 
-@pytest.mark.parametrize("name", ["t2d_mc"])
+@pytest.mark.parametrize("name", ["t2d_mc", "t2d_weno", "t3d_weno"])
 def test_steps_match_reference(name):
     d = load(name)
     g = make_grid(d)
