@@ -97,6 +97,7 @@ extern "C" int hg_ctx_destroy(hg_ctx *c) {
   if (c->ndt_bits) (void)hipFree(c->ndt_bits);
   if (c->counters) (void)hipFree(c->counters);
   if (c->err) (void)hipFree(c->err);
+  for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
   delete c;
   return 0;
 }
@@ -218,6 +219,77 @@ extern "C" int hg_bound_prim(hg_ctx *c, void *stream) {
   return check_err(c);
 }
 
+// ---------------------------------------------------------------------------
+// per-kernel-class timing with HIP events on the launch stream.  While a
+// profile is open, prof_mark(c, k, s) closes the running interval (if any)
+// and opens one of class k (k < 0: none); the events are read back and summed
+// by hg_profile_read.
+// ---------------------------------------------------------------------------
+static void prof_mark(hg_ctx *c, int cls, hipStream_t s) {
+  if (!c->prof_on) return;
+  if (c->prof_open >= 0) {
+    hipEvent_t stop;
+    if (c->prof_used < c->prof_events.size()) stop = c->prof_events[c->prof_used];
+    else {
+      if (hipEventCreate(&stop) != hipSuccess) { c->prof_on = false; return; }
+      c->prof_events.push_back(stop);
+    }
+    (void)hipEventRecord(stop, s);
+    c->prof_intervals.push_back({c->prof_open, c->prof_used - 1, c->prof_used});
+    c->prof_used++;
+    c->prof_open = -1;
+  }
+  if (cls < 0) return;
+  hipEvent_t start;
+  if (c->prof_used < c->prof_events.size()) start = c->prof_events[c->prof_used];
+  else {
+    if (hipEventCreate(&start) != hipSuccess) { c->prof_on = false; return; }
+    c->prof_events.push_back(start);
+  }
+  (void)hipEventRecord(start, s);
+  c->prof_used++;
+  c->prof_open = cls;
+}
+
+extern "C" int hg_profile(hg_ctx *c, int enable) {
+  if (!c) { set_error("hg_profile: null context"); return 1; }
+  c->prof_on = enable != 0;
+  c->prof_open = -1;
+  if (enable) {
+    c->prof_used = 0;
+    c->prof_intervals.clear();
+    // enough events for ~150 steps, created here rather than while timing
+    while (c->prof_events.size() < 4096) {
+      hipEvent_t e;
+      HIPCHECK(hipEventCreate(&e));
+      c->prof_events.push_back(e);
+    }
+  }
+  return 0;
+}
+
+extern "C" int hg_profile_read(hg_ctx *c, double ms[HG_PROF_NCLASS], int64_t launches[HG_PROF_NCLASS]) {
+  if (!c || !ms) { set_error("hg_profile_read: null argument"); return 1; }
+  for (int k = 0; k < HG_PROF_NCLASS; k++) {
+    ms[k] = 0.;
+    if (launches) launches[k] = 0;
+  }
+  for (const auto &iv : c->prof_intervals) {
+    HIPCHECK(hipEventSynchronize(c->prof_events[iv.stop]));
+    float t = 0.f;
+    HIPCHECK(hipEventElapsedTime(&t, c->prof_events[iv.start], c->prof_events[iv.stop]));
+    ms[iv.cls] += t;
+    if (launches) launches[iv.cls]++;
+  }
+  return 0;
+}
+
+extern "C" const char *hg_profile_class_name(int k) {
+  static const char *names[HG_PROF_NCLASS] = {"bound", "recon1", "recon2", "recon3", "flux1",
+                                              "flux2", "flux3", "emf",    "update"};
+  return (k >= 0 && k < HG_PROF_NCLASS) ? names[k] : "";
+}
+
 // fluxcalc (flux.c:7) on primitives pb: per-direction fluxes and ndt bits,
 // then the flux-CT EMFs (fluxct.c:10)
 static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
@@ -228,15 +300,22 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
   for (int dir = 1; dir <= 3; dir++) {
     if (nn[dir] <= 1) continue;
     if (f.lim == HG_LIM_MC) {
+      prof_mark(c, HG_PROF_FLUX1 + dir - 1, s);
       launch_flux_mc(dir, pb, c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,
                      c->ndt_bits + dir - 1, c->err, s);
+      prof_mark(c, -1, s);
     } else {
+      prof_mark(c, HG_PROF_RECON1 + dir - 1, s);
       launch_weno(dir, pb, c->geo, c->pl, c->pr, d, f.gam, s);
+      prof_mark(c, HG_PROF_FLUX1 + dir - 1, s);
       launch_flux_lr(dir, c->pl, c->pr, c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,
                      c->ndt_bits + dir - 1, c->err, s);
+      prof_mark(c, -1, s);
     }
   }
+  prof_mark(c, HG_PROF_EMF, s);
   launch_emf(c->F[0], c->F[1], c->F[2], c->geo, c->corn, c->emf[0], c->emf[1], c->emf[2], d, s);
+  prof_mark(c, -1, s);
 }
 
 // the field fluxes rewritten from the EMFs (fluxct.c:720-781), on the host,
@@ -310,14 +389,17 @@ static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, doubl
                             hipStream_t s) {
   const hg_config &f = c->cfg;
   enqueue_fluxcalc(c, pb, s);
+  prof_mark(c, HG_PROF_UPDATE, s);
   launch_update(pi, pb, pf, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2], c->geo, c->conn,
                 c->floors, c->dt, cuf2, finalstep, c->d, f.gam, f.dx, f.gammamax, c->counters, c->err, s);
   if (finalstep) launch_dt(c->ndt_bits, c->dt, f.defcon, f.safe, s);
   // step_ch.c:127 and :166 call bound_prim twice per substep; the second call
   // is not a repeat of the first: POLEDEATH has zeroed U2 in the rows next to
   // the axis, which the radial inflow check then sees in the corner ghosts
+  prof_mark(c, HG_PROF_BOUND, s);
   launch_bound(pf, c->geo, c->d, f.gammamax, c->err, s);
   launch_bound(pf, c->geo, c->d, f.gammamax, c->err, s);
+  prof_mark(c, -1, s);
 }
 
 // step_ch.c:442 setup_rktimestep, TIMEORDER 2: CUf[0][2]=0.5, CUf[1][2]=1
@@ -392,5 +474,11 @@ extern "C" int hg_utoprim_batch(int64_t n, double gam, const double *U, const do
   }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int hg_get_config(const hg_ctx *c, hg_config *out) {
+  if (!c || !out) { set_error("hg_get_config: null argument"); return 1; }
+  *out = c->cfg;
   return 0;
 }

@@ -1,6 +1,9 @@
 // Internal host-side definitions shared by the C-ABI translation units.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdarg>
+#include <vector>
 
 #include "harmgpu.h"
 #include "hg_grid.h"
@@ -17,6 +20,20 @@ struct hg_ctx {
   unsigned long long *ndt_bits = nullptr, *counters = nullptr;
   int *err = nullptr;
   bool have_geometry = false;
+  // modified Kerr-Schild grid parameters when the geometry was built by
+  // hg_build_mks_geometry (the torus set-up needs the coordinates)
+  bool have_mks = false;
+  double mks_a = 0, mks_Rin = 0, mks_Rout = 0, mks_R0 = 0, mks_hslope = 0;
+  // kernel-class timing (hg_profile): event pool and recorded intervals
+  struct ProfInterval {
+    int cls;
+    size_t start, stop;
+  };
+  bool prof_on = false;
+  int prof_open = -1;
+  size_t prof_used = 0;
+  std::vector<hipEvent_t> prof_events;
+  std::vector<ProfInterval> prof_intervals;
 };
 
 namespace hg {

@@ -31,4 +31,7 @@ void launch_c2p_batch(int64_t n, double gam, const double *U, const double *gcov
 void launch_weno(int dir, const double *p, const double *geo, double *pleft, double *pright, const Dims &d,
                  double gam, hipStream_t s);
 
+// initial time step over every zone (advance.c:957 set_dt), folded into *out
+void launch_set_dt(const double *p, const double *geo, const Dims &d, double gam, double cour, const double *dx,
+                   unsigned long long *out, int *err, hipStream_t s);
 }  // namespace hg

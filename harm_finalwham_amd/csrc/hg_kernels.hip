@@ -7,6 +7,7 @@
 //   k_update           conserved update + source + Noble 2D + fixup1zone (advance.c:58)
 //   k_dt               time-step update (advance.c:231, step_ch.c:243)
 //   k_c2p_batch        batched inversion (utoprimgen.c:23)
+//   k_set_dt           initial time step of the set-up (advance.c:957)
 #include <hip/hip_runtime.h>
 
 #include "hg_c2p.h"
@@ -607,6 +608,55 @@ void launch_c2p_batch(int64_t n, double gam, const double *U, const double *gcov
   if (n <= 0) return;
   hipLaunchKernelGGL(k_c2p_batch, dim3(nblocks(n, 256)), dim3(256), 0, s, n, gam, U, gcov, gcon, gdet, gcovpert,
                      pr, pflag, iters);
+}
+
+}  // namespace hg
+
+// ---------------------------------------------------------------------------
+// initial time step of the set-up
+// ---------------------------------------------------------------------------
+namespace hg {
+
+// advance.c:957 set_dt over every zone including ghosts
+__global__ void k_set_dt(const double *__restrict__ p, const double *__restrict__ geo, Dims d, double gam,
+                         double cour, double dx1, double dx2, double dx3, unsigned long long *__restrict__ out,
+                         int *__restrict__ err) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double ndt = 1.e+50;
+  if (t < d.spl) {
+    int ii = (int)(t % d.m1);
+    int64_t r = t / d.m1;
+    int jj = (int)(r % d.m2), kk = (int)(r / d.m2);
+    int i = ii - d.nb1, j = jj - d.nb2;
+    double pr[NPR];
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) pr[pl] = p[t + pl * d.spl];
+    Geom G;
+    load_geom(geo, GL_CENT, d.gplane, d.gidx(i, j), G);
+    State q;
+    (void)kk;
+    if (get_state(pr, G, q)) {
+      atomicOr(err, 8);
+    } else {
+      double dxs[4] = {0., dx1, dx2, dx3};
+      int nn[4] = {0, d.n1, d.n2, d.n3};
+      for (int dir = 1; dir <= 3; dir++) {
+        if (nn[dir] <= 1) continue;
+        double cmax, cmin;
+        if (vchar(pr, q, dir, G, gam, cmax, cmin)) { atomicOr(err, 8); continue; }
+        double a = fabs(cmax), b = fabs(cmin);
+        double dtij = cour * dxs[dir] / ((a > b) ? a : b);
+        if (dtij < ndt) ndt = dtij;
+      }
+    }
+  }
+  block_min_to(ndt, out);
+}
+
+void launch_set_dt(const double *p, const double *geo, const Dims &d, double gam, double cour, const double *dx,
+                   unsigned long long *out, int *err, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_dt, dim3(nblocks(d.spl, 256)), dim3(256), 0, s, p, geo, d, gam, cour, dx[1], dx[2],
+                     dx[3], out, err);
 }
 
 }  // namespace hg

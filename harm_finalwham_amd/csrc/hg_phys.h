@@ -10,6 +10,10 @@
 
 #include "hg_grid.h"
 
+// point physics is shared by the device kernels and the host-side set-up
+// (initial data), which must produce the same bits
+#define HG_HD __host__ __device__ __forceinline__
+
 namespace hg {
 
 constexpr double DBL_EPS = 2.2204460492503131e-16;
@@ -30,11 +34,11 @@ struct State {
 };
 
 // phys.c:1797 max / phys.c:1804 min
-__device__ __forceinline__ double rmax(double a, double b) { return (a > b) ? a : b; }
-__device__ __forceinline__ double rmin(double a, double b) { return (a > b) ? b : a; }
+HG_HD double rmax(double a, double b) { return (a > b) ? a : b; }
+HG_HD double rmin(double a, double b) { return (a > b) ? b : a; }
 
 // phys.c:766 lower_vec
-__device__ __forceinline__ void lower_vec(const double *v, const Geom &G, double *w) {
+HG_HD void lower_vec(const double *v, const Geom &G, double *w) {
   const double(*g)[4] = G.gcov;
   w[0] = g[0][0] * v[0] + g[0][1] * v[1] + g[0][2] * v[2] + g[0][3] * v[3];
   w[1] = g[0][1] * v[0] + g[1][1] * v[1] + g[1][2] * v[2] + g[1][3] * v[3];
@@ -43,7 +47,7 @@ __device__ __forceinline__ void lower_vec(const double *v, const Geom &G, double
 }
 
 // phys.c:825 raise_vec
-__device__ __forceinline__ void raise_vec(const double *v, const Geom &G, double *w) {
+HG_HD void raise_vec(const double *v, const Geom &G, double *w) {
   const double(*g)[4] = G.gcon;
   w[0] = g[0][0] * v[0] + g[0][1] * v[1] + g[0][2] * v[2] + g[0][3] * v[3];
   w[1] = g[0][1] * v[0] + g[1][1] * v[1] + g[1][2] * v[2] + g[1][3] * v[3];
@@ -52,7 +56,7 @@ __device__ __forceinline__ void raise_vec(const double *v, const Geom &G, double
 }
 
 // phys.c:1069 qsq_calc, phys.c:1040 gamma_calc_fromuconrel with JONCHECKS
-__device__ __forceinline__ bool gamma_calc(const double *pr, const Geom &G, double &gamma) {
+HG_HD bool gamma_calc(const double *pr, const Geom &G, double &gamma) {
   const double(*g)[4] = G.gcov;
   double u1 = pr[U1], u2 = pr[U2], u3 = pr[U3];
   double qsq = +g[1][1] * u1 * u1 + g[2][2] * u2 * u2 + g[3][3] * u3 * u3 +
@@ -66,7 +70,7 @@ __device__ __forceinline__ bool gamma_calc(const double *pr, const Geom &G, doub
 }
 
 // phys.c:1006 ucon_calc_rel4vel
-__device__ __forceinline__ bool ucon_calc(const double *pr, const Geom &G, double *ucon) {
+HG_HD bool ucon_calc(const double *pr, const Geom &G, double *ucon) {
   double alpha = 1. / sqrt(-G.gcon[0][0]);
   double beta1 = G.gcon[0][1] * alpha * alpha;
   double beta2 = G.gcon[0][2] * alpha * alpha;
@@ -81,7 +85,7 @@ __device__ __forceinline__ bool ucon_calc(const double *pr, const Geom &G, doubl
 }
 
 // phys.c:849 get_state (with phys.c:309 bcon_calc)
-__device__ __forceinline__ bool get_state(const double *pr, const Geom &G, State &q) {
+HG_HD bool get_state(const double *pr, const Geom &G, State &q) {
   if (ucon_calc(pr, G, q.ucon)) return true;
   lower_vec(q.ucon, G, q.ucov);
   q.bcon[0] = pr[B1] * q.ucov[1] + pr[B2] * q.ucov[2] + pr[B3] * q.ucov[3];
@@ -92,7 +96,7 @@ __device__ __forceinline__ bool get_state(const double *pr, const Geom &G, State
 }
 
 // phys.c:541 compute_1plusud0
-__device__ __forceinline__ double one_plus_ud0(const State &q, const Geom &G) {
+HG_HD double one_plus_ud0(const State &q, const Geom &G) {
   double vcon[4];
 #pragma unroll
   for (int j = 1; j < 4; j++) vcon[j] = q.ucon[j] / q.ucon[0];
@@ -119,7 +123,7 @@ struct Maxwell {
   double Mcon[4][4], Mcov[4][4], Msq;
 };
 
-__device__ __forceinline__ void maxwell(const double *pr, const State &q, const Geom &G, Maxwell &M) {
+HG_HD void maxwell(const double *pr, const State &q, const Geom &G, Maxwell &M) {
   const double(*g)[4] = G.gcov;
 #pragma unroll
   for (int j = 0; j < 4; j++) M.Mcon[j][j] = 0.0;
@@ -159,7 +163,7 @@ __device__ __forceinline__ void maxwell(const double *pr, const State &q, const 
 }
 
 // phys.c:442 mhd_calc_norestmass: row dir of T^mu_nu with rest mass removed from T^t_t
-__device__ __forceinline__ void mhd_row(const double *pr, int dir, const State &q, const Maxwell &M,
+HG_HD void mhd_row(const double *pr, int dir, const State &q, const Maxwell &M,
                                         double plus1ud0, double gam, double *mhd) {
   double rho = pr[RHO], u = pr[UU];
   double P = (gam - 1.) * u;  // eos.c:58
@@ -181,7 +185,7 @@ __device__ __forceinline__ void mhd_row(const double *pr, int dir, const State &
 }
 
 // phys.c:16 primtoflux, returntype UEVOLVE (x gdet)
-__device__ __forceinline__ void primtoflux(const double *pr, const State &q, const Maxwell &M,
+HG_HD void primtoflux(const double *pr, const State &q, const Maxwell &M,
                                           double plus1ud0, int dir, const Geom &G, double gam,
                                           double *flux) {
   double fl[NPR];
@@ -201,7 +205,7 @@ __device__ __forceinline__ void primtoflux(const double *pr, const State &q, con
 }
 
 // conserved variables U (phys.c:225 primtoU)
-__device__ __forceinline__ bool primtoU(const double *pr, const Geom &G, double gam, double *U) {
+HG_HD bool primtoU(const double *pr, const Geom &G, double gam, double *U) {
   State q;
   if (get_state(pr, G, q)) return true;
   Maxwell M;
@@ -212,7 +216,7 @@ __device__ __forceinline__ bool primtoU(const double *pr, const Geom &G, double 
 }
 
 // vchar.c:213 simplefast
-__device__ __forceinline__ bool simplefast(int dir, const Geom &G, const State &q, double cms2,
+HG_HD bool simplefast(int dir, const Geom &G, const State &q, double cms2,
                                           double &vmin, double &vmax) {
   double Acov[4] = {0., 0., 0., 0.}, Bcov[4] = {1., 0., 0., 0.}, Acon[4], Bcon[4];
   Acov[dir] = 1.;
@@ -240,7 +244,7 @@ __device__ __forceinline__ bool simplefast(int dir, const Geom &G, const State &
 }
 
 // vchar.c:37 vchar
-__device__ __forceinline__ bool vchar(const double *pr, const State &q, int dir, const Geom &G,
+HG_HD bool vchar(const double *pr, const State &q, int dir, const Geom &G,
                                      double gam, double &vmax, double &vmin) {
   double bsq = q.bcon[0] * q.bcov[0] + q.bcon[1] * q.bcov[1] + q.bcon[2] * q.bcov[2] + q.bcon[3] * q.bcov[3];
   double rho = pr[RHO], u = pr[UU];
@@ -274,7 +278,7 @@ __device__ __forceinline__ bool vchar(const double *pr, const State &q, int dir,
 }
 
 // fixup.c:1859 limit_gamma
-__device__ __forceinline__ int limit_gamma(double gammamax, double *pr, const Geom &G) {
+HG_HD int limit_gamma(double gammamax, double *pr, const Geom &G) {
   if (gammamax <= 1.0) return 0;
   double gamma;
   if (gamma_calc(pr, G, gamma)) return 1;
@@ -292,7 +296,7 @@ __device__ __forceinline__ int limit_gamma(double gammamax, double *pr, const Ge
 
 // fixup.c:2271 inflow_check_rel4vel, radial direction at zone centres;
 // side < 0: inner boundary (u^r > 0 is fixed), side > 0: outer (u^r < 0)
-__device__ __forceinline__ int inflow_check_r(double *pr, const Geom &G, int side, double gammamax) {
+HG_HD int inflow_check_r(double *pr, const Geom &G, int side, double gammamax) {
   double ucon[4];
   if (ucon_calc(pr, G, ucon)) return 1;
   if (!((side < 0 && ucon[1] > 0.) || (side > 0 && ucon[1] < 0.))) return 0;

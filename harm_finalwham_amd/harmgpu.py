@@ -16,10 +16,12 @@ HG_FLUX_LAXF, HG_FLUX_HLL = 0, 1
 HG_C2P_FP64, HG_C2P_MIXED = 0, 1
 
 # every symbol include/harmgpu.h declares
-EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_build_mks_geometry",
-           "hg_init_torus", "hg_set_prims", "hg_get_prims", "hg_bound_prim", "hg_fluxcalc",
+EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geometry_tables",
+           "hg_build_mks_geometry", "hg_get_config", "hg_init_torus", "hg_set_prims", "hg_get_prims", "hg_bound_prim", "hg_fluxcalc",
            "hg_step", "hg_step_async", "hg_set_dt", "hg_get_dt", "hg_get_counters",
-           "hg_utoprim_batch", "hg_make_c2p_batch", "hg_last_error"]
+           "hg_utoprim_batch", "hg_make_c2p_batch", "hg_last_error",
+           "hg_profile", "hg_profile_read", "hg_profile_class_name"]
+HG_PROF_NCLASS = 9
 
 
 class HarmError(RuntimeError):
@@ -51,7 +53,10 @@ def lib():
     L.hg_ctx_create.argtypes = [C.POINTER(HgConfig), C.POINTER(vp)]
     L.hg_ctx_destroy.argtypes = [vp]
     L.hg_upload_geometry.argtypes = [vp, d, d, d, d, d, d]
+    L.hg_mks_geometry_tables.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                         C.c_double, C.c_double, C.c_double, d, d, d, d, d, d, d]
     L.hg_build_mks_geometry.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.hg_get_config.argtypes = [vp, C.POINTER(HgConfig)]
     L.hg_init_torus.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                 C.c_uint64, d]
     L.hg_set_prims.argtypes = [vp, d]
@@ -67,6 +72,10 @@ def lib():
     L.hg_make_c2p_batch.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_uint64, vp, vp, vp, vp, vp,
                                     vp, vp, vp]
     L.hg_last_error.restype = C.c_char_p
+    L.hg_profile.argtypes = [vp, C.c_int]
+    L.hg_profile_read.argtypes = [vp, d, i64p]
+    L.hg_profile_class_name.argtypes = [C.c_int]
+    L.hg_profile_class_name.restype = C.c_char_p
     _lib = L
     return L
 
@@ -117,6 +126,12 @@ class Grid:
 
     def build_mks_geometry(self, a=0.9375, Rin=1.3, Rout=40.0, R0=0.0, hslope=0.3):
         _check(lib().hg_build_mks_geometry(self.h, a, Rin, Rout, R0, hslope))
+        self.cfg = self.config()
+
+    def config(self):
+        cfg = HgConfig()
+        _check(lib().hg_get_config(self.h, C.byref(cfg)))
+        return cfg
 
     def init_torus(self, rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0, randfact=4e-2, seed=7):
         dt = np.zeros(1)
@@ -156,10 +171,37 @@ class Grid:
     def set_dt(self, dt):
         _check(lib().hg_set_dt(self.h, dt))
 
+    def profile(self, enable=True):
+        _check(lib().hg_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self):
+        """{class name: (total ms, intervals)} of the open/last profile."""
+        ms = np.zeros(HG_PROF_NCLASS)
+        n = np.zeros(HG_PROF_NCLASS, dtype=np.int64)
+        _check(lib().hg_profile_read(self.h, _d(ms), n.ctypes.data_as(C.POINTER(C.c_int64))))
+        return {lib().hg_profile_class_name(k).decode(): (float(ms[k]), int(n[k])) for k in range(HG_PROF_NCLASS)}
+
     def get_dt(self):
         v = np.zeros(1)
         _check(lib().hg_get_dt(self.h, _d(v)))
         return float(v[0])
+
+
+def mks_geometry_tables(n1, n2, n3, nb=6, a=0.9375, Rin=1.3, Rout=40.0, R0=0.0, hslope=0.3):
+    """Host geometry tables of the modified Kerr-Schild grid (no GPU needed).
+    Returns a dict with gcov/gcon [g1,g2,7,4,4], gcovpert [g1,g2,7,4],
+    gdet [g1,g2,7] (slots FACE1 FACE2 FACE3 CORN1 CORN2 CORN3 CENT; g = m+1),
+    conn [m1,m2,4,4,4], r_cent [m1,m2] and dx[4]."""
+    nbs = [nb if n > 1 else 0 for n in (n1, n2)]
+    m1, m2 = n1 + 2 * nbs[0], n2 + 2 * nbs[1]
+    g1, g2 = m1 + 1, m2 + 1
+    out = dict(gcov=np.zeros((g1, g2, 7, 4, 4)), gcon=np.zeros((g1, g2, 7, 4, 4)),
+               gcovpert=np.zeros((g1, g2, 7, 4)), gdet=np.zeros((g1, g2, 7)),
+               conn=np.zeros((m1, m2, 4, 4, 4)), r_cent=np.zeros((m1, m2)), dx=np.zeros(4))
+    _check(lib().hg_mks_geometry_tables(n1, n2, n3, nb, a, Rin, Rout, R0, hslope,
+                                        *[_d(out[k]) for k in ("gcov", "gcon", "gcovpert", "gdet", "conn",
+                                                               "r_cent", "dx")]))
+    return out
 
 
 def utoprim_batch(U, gcov, gcon, gdet, gcovpert, pr, pflag, iters=None, gam=4.0 / 3.0, mode=HG_C2P_FP64,

@@ -77,11 +77,23 @@ int hg_upload_geometry(hg_ctx *ctx, const double *gcov, const double *gcon,
                        const double *gcovpert, const double *gdet, const double *conn,
                        const double *r_cent);
 
-/* Builds the geometry of the modified Kerr-Schild grid (coord.c:389 LOGRSINTH,
-   metric.c:697 KS metric) on the host and uploads it: spin a, radial range
-   [Rin, Rout] with R0, theta concentration hslope. */
+/* Geometry of the modified Kerr-Schild grid in the host layout of
+   hg_upload_geometry (set_grid.c:7 with coord.c:389 LOGRSINTH, metric.c:697
+   KS metric, metric.c:1388 numerical connection): spin a, radial range
+   [Rin, Rout] with R0, theta concentration hslope.  Host only, needs no GPU.
+   dx (may be NULL) receives the coordinate spacings (coord.c:1142). */
+int hg_mks_geometry_tables(int n1, int n2, int n3, int nb, double a, double Rin, double Rout,
+                           double R0, double hslope, double *gcov, double *gcon,
+                           double *gcovpert, double *gdet, double *conn, double *r_cent,
+                           double dx[4]);
+
+/* hg_mks_geometry_tables for the context's grid, uploaded; also sets the
+   context's dx[1..3] to the grid spacings (read them with hg_get_config). */
 int hg_build_mks_geometry(hg_ctx *ctx, double a, double Rin, double Rout, double R0,
                           double hslope);
+
+/* Copy of the context's configuration. */
+int hg_get_config(const hg_ctx *ctx, hg_config *out);
 
 /* Fishbone-Moncrief torus initial data (init.fishmon.c:237 init_primitives and
    initbase.c:5 init) on the grid built by hg_build_mks_geometry: r_in, r_pmax,
@@ -134,6 +146,22 @@ int hg_make_c2p_batch(int64_t n, double gam, double a, uint64_t seed, double *U,
                       double *p_guess, void *stream);
 
 const char *hg_last_error(void);
+
+/* Kernel-class timing, no counterpart in the reference (its timing is the
+   wall clock of the whole step, diag.c): while a profile is open, every
+   launch of hg_step/hg_step_async/hg_fluxcalc is bracketed by HIP events on
+   its stream.  hg_profile(ctx, 1) opens a profile (clearing the previous
+   one), hg_profile(ctx, 0) closes it; hg_profile_read waits for the events
+   and returns, per class, the summed milliseconds and the interval count. */
+#define HG_PROF_BOUND 0
+#define HG_PROF_RECON1 1 /* WENO5BND interface states, direction 1 (2, 3 follow) */
+#define HG_PROF_FLUX1 4  /* Riemann flux, direction 1 (MC: fused reconstruction) */
+#define HG_PROF_EMF 7
+#define HG_PROF_UPDATE 8 /* update + source + inversion + fixup (+ dt) */
+#define HG_PROF_NCLASS 9
+int hg_profile(hg_ctx *ctx, int enable);
+int hg_profile_read(hg_ctx *ctx, double ms[HG_PROF_NCLASS], int64_t launches[HG_PROF_NCLASS]);
+const char *hg_profile_class_name(int k);
 
 #ifdef __cplusplus
 }
