@@ -7,12 +7,15 @@
  *
  *   hg_bound_prim     bound_prim()        step_ch.c:568 -> bounds.fishmon.c:29
  *   hg_fluxcalc       fluxcalc()          flux.c:7 (which ends in flux_ct(), fluxct.c:10)
- *   hg_advance        advance()           advance.c:16 -> advance_standard advance.c:58
- *                     (flux update + source + Utoprimgen + fixup1zone per zone)
  *   hg_step           step_ch()           step_ch.c:16 -> step_ch_simplempi step_ch.c:35
- *                     (RK2 midpoint, two advance() + bound_prim() calls, dt update)
+ *                     (RK2 midpoint: two advance() calls, advance.c:58 -- flux
+ *                     update + source + Utoprimgen + fixup1zone per zone --
+ *                     each followed by bound_prim(), then the dt update)
  *   hg_utoprim_batch  Utoprimgen()        utoprimgen.c:23 with UTOPRIM2DFINAL
  *                     (utoprim_2d_final.c:34), on a batch of independent zones
+ *
+ * and the problem set-up of init.fishmon.c / initbase.c init()
+ * (hg_mks_geometry_tables, hg_build_mks_geometry, hg_init_torus).
  *
  * Conventions follow the reference: NPR=8 primitives
  * [rho, u, U1, U2, U3, B1, B2, B3] (relative 4-velocity, WHICHVEL VELREL4),
@@ -22,7 +25,7 @@
  * Every function returns 0 on success and >=1 on error (the MYFUN
  * convention of global.h:1700); hg_last_error() describes the last error.
  * Soft per-zone inversion failures are not errors: they are reported as
- * failure codes (hg_utoprim_batch) or counted (hg_step, hg_advance).
+ * failure codes (hg_utoprim_batch) or counted (hg_step, hg_get_counters).
  *
  * Host arrays use the reference's layouts:
  *   primitives  p[i][j][k][pl], i in [0,n1+2nb), ... (ghost zones included,
