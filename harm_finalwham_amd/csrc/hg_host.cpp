@@ -69,6 +69,7 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
   if (cfg->lim == HG_LIM_WENO5BND) {
     rc |= dalloc(&c->pl, ns);
     rc |= dalloc(&c->pr, ns);
+    rc |= dalloc(&c->aux, 2 * nz);
   }
   rc |= dalloc(&c->geo, (size_t)4 * GCOMP * d.gplane);
   rc |= dalloc(&c->corn, (size_t)3 * d.gplane);
@@ -91,7 +92,7 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
 extern "C" int hg_ctx_destroy(hg_ctx *c) {
   if (!c) return 0;
   double *bufs[] = {c->p, c->pk, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2],
-                    c->pl, c->pr, c->geo, c->corn, c->conn, c->floors, c->dt};
+                    c->pl, c->pr, c->aux, c->geo, c->corn, c->conn, c->floors, c->dt};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (c->ndt_bits) (void)hipFree(c->ndt_bits);
@@ -297,6 +298,11 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
   const hg_config &f = c->cfg;
   int nn[4] = {0, d.n1, d.n2, d.n3};
   launch_reset_ndt(c->ndt_bits, s);
+  if (f.lim != HG_LIM_MC) {
+    // the dP/P inputs are shared by the three directions; timed with direction 1
+    prof_mark(c, HG_PROF_RECON1, s);
+    launch_weno_prep(pb, c->geo, c->aux, d, f.gam, s);
+  }
   for (int dir = 1; dir <= 3; dir++) {
     if (nn[dir] <= 1) continue;
     if (f.lim == HG_LIM_MC) {
@@ -305,8 +311,8 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
                      c->ndt_bits + dir - 1, c->err, s);
       prof_mark(c, -1, s);
     } else {
-      prof_mark(c, HG_PROF_RECON1 + dir - 1, s);
-      launch_weno(dir, pb, c->geo, c->pl, c->pr, d, f.gam, s);
+      if (dir != 1) prof_mark(c, HG_PROF_RECON1 + dir - 1, s);
+      launch_weno(dir, pb, c->aux, c->pl, c->pr, d, s);
       prof_mark(c, HG_PROF_FLUX1 + dir - 1, s);
       launch_flux_lr(dir, c->pl, c->pr, c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,
                      c->ndt_bits + dir - 1, c->err, s);

@@ -26,10 +26,12 @@ void launch_c2p_batch(int64_t n, double gam, const double *U, const double *gcov
                       const double *gdet, const double *gcovpert, double *pr, int *pflag, int *iters,
                       hipStream_t s);
 
-// WENO5BND interface states along dir for every zone (hg_weno.hip); pleft/pright
-// have the layout of the primitives
-void launch_weno(int dir, const double *p, const double *geo, double *pleft, double *pright, const Dims &d,
-                 double gam, hipStream_t s);
+// WENO5BND (hg_weno.hip): the per-zone dP/P inputs (2 x spl doubles in aux),
+// then the interface states along dir for every zone; pleft/pright have the
+// layout of the primitives
+void launch_weno_prep(const double *p, const double *geo, double *aux, const Dims &d, double gam, hipStream_t s);
+void launch_weno(int dir, const double *p, const double *aux, double *pleft, double *pright, const Dims &d,
+                 hipStream_t s);
 
 // initial time step over every zone (advance.c:957 set_dt), folded into *out
 void launch_set_dt(const double *p, const double *geo, const Dims &d, double gam, double cour, const double *dx,

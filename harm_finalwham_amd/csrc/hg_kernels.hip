@@ -30,6 +30,22 @@ __device__ __forceinline__ void load_geom(const double *__restrict__ full, int l
   G.g = b[33 * gplane];
 }
 
+// Marks every geometry value as redefined.  The metric products that
+// maxwell() forms depend on the geometry alone; without this barrier the
+// compiler keeps the products of one state alive for the next state that
+// uses the same geometry, which costs more registers than recomputing them.
+__device__ __forceinline__ void geom_barrier(Geom &G) {
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      asm volatile("" : "+v"(G.gcov[a][c]));
+      asm volatile("" : "+v"(G.gcon[a][c]));
+    }
+  asm volatile("" : "+v"(G.gpert0));
+  asm volatile("" : "+v"(G.g));
+}
+
 // block-wide min of a positive double, folded into *out with an integer
 // atomic (positive IEEE doubles order like their bit patterns)
 __device__ __forceinline__ void block_min_to(double v, unsigned long long *out) {
@@ -154,33 +170,37 @@ __global__ void k_bound_x3(double *__restrict__ p, Dims d) {
 // ---------------------------------------------------------------------------
 // Riemann flux at one face from left/right states, fluxcompute.c:12
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool face_flux(int dir, const Geom &G, double *p_l, double *p_r, double gam,
+__device__ __forceinline__ bool face_flux(int dir, Geom &G, double *p_l, double *p_r, double gam,
                                           int fluxmethod, double *F, double &ctop) {
   // flux.c:629 BDIRCONT
   double bav = 0.5 * (p_l[B1 + dir - 1] + p_r[B1 + dir - 1]);
   p_l[B1 + dir - 1] = bav;
   p_r[B1 + dir - 1] = bav;
-  State ql, qr;
+  // each side is finished (fluxes, conserved, wave speeds) before the other
+  // starts, so only its results stay live
   double F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
-  if (get_state(p_l, G, ql)) return true;
+  double cmax_l, cmin_l, cmax_r, cmin_r;
   {
+    State ql;
+    if (get_state(p_l, G, ql)) return true;
     Maxwell M;
     maxwell(p_l, ql, G, M);
     double p1 = one_plus_ud0(ql, G);
     primtoflux(p_l, ql, M, p1, dir, G, gam, F_l);
     primtoflux(p_l, ql, M, p1, 0, G, gam, U_l);
+    if (vchar(p_l, ql, dir, G, gam, cmax_l, cmin_l)) return true;
   }
-  if (get_state(p_r, G, qr)) return true;
+  geom_barrier(G);
   {
+    State qr;
+    if (get_state(p_r, G, qr)) return true;
     Maxwell M;
     maxwell(p_r, qr, G, M);
     double p1 = one_plus_ud0(qr, G);
     primtoflux(p_r, qr, M, p1, dir, G, gam, F_r);
     primtoflux(p_r, qr, M, p1, 0, G, gam, U_r);
+    if (vchar(p_r, qr, dir, G, gam, cmax_r, cmin_r)) return true;
   }
-  double cmax_l, cmin_l, cmax_r, cmin_r;
-  if (vchar(p_l, ql, dir, G, gam, cmax_l, cmin_l)) return true;
-  if (vchar(p_r, qr, dir, G, gam, cmax_r, cmin_r)) return true;
   // wavespeeds.c:325 cminmax_calc
   double lmin = rmin(cmin_l, cmin_r);
   double lmax = rmax(cmax_l, cmax_r);
@@ -238,10 +258,12 @@ __global__ void __launch_bounds__(256) k_flux_mc(const double *__restrict__ p, c
   int64_t total = (int64_t)fr.ni * fr.nj * fr.nk;
   double dtij = 1.e+50;
   if (t < total) {
+    // i fastest, then k: the blocks in flight share one j row of the
+    // phi-independent geometry, which then stays in L2
     int i = fr.is + (int)(t % fr.ni);
     int64_t r = t / fr.ni;
-    int j = fr.js + (int)(r % fr.nj);
-    int k = fr.ks + (int)(r / fr.nj);
+    int k = fr.ks + (int)(r % fr.nk);
+    int j = fr.js + (int)(r / fr.nk);
     int64_t z = d.idx(i, j, k);
     int64_t s = (DIR == 1) ? 1 : (DIR == 2) ? d.sj : d.sk;
     double p_l[NPR], p_r[NPR];
@@ -278,10 +300,12 @@ __global__ void __launch_bounds__(256) k_flux_lr(const double *__restrict__ plef
   int64_t total = (int64_t)fr.ni * fr.nj * fr.nk;
   double dtij = 1.e+50;
   if (t < total) {
+    // i fastest, then k: the blocks in flight share one j row of the
+    // phi-independent geometry, which then stays in L2
     int i = fr.is + (int)(t % fr.ni);
     int64_t r = t / fr.ni;
-    int j = fr.js + (int)(r % fr.nj);
-    int k = fr.ks + (int)(r / fr.nj);
+    int k = fr.ks + (int)(r % fr.nk);
+    int j = fr.js + (int)(r / fr.nk);
     int64_t z = d.idx(i, j, k);
     int64_t s = (DIR == 1) ? 1 : (DIR == 2) ? d.sj : d.sk;
     double p_l[NPR], p_r[NPR];
@@ -318,8 +342,8 @@ __global__ void k_emf(const double *__restrict__ F1, const double *__restrict__ 
   if (t >= (int64_t)ni * nj * nk) return;
   int i = (int)(t % ni);
   int64_t r = t / ni;
-  int j = (int)(r % nj);
-  int k = (int)(r / nj);
+  int k = (int)(r % nk);
+  int j = (int)(r / nk);
   int64_t z = d.idx(i, j, k);
   int64_t zim = s1 ? z - 1 : z, zjm = s2 ? z - d.sj : z, zkm = s3 ? z - d.sk : z;
   const int64_t P = d.spl;
@@ -374,10 +398,11 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
   int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
   unsigned long long nfail = 0, nit = 0;
   if (t < total) {
+    // i fastest, then k (geometry of one j row stays in L2)
     int i = (int)(t % d.n1);
     int64_t r = t / d.n1;
-    int j = (int)(r % d.n2);
-    int k = (int)(r / d.n2);
+    int k = (int)(r % d.n3);
+    int j = (int)(r / d.n3);
     int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
     int64_t z = d.idx(i, j, k);
     const int64_t P = d.spl;
@@ -391,6 +416,7 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
     }
     double Ui[NPR], dUg[NPR];
     bool bad = primtoU(pri, G, gam, Ui);
+    geom_barrier(G);
     // phys.c:578 source -> phys.c:623 source_conn
     {
       State q;
@@ -398,17 +424,19 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
       Maxwell M;
       maxwell(prb, q, G, M);
       double p1 = one_plus_ud0(q, G);
-      double mhd[4][4];
-#pragma unroll
-      for (int jj = 0; jj < 4; jj++) mhd_row(prb, jj, q, M, p1, gam, mhd[jj]);
       const double *cn = conn + d.cidx(i, j);
       double dUc[4] = {0., 0., 0., 0.};
+      // dU_l = sum_jj sum_kk T^jj_kk Gamma^kk_{l jj}: one row of T at a time;
+      // every dU_l still accumulates in (jj, kk) order
 #pragma unroll
-      for (int l = 0; l < 4; l++)
+      for (int jj = 0; jj < 4; jj++) {
+        double mrow[4];
+        mhd_row(prb, jj, q, M, p1, gam, mrow);
 #pragma unroll
-        for (int jj = 0; jj < 4; jj++)
+        for (int l = 0; l < 4; l++)
 #pragma unroll
-          for (int kk = 0; kk < 4; kk++) dUc[l] += mhd[jj][kk] * cn[(kk * 16 + l * 4 + jj) * d.cplane];
+          for (int kk = 0; kk < 4; kk++) dUc[l] += mrow[kk] * cn[(kk * 16 + l * 4 + jj) * d.cplane];
+      }
 #pragma unroll
       for (int l = 0; l < 4; l++)
 #pragma unroll
@@ -420,6 +448,7 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
       dUg[B2] = 0.0 + G.g * 0;
       dUg[B3] = 0.0 + G.g * 0;
     }
+    geom_barrier(G);
     // advance.c:741 flux2dUavg, field components from the flux-CT EMFs (fluxct.c:720-781)
     double Uf[NPR];
     const double dt = dtp[0];

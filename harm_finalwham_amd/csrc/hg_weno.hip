@@ -1,19 +1,28 @@
 // WENO5BND interface reconstruction on gfx950 (interpline.c:517
 // slope_lim_linetype_c2e with lim = WENO5BND, PRIMTOINTERP, DEFAULT reduction).
 //
-// One workgroup owns TL pencils along DIR.  The pencil algorithm has data
-// dependencies along the line (weights at s+-2, lower-order fractions at
-// s+-1), so every stage is evaluated once per point into LDS and the next
-// stage reads its neighbours there:
-//   A  dP/P jump indicator (interpline.c:2148), shared by the 8 primitives
-//   per primitive:
-//   B  load the line
+// The pencil algorithm has data dependencies along the line, evaluated in
+// stages per point and primitive:
+//   A  dP/P jump indicator (interpline.c:2148) from the zone's p u^t (1+u_t)
+//      and the enthalpy-like term at the neighbours; the two per-zone terms do
+//      not depend on the direction and are computed once per substep
+//      (k_weno_prep)
 //   C  smooth-monotonicity indicator + order-3 weights (smonointerp.c:15,
-//      reconstructeno.c:1406, :702)
-//   D  weight ratios -> lower-order fraction (reconstructeno.c:1962)
+//      reconstructeno.c:1406, :702), from y[s-2..s+2]
+//   D  weight ratios -> lower-order fraction (reconstructeno.c:1962), from
+//      the weights at s-2, s, s+2
 //   E  post/pre-shock reduction, optimised weights, order-3/order-2 blend,
 //      stencil evaluation and the ENO5 blend (reconstructeno.c:650, :1617,
-//      :2087, :730, :470-490) -> pleft/pright
+//      :2087, :730, :470-490) -> pleft/pright, from y[s-2..s+2] and the
+//      fractions at s-1, s, s+1
+// Two kernel shapes, both with the radial index i on consecutive lanes so
+// that every global load and store is coalesced:
+//   along i (k_weno_x): one lane per point and primitive; the s+-1, s+-2
+//     neighbours come from other lanes of the wavefront (cross-lane
+//     permutes), a 64-lane wavefront yields 58 interface pairs
+//   along j or k (k_weno_m): one lane per (i, transverse index, primitive)
+//     marches along the line keeping the stage values of the last points
+//     in registers
 // Only the pencils and points the fluxes read are computed: transverse
 // indices -1..n and points -1..n along the line (weights at -4..n+3).
 // All arithmetic is FP64 in the reference's operation order.
@@ -232,187 +241,266 @@ __device__ __forceinline__ double eno5(bool right, const double *y) {
   return sum5(d);
 }
 
-// LDS arrays per line, each of length LS (points -nb .. n+nb-1 at offset nb)
-enum { A_Y, A_W0, A_W1, A_W2, A_M0, A_M1, A_LOF0, A_LOFD, A_PAR, A_UPAR, A_RHO, A_DP, A_NUM };
-
-template <int DIR, int TL>
-__global__ void __launch_bounds__(256) k_weno(const double *__restrict__ p, const double *__restrict__ geo,
-                                              double *__restrict__ pleft, double *__restrict__ pright, Dims d,
-                                              double gam, int LS) {
-  extern __shared__ double sm[];
-  const int n = (DIR == 1) ? d.n1 : (DIR == 2) ? d.n2 : d.n3;
-  const int nb = (DIR == 1) ? d.nb1 : (DIR == 2) ? d.nb2 : d.nb3;
-  // transverse extents: the flux faces read one layer beyond the active zones
-  int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
-  int c1, c2, lo1, lo2;
-  if (DIR == 1) { c1 = d.n2 + 2 * s2; lo1 = -s2; c2 = d.n3 + 2 * s3; lo2 = -s3; }
-  else if (DIR == 2) { c1 = d.n1 + 2 * s1; lo1 = -s1; c2 = d.n3 + 2 * s3; lo2 = -s3; }
-  else { c1 = d.n1 + 2 * s1; lo1 = -s1; c2 = d.n2 + 2 * s2; lo2 = -s2; }
-  const int nlines = c1 * c2;
-  const int tid = threadIdx.x;
-  const int l = tid % TL, sg = tid / TL, nsg = 256 / TL;
-  const int line = blockIdx.x * TL + l;
-  const bool active = line < nlines;
-  int t1 = lo1 + (active ? line % c1 : 0), t2 = lo2 + (active ? line / c1 : 0);
-  int i0, j0, k0;
-  if (DIR == 1) { i0 = 0; j0 = t1; k0 = t2; }
-  else if (DIR == 2) { i0 = t1; j0 = 0; k0 = t2; }
-  else { i0 = t1; j0 = t2; k0 = 0; }
-  const int64_t z0 = d.idx(i0, j0, k0);
-  const int64_t sd = (DIR == 1) ? 1 : (DIR == 2) ? d.sj : d.sk;
-  double *A = sm + (int64_t)l * A_NUM * LS;
-#define ARR(a, s) A[(a)*LS + (s) + nb]
-
-  // ---- stage A: dP/P indicator (interpline.c:2148 compute_df_line_new) ----
-  if (active) {
-    for (int s = -2 + sg; s <= n + 1; s += nsg) {
-      double pr[NPR];
+// ---------------------------------------------------------------------------
+// the stages at one point
+// ---------------------------------------------------------------------------
+// C: y points at the cell, y[-2..2] are read
+__device__ __forceinline__ void stage_c(const double *y, double w[3], double &m0, double &lof0) {
+  double m1;
+  if (cusp(y)) { m0 = 0; m1 = 1; }
+  else { m0 = mono_eno5(y); m1 = m0; }
+  lof0 = weights3(m0, m1, y, w);
+  if (m0 > 0.0 && m0 < 1.0) {  // reconstructeno.c:702 mono_adjust_unoptimized_weights
+    double simpleweight = 1.0 / 3.0;
 #pragma unroll
-      for (int pl = 0; pl < NPR; pl++) pr[pl] = p[z0 + s * sd + pl * d.spl];
-      int gi = (DIR == 1) ? s : i0, gj = (DIR == 2) ? s : j0;
-      Geom G;
-      {
-        const double *b = geo + (int64_t)GL_CENT * GCOMP * d.gplane + d.gidx(gi, gj);
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-          for (int c = 0; c < 4; c++) {
-            G.gcov[a][c] = b[(a * 4 + c) * d.gplane];
-            G.gcon[a][c] = b[(16 + a * 4 + c) * d.gplane];
-          }
-        G.gpert0 = b[32 * d.gplane];
-        G.g = b[33 * d.gplane];
-      }
-      State q;
-      get_state(pr, G, q);
-      double p1 = one_plus_ud0(q, G);
-      ARR(A_PAR, s) = p1 * (q.ucon[0]);
-      ARR(A_UPAR, s) = gam * pr[UU] * (q.ucon[0] * q.ucov[0]) + (gam - 1.0) * pr[UU];
-      ARR(A_RHO, s) = pr[RHO];
-    }
+    for (int c = 0; c < 3; c++) w[c] = (1.0 - m0) * w[c] + m0 * simpleweight;
   }
-  __syncthreads();
-  if (active) {
-    for (int s = -1 + sg; s <= n; s += nsg) {
-      double rho = ARR(A_RHO, s);
-      ARR(A_DP, s) = (fabs(rho * (ARR(A_PAR, s + 1) - ARR(A_PAR, s - 1))) +
-                      fabs(ARR(A_UPAR, s + 1) - ARR(A_UPAR, s - 1))) /
-                     (fabs(rho * ARR(A_PAR, s)) + fabs(ARR(A_UPAR, s)) + DBL_MIN);
-    }
-  }
-
-  for (int pl = 0; pl < NPR; pl++) {
-    __syncthreads();
-    // ---- stage B: the line ----
-    if (active)
-      for (int s = -nb + sg; s <= n + nb - 1; s += nsg) ARR(A_Y, s) = p[z0 + s * sd + pl * d.spl];
-    __syncthreads();
-    // ---- stage C: monotonicity indicator and order-3 weights at -4..n+3 ----
-    if (active) {
-      for (int s = -4 + sg; s <= n + 3; s += nsg) {
-        const double *y = &ARR(A_Y, s);
-        double m0, m1;
-        if (cusp(y)) { m0 = 0; m1 = 1; }
-        else { m0 = mono_eno5(y); m1 = m0; }
-        double w[3];
-        double lof0 = weights3(m0, m1, y, w);
-        if (m0 > 0.0 && m0 < 1.0) {  // reconstructeno.c:702 mono_adjust_unoptimized_weights
-          double simpleweight = 1.0 / 3.0;
-#pragma unroll
-          for (int c = 0; c < 3; c++) w[c] = (1.0 - m0) * w[c] + m0 * simpleweight;
-        }
-        ARR(A_W0, s) = w[0];
-        ARR(A_W1, s) = w[1];
-        ARR(A_W2, s) = w[2];
-        ARR(A_M0, s) = m0;
-        ARR(A_M1, s) = m1;
-        ARR(A_LOF0, s) = lof0;
-      }
-    }
-    __syncthreads();
-    // ---- stage D: weight ratios at -2..n+1 (reconstructeno.c:1962) ----
-    if (active) {
-      for (int s = -2 + sg; s <= n + 1; s += nsg) {
-        double wl = (ARR(A_W2, s) + WENO_EPS) / (ARR(A_W0, s - 2) + WENO_EPS);
-        double wr = (ARR(A_W0, s) + WENO_EPS) / (ARR(A_W2, s + 2) + WENO_EPS);
-        double lof = (wmin(wl, wr) - MIN_TR) / (MAX_TR - MIN_TR);
-        lof = wmax(0., wmin(lof, 1.0));
-        ARR(A_LOFD, s) = wmax(lof, ARR(A_LOF0, s));
-      }
-    }
-    __syncthreads();
-    // ---- stage E: reduction, optimised weights, outputs at -1..n ----
-    if (active) {
-      for (int s = -1 + sg; s <= n; s += nsg) {
-        const double *y = &ARR(A_Y, s);
-        double m0 = ARR(A_M0, s);
-        double lof = ARR(A_LOFD, s), dPi = ARR(A_DP, s);
-        double shock = wmin((fabs(dPi) - 0.5 * REDUCE_DP) / (0.5 * REDUCE_DP), 1.);
-        lof = wmax(lof, shock * ARR(A_LOFD, s + 1));
-        lof = wmax(lof, shock * ARR(A_LOFD, s - 1));
-        double outv[2];
-        if (m0 == 1) {
-          outv[0] = eno5(false, y);
-          outv[1] = eno5(true, y);
-        } else {
-          // compute_optimized_cf_stencil_weights (order 3)
-          double w[3] = {ARR(A_W0, s), ARR(A_W1, s), ARR(A_W2, s)};
-          double wl[3] = {w[0] * (1. / 16.), w[1] * (5. / 8.), w[2] * (5. / 16.)};
-          double wr[3] = {w[0] * (5. / 16.), w[1] * (5. / 8.), w[2] * (1. / 16.)};
-          double sl = sum3(wl[0], wl[1], wl[2]), sr = sum3(wr[0], wr[1], wr[2]);
-#pragma unroll
-          for (int c = 0; c < 3; c++) { wl[c] /= (sl + DBL_MIN); wr[c] /= (sr + DBL_MIN); }
-          double l2[2] = {0., 0.}, r2[2] = {0., 0.};
-          bool two = (0.0 < lof);
-          if (two) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) { wl[c] *= (1. - lof); wr[c] *= (1. - lof); }
-            weights2(m0, y, l2, r2);
-            l2[0] *= lof; l2[1] *= lof; r2[0] *= lof; r2[1] *= lof;
-          }
-#pragma unroll
-          for (int side = 0; side < 2; side++) {
-            double prev = 0.0;
-            if (m0 != 0.0) prev = eno5(side == 1, y);
-            double ov = 0.0;
-            ov += cvt3(side == 1, side ? wr : wl, y);
-            if (two) ov += cvt2(side == 1, side ? r2 : l2, y);
-            outv[side] = (1.0 - m0) * ov + m0 * prev;
-          }
-        }
-        int64_t z = z0 + s * sd + pl * d.spl;
-        pleft[z] = outv[0];
-        pright[z] = outv[1];
-      }
-    }
-  }
-#undef ARR
 }
 
-void launch_weno(int dir, const double *p, const double *geo, double *pleft, double *pright, const Dims &d,
-                 double gam, hipStream_t s) {
+// D: reconstructeno.c:1962 compute_weights_ratios and the cusp fraction
+__device__ __forceinline__ double stage_d(double w2_s, double w0_sm2, double w0_s, double w2_sp2, double lof0) {
+  double wl = (w2_s + WENO_EPS) / (w0_sm2 + WENO_EPS);
+  double wr = (w0_s + WENO_EPS) / (w2_sp2 + WENO_EPS);
+  double lof = (wmin(wl, wr) - MIN_TR) / (MAX_TR - MIN_TR);
+  lof = wmax(0., wmin(lof, 1.0));
+  return wmax(lof, lof0);
+}
+
+// A: interpline.c:2148 compute_df_line_new, dP/P at s from s-1, s, s+1
+__device__ __forceinline__ double dp_jump(double rho, double parm, double par0, double parp, double uparm,
+                                          double upar0, double uparp) {
+  return (fabs(rho * (parp - parm)) + fabs(uparp - uparm)) / (fabs(rho * par0) + fabs(upar0) + DBL_MIN);
+}
+
+// E: y points at the cell; lofm/lof0/lofp are the stage-D fractions at s-1, s, s+1
+__device__ __forceinline__ void stage_e(const double *y, double m0, double w0, double w1, double w2, double lofm,
+                                        double lof0, double lofp, double dPi, double &outl, double &outr) {
+  double shock = wmin((fabs(dPi) - 0.5 * REDUCE_DP) / (0.5 * REDUCE_DP), 1.);
+  double lof = lof0;
+  lof = wmax(lof, shock * lofp);
+  lof = wmax(lof, shock * lofm);
+  if (m0 == 1) {
+    outl = eno5(false, y);
+    outr = eno5(true, y);
+    return;
+  }
+  // compute_optimized_cf_stencil_weights (order 3)
+  double wl[3] = {w0 * (1. / 16.), w1 * (5. / 8.), w2 * (5. / 16.)};
+  double wr[3] = {w0 * (5. / 16.), w1 * (5. / 8.), w2 * (1. / 16.)};
+  double sl = sum3(wl[0], wl[1], wl[2]), sr = sum3(wr[0], wr[1], wr[2]);
+#pragma unroll
+  for (int c = 0; c < 3; c++) { wl[c] /= (sl + DBL_MIN); wr[c] /= (sr + DBL_MIN); }
+  double l2[2] = {0., 0.}, r2[2] = {0., 0.};
+  bool two = (0.0 < lof);
+  if (two) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) { wl[c] *= (1. - lof); wr[c] *= (1. - lof); }
+    weights2(m0, y, l2, r2);
+    l2[0] *= lof; l2[1] *= lof; r2[0] *= lof; r2[1] *= lof;
+  }
+  double outv[2];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    double prev = 0.0;
+    if (m0 != 0.0) prev = eno5(side == 1, y);
+    double ov = 0.0;
+    ov += cvt3(side == 1, side ? wr : wl, y);
+    if (two) ov += cvt2(side == 1, side ? r2 : l2, y);
+    outv[side] = (1.0 - m0) * ov + m0 * prev;
+  }
+  outl = outv[0];
+  outr = outv[1];
+}
+
+// ---------------------------------------------------------------------------
+// stage A inputs per zone: par = (1 + u_t) u^t, upar = gam u u^t u_t + (gam-1) u
+// over the box [-2, n+1] of every direction with more than one zone
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_weno_prep(const double *__restrict__ p, const double *__restrict__ geo,
+                                                   double *__restrict__ aux, Dims d, double gam) {
+  const int e1 = d.n1 > 1 ? 2 : 0, e2 = d.n2 > 1 ? 2 : 0, e3 = d.n3 > 1 ? 2 : 0;
+  const int b1 = d.n1 + 2 * e1, b2 = d.n2 + 2 * e2, b3 = d.n3 + 2 * e3;
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)b1 * b2 * b3) return;
+  int i = -e1 + (int)(t % b1);
+  int64_t r = t / b1;
+  int k = -e3 + (int)(r % b3), j = -e2 + (int)(r / b3);
+  int64_t z = d.idx(i, j, k);
+  double pr[NPR];
+#pragma unroll
+  for (int pl = 0; pl < NPR; pl++) pr[pl] = p[z + pl * d.spl];
+  Geom G;
+  {
+    const double *b = geo + (int64_t)GL_CENT * GCOMP * d.gplane + d.gidx(i, j);
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        G.gcov[a][c] = b[(a * 4 + c) * d.gplane];
+        G.gcon[a][c] = b[(16 + a * 4 + c) * d.gplane];
+      }
+    G.gpert0 = b[32 * d.gplane];
+    G.g = b[33 * d.gplane];
+  }
+  State q;
+  get_state(pr, G, q);
+  double p1 = one_plus_ud0(q, G);
+  aux[z] = p1 * (q.ucon[0]);
+  aux[z + d.spl] = gam * pr[UU] * (q.ucon[0] * q.ucov[0]) + (gam - 1.0) * pr[UU];
+}
+
+// ---------------------------------------------------------------------------
+// along i: one lane per point; wavefront w covers points s0-3 .. s0+60 and
+// writes the interface pair of s0 .. s0+57
+// ---------------------------------------------------------------------------
+constexpr int XOUT = 58;
+
+__global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, const double *__restrict__ aux,
+                                                double *__restrict__ pleft, double *__restrict__ pright, Dims d,
+                                                int nseg) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wid = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int n = d.n1;
+  const int s2 = d.n2 > 1, s3 = d.n3 > 1;
+  const int c1 = d.n2 + 2 * s2, c2 = d.n3 + 2 * s3;
+  if (wid >= (int64_t)nseg * NPR * c1 * c2) return;  // uniform per wavefront
+  const int seg = (int)(wid % nseg);
+  int64_t r = wid / nseg;
+  const int pl = (int)(r % NPR);
+  r /= NPR;
+  const int j = -s2 + (int)(r % c1), k = -s3 + (int)(r / c1);
+  const int s0 = -1 + seg * XOUT;
+  const int s = s0 - 3 + lane;
+  const int64_t zl = d.idx(0, j, k);
+  const double *yl = p + (int64_t)pl * d.spl + zl;
+  double y[5] = {0., 0., 0., 0., 0.}, w[3] = {1., 1., 1.}, m0 = 0., lof0 = 0.;
+  if (s <= n + 3) {
+#pragma unroll
+    for (int a = 0; a < 5; a++) y[a] = yl[s - 2 + a];
+    stage_c(y + 2, w, m0, lof0);
+  }
+  // stage D needs the weights of s-2 and s+2 (lanes -2 and +2)
+  double w0m2 = __shfl_up(w[0], 2, 64);
+  double w2p2 = __shfl_down(w[2], 2, 64);
+  double lofd = stage_d(w[2], w0m2, w[0], w2p2, lof0);
+  double lofm = __shfl_up(lofd, 1, 64);
+  double lofp = __shfl_down(lofd, 1, 64);
+  if (lane >= 3 && lane < 3 + XOUT && s <= n) {
+    const double *a0 = aux + zl, *a1 = aux + d.spl + zl;
+    double dPi = dp_jump(p[RHO * d.spl + zl + s], a0[s - 1], a0[s], a0[s + 1], a1[s - 1], a1[s], a1[s + 1]);
+    double ol, orr;
+    stage_e(y + 2, m0, w[0], w[1], w[2], lofm, lofd, lofp, dPi, ol, orr);
+    int64_t z = (int64_t)pl * d.spl + zl + s;
+    pleft[z] = ol;
+    pright[z] = orr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// along j (DIR 2) or k (DIR 3): lane = (i, transverse index) flattened with i
+// fastest; each lane marches over its segment of the line: at step t it
+// evaluates C(t), D(t-2) and E(t-3), keeping the last values in registers
+// ---------------------------------------------------------------------------
+template <int DIR>
+__global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, const double *__restrict__ aux,
+                                                double *__restrict__ pleft, double *__restrict__ pright, Dims d,
+                                                int nseg, int seglen) {
+  const int n = (DIR == 2) ? d.n2 : d.n3;
+  const int s1 = d.n1 > 1;
+  const int so = (DIR == 2) ? (d.n3 > 1) : (d.n2 > 1);
+  const int ci = d.n1 + 2 * s1, co = ((DIR == 2) ? d.n3 : d.n2) + 2 * so;
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= (int64_t)ci * co) return;
+  const int i = -s1 + (int)(g % ci);
+  const int o = -so + (int)(g / ci);
+  const int seg = blockIdx.y % nseg;
+  const int pl = blockIdx.y / nseg;
+  const int e0 = -1 + seg * seglen;
+  const int e1 = (e0 + seglen - 1 < n) ? e0 + seglen - 1 : n;
+  const int64_t zl = (DIR == 2) ? d.idx(i, 0, o) : d.idx(i, o, 0);
+  const int64_t sd = (DIR == 2) ? d.sj : d.sk;
+  const double *yl = p + (int64_t)pl * d.spl + zl;
+  const double *rl = p + (int64_t)RHO * d.spl + zl;
+  const double *a0 = aux + zl, *a1 = aux + d.spl + zl;
+  double *ol = pleft + (int64_t)pl * d.spl + zl, *orr = pright + (int64_t)pl * d.spl + zl;
+  // windows: yw = y(t-5..t+2), W0 = w0(t-4..t), W1/W2/M0 = (t-3..t), L0 = lof0(t-2..t),
+  // LD = stage-D fraction (t-4..t-2), PA/UA = par/upar (t-4..t-2)
+  double yw[8], W0[5], W1[4], W2[4], M0[4], L0[3], LD[3], PA[3], UA[3];
+#pragma unroll
+  for (int a = 0; a < 8; a++) yw[a] = 0.;
+#pragma unroll
+  for (int a = 0; a < 5; a++) W0[a] = 1.;
+#pragma unroll
+  for (int a = 0; a < 4; a++) { W1[a] = 1.; W2[a] = 1.; M0[a] = 0.; }
+#pragma unroll
+  for (int a = 0; a < 3; a++) { L0[a] = 0.; LD[a] = 0.; PA[a] = 1.; UA[a] = 1.; }
+  // before step t = e0-3 the y window holds y(t-6..t+1); points below e0-5 are never read
+#pragma unroll
+  for (int a = 4; a < 8; a++) yw[a] = yl[(int64_t)(e0 - 9 + a) * sd];
+  // par/upar window before step t: (t-5..t-3); loaded from t-2 on
+  for (int t = e0 - 3; t <= e1 + 3; t++) {
+#pragma unroll
+    for (int a = 0; a < 7; a++) yw[a] = yw[a + 1];
+    yw[7] = yl[(int64_t)(t + 2) * sd];
+    double w[3], m0, lof0;
+    stage_c(yw + 5, w, m0, lof0);
+#pragma unroll
+    for (int a = 0; a < 4; a++) W0[a] = W0[a + 1];
+    W0[4] = w[0];
+#pragma unroll
+    for (int a = 0; a < 3; a++) { W1[a] = W1[a + 1]; W2[a] = W2[a + 1]; M0[a] = M0[a + 1]; }
+    W1[3] = w[1];
+    W2[3] = w[2];
+    M0[3] = m0;
+    L0[0] = L0[1];
+    L0[1] = L0[2];
+    L0[2] = lof0;
+    // D(t-2)
+    LD[0] = LD[1];
+    LD[1] = LD[2];
+    LD[2] = stage_d(W2[1], W0[0], W0[2], W2[3], L0[0]);
+    PA[0] = PA[1];
+    PA[1] = PA[2];
+    UA[0] = UA[1];
+    UA[1] = UA[2];
+    if (t - 2 >= -2) {
+      PA[2] = a0[(int64_t)(t - 2) * sd];
+      UA[2] = a1[(int64_t)(t - 2) * sd];
+    }
+    // E(t-3)
+    if (t - 3 >= e0) {
+      double dPi = dp_jump(rl[(int64_t)(t - 3) * sd], PA[0], PA[1], PA[2], UA[0], UA[1], UA[2]);
+      double outl, outr;
+      stage_e(yw + 2, M0[0], W0[1], W1[0], W2[0], LD[0], LD[1], LD[2], dPi, outl, outr);
+      ol[(int64_t)(t - 3) * sd] = outl;
+      orr[(int64_t)(t - 3) * sd] = outr;
+    }
+  }
+}
+
+void launch_weno_prep(const double *p, const double *geo, double *aux, const Dims &d, double gam, hipStream_t s) {
+  int64_t n = (int64_t)(d.n1 + (d.n1 > 1 ? 4 : 0)) * (d.n2 + (d.n2 > 1 ? 4 : 0)) * (d.n3 + (d.n3 > 1 ? 4 : 0));
+  hipLaunchKernelGGL(k_weno_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, geo, aux, d, gam);
+}
+
+void launch_weno(int dir, const double *p, const double *aux, double *pleft, double *pright, const Dims &d,
+                 hipStream_t s) {
   int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
   if (dir == 1) {
-    constexpr int TL = 1;
-    int LS = d.m1;
-    int64_t nl = (int64_t)(d.n2 + 2 * s2) * (d.n3 + 2 * s3);
-    size_t shm = (size_t)TL * A_NUM * LS * sizeof(double);
-    hipLaunchKernelGGL((k_weno<1, TL>), dim3((unsigned)((nl + TL - 1) / TL)), dim3(256), shm, s, p, geo, pleft,
-                       pright, d, gam, LS);
-  } else if (dir == 2) {
-    constexpr int TL = 4;
-    int LS = d.m2;
-    int64_t nl = (int64_t)(d.n1 + 2 * s1) * (d.n3 + 2 * s3);
-    size_t shm = (size_t)TL * A_NUM * LS * sizeof(double);
-    hipLaunchKernelGGL((k_weno<2, TL>), dim3((unsigned)((nl + TL - 1) / TL)), dim3(256), shm, s, p, geo, pleft,
-                       pright, d, gam, LS);
+    int nseg = (d.n1 + 2 + XOUT - 1) / XOUT;
+    int64_t nw = (int64_t)nseg * NPR * (d.n2 + 2 * s2) * (d.n3 + 2 * s3);
+    hipLaunchKernelGGL(k_weno_x, dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, s, p, aux, pleft, pright, d, nseg);
   } else {
-    constexpr int TL = 8;
-    int LS = d.m3;
-    int64_t nl = (int64_t)(d.n1 + 2 * s1) * (d.n2 + 2 * s2);
-    size_t shm = (size_t)TL * A_NUM * LS * sizeof(double);
-    hipLaunchKernelGGL((k_weno<3, TL>), dim3((unsigned)((nl + TL - 1) / TL)), dim3(256), shm, s, p, geo, pleft,
-                       pright, d, gam, LS);
+    int n = (dir == 2) ? d.n2 : d.n3;
+    int64_t lanes = (int64_t)(d.n1 + 2 * s1) * ((dir == 2) ? d.n3 + 2 * s3 : d.n2 + 2 * s2);
+    // split long lines so that the grid holds enough wavefronts
+    int nseg = (n + 2 >= 96) ? 2 : 1;
+    int seglen = (n + 2 + nseg - 1) / nseg;
+    dim3 grid((unsigned)((lanes + 255) / 256), (unsigned)(NPR * nseg));
+    if (dir == 2)
+      hipLaunchKernelGGL(k_weno_m<2>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen);
+    else
+      hipLaunchKernelGGL(k_weno_m<3>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen);
   }
 }
 
