@@ -64,6 +64,7 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
   int rc = 0;
   rc |= dalloc(&c->p, ns);
   rc |= dalloc(&c->pk, ns);
+  rc |= dalloc(&c->uf, ns);
   for (int k = 0; k < 3; k++) rc |= dalloc(&c->F[k], ns);
   for (int k = 0; k < 3; k++) rc |= dalloc(&c->emf[k], nz);
   if (cfg->lim == HG_LIM_WENO5BND) {
@@ -91,7 +92,7 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
 
 extern "C" int hg_ctx_destroy(hg_ctx *c) {
   if (!c) return 0;
-  double *bufs[] = {c->p, c->pk, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2],
+  double *bufs[] = {c->p, c->pk, c->uf, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2],
                     c->pl, c->pr, c->aux, c->geo, c->corn, c->conn, c->floors, c->dt};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
@@ -396,7 +397,7 @@ static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, doubl
   const hg_config &f = c->cfg;
   enqueue_fluxcalc(c, pb, s);
   prof_mark(c, HG_PROF_UPDATE, s);
-  launch_update(pi, pb, pf, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2], c->geo, c->conn,
+  launch_update(pi, pb, pf, c->uf, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2], c->geo, c->conn,
                 c->floors, c->dt, cuf2, finalstep, c->d, f.gam, f.dx, f.gammamax, c->counters, c->err, s);
   if (finalstep) launch_dt(c->ndt_bits, c->dt, f.defcon, f.safe, s);
   // step_ch.c:127 and :166 call bound_prim twice per substep; the second call

@@ -16,6 +16,7 @@ struct hg_ctx {
   double *emf[3] = {nullptr, nullptr, nullptr};
   double *pl = nullptr, *pr = nullptr;       // WENO5BND interface states (per direction, reused)
   double *aux = nullptr;                     // WENO5BND per-zone dP/P inputs
+  double *uf = nullptr;                      // updated conserved variables before the inversion
   double *geo = nullptr, *corn = nullptr, *conn = nullptr, *floors = nullptr;
   double *dt = nullptr;
   unsigned long long *ndt_bits = nullptr, *counters = nullptr;

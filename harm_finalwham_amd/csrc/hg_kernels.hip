@@ -46,6 +46,13 @@ __device__ __forceinline__ void geom_barrier(Geom &G) {
   asm volatile("" : "+v"(G.g));
 }
 
+// Pins n values at this point of the program: everything that computes
+// them completes before, and nothing that reads them starts earlier.
+__device__ __forceinline__ void value_barrier(double *v, int n) {
+#pragma unroll
+  for (int a = 0; a < n; a++) asm volatile("" : "+v"(v[a]));
+}
+
 // block-wide min of a positive double, folded into *out with an integer
 // atomic (positive IEEE doubles order like their bit patterns)
 __device__ __forceinline__ void block_min_to(double v, unsigned long long *out) {
@@ -382,21 +389,23 @@ __global__ void k_emf(const double *__restrict__ F1, const double *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// per-zone update, advance.c:143-213: U(pi) + dt * (flux divergence + source),
-// Noble 2D inversion with guess pb, fixup1zone on the final substep
+// per-zone update, advance.c:143-213, in two kernels:
+//   k_update: U(pi) + dt * (flux divergence + source) -> Uf
+//   k_invert: Noble 2D inversion of Uf with guess pb, fixup1zone on the
+//             final substep -> pf
+// (one kernel holding both needs more registers than either, and the Uf
+// round trip through HBM costs less than the occupancy it loses)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, const double *__restrict__ pb,
-                                                double *__restrict__ pf, const double *__restrict__ F1,
+                                                double *__restrict__ ubuf, const double *__restrict__ F1,
                                                 const double *__restrict__ F2, const double *__restrict__ F3,
                                                 const double *__restrict__ emf1, const double *__restrict__ emf2,
                                                 const double *__restrict__ emf3, const double *__restrict__ geo,
-                                                const double *__restrict__ conn, const double *__restrict__ floors,
-                                                const double *__restrict__ dtp, double cuf2, int finalstep, Dims d,
-                                                double gam, double idx1, double idx2, double idx3, double gammamax,
-                                                unsigned long long *__restrict__ counters, int *__restrict__ err) {
+                                                const double *__restrict__ conn, const double *__restrict__ dtp,
+                                                double cuf2, Dims d, double gam, double idx1, double idx2,
+                                                double idx3, int *__restrict__ err) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
-  unsigned long long nfail = 0, nit = 0;
   if (t < total) {
     // i fastest, then k (geometry of one j row stays in L2)
     int i = (int)(t % d.n1);
@@ -408,15 +417,21 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
     const int64_t P = d.spl;
     Geom G;
     load_geom(geo, GL_CENT, d.gplane, d.gidx(i, j), G);
-    double pri[NPR], prb[NPR];
+    double Ui[NPR], dUg[NPR], prb[NPR];
+    bool bad;
+    {
+      double pri[NPR];
 #pragma unroll
-    for (int pl = 0; pl < NPR; pl++) {
-      pri[pl] = pi[z + pl * P];
-      prb[pl] = pb[z + pl * P];
+      for (int pl = 0; pl < NPR; pl++) pri[pl] = pi[z + pl * P];
+      bad = primtoU(pri, G, gam, Ui);
     }
-    double Ui[NPR], dUg[NPR];
-    bool bad = primtoU(pri, G, gam, Ui);
+    // the three phases (U of p_i, source of p_b, flux divergence) are kept
+    // apart: interleaved, their live values exceed the register file
+    value_barrier(Ui, NPR);
     geom_barrier(G);
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) prb[pl] = pb[z + pl * P];
     // phys.c:578 source -> phys.c:623 source_conn
     {
       State q;
@@ -448,7 +463,9 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
       dUg[B2] = 0.0 + G.g * 0;
       dUg[B3] = 0.0 + G.g * 0;
     }
+    value_barrier(dUg, NPR);
     geom_barrier(G);
+    asm volatile("" ::: "memory");
     // advance.c:741 flux2dUavg, field components from the flux-CT EMFs (fluxct.c:720-781)
     double Uf[NPR];
     const double dt = dtp[0];
@@ -490,9 +507,36 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
       // Uf on the final substep (Cunew = {0,0,1})
       Uf[pl] = 1.0 * Ui[pl] + 0.0 * 0.0 + cuf2 * dt * (dUr + dUg[pl]);
     }
-    double pr[NPR];
+    if (bad) atomicOr(err, 4);
 #pragma unroll
-    for (int pl = 0; pl < NPR; pl++) pr[pl] = prb[pl];
+    for (int pl = 0; pl < NPR; pl++) ubuf[z + pl * P] = Uf[pl];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_invert(const double *__restrict__ ubuf, const double *__restrict__ pb,
+                                                double *__restrict__ pf, const double *__restrict__ geo,
+                                                const double *__restrict__ floors, int finalstep, Dims d,
+                                                double gam, double gammamax,
+                                                unsigned long long *__restrict__ counters, int *__restrict__ err) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
+  unsigned long long nfail = 0, nit = 0;
+  if (t < total) {
+    int i = (int)(t % d.n1);
+    int64_t r = t / d.n1;
+    int k = (int)(r % d.n3);
+    int j = (int)(r / d.n3);
+    int64_t z = d.idx(i, j, k);
+    const int64_t P = d.spl;
+    Geom G;
+    load_geom(geo, GL_CENT, d.gplane, d.gidx(i, j), G);
+    double Uf[NPR], pr[NPR];
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) {
+      Uf[pl] = ubuf[z + pl * P];
+      pr[pl] = pb[z + pl * P];
+    }
+    bool bad = false;
     int iters = 0;
     int flag = utoprimgen_fp64(Uf, G, gam, pr, iters);
     nit = (unsigned long long)iters;
@@ -612,15 +656,16 @@ void launch_emf(const double *F1, const double *F2, const double *F3, const doub
   hipLaunchKernelGGL(k_emf, dim3(nblocks(n, 256)), dim3(256), 0, s, F1, F2, F3, geo, corn, e1, e2, e3, d);
 }
 
-void launch_update(const double *pi, const double *pb, double *pf, const double *F1, const double *F2,
-                   const double *F3, const double *e1, const double *e2, const double *e3, const double *geo,
-                   const double *conn, const double *floors, const double *dtp, double cuf2, int finalstep,
-                   const Dims &d, double gam, const double *dx, double gammamax, unsigned long long *counters,
-                   int *err, hipStream_t s) {
+void launch_update(const double *pi, const double *pb, double *pf, double *ubuf, const double *F1,
+                   const double *F2, const double *F3, const double *e1, const double *e2, const double *e3,
+                   const double *geo, const double *conn, const double *floors, const double *dtp, double cuf2,
+                   int finalstep, const Dims &d, double gam, const double *dx, double gammamax,
+                   unsigned long long *counters, int *err, hipStream_t s) {
   int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
-  hipLaunchKernelGGL(k_update, dim3(nblocks(n, 256)), dim3(256), 0, s, pi, pb, pf, F1, F2, F3, e1, e2, e3, geo,
-                     conn, floors, dtp, cuf2, finalstep, d, gam, 1.0 / dx[1], 1.0 / dx[2], 1.0 / dx[3], gammamax,
-                     counters, err);
+  hipLaunchKernelGGL(k_update, dim3(nblocks(n, 256)), dim3(256), 0, s, pi, pb, ubuf, F1, F2, F3, e1, e2, e3, geo,
+                     conn, dtp, cuf2, d, gam, 1.0 / dx[1], 1.0 / dx[2], 1.0 / dx[3], err);
+  hipLaunchKernelGGL(k_invert, dim3(nblocks(n, 256)), dim3(256), 0, s, ubuf, pb, pf, geo, floors, finalstep, d,
+                     gam, gammamax, counters, err);
 }
 
 void launch_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe, hipStream_t s) {
