@@ -70,6 +70,89 @@ def cpu_baseline(timeout_s=240):
                       f"zones, {res['steps']} RK2 step, {res['seconds']:.1f} s timed"}
 
 
+def c2p_bench(n, reps=3):
+    """Batched inversions (the second half of the headline metric): the
+    benchmark distribution for gam = 4/3 and 5/3 generated on the GPU,
+    inverted by the plain FP64 kernel and by the mixed-precision persistent
+    kernel; best-of-reps kernel time with HIP events on the launch stream."""
+    import torch
+    from harm_finalwham_amd import harmgpu
+    res = {"n": n, "gam": [], "fp64_ms": [], "mixed_ms": [], "flags_identical": True, "fallbacks": 0,
+           "max_err": [0.0, 0.0, 0.0], "p999_u_rel": 0.0, "fp64_iters_per_zone": [], "mixed_iters_per_zone": []}
+    for gam in (4.0 / 3.0, 5.0 / 3.0):
+        b = harmgpu.make_c2p_batch(n, gam=gam, seed=2024)
+        out = {}
+        for mode, key in ((harmgpu.HG_C2P_FP64, "fp64"), (harmgpu.HG_C2P_MIXED, "mixed")):
+            best = None
+            for _ in range(reps):
+                pr = b["p_guess"].clone()
+                pflag = torch.zeros(n, dtype=torch.int32, device="cuda")
+                iters = torch.zeros_like(pflag)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                torch.cuda.synchronize()
+                e0.record()
+                harmgpu.utoprim_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], b["gcovpert"], pr, pflag, iters,
+                                      gam=gam, mode=mode)
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1)
+                best = ms if best is None else min(best, ms)
+            out[key] = (pr, pflag, iters, best)
+            if key == "mixed":
+                res["fallbacks"] += harmgpu.c2p_fallbacks()
+        p64, f64, i64, t64 = out["fp64"]
+        pmx, fmx, imx, tmx = out["mixed"]
+        res["gam"].append(gam)
+        res["fp64_ms"].append(t64)
+        res["mixed_ms"].append(tmx)
+        res["fp64_iters_per_zone"].append(float(i64.double().mean()))
+        res["mixed_iters_per_zone"].append(float(imx.double().mean()))
+        res["flags_identical"] &= bool(torch.equal(f64, fmx))
+        ok = f64 == 0
+        d = (pmx - p64)[:, ok].abs()
+        ref = p64[:, ok].abs()
+        err = [float((d[0] / ref[0]).max()), float((d[1] / (ref[0] + gam * ref[1])).max()),
+               float((d[2:5].amax(dim=0) / ref[2:5].amax(dim=0)).max())]
+        res["max_err"] = [max(a, c) for a, c in zip(res["max_err"], err)]
+        u_rel = d[1] / ref[1]
+        res["p999_u_rel"] = max(res["p999_u_rel"], float(torch.quantile(u_rel[: 1 << 20], 0.999)))
+        del b, out
+    tot = 2 * n
+    res["fp64_solves_per_s"] = tot / (sum(res["fp64_ms"]) * 1e-3)
+    res["mixed_solves_per_s"] = tot / (sum(res["mixed_ms"]) * 1e-3)
+    res["speedup"] = res["mixed_solves_per_s"] / res["fp64_solves_per_s"]
+    # algorithmic bytes per inversion of this layout: U 8, gcov 16, gcon 16,
+    # gdet, gcov'_tt, guess 8 doubles in; p 8 doubles + flag + iterations out
+    bpi = (8 + 16 + 16 + 1 + 1 + 8 + 8) * 8 + 4 + 4
+    res["bytes_per_inversion"] = bpi
+    res["mixed_hbm_frac"] = res["mixed_solves_per_s"] * bpi / (HBM_PEAK_GBS * 1e9)
+    res["err_note"] = ("max errors vs FP64 over successful zones: rho relative, u relative to rho + gam u, "
+                       "velocity relative to the largest component; p999_u_rel = 99.9th percentile of u relative")
+    return res
+
+
+def c2p_cpu_baseline(n=1 << 16):
+    """The oracle's Noble 2D inversion (C, gcc -O2, one thread; bit-identical
+    to the reference's Utoprimgen) on n inversions of the same distribution."""
+    import numpy as np
+    import torch
+    from harm_finalwham_amd import harmgpu
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import pyoracle
+    gam = 4.0 / 3.0
+    b = harmgpu.make_c2p_batch(n, gam=gam, seed=2025)
+    h = {k: v.cpu().numpy() for k, v in b.items()}
+    gpert = np.zeros((n, 4))
+    gpert[:, 0] = h["gcovpert"]
+    args = (h["gcov"].T.reshape(n, 4, 4).copy(), h["gcon"].T.reshape(n, 4, 4).copy(), h["gdet"].copy(), gpert,
+            h["U"].T.copy(), gam, h["p_guess"].T.copy())
+    t0 = time.perf_counter()
+    pyoracle.c2p_batch(*args)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "inversions/s", "cores": 1, "kind": "port",
+            "sample": f"oracle Noble 2D (C, gcc -O2, bit-identical to the reference) on {n} inversions, {dt:.2f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +160,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--zones", default="256,128,64", help="n1,n2,n3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--c2p-n", type=int, default=1 << 24, help="inversions per gam for the con2prim leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -125,6 +209,13 @@ def main():
     for k, v in summ.items():
         print(f"#   {k:12s} {json.dumps(v)}", file=sys.stderr, flush=True)
 
+    c2p = None
+    if args.c2p_n > 0:
+        g.close()
+        c2p = c2p_bench(args.c2p_n)
+        if not args.no_cpu_baseline:
+            c2p["cpu_baseline"] = c2p_cpu_baseline()
+        print(f"# con2prim: {json.dumps(c2p)}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": "zone-updates/s full GRMHD step",
@@ -146,6 +237,7 @@ def main():
                      "bytes_per_zone": dk["bytes_per_zone"], "mean_launch_ms": dk["mean_ms"]},
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v["total_ms"] / args.steps for k, v in summ.items()},
+        "con2prim": c2p,
         "failed_inversions": nf.value,
         "newton_iterations_per_zone_update": it.value / (2.0 * zones * args.steps),
     }

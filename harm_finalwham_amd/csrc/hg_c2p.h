@@ -236,3 +236,150 @@ __device__ __forceinline__ int utoprimgen_fp64(const double *U, const Geom &G, d
 }
 
 }  // namespace hg
+
+namespace hg {
+
+// ---------------------------------------------------------------------------
+// Mixed precision: the Newton iteration of utoprimgen_fp64 run in FP32 from
+// the same FP64 set-up (metric contractions, invariants, guess), then FP64
+// Newton polishes from the FP32 root until the reference's tolerance holds,
+// then the FP64 post-solve.  Whenever the FP32 phase does not converge, meets
+// a non-finite value, or the polish needs more than MIXED_MAX_POLISH steps,
+// the zone is inverted again by utoprimgen_fp64 from the original guess, so
+// every hard case keeps the reference's iterates and failure codes.
+// ---------------------------------------------------------------------------
+constexpr int MIXED_MAX_F32 = 40;       // FP32 Newton steps before falling back
+constexpr float MIXED_TOL_F32 = 2e-6f;  // |dW/W| that ends the FP32 phase
+constexpr int MIXED_MAX_POLISH = 4;     // FP64 steps allowed after it (2 expected)
+
+struct C2PInvF {
+  float Bsq, QdotBsq, Qtsq, Qdotn, D;
+};
+
+// newton_dx in FP32 (same algebra) with the hardware reciprocal and
+// reciprocal square root (about 1 ulp): the FP32 phase only has to bring W
+// within MIXED_TOL_F32 of the root, the FP64 polish does the rest
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ void newton_dx_f32(const C2PInvF &c, float igr, float W, float vsq, float &dx0,
+                                              float &dx1) {
+  float Wsq = W * W, D = c.D, Bsq = c.Bsq, QdotBsq = c.QdotBsq;
+  float Wp = W - D;
+  float gamma = __builtin_amdgcn_rsqf(1.0f - vsq);
+  float gammasq = gamma * gamma;
+  float wmrho0 = Wp * frcp(gammasq) - D * vsq * frcp(1.0f + gamma);
+  float p_tmp = igr * wmrho0;
+  float dPdW = (1.0f - vsq) * igr;
+  float dwmrho0dvsq = (D * (gamma * 0.5f - 1.0f) - Wp);
+  float dPdvsq = dwmrho0dvsq * igr;
+  float t2 = -0.5f * Bsq + dPdvsq;
+  float t3 = Bsq + W;
+  float t4 = t3 * t3;
+  float t9 = frcp(Wsq);
+  float t11 = c.Qtsq - vsq * t4 + QdotBsq * (Bsq + 2.0f * W) * t9;
+  float t16 = QdotBsq * t9;
+  float t18 = -c.Qdotn - 0.5f * Bsq * (1.0f + vsq) + 0.5f * t16 - W + p_tmp;
+  float t21 = frcp(t3);
+  float t23 = frcp(W);
+  float t24 = t16 * t23;
+  float t25 = -1.0f + dPdW - t24;
+  float t35 = t25 * t3 + (Bsq - 2.0f * dPdvsq) * (QdotBsq + vsq * Wsq * W) * t9 * t23;
+  float t36 = frcp(t35);
+  dx0 = -(t2 * t11 + t4 * t18) * t21 * t36;
+  float t40 = (vsq + t24) * t3;
+  dx1 = -(-t25 * t11 - 2.0f * t40 * t18) * t21 * t36;
+}
+
+// Returns the failure code; iters counts FP32 + FP64 Newton steps (or the
+// FP64 path's steps after a fallback); *fell_back tells which path answered.
+__device__ __forceinline__ int utoprimgen_mixed(const double *U, const Geom &G, double gam, double *pr,
+                                                int &iters, bool &fell_back) {
+  double guess[NPR];
+#pragma unroll
+  for (int i = 0; i < NPR; i++) guess[i] = pr[i];
+  fell_back = false;
+  double Ug[NPR], Ut[NPR];
+#pragma unroll
+  for (int i = 0; i < NPR; i++) Ug[i] = U[i] / G.g;
+  Ug[UU] -= Ug[RHO];
+  double alpha = 1.0 / sqrt(-G.gcon[0][0]);
+#pragma unroll
+  for (int i = 0; i < NPR; i++) Ut[i] = alpha * Ug[i];
+  const double igr = (gam - 1.0) / gam;
+  const double gsq_big = GAMMAFAIL * GAMMAFAIL;
+  const double vsq_big = (gsq_big - 1.0) / (gsq_big);
+  C2PInv c;
+  double W_last;
+  iters = 0;
+  int code = c2p_setup(Ut, G, guess, gam, c, W_last);
+  bool ok = (code == 0);
+  double x0 = 0., x1 = 0.;
+  if (ok) {
+    // FP32 bulk
+    C2PInvF cf = {(float)c.Bsq, (float)c.QdotBsq, (float)c.Qtsq, (float)c.Qdotn, (float)c.D};
+    const float igrf = (float)igr, wf = (float)c.w;
+    const float gsq_bigf = (float)gsq_big, vsq_bigf = (float)vsq_big;
+    float w0 = (float)fabs(W_last);
+    double v0 = vsq_of_W(c, W_last);
+    float w1 = (float)((v0 > 1.) ? vsq_big : v0);
+    int n32 = 0;
+    bool conv = false;
+    while (n32 < MIXED_MAX_F32) {
+      float d0, d1;
+      newton_dx_f32(cf, igrf, w0, w1, d0, d1);
+      float wo = w0;
+      w0 += d0;
+      w1 += d1;
+      float err = (w0 == 0.f) ? fabsf(d0) : fabsf(d0 * frcp(w0));
+      w0 = fabsf(w0);
+      w0 = (w0 > gsq_bigf * wf) ? wo : w0;
+      w1 = (w1 > 1.f) ? vsq_bigf : w1;
+      n32++;
+      if (!isfinite(w0) || !isfinite(w1)) break;
+      if (err <= MIXED_TOL_F32) { conv = true; break; }
+    }
+    ok = conv;
+    iters = n32;
+    x0 = (double)w0;
+    x1 = (double)w1;
+  }
+  if (ok) {
+    // FP64 polish with the reference's stopping rule
+    double f = 1., df = 1., errx = 1.;
+    int np = 0;
+    bool done = false;
+    while (np < MIXED_MAX_POLISH) {
+      double dx0, dx1;
+      newton_dx(c, igr, x0, x1, dx0, dx1, f, df);
+      double xo0 = x0;
+      x0 += dx0;
+      x1 += dx1;
+      errx = (x0 == 0.) ? fabs(dx0) : fabs(dx0 / x0);
+      x0 = fabs(x0);
+      x0 = (x0 / c.w > gsq_big) ? xo0 : x0;
+      x1 = (x1 > 1.) ? vsq_big : x1;
+      np++;
+      if (fabs(errx) <= NEWT_TOL) { done = true; break; }
+    }
+    iters += np;
+    ok = done && isfinite(f) && isfinite(df) && isfinite(x0) && isfinite(x1);
+  }
+  if (!ok) {
+    // the reference path from the original guess
+    fell_back = true;
+#pragma unroll
+    for (int i = 0; i < NPR; i++) pr[i] = guess[i];
+    return utoprimgen_fp64(U, G, gam, pr, iters);
+  }
+#pragma unroll
+  for (int i = B1; i <= B3; i++) pr[i] = Ug[i];
+  double out[NPR];
+  bool assign;
+  code = c2p_finish(c, x0, 0, igr, out, assign);
+  if (assign) {
+#pragma unroll
+    for (int i = 0; i < B1; i++) pr[i] = out[i];
+  }
+  return code;
+}
+
+}  // namespace hg

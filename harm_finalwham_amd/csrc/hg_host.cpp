@@ -473,8 +473,7 @@ extern "C" int hg_utoprim_batch(int64_t n, double gam, const double *U, const do
   if (mode == HG_C2P_FP64) {
     launch_c2p_batch(n, gam, U, gcov, gcon, gdet, gcovpert, pr, pflag, iters, s);
   } else if (mode == HG_C2P_MIXED) {
-    set_error("hg_utoprim_batch: mixed-precision mode not available in this build");
-    return 1;
+    if (launch_c2p_batch_mixed(n, gam, U, gcov, gcon, gdet, gcovpert, pr, pflag, iters, s)) return 1;
   } else {
     set_error("hg_utoprim_batch: unknown mode %d", mode);
     return 1;

@@ -44,7 +44,8 @@ int upload_geometry_tables(hg_ctx *c, const double *gcov, const double *gcon, co
                            const double *gdet, const double *conn, const double *r_cent);
 int copy_prims(hg_ctx *c, double *dev, const double *host);
 int fetch_prims(hg_ctx *c, const double *dev, double *host);
-void launch_c2p_batch_mixed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
-                            const double *gdet, const double *gcovpert, double *pr, int *pflag, int *iters,
-                            hipStream_t s);
+// returns 1 (with the error set) when the work counter cannot be set up
+int launch_c2p_batch_mixed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
+                           const double *gdet, const double *gcovpert, double *pr, int *pflag, int *iters,
+                           hipStream_t s);
 }  // namespace hg

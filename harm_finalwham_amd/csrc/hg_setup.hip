@@ -729,24 +729,3 @@ extern "C" int hg_init_torus(hg_ctx *c, double rin, double rmax, double kappa, d
   *dt = ndt;
   return hg_set_dt(c, ndt);
 }
-
-// ---------------------------------------------------------------------------
-// batched-inversion benchmark generator (device)
-// ---------------------------------------------------------------------------
-extern "C" int hg_make_c2p_batch(int64_t n, double gam, double a, uint64_t seed, double *U, double *gcov,
-                                 double *gcon, double *gdet, double *gcovpert, double *p_true, double *p_guess,
-                                 void *stream) {
-  (void)n; (void)gam; (void)a; (void)seed; (void)U; (void)gcov; (void)gcon; (void)gdet; (void)gcovpert;
-  (void)p_true; (void)p_guess; (void)stream;
-  set_error("hg_make_c2p_batch: not available in this build");
-  return 1;
-}
-
-namespace hg {
-void launch_c2p_batch_mixed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
-                            const double *gdet, const double *gcovpert, double *pr, int *pflag, int *iters,
-                            hipStream_t s) {
-  (void)n; (void)gam; (void)U; (void)gcov; (void)gcon; (void)gdet; (void)gcovpert; (void)pr; (void)pflag;
-  (void)iters; (void)s;
-}
-}  // namespace hg

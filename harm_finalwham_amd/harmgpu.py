@@ -20,7 +20,7 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_build_mks_geometry", "hg_get_config", "hg_init_torus", "hg_set_prims", "hg_get_prims", "hg_bound_prim", "hg_fluxcalc",
            "hg_step", "hg_step_async", "hg_set_dt", "hg_get_dt", "hg_get_counters",
            "hg_utoprim_batch", "hg_make_c2p_batch", "hg_last_error",
-           "hg_profile", "hg_profile_read", "hg_profile_class_name"]
+           "hg_profile", "hg_profile_read", "hg_profile_class_name", "hg_c2p_fallbacks"]
 HG_PROF_NCLASS = 9
 
 
@@ -72,6 +72,7 @@ def lib():
     L.hg_make_c2p_batch.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_uint64, vp, vp, vp, vp, vp,
                                     vp, vp, vp]
     L.hg_last_error.restype = C.c_char_p
+    L.hg_c2p_fallbacks.argtypes = [i64p]
     L.hg_profile.argtypes = [vp, C.c_int]
     L.hg_profile_read.argtypes = [vp, d, i64p]
     L.hg_profile_class_name.argtypes = [C.c_int]
@@ -214,3 +215,27 @@ def utoprim_batch(U, gcov, gcon, gdet, gcovpert, pr, pflag, iters=None, gam=4.0 
     ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
     _check(lib().hg_utoprim_batch(n, gam, ptr(U), ptr(gcov), ptr(gcon), ptr(gdet), ptr(gcovpert), ptr(pr),
                                   ptr(pflag), ptr(iters), mode, stream))
+
+
+def make_c2p_batch(n, gam=4.0 / 3.0, a=0.9375, seed=1, device="cuda"):
+    """The batched-inversion benchmark distribution generated on the GPU
+    (hg_make_c2p_batch).  Returns a dict of torch float64 tensors in the
+    layout of utoprim_batch: U[8,n], gcov[16,n], gcon[16,n], gdet[n],
+    gcovpert[n], p_true[8,n], p_guess[8,n]."""
+    import torch
+    f = dict(dtype=torch.float64, device=device)
+    out = dict(U=torch.empty(8, n, **f), gcov=torch.empty(16, n, **f), gcon=torch.empty(16, n, **f),
+               gdet=torch.empty(n, **f), gcovpert=torch.empty(n, **f), p_true=torch.empty(8, n, **f),
+               p_guess=torch.empty(8, n, **f))
+    ptr = lambda t: C.c_void_p(t.data_ptr())
+    _check(lib().hg_make_c2p_batch(n, gam, a, seed, *[ptr(out[k]) for k in ("U", "gcov", "gcon", "gdet",
+                                                                            "gcovpert", "p_true", "p_guess")],
+                                   None))
+    return out
+
+
+def c2p_fallbacks():
+    """Zones of the last HG_C2P_MIXED batch that took the FP64 path."""
+    v = C.c_int64()
+    _check(lib().hg_c2p_fallbacks(C.byref(v)))
+    return v.value

@@ -133,8 +133,10 @@ int hg_get_counters(hg_ctx *ctx, int64_t *nfail, int64_t *iters);
      U[8][n] evolved conserved variables (with gdet), gcov[16][n], gcon[16][n],
      gdet[n], gcovpert[n] (the tt component), pr[8][n] guess in / solution out,
      pflag[n] failure code out, iters[n] Newton iterations out (may be NULL).
-   mode: HG_C2P_FP64 (reference arithmetic) or HG_C2P_MIXED (FP32 bulk Newton,
-   FP64 residual and final polish). */
+   mode: HG_C2P_FP64 (reference arithmetic) or HG_C2P_MIXED (FP64 set-up and
+   metric contractions, FP32 Newton to |dW/W| <= 2e-6, FP64 Newton polish to
+   the reference tolerance and FP64 post-solve; any zone the FP32 phase does
+   not settle is inverted again by the FP64 path, so failure codes match). */
 #define HG_C2P_FP64 0
 #define HG_C2P_MIXED 1
 int hg_utoprim_batch(int64_t n, double gam, const double *U, const double *gcov,
@@ -147,6 +149,11 @@ int hg_utoprim_batch(int64_t n, double gam, const double *U, const double *gcov,
 int hg_make_c2p_batch(int64_t n, double gam, double a, uint64_t seed, double *U, double *gcov,
                       double *gcon, double *gdet, double *gcovpert, double *p_true,
                       double *p_guess, void *stream);
+
+/* Number of zones of the last HG_C2P_MIXED batch on the current device that
+   were inverted by the FP64 path (the FP32 phase did not converge or the FP64
+   polish needed more than 4 steps).  No counterpart in the reference. */
+int hg_c2p_fallbacks(int64_t *nfallback);
 
 const char *hg_last_error(void);
 
