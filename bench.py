@@ -153,6 +153,71 @@ def c2p_cpu_baseline(n=1 << 16):
             "sample": f"oracle Noble 2D (C, gcc -O2, bit-identical to the reference) on {n} inversions, {dt:.2f} s"}
 
 
+def batched_bench(ntori=48, nsteps=10, seed=5):
+    """48 independent 2-D tori of mixed sizes, spins a ~ U[0, 0.98] and
+    pressure-maximum radii r_pmax ~ U[10, 14] (WENO5BND + HLL + flux-CT +
+    Noble 2D): nsteps RK2 steps of all of them in one HIP graph launch, with
+    launch shapes autotuned once per grid size, against the same grids stepped
+    one after the other; every grid is compared bit for bit."""
+    import math
+    import numpy as np
+    import torch
+    from harm_finalwham_amd import harmgpu
+    rng = np.random.default_rng(seed)
+    sizes = [(64, 64), (96, 128), (128, 128), (192, 192), (256, 256)]
+    specs = [(sizes[int(rng.integers(len(sizes)))], float(rng.uniform(0.0, 0.98)), float(rng.uniform(10.0, 14.0)),
+              int(rng.integers(1, 1 << 30))) for _ in range(ntori)]
+
+    def make(spec):
+        (n1, n2), a, rmax, sd = spec
+        g = harmgpu.Grid(n1, n2, 1, nb=6, lim=harmgpu.HG_LIM_WENO5BND, fluxmethod=harmgpu.HG_FLUX_HLL)
+        rh = 1.0 + math.sqrt(1.0 - a * a)
+        g.build_mks_geometry(a=a, Rin=0.9 * rh, Rout=40.0, R0=0.0, hslope=0.3)
+        g.init_torus(rin=6.0, rmax=rmax, kappa=1e-3, beta=100.0, randfact=4e-2, seed=sd)
+        return g
+
+    t0 = time.perf_counter()
+    batch = [make(sp) for sp in specs]
+    alone = [make(sp) for sp in specs]
+    t_setup = time.perf_counter() - t0
+    # autotune once per grid size, share the shapes
+    tuned = {}
+    for g, sp in zip(batch, specs):
+        if sp[0] not in tuned:
+            tuned[sp[0]] = g.autotune(nsteps=2)
+        g.set_tune(tuned[sp[0]])
+    for g, sp in zip(alone, specs):
+        g.set_tune(tuned[sp[0]])
+    zones = sum(n1 * n2 for (n1, n2), _, _, _ in specs)
+    graph = harmgpu.Graph(batch, nsteps)
+    graph.launch()  # warm-up replay, repeated on the single grids below
+    for g in alone:
+        g.step_async(nsteps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    graph.launch()
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for g in alone:
+        g.step_async(nsteps)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    identical = all(gb.get_dt() == ga.get_dt() and np.array_equal(gb.get_prims(), ga.get_prims())
+                    for gb, ga in zip(batch, alone))
+    graph.close()
+    res = {"tori": ntori, "steps": nsteps, "zones": zones,
+           "sizes": {f"{a}x{b}": sum(1 for sp in specs if sp[0] == (a, b)) for a, b in sizes},
+           "graph_zone_updates_per_s": zones * nsteps / t_graph,
+           "sequential_zone_updates_per_s": zones * nsteps / t_seq,
+           "graph_speedup": t_seq / t_graph, "bitwise_identical_to_single_runs": identical,
+           "tuned_shapes": {f"{k[0]}x{k[1]}": v.as_dict() for k, v in tuned.items()},
+           "setup_s": t_setup}
+    for g in batch + alone:
+        g.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +226,7 @@ def main():
     ap.add_argument("--zones", default="256,128,64", help="n1,n2,n3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--c2p-n", type=int, default=1 << 24, help="inversions per gam for the con2prim leg (0: skip)")
+    ap.add_argument("--batch-tori", type=int, default=48, help="tori of the batched-graph leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -216,6 +282,10 @@ def main():
         if not args.no_cpu_baseline:
             c2p["cpu_baseline"] = c2p_cpu_baseline()
         print(f"# con2prim: {json.dumps(c2p)}", file=sys.stderr, flush=True)
+    batched = None
+    if args.batch_tori > 0:
+        batched = batched_bench(args.batch_tori)
+        print(f"# batched: {json.dumps(batched)}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": "zone-updates/s full GRMHD step",
@@ -238,6 +308,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v["total_ms"] / args.steps for k, v in summ.items()},
         "con2prim": c2p,
+        "batched_tori": batched,
         "failed_inversions": nf.value,
         "newton_iterations_per_zone_update": it.value / (2.0 * zones * args.steps),
     }

@@ -1,0 +1,268 @@
+// Many grids in one HIP graph, and the launch-shape autotuner.
+//
+// hg_graph_create captures nsteps RK2 steps of every grid into one graph:
+// each grid gets its own branch (a stream forked from the capture stream and
+// joined back), so the GPU runs the grids' kernels side by side and one
+// hipGraphLaunch advances all of them.  dt stays on the device per grid, so
+// replays need no host work.  The step enqueues kernels only (no memset or
+// copy nodes).
+//
+// hg_autotune times the step under different launch shapes (block sizes,
+// reconstruction wavefronts per block and line segments) and keeps the
+// fastest; the shapes do not change any result bit.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "harmgpu.h"
+#include "hg_host.h"
+
+using namespace hg;
+
+struct hg_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+#define HIPTRY(call)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess && rc == 0) {                                             \
+      set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      rc = 1;                                                                      \
+    }                                                                              \
+  } while (0)
+
+extern "C" int hg_graph_create(hg_ctx *const *ctxs, int n, int nsteps, hg_graph **out) {
+  if (!ctxs || !out || n < 1 || nsteps < 1) {
+    set_error("hg_graph_create: need n >= 1 contexts, nsteps >= 1 and an output");
+    return 1;
+  }
+  *out = nullptr;
+  for (int i = 0; i < n; i++) {
+    if (!ctxs[i] || !ctxs[i]->have_geometry) {
+      set_error("hg_graph_create: context %d missing or without geometry", i);
+      return 1;
+    }
+    if (ctxs[i]->prof_on) {
+      set_error("hg_graph_create: context %d has a profile open", i);
+      return 1;
+    }
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == ctxs[i]) {
+        set_error("hg_graph_create: context %d appears twice", i);
+        return 1;
+      }
+  }
+  int rc = 0;
+  hipStream_t cap = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipStream_t> st(n, nullptr);
+  std::vector<hipEvent_t> join(n, nullptr);
+  HIPTRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  HIPTRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  for (int i = 0; i < n && !rc; i++) {
+    HIPTRY(hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking));
+    HIPTRY(hipEventCreateWithFlags(&join[i], hipEventDisableTiming));
+  }
+  hg_graph *g = nullptr;
+  if (!rc) {
+    g = new hg_graph();
+    HIPTRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    if (!rc) {
+      HIPTRY(hipEventRecord(fork, cap));
+      for (int i = 0; i < n && !rc; i++) {
+        HIPTRY(hipStreamWaitEvent(st[i], fork, 0));
+        for (int k = 0; k < nsteps; k++) enqueue_step(ctxs[i], st[i]);
+        HIPTRY(hipGetLastError());
+        HIPTRY(hipEventRecord(join[i], st[i]));
+        HIPTRY(hipStreamWaitEvent(cap, join[i], 0));
+      }
+      // always end the capture, also after an error, so the stream is usable
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(cap, &graph);
+      if (e != hipSuccess && !rc) {
+        set_error("hg_graph_create: capture failed: %s", hipGetErrorString(e));
+        rc = 1;
+      }
+      g->graph = graph;
+      if (!rc) HIPTRY(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    if (st[i]) (void)hipStreamDestroy(st[i]);
+    if (join[i]) (void)hipEventDestroy(join[i]);
+  }
+  if (fork) (void)hipEventDestroy(fork);
+  if (cap) (void)hipStreamDestroy(cap);
+  if (rc) {
+    if (g) {
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+      delete g;
+    }
+    return 1;
+  }
+  *out = g;
+  return 0;
+}
+
+extern "C" int hg_graph_launch(hg_graph *g, void *stream) {
+  if (!g || !g->exec) {
+    set_error("hg_graph_launch: no graph");
+    return 1;
+  }
+  hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+  if (e != hipSuccess) {
+    set_error("hg_graph_launch: %s", hipGetErrorString(e));
+    return 1;
+  }
+  return 0;
+}
+
+extern "C" int hg_graph_destroy(hg_graph *g) {
+  if (!g) return 0;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// launch shapes
+// ---------------------------------------------------------------------------
+static bool valid_bs(int b) { return b == 64 || b == 128 || b == 256; }
+
+static int check_tune(const hg_tune *t) {
+  if (!valid_bs(t->bs_flux) || !valid_bs(t->bs_emf) || !valid_bs(t->bs_update) || !valid_bs(t->bs_invert)) {
+    set_error("hg_set_tune: block sizes must be 64, 128 or 256");
+    return 1;
+  }
+  if (t->weno_xwaves != 1 && t->weno_xwaves != 2 && t->weno_xwaves != 4) {
+    set_error("hg_set_tune: weno_xwaves must be 1, 2 or 4");
+    return 1;
+  }
+  if (t->weno_nseg < 0 || t->weno_nseg > 64) {
+    set_error("hg_set_tune: weno_nseg must be in [0, 64]");
+    return 1;
+  }
+  return 0;
+}
+
+static void to_c(const Tune &a, hg_tune *t) {
+  t->bs_flux = a.bs_flux;
+  t->bs_emf = a.bs_emf;
+  t->bs_update = a.bs_update;
+  t->bs_invert = a.bs_invert;
+  t->weno_xwaves = a.weno_xwaves;
+  t->weno_nseg = a.weno_nseg;
+}
+
+static void from_c(const hg_tune *t, Tune &a) {
+  a.bs_flux = t->bs_flux;
+  a.bs_emf = t->bs_emf;
+  a.bs_update = t->bs_update;
+  a.bs_invert = t->bs_invert;
+  a.weno_xwaves = t->weno_xwaves;
+  a.weno_nseg = t->weno_nseg;
+}
+
+extern "C" int hg_get_tune(const hg_ctx *c, hg_tune *t) {
+  if (!c || !t) {
+    set_error("hg_get_tune: null argument");
+    return 1;
+  }
+  to_c(c->tune, t);
+  return 0;
+}
+
+extern "C" int hg_set_tune(hg_ctx *c, const hg_tune *t) {
+  if (!c || !t) {
+    set_error("hg_set_tune: null argument");
+    return 1;
+  }
+  if (check_tune(t)) return 1;
+  from_c(t, c->tune);
+  return 0;
+}
+
+// time nsteps steps from the saved state; restores the state afterwards
+static int time_steps(hg_ctx *c, const double *p0, double dt0, int nsteps, hipStream_t s, hipEvent_t e0,
+                      hipEvent_t e1, float &ms) {
+  int rc = 0;
+  size_t bytes = (size_t)c->d.spl * NPR * sizeof(double);
+  HIPTRY(hipMemcpyAsync(c->p, p0, bytes, hipMemcpyDeviceToDevice, s));
+  HIPTRY(hipMemcpyAsync(c->dt, &dt0, sizeof(double), hipMemcpyHostToDevice, s));
+  HIPTRY(hipEventRecord(e0, s));
+  for (int k = 0; k < nsteps; k++) enqueue_step(c, s);
+  HIPTRY(hipGetLastError());
+  HIPTRY(hipEventRecord(e1, s));
+  HIPTRY(hipEventSynchronize(e1));
+  HIPTRY(hipEventElapsedTime(&ms, e0, e1));
+  return rc;
+}
+
+extern "C" int hg_autotune(hg_ctx *c, int nsteps, hg_tune *best, void *stream) {
+  if (!c || !c->have_geometry) {
+    set_error("hg_autotune: context missing or without geometry");
+    return 1;
+  }
+  if (nsteps < 1) nsteps = 1;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = 0;
+  size_t bytes = (size_t)c->d.spl * NPR * sizeof(double);
+  double *p0 = nullptr;
+  double dt0 = 0.;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPTRY(hipMalloc((void **)&p0, bytes));
+  HIPTRY(hipEventCreate(&e0));
+  HIPTRY(hipEventCreate(&e1));
+  HIPTRY(hipStreamSynchronize(s));
+  HIPTRY(hipMemcpy(p0, c->p, bytes, hipMemcpyDeviceToDevice));
+  HIPTRY(hipMemcpy(&dt0, c->dt, sizeof(double), hipMemcpyDeviceToHost));
+  unsigned long long cnt[2] = {0, 0};
+  HIPTRY(hipMemcpy(cnt, c->counters, sizeof cnt, hipMemcpyDeviceToHost));
+  Tune cur = c->tune;
+  if (!rc) {
+    // one warm-up, then coordinate descent over the shapes, one at a time
+    float ms = 0.f;
+    rc = time_steps(c, p0, dt0, nsteps, s, e0, e1, ms);
+    float best_ms = 0.f;
+    if (!rc) rc = time_steps(c, p0, dt0, nsteps, s, e0, e1, best_ms);
+    struct Axis {
+      int Tune::*field;
+      std::vector<int> values;
+    };
+    std::vector<Axis> axes = {{&Tune::bs_flux, {64, 128, 256}},   {&Tune::bs_update, {64, 128, 256}},
+                              {&Tune::bs_invert, {64, 128, 256}}, {&Tune::bs_emf, {128, 256}},
+                              {&Tune::weno_xwaves, {1, 2, 4}},    {&Tune::weno_nseg, {1, 2, 4}}};
+    if (c->cfg.lim == HG_LIM_MC) axes.resize(4);
+    for (const Axis &ax : axes) {
+      if (rc) break;
+      for (int v : ax.values) {
+        if (cur.*(ax.field) == v) continue;
+        Tune trial = cur;
+        trial.*(ax.field) = v;
+        c->tune = trial;
+        rc = time_steps(c, p0, dt0, nsteps, s, e0, e1, ms);
+        if (rc) break;
+        if (ms < best_ms) {
+          best_ms = ms;
+          cur = trial;
+        }
+      }
+    }
+    c->tune = cur;
+  }
+  // the state, dt and counters as they were
+  if (p0) {
+    HIPTRY(hipMemcpy(c->p, p0, bytes, hipMemcpyDeviceToDevice));
+    HIPTRY(hipMemcpy(c->dt, &dt0, sizeof(double), hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(c->counters, cnt, sizeof cnt, hipMemcpyHostToDevice));
+  }
+  if (p0) (void)hipFree(p0);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (!rc && best) to_c(c->tune, best);
+  return rc;
+}

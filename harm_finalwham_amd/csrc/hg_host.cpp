@@ -309,19 +309,19 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
     if (f.lim == HG_LIM_MC) {
       prof_mark(c, HG_PROF_FLUX1 + dir - 1, s);
       launch_flux_mc(dir, pb, c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,
-                     c->ndt_bits + dir - 1, c->err, s);
+                     c->ndt_bits + dir - 1, c->err, c->tune, s);
       prof_mark(c, -1, s);
     } else {
       if (dir != 1) prof_mark(c, HG_PROF_RECON1 + dir - 1, s);
-      launch_weno(dir, pb, c->aux, c->pl, c->pr, d, s);
+      launch_weno(dir, pb, c->aux, c->pl, c->pr, d, c->tune, s);
       prof_mark(c, HG_PROF_FLUX1 + dir - 1, s);
       launch_flux_lr(dir, c->pl, c->pr, c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,
-                     c->ndt_bits + dir - 1, c->err, s);
+                     c->ndt_bits + dir - 1, c->err, c->tune, s);
       prof_mark(c, -1, s);
     }
   }
   prof_mark(c, HG_PROF_EMF, s);
-  launch_emf(c->F[0], c->F[1], c->F[2], c->geo, c->corn, c->emf[0], c->emf[1], c->emf[2], d, s);
+  launch_emf(c->F[0], c->F[1], c->F[2], c->geo, c->corn, c->emf[0], c->emf[1], c->emf[2], d, c->tune, s);
   prof_mark(c, -1, s);
 }
 
@@ -398,7 +398,7 @@ static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, doubl
   enqueue_fluxcalc(c, pb, s);
   prof_mark(c, HG_PROF_UPDATE, s);
   launch_update(pi, pb, pf, c->uf, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2], c->geo, c->conn,
-                c->floors, c->dt, cuf2, finalstep, c->d, f.gam, f.dx, f.gammamax, c->counters, c->err, s);
+                c->floors, c->dt, cuf2, finalstep, c->d, f.gam, f.dx, f.gammamax, c->counters, c->err, c->tune, s);
   if (finalstep) launch_dt(c->ndt_bits, c->dt, f.defcon, f.safe, s);
   // step_ch.c:127 and :166 call bound_prim twice per substep; the second call
   // is not a repeat of the first: POLEDEATH has zeroed U2 in the rows next to
@@ -410,7 +410,7 @@ static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, doubl
 }
 
 // step_ch.c:442 setup_rktimestep, TIMEORDER 2: CUf[0][2]=0.5, CUf[1][2]=1
-static void enqueue_step(hg_ctx *c, hipStream_t s) {
+void hg::enqueue_step(hg_ctx *c, hipStream_t s) {
   enqueue_substep(c, c->p, c->p, c->pk, 0.5, 0, s);
   enqueue_substep(c, c->p, c->pk, c->p, 1.0, 1, s);
 }

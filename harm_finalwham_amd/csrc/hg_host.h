@@ -7,6 +7,7 @@
 
 #include "harmgpu.h"
 #include "hg_grid.h"
+#include "hg_kernels.h"
 
 struct hg_ctx {
   hg_config cfg;
@@ -22,6 +23,7 @@ struct hg_ctx {
   unsigned long long *ndt_bits = nullptr, *counters = nullptr;
   int *err = nullptr;
   bool have_geometry = false;
+  hg::Tune tune;                             // launch shapes (hg_autotune)
   // modified Kerr-Schild grid parameters when the geometry was built by
   // hg_build_mks_geometry (the torus set-up needs the coordinates)
   bool have_mks = false;
@@ -43,6 +45,8 @@ void set_error(const char *fmt, ...);
 int upload_geometry_tables(hg_ctx *c, const double *gcov, const double *gcon, const double *gcovpert,
                            const double *gdet, const double *conn, const double *r_cent);
 int copy_prims(hg_ctx *c, double *dev, const double *host);
+// one RK2 step of the context on stream s (kernels only: capturable)
+void enqueue_step(hg_ctx *c, hipStream_t s);
 int fetch_prims(hg_ctx *c, const double *dev, double *host);
 // returns 1 (with the error set) when the work counter cannot be set up
 int launch_c2p_batch_mixed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,

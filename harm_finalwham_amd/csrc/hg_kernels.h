@@ -6,22 +6,32 @@
 
 namespace hg {
 
+// launch shapes that do not change results (every zone's arithmetic is
+// independent of the block it runs in; the block reductions are a minimum and
+// integer sums): block sizes (64, 128 or 256 threads), wavefronts per block of
+// the i-direction reconstruction, segments per line of the j/k-direction
+// reconstruction (0: chosen from the line length)
+struct Tune {
+  int bs_flux = 256, bs_emf = 256, bs_update = 256, bs_invert = 256;
+  int weno_xwaves = 4, weno_nseg = 0;
+};
+
 void launch_bound(double *p, const double *geo, const Dims &d, double gammamax, int *err, hipStream_t s);
 void launch_flux_mc(int dir, const double *p, const double *geo, double *F, const Dims &d, double gam,
                     double cour, double dxdir, int fluxmethod, unsigned long long *ndt_bits, int *err,
-                    hipStream_t s);
+                    const Tune &tu, hipStream_t s);
 void launch_flux_lr(int dir, const double *pleft, const double *pright, const double *geo, double *F,
                     const Dims &d, double gam, double cour, double dxdir, int fluxmethod,
-                    unsigned long long *ndt_bits, int *err, hipStream_t s);
+                    unsigned long long *ndt_bits, int *err, const Tune &tu, hipStream_t s);
 void launch_emf(const double *F1, const double *F2, const double *F3, const double *geo, const double *corn,
-                double *e1, double *e2, double *e3, const Dims &d, hipStream_t s);
+                double *e1, double *e2, double *e3, const Dims &d, const Tune &tu, hipStream_t s);
 // per-zone update (-> ubuf) then inversion + fixup (-> pf); ubuf has the
 // layout of the primitives
 void launch_update(const double *pi, const double *pb, double *pf, double *ubuf, const double *F1,
                    const double *F2, const double *F3, const double *e1, const double *e2, const double *e3,
                    const double *geo, const double *conn, const double *floors, const double *dtp, double cuf2,
                    int finalstep, const Dims &d, double gam, const double *dx, double gammamax,
-                   unsigned long long *counters, int *err, hipStream_t s);
+                   unsigned long long *counters, int *err, const Tune &tu, hipStream_t s);
 void launch_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe, hipStream_t s);
 void launch_reset_ndt(unsigned long long *ndt_bits, hipStream_t s);
 void launch_c2p_batch(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
@@ -33,7 +43,7 @@ void launch_c2p_batch(int64_t n, double gam, const double *U, const double *gcov
 // layout of the primitives
 void launch_weno_prep(const double *p, const double *geo, double *aux, const Dims &d, double gam, hipStream_t s);
 void launch_weno(int dir, const double *p, const double *aux, double *pleft, double *pright, const Dims &d,
-                 hipStream_t s);
+                 const Tune &tu, hipStream_t s);
 
 // initial time step over every zone (advance.c:957 set_dt), folded into *out
 void launch_set_dt(const double *p, const double *geo, const Dims &d, double gam, double cour, const double *dx,

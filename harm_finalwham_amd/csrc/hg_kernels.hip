@@ -629,10 +629,10 @@ void launch_bound(double *p, const double *geo, const Dims &d, double gammamax, 
 
 void launch_flux_mc(int dir, const double *p, const double *geo, double *F, const Dims &d, double gam,
                     double cour, double dxdir, int fluxmethod, unsigned long long *ndt_bits, int *err,
-                    hipStream_t s) {
+                    const Tune &tu, hipStream_t s) {
   FaceRange fr = face_range(d, dir);
   int64_t n = (int64_t)fr.ni * fr.nj * fr.nk;
-  dim3 g(nblocks(n, 256)), b(256);
+  dim3 g(nblocks(n, tu.bs_flux)), b(tu.bs_flux);
   if (dir == 1) hipLaunchKernelGGL(k_flux_mc<1>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
   if (dir == 2) hipLaunchKernelGGL(k_flux_mc<2>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
   if (dir == 3) hipLaunchKernelGGL(k_flux_mc<3>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
@@ -640,31 +640,31 @@ void launch_flux_mc(int dir, const double *p, const double *geo, double *F, cons
 
 void launch_flux_lr(int dir, const double *pleft, const double *pright, const double *geo, double *F,
                     const Dims &d, double gam, double cour, double dxdir, int fluxmethod,
-                    unsigned long long *ndt_bits, int *err, hipStream_t s) {
+                    unsigned long long *ndt_bits, int *err, const Tune &tu, hipStream_t s) {
   FaceRange fr = face_range(d, dir);
   int64_t n = (int64_t)fr.ni * fr.nj * fr.nk;
-  dim3 g(nblocks(n, 256)), b(256);
+  dim3 g(nblocks(n, tu.bs_flux)), b(tu.bs_flux);
   if (dir == 1) hipLaunchKernelGGL(k_flux_lr<1>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
   if (dir == 2) hipLaunchKernelGGL(k_flux_lr<2>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
   if (dir == 3) hipLaunchKernelGGL(k_flux_lr<3>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
 void launch_emf(const double *F1, const double *F2, const double *F3, const double *geo, const double *corn,
-                double *e1, double *e2, double *e3, const Dims &d, hipStream_t s) {
+                double *e1, double *e2, double *e3, const Dims &d, const Tune &tu, hipStream_t s) {
   int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
   int64_t n = (int64_t)(d.n1 + s1) * (d.n2 + s2) * (d.n3 + s3);
-  hipLaunchKernelGGL(k_emf, dim3(nblocks(n, 256)), dim3(256), 0, s, F1, F2, F3, geo, corn, e1, e2, e3, d);
+  hipLaunchKernelGGL(k_emf, dim3(nblocks(n, tu.bs_emf)), dim3(tu.bs_emf), 0, s, F1, F2, F3, geo, corn, e1, e2, e3, d);
 }
 
 void launch_update(const double *pi, const double *pb, double *pf, double *ubuf, const double *F1,
                    const double *F2, const double *F3, const double *e1, const double *e2, const double *e3,
                    const double *geo, const double *conn, const double *floors, const double *dtp, double cuf2,
                    int finalstep, const Dims &d, double gam, const double *dx, double gammamax,
-                   unsigned long long *counters, int *err, hipStream_t s) {
+                   unsigned long long *counters, int *err, const Tune &tu, hipStream_t s) {
   int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
-  hipLaunchKernelGGL(k_update, dim3(nblocks(n, 256)), dim3(256), 0, s, pi, pb, ubuf, F1, F2, F3, e1, e2, e3, geo,
+  hipLaunchKernelGGL(k_update, dim3(nblocks(n, tu.bs_update)), dim3(tu.bs_update), 0, s, pi, pb, ubuf, F1, F2, F3, e1, e2, e3, geo,
                      conn, dtp, cuf2, d, gam, 1.0 / dx[1], 1.0 / dx[2], 1.0 / dx[3], err);
-  hipLaunchKernelGGL(k_invert, dim3(nblocks(n, 256)), dim3(256), 0, s, ubuf, pb, pf, geo, floors, finalstep, d,
+  hipLaunchKernelGGL(k_invert, dim3(nblocks(n, tu.bs_invert)), dim3(tu.bs_invert), 0, s, ubuf, pb, pf, geo, floors, finalstep, d,
                      gam, gammamax, counters, err);
 }
 

@@ -33,6 +33,13 @@ struct Mks {
   double startx[4], dx[4];
 };
 
+// sin and cos of one angle.  Where the reference evaluates both in one
+// function (the metric components, the torus solution) its gcc build computes
+// them with a single libm sincos() call, whose results can differ in the last
+// bit from separate sin() and cos() calls; the set-up does the same there.
+// Lone sin or cos (the coordinate map and its Jacobian) stay separate calls.
+inline void sincos_ref(double x, double &s, double &c) { sincos(x, &s, &c); }
+
 // coord.c:1142 set_points (LOGRSINTH)
 Mks make_mks(double a, double Rin, double Rout, double R0, double hslope, int n1, int n2, int n3) {
   Mks m;
@@ -88,7 +95,8 @@ void dxdxp_diag(const Mks &m, const double X[4], const double V[4], double d[4])
 
 // metric.c:697 set_gcov_ksmetric (Kerr-Schild g_{mu nu} at r, theta)
 void ks_gcov(double a, double r, double th, double g[4][4]) {
-  double sth = sin(th), cth = cos(th);
+  double sth, cth;
+  sincos_ref(th, sth, cth);
   double s2 = sth * sth;
   double rho2 = r * r + a * a * cth * cth;
   for (int p = 0; p < 4; p++)
@@ -104,7 +112,8 @@ void ks_gcov(double a, double r, double th, double g[4][4]) {
 
 // metric.c:863 set_gcon_ksmetric (Kerr-Schild g^{mu nu})
 void ks_gcon(double a, double r, double th, double g[4][4]) {
-  double cth = cos(th), sth = sin(th);
+  double cth, sth;
+  sincos_ref(th, sth, cth);
   double s2 = sth * sth;
   double rho2 = r * r + a * a * cth * cth;
   for (int p = 0; p < 4; p++)
@@ -120,7 +129,8 @@ void ks_gcon(double a, double r, double th, double g[4][4]) {
 // metric.c:777 set_gcov_blmetric (Boyer-Lindquist g_{mu nu}); only the rows
 // the 4-velocity normalisation reads
 void bl_gcov(double a, double r, double th, double g[4][4]) {
-  double cth = cos(th), sth = sin(th);
+  double cth, sth;
+  sincos_ref(th, sth, cth);
   double s2 = sth * sth, a2 = a * a, r2 = r * r;
   double DDpert = -2. / r + a2 / r2;
   double DD = 1. + DDpert;
@@ -467,7 +477,8 @@ extern "C" int hg_init_torus(hg_ctx *c, double rin, double rmax, double kappa, d
       // fixup.c:1562 set_density_floors (rescaletype 0) and fixup.c:1686 set_atmosphere
       C.floor_rho = f.rhomin * pow(r, -1.5);
       C.floor_u = f.uumin * pow(r, -2.5);
-      double sth = sin(th), cth = cos(th);
+      double sth, cth;
+      sincos_ref(th, sth, cth);
       double DD = r * r - 2. * r + a * a;
       double AA = (r * r + a * a) * (r * r + a * a) - DD * a * a * sth * sth;
       double SS = r * r + a * a * cth * cth;

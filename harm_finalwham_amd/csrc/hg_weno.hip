@@ -484,18 +484,22 @@ void launch_weno_prep(const double *p, const double *geo, double *aux, const Dim
 }
 
 void launch_weno(int dir, const double *p, const double *aux, double *pleft, double *pright, const Dims &d,
-                 hipStream_t s) {
+                 const Tune &tu, hipStream_t s) {
   int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
   if (dir == 1) {
     int nseg = (d.n1 + 2 + XOUT - 1) / XOUT;
     int64_t nw = (int64_t)nseg * NPR * (d.n2 + 2 * s2) * (d.n3 + 2 * s3);
-    hipLaunchKernelGGL(k_weno_x, dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, s, p, aux, pleft, pright, d, nseg);
+    int wpb = tu.weno_xwaves;
+    hipLaunchKernelGGL(k_weno_x, dim3((unsigned)((nw + wpb - 1) / wpb)), dim3(64 * wpb), 0, s, p, aux, pleft,
+                       pright, d, nseg);
   } else {
     int n = (dir == 2) ? d.n2 : d.n3;
     int64_t lanes = (int64_t)(d.n1 + 2 * s1) * ((dir == 2) ? d.n3 + 2 * s3 : d.n2 + 2 * s2);
     // split long lines so that the grid holds enough wavefronts
-    int nseg = (n + 2 >= 96) ? 2 : 1;
+    int nseg = tu.weno_nseg > 0 ? tu.weno_nseg : ((n + 2 >= 96) ? 2 : 1);
+    if (nseg > n + 2) nseg = n + 2;
     int seglen = (n + 2 + nseg - 1) / nseg;
+    nseg = (n + 2 + seglen - 1) / seglen;  // every segment starts at or before point n
     dim3 grid((unsigned)((lanes + 255) / 256), (unsigned)(NPR * nseg));
     if (dir == 2)
       hipLaunchKernelGGL(k_weno_m<2>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen);

@@ -20,12 +20,21 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_build_mks_geometry", "hg_get_config", "hg_init_torus", "hg_set_prims", "hg_get_prims", "hg_bound_prim", "hg_fluxcalc",
            "hg_step", "hg_step_async", "hg_set_dt", "hg_get_dt", "hg_get_counters",
            "hg_utoprim_batch", "hg_make_c2p_batch", "hg_last_error",
-           "hg_profile", "hg_profile_read", "hg_profile_class_name", "hg_c2p_fallbacks"]
+           "hg_profile", "hg_profile_read", "hg_profile_class_name", "hg_c2p_fallbacks",
+           "hg_graph_create", "hg_graph_launch", "hg_graph_destroy", "hg_get_tune", "hg_set_tune", "hg_autotune"]
 HG_PROF_NCLASS = 9
 
 
 class HarmError(RuntimeError):
     pass
+
+
+class HgTune(C.Structure):
+    _fields_ = [("bs_flux", C.c_int), ("bs_emf", C.c_int), ("bs_update", C.c_int), ("bs_invert", C.c_int),
+                ("weno_xwaves", C.c_int), ("weno_nseg", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class HgConfig(C.Structure):
@@ -73,6 +82,12 @@ def lib():
                                     vp, vp, vp]
     L.hg_last_error.restype = C.c_char_p
     L.hg_c2p_fallbacks.argtypes = [i64p]
+    L.hg_graph_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.POINTER(vp)]
+    L.hg_graph_launch.argtypes = [vp, vp]
+    L.hg_graph_destroy.argtypes = [vp]
+    L.hg_get_tune.argtypes = [vp, C.POINTER(HgTune)]
+    L.hg_set_tune.argtypes = [vp, C.POINTER(HgTune)]
+    L.hg_autotune.argtypes = [vp, C.c_int, C.POINTER(HgTune), vp]
     L.hg_profile.argtypes = [vp, C.c_int]
     L.hg_profile_read.argtypes = [vp, d, i64p]
     L.hg_profile_class_name.argtypes = [C.c_int]
@@ -182,6 +197,19 @@ class Grid:
         _check(lib().hg_profile_read(self.h, _d(ms), n.ctypes.data_as(C.POINTER(C.c_int64))))
         return {lib().hg_profile_class_name(k).decode(): (float(ms[k]), int(n[k])) for k in range(HG_PROF_NCLASS)}
 
+    def tune(self):
+        t = HgTune()
+        _check(lib().hg_get_tune(self.h, C.byref(t)))
+        return t
+
+    def set_tune(self, t):
+        _check(lib().hg_set_tune(self.h, C.byref(t)))
+
+    def autotune(self, nsteps=2, stream=None):
+        t = HgTune()
+        _check(lib().hg_autotune(self.h, nsteps, C.byref(t), stream))
+        return t
+
     def get_dt(self):
         v = np.zeros(1)
         _check(lib().hg_get_dt(self.h, _d(v)))
@@ -239,3 +267,28 @@ def c2p_fallbacks():
     v = C.c_int64()
     _check(lib().hg_c2p_fallbacks(C.byref(v)))
     return v.value
+
+
+class Graph:
+    """hg_graph: nsteps RK2 steps of several grids in one HIP graph."""
+
+    def __init__(self, grids, nsteps):
+        self.grids = list(grids)  # keep the contexts alive
+        arr = (C.c_void_p * len(self.grids))(*[g.h for g in self.grids])
+        h = C.c_void_p()
+        _check(lib().hg_graph_create(arr, len(self.grids), nsteps, C.byref(h)))
+        self.h = h
+
+    def launch(self, stream=None):
+        _check(lib().hg_graph_launch(self.h, stream))
+
+    def close(self):
+        if self.h:
+            lib().hg_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

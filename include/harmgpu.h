@@ -155,6 +155,32 @@ int hg_make_c2p_batch(int64_t n, double gam, double a, uint64_t seed, double *U,
    polish needed more than 4 steps).  No counterpart in the reference. */
 int hg_c2p_fallbacks(int64_t *nfallback);
 
+/* Several grids advanced by one HIP graph, no counterpart in the reference
+   (which runs one grid per process): the graph holds nsteps RK2 steps of
+   every context, one concurrent branch per context, and each launch advances
+   all of them on the device (dt stays per context on the device).  Every
+   grid evolves exactly as it would alone.  The contexts must outlive the
+   graph and must not be stepped otherwise while a launch is in flight. */
+typedef struct hg_graph hg_graph;
+int hg_graph_create(hg_ctx *const *ctxs, int n, int nsteps, hg_graph **out);
+int hg_graph_launch(hg_graph *g, void *stream);
+int hg_graph_destroy(hg_graph *g);
+
+/* Launch shapes of the step (no effect on results): block sizes of the flux,
+   EMF, update and inversion kernels (64, 128 or 256), wavefronts per block of
+   the i-direction reconstruction (1, 2 or 4) and segments per line of the j/k
+   reconstruction (0 = by line length).  hg_autotune times nsteps steps per
+   candidate (coordinate descent) from the current state, restores the state,
+   dt and counters, keeps the fastest shapes and returns them in *best (may be
+   NULL).  No counterpart in the reference. */
+typedef struct {
+  int bs_flux, bs_emf, bs_update, bs_invert;
+  int weno_xwaves, weno_nseg;
+} hg_tune;
+int hg_get_tune(const hg_ctx *ctx, hg_tune *t);
+int hg_set_tune(hg_ctx *ctx, const hg_tune *t);
+int hg_autotune(hg_ctx *ctx, int nsteps, hg_tune *best, void *stream);
+
 const char *hg_last_error(void);
 
 /* Kernel-class timing, no counterpart in the reference (its timing is the

@@ -17,7 +17,16 @@ Fixtures
 Each torus fixture carries the reference geometry tables, the initial
 primitives, the reconstruction states (or MC slopes) per direction, the
 fluxes after one fluxcalc()+flux_ct() and the primitives after steps.
+
+With --long (and --full):
+  c1_mc_100.npz    2-D torus 128x128, MC + LaxF, 100 RK2 steps: dt and
+        failure sequences, digests of the initial and final states and of
+        the geometry tables, the final state
+  f3d_weno_1.npz   3-D torus 256x128x64, WENO5BND + HLL, 1 RK2 step: dt,
+        digests (states, geometry) and per-primitive sums only (the states
+        are 182 MB)
 """
+import hashlib
 import os
 import subprocess
 import sys
@@ -160,7 +169,45 @@ def run_torus(harness, weno, nsteps):
     return d
 
 
+def state_digest(p):
+    """SHA-256 of a state array with -0.0 folded into +0.0 (array_equal
+    semantics), for comparing states too large to keep as fixtures."""
+    a = np.ascontiguousarray(p, dtype="<f8") + 0.0
+    return hashlib.sha256(a.tobytes()).hexdigest()
+
+
+def run_torus_long(harness, weno, nsteps, keep_final):
+    """A longer or larger run: the parameters, the dt and failure sequences,
+    digests of the initial and final states, per-primitive sums of the final
+    state, and the final state itself when keep_final."""
+    d = run_torus(harness, weno, nsteps)
+    out = {k: d[k] for k in ("params", "dt_seq", "fail_seq", "weno", "nsteps")}
+    out["digest_init"] = np.array(state_digest(d["p_init"]))
+    out["digest_final"] = np.array(state_digest(d["p_final"]))
+    out["sum_final"] = d["p_final"].reshape(-1, 8).sum(axis=0)
+    for k in ("gcov", "gcon", "gdet", "conn"):
+        out["digest_" + k] = np.array(state_digest(d[k]))
+    if keep_final:
+        out["p_final"] = d["p_final"]
+    return out
+
+
+def main_long(full):
+    """128x128 MC/LaxF torus over 100 steps; with full, also the 256x128x64
+    WENO5BND/HLL torus over one step (about ten minutes and 14 GB here)."""
+    runs = [("c1", (128, 128, 1), 0, 100, True, "c1_mc_100.npz")]
+    if full:
+        runs.append(("f3d", (256, 128, 64), 1, 1, False, "f3d_weno_1.npz"))
+    for tag, (n1, n2, n3), weno, nsteps, keep, name in runs:
+        subprocess.run([BUILD, tag, str(n1), str(n2), str(n3)], check=True, stdout=subprocess.DEVNULL)
+        d = run_torus_long(os.path.join(REFDIR, "harness_" + tag), weno, nsteps, keep)
+        np.savez_compressed(os.path.join(HERE, name), **d)
+        print(name, os.path.getsize(os.path.join(HERE, name)))
+
+
 def main():
+    if "--long" in sys.argv:
+        return main_long("--full" in sys.argv)
     builds = {"t2d": (32, 32, 1), "t3d": (16, 16, 8)}
     for tag, (n1, n2, n3) in builds.items():
         subprocess.run([BUILD, tag, str(n1), str(n2), str(n3)], check=True, stdout=subprocess.DEVNULL)

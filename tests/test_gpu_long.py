@@ -1,0 +1,64 @@
+"""Longer and larger runs of the library against the reference:
+  the 128x128 MC/LaxF torus (configuration 1) over 100 RK2 steps, from the
+  library's own initial data: every dt, no failed inversion, the final state
+  bit for bit, div B at round-off;
+  the 256x128x64 WENO5BND/HLL torus (the benchmark grid) over one RK2 step:
+  digests of the initial and final states and the dt sequence."""
+import os
+
+import numpy as np
+import pytest
+
+from golden_util import GOLDEN, divb_fluxct, load, state_digest
+
+pytestmark = pytest.mark.gpu
+
+
+def _own(d, nb):
+    from harm_finalwham_amd import harmgpu
+    par = d["params"]
+    n1, n2, n3 = (int(x) for x in par[:3])
+    weno = int(d["weno"])
+    g = harmgpu.Grid(n1, n2, n3, nb=nb, lim=harmgpu.HG_LIM_WENO5BND if weno else harmgpu.HG_LIM_MC,
+                     fluxmethod=harmgpu.HG_FLUX_HLL if weno else harmgpu.HG_FLUX_LAXF, gam=par[5], cour=par[10])
+    g.build_mks_geometry(a=par[4], Rin=par[6], Rout=par[7], R0=par[9], hslope=par[8])
+    dt = g.init_torus(rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0, randfact=4e-2, seed=7)
+    return g, dt
+
+
+def test_config1_100_steps_match_reference():
+    from harm_finalwham_amd import harmgpu
+    d = load("c1_mc_100")
+    nb = int(d["params"][3])
+    print("case 128x128 MC/LaxF, 100 steps", flush=True)
+    g, dt = _own(d, nb)
+    assert dt == d["dt_seq"][0]
+    assert state_digest(g.get_prims()) == str(d["digest_init"])
+    for n in range(1, int(d["nsteps"]) + 1):
+        dt, nfail, _ = g.step_ch(dt)
+        assert nfail == d["fail_seq"][n]
+        assert dt == d["dt_seq"][n], n
+    p = g.get_prims()
+    assert np.array_equal(p, d["p_final"])
+    geo = harmgpu.mks_geometry_tables(128, 128, 1, nb)
+    m = 128 + 2 * nb
+    div = divb_fluxct(p, geo["gdet"][:m, :m, 6], geo["gdet"][:m, :m, 5], geo["dx"], nb)
+    # the corner rows next to the axis carry POLEDEATH's zeroing of B2
+    assert div[:, 1:-1].max() < 1e-11
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "f3d_weno_1.npz")),
+                    reason="full-size fixture not generated (make_golden.py --long --full)")
+def test_benchmark_grid_one_step_matches_reference():
+    d = load("f3d_weno_1")
+    nb = int(d["params"][3])
+    print("case 256x128x64 WENO5BND/HLL, 1 step", flush=True)
+    g, dt = _own(d, nb)
+    assert dt == d["dt_seq"][0]
+    assert state_digest(g.get_prims()) == str(d["digest_init"])
+    dt, nfail, _ = g.step_ch(dt)
+    assert nfail == d["fail_seq"][1]
+    assert dt == d["dt_seq"][1]
+    p = g.get_prims()
+    assert np.array_equal(p.reshape(-1, 8).sum(axis=0), d["sum_final"])
+    assert state_digest(p) == str(d["digest_final"])

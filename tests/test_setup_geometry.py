@@ -43,3 +43,22 @@ def test_mks_geometry_rejects_bad_arguments():
     from harm_finalwham_amd import harmgpu
     with pytest.raises(harmgpu.HarmError):
         harmgpu.mks_geometry_tables(16, 16, 1, 3, Rin=0.0, R0=0.0)
+
+
+@pytest.mark.parametrize("name", ["c1_mc_100", "f3d_weno_1"])
+def test_mks_geometry_digests_match_reference(name):
+    """The 128x128 and 256x128 (x64) grids: digests of the reference's tables.
+    These grids reach angles where a libm sincos() call and separate sin()
+    and cos() calls round differently in the last bit."""
+    import os
+    from golden_util import GOLDEN, state_digest
+    from harm_finalwham_amd import harmgpu
+    if not os.path.exists(os.path.join(GOLDEN, name + ".npz")):
+        pytest.skip("fixture not generated")
+    d = load(name)
+    par = d["params"]
+    n1, n2, n3, nb = (int(x) for x in par[:4])
+    g = harmgpu.mks_geometry_tables(n1, n2, n3, nb, a=par[4], Rin=par[6], Rout=par[7], R0=par[9],
+                                    hslope=par[8])
+    for key in ("gcov", "gcon", "gdet", "conn"):
+        assert state_digest(g[key]) == str(d["digest_" + key]), key
