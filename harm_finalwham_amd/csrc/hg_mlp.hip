@@ -1,0 +1,715 @@
+// Learned Newton initial guess for the Noble 2D inversion.
+//
+// A multilayer perceptron 8 -> 64 -> 64 -> 1 (tanh) maps dimensionless
+// invariants of the conserved state, the metric and the plain guess to the
+// correction log(W / W_guess), W being the enthalpy-like unknown of the 2D
+// scheme and W_guess the reference's start (from the previous primitives).
+// Weights live in FP32 (training) with BF16 copies for the matrix cores: the
+// two hidden layers run on gfx950 MFMA (v_mfma_f32_16x16x32_bf16, FP32
+// accumulate), 16 inversions per tile.
+//
+//   k_mlp_features  FP64 set-up of the inversion -> 8 features per zone, and
+//                   the training target log(W_solved / W_guess)
+//   k_mlp_forward   features -> prediction (MFMA)
+//   k_mlp_backward  forward + backward of a 64-sample chunk on the vector
+//                   units (FP32), per-block weight gradients added to grad[]
+//   k_mlp_adam      Adam update of the FP32 weights
+//   k_mlp_refresh   BF16 copies of the hidden-layer weights
+//   k_c2p_learned   the FP64 inversion from the better of the learned and
+//                   the plain start; a learned start that does not converge
+//                   falls back to the reference path from the plain guess
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "harmgpu.h"
+#include "hg_c2p.h"
+#include "hg_host.h"
+
+using namespace hg;
+
+namespace {
+
+constexpr int NIN = 8, NH = 64;
+// parameter block (FP32): W1[8][64] b1[64] W2[64][64] b2[64] W3[64] b3 mu[8] sd[8]
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NIN * NH, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
+              OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NH, OFF_MU = OFF_B3 + 1, OFF_SD = OFF_MU + NIN,
+              NPARAM = OFF_SD + NIN;
+// trainable parameters are [0, OFF_MU)
+constexpr int NTRAIN = OFF_MU;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// features of one inversion from its FP64 set-up (scale free: ratios to D)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void features(const C2PInv &c, double W_last, double alpha, double gam, float *x) {
+  const double tiny = 1e-300;
+  double D = c.D;
+  double E = fmax(-c.Qdotn, tiny);
+  x[0] = (float)log(E / D);                                   // energy per rest mass
+  x[1] = (float)log1p(c.Qtsq / (E * E));                      // momentum^2 / energy^2
+  x[2] = (float)log1p(c.Bsq / D);                             // magnetisation
+  x[3] = (float)(c.QdotBsq / (c.Bsq * c.Qtsq + tiny));        // cos^2 (S, B)
+  x[4] = (float)log10(D);                                     // scale
+  x[5] = (float)alpha;                                        // lapse
+  x[6] = (float)log(fmax(W_last, tiny) / D);                  // the plain guess
+  x[7] = (float)gam;                                          // adiabatic index
+}
+
+// the FP64 set-up shared by features and the learned inversion
+__device__ __forceinline__ int setup_zone(const double *U, const Geom &G, double gam, const double *guess,
+                                          C2PInv &c, double &W_last, double &alpha, double *Ug) {
+  double Ut[NPR];
+#pragma unroll
+  for (int i = 0; i < NPR; i++) Ug[i] = U[i] / G.g;
+  Ug[UU] -= Ug[RHO];
+  alpha = 1.0 / sqrt(-G.gcon[0][0]);
+#pragma unroll
+  for (int i = 0; i < NPR; i++) Ut[i] = alpha * Ug[i];
+  return c2p_setup(Ut, G, guess, gam, c, W_last);
+}
+
+__device__ __forceinline__ void load_zone(int64_t n, int64_t t, const double *U, const double *gcov,
+                                          const double *gcon, const double *gdet, const double *pr, Geom &G,
+                                          double *u, double *p) {
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      G.gcov[a][b] = gcov[(a * 4 + b) * n + t];
+      G.gcon[a][b] = gcon[(a * 4 + b) * n + t];
+    }
+  G.gpert0 = 0.;
+  G.g = gdet[t];
+#pragma unroll
+  for (int pl = 0; pl < NPR; pl++) {
+    u[pl] = U[pl * n + t];
+    p[pl] = pr[pl * n + t];
+  }
+}
+
+// W = (rho + gam u) (1 + u~^2) of primitives p
+__device__ __forceinline__ double W_of_prims(const Geom &G, const double *p, double gam) {
+  double utsq = 0.;
+#pragma unroll
+  for (int i = 1; i < 4; i++)
+#pragma unroll
+    for (int j = 1; j < 4; j++) utsq += G.gcov[i][j] * p[U1 + i - 1] * p[U1 + j - 1];
+  return (p[RHO] + gam * p[UU]) * (1.0 + utsq);
+}
+
+// features X[n][8] from the plain guess; with solved primitives, the
+// training target y = log(W_solved / W_guess) in FP64 rounded once to FP32
+__global__ void __launch_bounds__(256) k_mlp_features(int64_t n, double gam, const double *__restrict__ U,
+                                                      const double *__restrict__ gcov,
+                                                      const double *__restrict__ gcon,
+                                                      const double *__restrict__ gdet,
+                                                      const double *__restrict__ guess,
+                                                      const double *__restrict__ solved, float *__restrict__ X,
+                                                      float *__restrict__ y, int *__restrict__ ok) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  Geom G;
+  double u[NPR], p[NPR], Ug[NPR];
+  load_zone(n, t, U, gcov, gcon, gdet, guess, G, u, p);
+  C2PInv c;
+  double W_last, alpha;
+  int code = setup_zone(u, G, gam, p, c, W_last, alpha, Ug);
+  float x[NIN];
+  features(c, W_last, alpha, gam, x);
+  bool good = (code == 0);
+#pragma unroll
+  for (int k = 0; k < NIN; k++) {
+    good = good && isfinite(x[k]);
+    X[t * NIN + k] = x[k];
+  }
+  if (solved) {
+    double ps[NPR];
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) ps[pl] = solved[pl * n + t];
+    double yt = log(W_of_prims(G, ps, gam) / W_last);
+    good = good && isfinite(yt);
+    if (y) y[t] = (float)yt;
+  }
+  if (ok) ok[t] = good ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// forward pass on the matrix cores: one wavefront, 16 samples per MFMA tile.
+// v_mfma_f32_16x16x32_bf16 lane layout (lane l, q = l / 16, r = l % 16):
+//   A (16 x 32):  A[r][8q + j], j = 0..7
+//   B (32 x 16):  B[8q + j][r]
+//   C (16 x 16):  C[4q + i][r], i = 0..3
+// Each layer input v enters as a pair of BF16 numbers hi = bf16(v),
+// lo = bf16(v - hi) against the same BF16 weights in both halves of K, so
+// the product keeps ~16 bits of the input (the weights are BF16, the
+// accumulation FP32).  Layer 1: K = 32 = [x_hi(8) x_lo(8) 0(16)];
+// layer 2: K = 128 = [h_hi(64) h_lo(64)].
+// ---------------------------------------------------------------------------
+struct MlpDev {
+  const float *w;       // FP32 parameters (biases, W3, normalisation)
+  const __bf16 *w1b;    // BF16 W1 as [32][64]: rows 0..7 and 8..15 W1, rest 0
+  const __bf16 *w2b;    // BF16 W2 [64][64]
+};
+
+__device__ __forceinline__ __bf16 bf_hi(float v) { return (__bf16)v; }
+__device__ __forceinline__ __bf16 bf_lo(float v) { return (__bf16)(v - (float)(__bf16)v); }
+
+// 16 samples: xin[16][8] (normalised) -> y[16]; h[16][64] float LDS scratch
+__device__ __forceinline__ void forward16(const MlpDev &m, const float (*xin)[NIN], float (*h)[NH], float *y) {
+  const int l = threadIdx.x & 63, q = l >> 4, r = l & 15;
+  // layer 1
+  bf16x8 a;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    float v = (q < 2) ? xin[r][j] : 0.0f;
+    a[j] = (q == 0) ? bf_hi(v) : bf_lo(v);  // q >= 2: bf16(0 - 0) = 0
+  }
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+    bf16x8 b;
+#pragma unroll
+    for (int j = 0; j < 8; j++) b[j] = m.w1b[(8 * q + j) * NH + 16 * nt + r];
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; i++) h[4 * q + i][16 * nt + r] = tanhf(acc[i] + m.w[OFF_B1 + 16 * nt + r]);
+  }
+  __syncthreads();
+  // layer 2: K steps 0,1 take h_hi, steps 2,3 h_lo
+  f32x4 acc2[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) acc2[nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < 4; ks++) {
+    const int kb = 32 * (ks & 1) + 8 * q;
+    bf16x8 a2;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      float v = h[r][kb + j];
+      a2[j] = (ks < 2) ? bf_hi(v) : bf_lo(v);
+    }
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+      bf16x8 b2;
+#pragma unroll
+      for (int j = 0; j < 8; j++) b2[j] = m.w2b[(kb + j) * NH + 16 * nt + r];
+      acc2[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2, acc2[nt], 0, 0, 0);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) h[4 * q + i][16 * nt + r] = tanhf(acc2[nt][i] + m.w[OFF_B2 + 16 * nt + r]);
+  __syncthreads();
+  // layer 3 on the vector units: lanes 0..15 one sample each
+  if (l < 16) {
+    float s = m.w[OFF_B3];
+    for (int k = 0; k < NH; k++) s += h[l][k] * m.w[OFF_W3 + k];
+    y[l] = s;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(64) k_mlp_forward(int64_t n, const float *__restrict__ X, float *__restrict__ y,
+                                                    MlpDev m) {
+  __shared__ float xin[16][NIN];
+  __shared__ float h[16][NH];
+  __shared__ float yo[16];
+  const int l = threadIdx.x;
+  for (int64_t base = (int64_t)blockIdx.x * 64; base < (int64_t)blockIdx.x * 64 + 64; base += 16) {
+    if (base >= n) break;
+    if (l < 16) {
+      int64_t t = base + l;
+#pragma unroll
+      for (int k = 0; k < NIN; k++)
+        xin[l][k] = (t < n) ? (X[t * NIN + k] - m.w[OFF_MU + k]) / m.w[OFF_SD + k] : 0.f;
+    }
+    __syncthreads();
+    forward16(m, xin, h, yo);
+    __syncthreads();
+    if (l < 16 && base + l < n) y[base + l] = yo[l];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// backward pass, 64 samples per block of one wavefront, FP32 on the vector
+// units.  The forward is recomputed from the same BF16-rounded weights and
+// hi + lo layer inputs the matrix-core forward uses (the weight rounding is
+// treated as the identity when differentiating: the gradient is that of the
+// FP32 master weights).  Activations are staged in LDS (row stride 65 floats, so both
+// per-sample rows and per-unit columns are free of bank conflicts):
+//   A  thread = sample: x, h1, h2, dy
+//   B  thread = unit j: dW3[j], dz2[s][j] in place of h2, db2[j]
+//   C  thread = sample: dz1[s][k] = (W2 dz2)[k] (1 - h1^2)
+//   D  thread = unit k: dW2[k][:], dW1[:][k], db1[k]
+// Each block adds its sums into grad[] once per parameter.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float bfr(float x) { return (float)(__bf16)x; }
+// the value a layer input carries into the matrix cores: hi + lo
+__device__ __forceinline__ float eff(float v) { return (float)bf_hi(v) + (float)bf_lo(v); }
+
+constexpr int LS = NH + 1;  // LDS row stride
+
+__global__ void __launch_bounds__(64) k_mlp_backward(int64_t n, const float *__restrict__ X,
+                                                     const float *__restrict__ Y, const float *__restrict__ w,
+                                                     float inv_n, float loss_scale, float *__restrict__ grad,
+                                                     float *__restrict__ loss) {
+  __shared__ float xs[64][NIN + 1];
+  __shared__ float h1s[64][LS];
+  __shared__ float h2s[64][LS];  // h2, then dz2
+  __shared__ float d1s[64][LS];  // dz1
+  __shared__ float dys[64];
+  const int l = threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * 64 + l;
+  const bool act = t < n;
+  // A: forward of sample l
+  {
+    float x[NIN];
+#pragma unroll
+    for (int i = 0; i < NIN; i++) {
+      x[i] = act ? eff((X[t * NIN + i] - w[OFF_MU + i]) / w[OFF_SD + i]) : 0.f;
+      xs[l][i] = x[i];
+    }
+    for (int j = 0; j < NH; j++) {
+      float s = 0.f;
+#pragma unroll
+      for (int i = 0; i < NIN; i++) s += x[i] * bfr(w[OFF_W1 + i * NH + j]);
+      h1s[l][j] = tanhf(s + w[OFF_B1 + j]);
+    }
+    float yp = w[OFF_B3];
+    for (int j = 0; j < NH; j++) {
+      float s = 0.f;
+      for (int k = 0; k < NH; k++) s += eff(h1s[l][k]) * bfr(w[OFF_W2 + k * NH + j]);
+      float h2 = tanhf(s + w[OFF_B2 + j]);
+      h2s[l][j] = h2;
+      yp += h2 * w[OFF_W3 + j];
+    }
+    float r = act ? yp - Y[t] : 0.f;
+    dys[l] = 2.f * r * inv_n;
+    float lsum = r * r * loss_scale;
+    for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+    if (l == 0) atomicAdd(loss, lsum);
+  }
+  __syncthreads();
+  // B: unit j = l of layer 3 / layer 2 output
+  {
+    const int j = l;
+    const float w3 = w[OFF_W3 + j];
+    float gw3 = 0.f, gb2 = 0.f;
+    for (int s = 0; s < 64; s++) {
+      float h2 = h2s[s][j], dy = dys[s];
+      gw3 += dy * h2;
+      float dz2 = dy * w3 * (1.f - h2 * h2);
+      h2s[s][j] = dz2;
+      gb2 += dz2;
+    }
+    atomicAdd(&grad[OFF_W3 + j], gw3);
+    atomicAdd(&grad[OFF_B2 + j], gb2);
+    if (l == 0) {
+      float gb3 = 0.f;
+      for (int s = 0; s < 64; s++) gb3 += dys[s];
+      atomicAdd(&grad[OFF_B3], gb3);
+    }
+  }
+  __syncthreads();
+  // C: sample l, back through W2 and the first tanh
+  for (int k = 0; k < NH; k++) {
+    float dh = 0.f;
+    for (int j = 0; j < NH; j++) dh += bfr(w[OFF_W2 + k * NH + j]) * h2s[l][j];
+    float h1 = h1s[l][k];
+    d1s[l][k] = dh * (1.f - h1 * h1);
+  }
+  __syncthreads();
+  // D: unit k = l of the first hidden layer
+  {
+    const int k = l;
+    float g2[NH];
+#pragma unroll
+    for (int j = 0; j < NH; j++) g2[j] = 0.f;
+    float g1[NIN];
+#pragma unroll
+    for (int i = 0; i < NIN; i++) g1[i] = 0.f;
+    float gb1 = 0.f;
+    for (int s = 0; s < 64; s++) {
+      float hk = eff(h1s[s][k]);
+#pragma unroll
+      for (int j = 0; j < NH; j++) g2[j] += hk * h2s[s][j];
+      float dz1 = d1s[s][k];
+      gb1 += dz1;
+#pragma unroll
+      for (int i = 0; i < NIN; i++) g1[i] += xs[s][i] * dz1;
+    }
+#pragma unroll
+    for (int j = 0; j < NH; j++) atomicAdd(&grad[OFF_W2 + k * NH + j], g2[j]);
+#pragma unroll
+    for (int i = 0; i < NIN; i++) atomicAdd(&grad[OFF_W1 + i * NH + k], g1[i]);
+    atomicAdd(&grad[OFF_B1 + k], gb1);
+  }
+}
+
+// BF16 copies of W1 (as [32][64], rows 0..7 and 8..15 both W1) and W2
+__device__ __forceinline__ void refresh_bf16(int e, const float *w, __bf16 *w1b, __bf16 *w2b) {
+  if (e < 32 * NH) {
+    int row = e / NH, col = e % NH;
+    w1b[e] = (row < 2 * NIN) ? (__bf16)w[OFF_W1 + (row % NIN) * NH + col] : (__bf16)0.0f;
+  }
+  if (e < NH * NH) w2b[e] = (__bf16)w[OFF_W2 + e];
+}
+
+__global__ void k_mlp_adam(float *__restrict__ w, float *__restrict__ g, float *__restrict__ m1,
+                           float *__restrict__ m2, float lr, float b1t, float b2t) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < NTRAIN) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    float gi = g[e];
+    m1[e] = b1 * m1[e] + (1.f - b1) * gi;
+    m2[e] = b2 * m2[e] + (1.f - b2) * gi * gi;
+    float mh = m1[e] / (1.f - b1t), vh = m2[e] / (1.f - b2t);
+    w[e] -= lr * mh / (sqrtf(vh) + eps);
+    g[e] = 0.f;
+  }
+}
+
+__global__ void k_mlp_refresh(const float *__restrict__ w, __bf16 *__restrict__ w1b, __bf16 *__restrict__ w2b) {
+  refresh_bf16(blockIdx.x * blockDim.x + threadIdx.x, w, w1b, w2b);
+}
+
+// ---------------------------------------------------------------------------
+// inversion with the learned start.  The residual of the Newton system is
+// evaluated at the plain start (the reference's W from the previous
+// primitives) and at the learned one, W_guess exp(y); the iteration starts
+// from the smaller.  From the plain start it is the reference iteration of
+// utoprimgen_fp64 (same arithmetic, same result bits).  From the learned
+// start, a zone that does not reach the reference tolerance within
+// LEARNED_MAX_ITER steps, or meets a non-finite value, is inverted again by
+// utoprimgen_fp64 from the plain guess.
+// ---------------------------------------------------------------------------
+constexpr int LEARNED_MAX_ITER = 30;
+
+// Newton loop of utoprimgen_fp64 from (x0, x1) whose first step (dx0, dx1,
+// f, df) is already evaluated; `iters` counts evaluations
+__device__ __forceinline__ void newton_continue(const C2PInv &c, double igr, double &x0, double &x1, double dx0,
+                                                double dx1, double &f, double &df, int max_iter, int &iters,
+                                                double &errx) {
+  const double gsq_big = GAMMAFAIL * GAMMAFAIL;
+  const double vsq_big = (gsq_big - 1.0) / (gsq_big);
+  int n_iter = 0;
+  while (true) {
+    double xo0 = x0;
+    x0 += dx0;
+    x1 += dx1;
+    errx = (x0 == 0.) ? fabs(dx0) : fabs(dx0 / x0);
+    x0 = fabs(x0);
+    x0 = (x0 / c.w > gsq_big) ? xo0 : x0;
+    x1 = (x1 > 1.) ? vsq_big : x1;
+    bool stop = (fabs(errx) <= NEWT_TOL) || (n_iter >= max_iter - 1);
+    n_iter++;
+    if (stop) break;
+    iters++;
+    newton_dx(c, igr, x0, x1, dx0, dx1, f, df);
+  }
+}
+
+__global__ void __launch_bounds__(64) k_c2p_learned(int64_t n, double gam, const double *__restrict__ U,
+                                                    const double *__restrict__ gcov,
+                                                    const double *__restrict__ gcon,
+                                                    const double *__restrict__ gdet,
+                                                    const double *__restrict__ gcovpert, double *__restrict__ pr,
+                                                    int *__restrict__ pflag, int *__restrict__ iters, MlpDev m,
+                                                    unsigned long long *__restrict__ counts) {
+  __shared__ float xin[64][NIN];
+  __shared__ float h[16][NH];
+  __shared__ float yo[64];
+  const int l = threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * 64 + l;
+  const bool act = t < n;
+  Geom G;
+  double u[NPR], p[NPR], Ug[NPR];
+  C2PInv c;
+  double W_last = 1., alpha = 1.;
+  int code = 1;
+  if (act) {
+    load_zone(n, t, U, gcov, gcon, gdet, pr, G, u, p);
+    G.gpert0 = gcovpert[t];
+    code = setup_zone(u, G, gam, p, c, W_last, alpha, Ug);
+    float x[NIN];
+    features(c, W_last, alpha, gam, x);
+#pragma unroll
+    for (int k = 0; k < NIN; k++) xin[l][k] = (x[k] - m.w[OFF_MU + k]) / m.w[OFF_SD + k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NIN; k++) xin[l][k] = 0.f;
+  }
+  __syncthreads();
+  for (int g16 = 0; g16 < 4; g16++) {
+    forward16(m, (const float(*)[NIN]) & xin[16 * g16][0], h, &yo[16 * g16]);
+    __syncthreads();
+  }
+  if (!act) return;
+  const double igr = (gam - 1.0) / gam;
+  const double gsq_big = GAMMAFAIL * GAMMAFAIL;
+  const double vsq_big = (gsq_big - 1.0) / (gsq_big);
+  int it = 0, flag;
+  bool learned = false, ref_done = false;
+  double x0 = 0., x1 = 0., errx = 1., f = 1., df = 1.;
+  if (code == 0) {
+    // the plain start and its first step
+    double xp0 = fabs(W_last), vp = vsq_of_W(c, W_last);
+    double xp1 = (vp > 1.) ? vsq_big : vp;
+    double dp0, dp1, fp, dfp;
+    newton_dx(c, igr, xp0, xp1, dp0, dp1, fp, dfp);
+    // the learned start and its first step
+    double Wm = W_last * exp((double)yo[l]);
+    double xm0 = Wm, vm = vsq_of_W(c, Wm);
+    double xm1 = (vm > 1.) ? vsq_big : vm;
+    double dm0 = 0., dm1 = 0., fm = 0., dfm = 0.;
+    bool cand = isfinite(Wm) && Wm > 0.;
+    if (cand) newton_dx(c, igr, xm0, xm1, dm0, dm1, fm, dfm);
+    cand = cand && isfinite(fm) && isfinite(dm0) && isfinite(dm1) && (fm < fp || !isfinite(fp));
+    it = cand ? 2 : 1;
+    if (cand) {
+      x0 = xm0;
+      x1 = xm1;
+      f = fm;
+      df = dfm;
+      newton_continue(c, igr, x0, x1, dm0, dm1, f, df, LEARNED_MAX_ITER, it, errx);
+      learned = (fabs(errx) <= NEWT_TOL) && isfinite(f) && isfinite(df) && isfinite(x0) && isfinite(x1);
+    } else {
+      // the reference iteration, continued from its own first step
+      x0 = xp0;
+      x1 = xp1;
+      f = fp;
+      df = dfp;
+      newton_continue(c, igr, x0, x1, dp0, dp1, f, df, MAX_NEWT_ITER, it, errx);
+      ref_done = true;
+    }
+  }
+  if (learned || ref_done) {
+    int retval = 0;
+    if (!isfinite(f) || !isfinite(df) || !isfinite(x0) || !isfinite(x1)) retval = 1;
+    else if (fabs(errx) > MIN_NEWT_TOL) retval = 1;
+#pragma unroll
+    for (int i = B1; i <= B3; i++) p[i] = Ug[i];
+    double out[NPR];
+    bool assign;
+    flag = c2p_finish(c, x0, retval, igr, out, assign);
+    if (assign)
+#pragma unroll
+      for (int i = 0; i < B1; i++) p[i] = out[i];
+  } else {
+    // set-up failure, or the learned start did not converge: the plain
+    // guess and the reference arithmetic
+    int it2 = 0;
+    for (int pl = 0; pl < NPR; pl++) p[pl] = pr[pl * n + t];
+    flag = utoprimgen_fp64(u, G, gam, p, it2);
+    if (code == 0) atomicAdd(&counts[1], 1ull);
+    it += it2;
+  }
+  if (learned) atomicAdd(&counts[0], 1ull);
+#pragma unroll
+  for (int pl = 0; pl < NPR; pl++) pr[pl * n + t] = p[pl];
+  pflag[t] = flag;
+  if (iters) iters[t] = it;
+}
+
+}  // namespace
+
+struct hg_mlp {
+  float *w = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr, *loss = nullptr;
+  __bf16 *w1b = nullptr, *w2b = nullptr;
+  unsigned long long *counts = nullptr;  // [learned starts, fallbacks]
+  int step = 0;
+};
+
+static MlpDev dev_view(const hg_mlp *m) { return MlpDev{m->w, m->w1b, m->w2b}; }
+
+#define MLPCHECK(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return 1;                                                                     \
+    }                                                                               \
+  } while (0)
+
+extern "C" int hg_mlp_destroy(hg_mlp *m) {
+  if (!m) return 0;
+  void *b[] = {m->w, m->g, m->m1, m->m2, m->loss, m->w1b, m->w2b, m->counts};
+  for (void *p : b)
+    if (p) (void)hipFree(p);
+  delete m;
+  return 0;
+}
+
+extern "C" int hg_mlp_create(uint64_t seed, hg_mlp **out) {
+  if (!out) { set_error("hg_mlp_create: null output"); return 1; }
+  *out = nullptr;
+  hg_mlp *m = new hg_mlp();
+  std::vector<float> w(NPARAM, 0.f);
+  // Glorot-uniform hidden weights from a 64-bit LCG; unit normalisation
+  uint64_t s = seed * 6364136223846793005ull + 1442695040888963407ull;
+  auto uni = [&]() {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    return (double)(s >> 11) * 0x1.0p-53;
+  };
+  double a1 = sqrt(6.0 / (NIN + NH)), a2 = sqrt(6.0 / (NH + NH)), a3 = sqrt(6.0 / (NH + 1));
+  for (int e = 0; e < NIN * NH; e++) w[OFF_W1 + e] = (float)(a1 * (2 * uni() - 1));
+  for (int e = 0; e < NH * NH; e++) w[OFF_W2 + e] = (float)(a2 * (2 * uni() - 1));
+  for (int e = 0; e < NH; e++) w[OFF_W3 + e] = (float)(a3 * (2 * uni() - 1));
+  for (int k = 0; k < NIN; k++) w[OFF_SD + k] = 1.f;
+  hipError_t e = hipMalloc((void **)&m->w, NPARAM * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void **)&m->g, NPARAM * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void **)&m->m1, NPARAM * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void **)&m->m2, NPARAM * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void **)&m->loss, sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void **)&m->w1b, 32 * NH * sizeof(__bf16));
+  if (e == hipSuccess) e = hipMalloc((void **)&m->w2b, NH * NH * sizeof(__bf16));
+  if (e == hipSuccess) e = hipMalloc((void **)&m->counts, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemcpy(m->w, w.data(), NPARAM * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(m->g, 0, NPARAM * sizeof(float));
+  if (e == hipSuccess) e = hipMemset(m->m1, 0, NPARAM * sizeof(float));
+  if (e == hipSuccess) e = hipMemset(m->m2, 0, NPARAM * sizeof(float));
+  if (e != hipSuccess) {
+    set_error("hg_mlp_create: %s", hipGetErrorString(e));
+    hg_mlp_destroy(m);
+    return 1;
+  }
+  hipLaunchKernelGGL(k_mlp_refresh, dim3((NH * NH + 255) / 256), dim3(256), 0, 0, m->w, m->w1b, m->w2b);
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    set_error("hg_mlp_create: %s", hipGetErrorString(e));
+    hg_mlp_destroy(m);
+    return 1;
+  }
+  *out = m;
+  return 0;
+}
+
+extern "C" int hg_mlp_nparams(void) { return NPARAM; }
+
+extern "C" int hg_mlp_get_params(const hg_mlp *m, float *host) {
+  if (!m || !host) { set_error("hg_mlp_get_params: null argument"); return 1; }
+  MLPCHECK(hipMemcpy(host, m->w, NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int hg_mlp_set_params(hg_mlp *m, const float *host) {
+  if (!m || !host) { set_error("hg_mlp_set_params: null argument"); return 1; }
+  for (int k = 0; k < NIN; k++)
+    if (!(host[OFF_SD + k] > 0.f)) { set_error("hg_mlp_set_params: feature scales must be > 0"); return 1; }
+  MLPCHECK(hipMemcpy(m->w, host, NPARAM * sizeof(float), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_mlp_refresh, dim3((NH * NH + 255) / 256), dim3(256), 0, 0, m->w, m->w1b, m->w2b);
+  MLPCHECK(hipGetLastError());
+  MLPCHECK(hipDeviceSynchronize());
+  return 0;
+}
+
+extern "C" int hg_mlp_features(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
+                               const double *gdet, const double *p_guess, const double *p_solved, float *X,
+                               float *y, int32_t *ok, void *stream) {
+  if (n < 0 || (n > 0 && (!U || !gcov || !gcon || !gdet || !p_guess || !X || (y && !p_solved)))) {
+    set_error("hg_mlp_features: bad arguments");
+    return 1;
+  }
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_mlp_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, gam, U, gcov, gcon,
+                     gdet, p_guess, p_solved, X, y, ok);
+  MLPCHECK(hipGetLastError());
+  MLPCHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int hg_mlp_forward(const hg_mlp *m, int64_t n, const float *X, float *y, void *stream) {
+  if (!m || n < 0 || (n > 0 && (!X || !y))) { set_error("hg_mlp_forward: bad arguments"); return 1; }
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, X, y, dev_view(m));
+  MLPCHECK(hipGetLastError());
+  MLPCHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
+// gradients of the mean squared error over n samples (no update); grad_host
+// (may be NULL) receives them, *loss the error
+extern "C" int hg_mlp_gradient(hg_mlp *m, int64_t n, const float *X, const float *Y, float *grad_host,
+                               float *loss, void *stream) {
+  if (!m || n <= 0 || !X || !Y) { set_error("hg_mlp_gradient: bad arguments"); return 1; }
+  hipStream_t s = (hipStream_t)stream;
+  MLPCHECK(hipMemsetAsync(m->g, 0, NPARAM * sizeof(float), s));
+  MLPCHECK(hipMemsetAsync(m->loss, 0, sizeof(float), s));
+  hipLaunchKernelGGL(k_mlp_backward, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, X, Y, m->w,
+                     (float)(1.0 / (double)n), (float)(1.0 / (double)n), m->g, m->loss);
+  MLPCHECK(hipGetLastError());
+  MLPCHECK(hipStreamSynchronize(s));
+  if (grad_host) MLPCHECK(hipMemcpy(grad_host, m->g, NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+  if (loss) MLPCHECK(hipMemcpy(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost));
+  MLPCHECK(hipMemset(m->g, 0, NPARAM * sizeof(float)));
+  return 0;
+}
+
+// epochs of Adam over the n samples in minibatches of `batch` (in order);
+// *loss receives the mean minibatch loss of the last epoch
+extern "C" int hg_mlp_train(hg_mlp *m, int64_t n, const float *X, const float *Y, int epochs, int batch,
+                            float lr, float *loss, void *stream) {
+  if (!m || n <= 0 || !X || !Y || epochs < 0 || batch < 64) {
+    set_error("hg_mlp_train: bad arguments (batch >= 64)");
+    return 1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int64_t nb = (n + batch - 1) / batch;
+  float last = 0.f;
+  for (int ep = 0; ep < epochs; ep++) {
+    MLPCHECK(hipMemsetAsync(m->loss, 0, sizeof(float), s));
+    for (int64_t b = 0; b < nb; b++) {
+      int64_t off = b * batch, cnt = (off + batch <= n) ? batch : n - off;
+      hipLaunchKernelGGL(k_mlp_backward, dim3((unsigned)((cnt + 63) / 64)), dim3(64), 0, s, cnt,
+                         X + off * NIN, Y + off, m->w, (float)(1.0 / (double)cnt),
+                         (float)(1.0 / (double)cnt / (double)nb), m->g, m->loss);
+      m->step++;
+      float b1t = powf(0.9f, (float)m->step), b2t = powf(0.999f, (float)m->step);
+      hipLaunchKernelGGL(k_mlp_adam, dim3((NTRAIN + 255) / 256), dim3(256), 0, s, m->w, m->g, m->m1, m->m2, lr,
+                         b1t, b2t);
+      hipLaunchKernelGGL(k_mlp_refresh, dim3((NH * NH + 255) / 256), dim3(256), 0, s, m->w, m->w1b, m->w2b);
+    }
+    MLPCHECK(hipGetLastError());
+    MLPCHECK(hipMemcpyAsync(&last, m->loss, sizeof(float), hipMemcpyDeviceToHost, s));
+    MLPCHECK(hipStreamSynchronize(s));
+  }
+  if (loss) *loss = last;
+  return 0;
+}
+
+extern "C" int hg_utoprim_batch_learned(const hg_mlp *m, int64_t n, double gam, const double *U,
+                                        const double *gcov, const double *gcon, const double *gdet,
+                                        const double *gcovpert, double *pr, int32_t *pflag, int32_t *iters,
+                                        int64_t *counts, void *stream) {
+  if (!m || n < 0) { set_error("hg_utoprim_batch_learned: bad arguments"); return 1; }
+  if (n == 0) {
+    if (counts) counts[0] = counts[1] = 0;
+    return 0;
+  }
+  if (!U || !gcov || !gcon || !gdet || !gcovpert || !pr || !pflag) {
+    set_error("hg_utoprim_batch_learned: null argument");
+    return 1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  MLPCHECK(hipMemsetAsync(m->counts, 0, 2 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_c2p_learned, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, gam, U, gcov, gcon, gdet,
+                     gcovpert, pr, pflag, iters, dev_view(m), m->counts);
+  MLPCHECK(hipGetLastError());
+  MLPCHECK(hipStreamSynchronize(s));
+  if (counts) {
+    unsigned long long h[2] = {0, 0};
+    MLPCHECK(hipMemcpy(h, m->counts, sizeof h, hipMemcpyDeviceToHost));
+    counts[0] = (int64_t)h[0];
+    counts[1] = (int64_t)h[1];
+  }
+  return 0;
+}

@@ -21,7 +21,10 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_step", "hg_step_async", "hg_set_dt", "hg_get_dt", "hg_get_counters",
            "hg_utoprim_batch", "hg_make_c2p_batch", "hg_last_error",
            "hg_profile", "hg_profile_read", "hg_profile_class_name", "hg_c2p_fallbacks",
-           "hg_graph_create", "hg_graph_launch", "hg_graph_destroy", "hg_get_tune", "hg_set_tune", "hg_autotune"]
+           "hg_graph_create", "hg_graph_launch", "hg_graph_destroy", "hg_get_tune", "hg_set_tune", "hg_autotune",
+           "hg_mlp_create", "hg_mlp_destroy", "hg_mlp_nparams", "hg_mlp_get_params", "hg_mlp_set_params",
+           "hg_mlp_features", "hg_mlp_forward", "hg_mlp_gradient", "hg_mlp_train",
+           "hg_utoprim_batch_learned"]
 HG_PROF_NCLASS = 9
 
 
@@ -89,6 +92,17 @@ def lib():
     L.hg_set_tune.argtypes = [vp, C.POINTER(HgTune)]
     L.hg_autotune.argtypes = [vp, C.c_int, C.POINTER(HgTune), vp]
     L.hg_profile.argtypes = [vp, C.c_int]
+    fp = C.POINTER(C.c_float)
+    L.hg_mlp_create.argtypes = [C.c_uint64, C.POINTER(vp)]
+    L.hg_mlp_destroy.argtypes = [vp]
+    L.hg_mlp_nparams.argtypes = []
+    L.hg_mlp_get_params.argtypes = [vp, fp]
+    L.hg_mlp_set_params.argtypes = [vp, fp]
+    L.hg_mlp_features.argtypes = [C.c_int64, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.hg_mlp_forward.argtypes = [vp, C.c_int64, vp, vp, vp]
+    L.hg_mlp_gradient.argtypes = [vp, C.c_int64, vp, vp, fp, fp, vp]
+    L.hg_mlp_train.argtypes = [vp, C.c_int64, vp, vp, C.c_int, C.c_int, C.c_float, fp, vp]
+    L.hg_utoprim_batch_learned.argtypes = [vp, C.c_int64, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp, i64p, vp]
     L.hg_profile_read.argtypes = [vp, d, i64p]
     L.hg_profile_class_name.argtypes = [C.c_int]
     L.hg_profile_class_name.restype = C.c_char_p
@@ -267,6 +281,105 @@ def c2p_fallbacks():
     v = C.c_int64()
     _check(lib().hg_c2p_fallbacks(C.byref(v)))
     return v.value
+
+
+MLP_NIN, MLP_NH = 8, 64
+# offsets into the parameter block (FP32): W1[8][64] b1[64] W2[64][64] b2[64] W3[64] b3 mu[8] sd[8]
+MLP_OFF = {"W1": 0, "b1": 512, "W2": 576, "b2": 4672, "W3": 4736, "b3": 4800, "mu": 4801, "sd": 4809}
+MLP_NPARAM = 4817
+
+
+class Mlp:
+    """hg_mlp: the learned initial guess of the 2D inversion (8 -> 64 -> 64 -> 1,
+    tanh; BF16 weights on the matrix cores, FP32 master copy)."""
+
+    def __init__(self, seed=1):
+        h = C.c_void_p()
+        _check(lib().hg_mlp_create(seed, C.byref(h)))
+        self.h = h
+        assert lib().hg_mlp_nparams() == MLP_NPARAM
+
+    def params(self):
+        import numpy as np
+        w = np.empty(MLP_NPARAM, np.float32)
+        _check(lib().hg_mlp_get_params(self.h, w.ctypes.data_as(C.POINTER(C.c_float))))
+        return w
+
+    def set_params(self, w):
+        import numpy as np
+        w = np.ascontiguousarray(w, np.float32)
+        if w.shape != (MLP_NPARAM,):
+            raise ValueError("parameter block must have %d entries" % MLP_NPARAM)
+        _check(lib().hg_mlp_set_params(self.h, w.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def set_normalisation(self, mu, sd):
+        w = self.params()
+        w[MLP_OFF["mu"]:MLP_OFF["mu"] + MLP_NIN] = mu
+        w[MLP_OFF["sd"]:MLP_OFF["sd"] + MLP_NIN] = sd
+        self.set_params(w)
+
+    def forward(self, X, stream=None):
+        """X: float32 device tensor [n, 8] of raw features -> log(W/D) [n]."""
+        import torch
+        y = torch.empty(X.shape[0], dtype=torch.float32, device=X.device)
+        _check(lib().hg_mlp_forward(self.h, X.shape[0], C.c_void_p(X.data_ptr()), C.c_void_p(y.data_ptr()),
+                                    stream))
+        return y
+
+    def gradient(self, X, Y, stream=None):
+        """Gradient of the mean squared error over (X, Y) w.r.t. the parameter
+        block (normalisation entries zero) and the loss; no update."""
+        import numpy as np
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_float()
+        _check(lib().hg_mlp_gradient(self.h, X.shape[0], C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()),
+                                     g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss), stream))
+        return g, loss.value
+
+    def train(self, X, Y, epochs=1, batch=4096, lr=1e-3, stream=None):
+        """Adam over the samples in order, minibatches of `batch`; returns the
+        mean loss of the last epoch."""
+        loss = C.c_float()
+        _check(lib().hg_mlp_train(self.h, X.shape[0], C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()),
+                                  epochs, batch, lr, C.byref(loss), stream))
+        return loss.value
+
+    def close(self):
+        if self.h:
+            lib().hg_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mlp_features(U, gcov, gcon, gdet, p_guess, p_solved=None, gam=4.0 / 3.0, stream=None):
+    """Features of each inversion (float32 [n, 8]), with p_solved the targets
+    log(W_solved / W_guess) (float32 [n], else None), and a validity mask."""
+    import torch
+    n = U.shape[1]
+    X = torch.empty(n, MLP_NIN, dtype=torch.float32, device=U.device)
+    y = torch.empty(n, dtype=torch.float32, device=U.device) if p_solved is not None else None
+    ok = torch.empty(n, dtype=torch.int32, device=U.device)
+    ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+    _check(lib().hg_mlp_features(n, gam, ptr(U), ptr(gcov), ptr(gcon), ptr(gdet), ptr(p_guess), ptr(p_solved),
+                                 ptr(X), ptr(y), ptr(ok), stream))
+    return X, y, ok
+
+
+def utoprim_batch_learned(mlp, U, gcov, gcon, gdet, gcovpert, pr, pflag, iters=None, gam=4.0 / 3.0,
+                          stream=None):
+    """utoprim_batch from the better of the learned and the plain start;
+    returns (zones solved from the learned start, fallbacks)."""
+    n = U.shape[1]
+    counts = (C.c_int64 * 2)()
+    ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+    _check(lib().hg_utoprim_batch_learned(mlp.h, n, gam, ptr(U), ptr(gcov), ptr(gcon), ptr(gdet), ptr(gcovpert),
+                                          ptr(pr), ptr(pflag), ptr(iters), counts, stream))
+    return counts[0], counts[1]
 
 
 class Graph:

@@ -155,6 +155,57 @@ int hg_make_c2p_batch(int64_t n, double gam, double a, uint64_t seed, double *U,
    polish needed more than 4 steps).  No counterpart in the reference. */
 int hg_c2p_fallbacks(int64_t *nfallback);
 
+/* Learned initial guess of the Noble-2D Newton iteration, no counterpart in
+   the reference (whose guess is the previous primitives, utoprim_2d_final.c
+   Utoprim_new_body).  A perceptron 8 -> 64 -> 64 -> 1 (tanh) maps invariants
+   of the conserved state and the plain guess to y = log(W / W_guess); the
+   hidden layers run on the matrix cores with BF16 weights (each input as a
+   hi + lo BF16 pair) and FP32 accumulation, the FP32 master weights are
+   trained on the GPU (Adam) from inversions this library solved.
+   Parameter block (FP32, hg_mlp_nparams() = 4817 entries):
+     W1[8][64] b1[64] W2[64][64] b2[64] W3[64] b3 mu[8] sd[8]
+   where the network sees (x - mu) / sd.  Features x per zone:
+     log(-Q.n / D), log1p(Qt^2 / (Q.n)^2), log1p(B^2 / D), (Q.B)^2/(B^2 Qt^2),
+     log10(D), alpha, log(W_guess / D), gam
+   with D = alpha rho u^t and W_guess the plain guess's W.
+   All arrays are device memory; X is [n][8] float, y [n] float, the
+   inversion arrays are laid out as for hg_utoprim_batch. */
+typedef struct hg_mlp hg_mlp;
+int hg_mlp_create(uint64_t seed, hg_mlp **out);
+int hg_mlp_destroy(hg_mlp *m);
+int hg_mlp_nparams(void);
+int hg_mlp_get_params(const hg_mlp *m, float *host);
+int hg_mlp_set_params(hg_mlp *m, const float *host);
+/* features X[n][8] of n inversions from their plain guess p_guess[8][n];
+   with solved primitives p_solved[8][n] (may be NULL) also the training
+   target y[n] = log(W_solved / W_guess), which the network predicts; ok[n]
+   (may be NULL) is 1 where the set-up succeeded and every value is finite */
+int hg_mlp_features(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
+                    const double *gdet, const double *p_guess, const double *p_solved, float *X, float *y,
+                    int32_t *ok, void *stream);
+int hg_mlp_forward(const hg_mlp *m, int64_t n, const float *X, float *y, void *stream);
+/* gradient of mean((y - Y)^2) with respect to the parameter block (host
+   copy in grad_host, may be NULL; normalisation entries are 0), the loss in
+   *loss; the weights are not changed */
+int hg_mlp_gradient(hg_mlp *m, int64_t n, const float *X, const float *Y, float *grad_host, float *loss,
+                    void *stream);
+/* `epochs` passes of Adam over minibatches of `batch` (>= 64) samples in order */
+int hg_mlp_train(hg_mlp *m, int64_t n, const float *X, const float *Y, int epochs, int batch, float lr,
+                 float *loss, void *stream);
+/* hg_utoprim_batch (HG_C2P_FP64 arithmetic) with a learned start.  The
+   residual of the Newton system is evaluated at the plain start and at
+   W_guess exp(y); the iteration starts from the smaller.  From the plain
+   start the zone follows the reference iteration bit for bit.  From the
+   learned start, a zone whose iteration does not reach the reference
+   tolerance within 30 steps, or meets a non-finite value, is inverted again
+   from its plain guess by the reference path.  Learned zones agree with the
+   plain path to the Newton tolerance, not bit for bit (another start).
+   iters counts residual evaluations, both starts included.  counts (may be
+   NULL) receives [zones solved from the learned start, fallbacks]. */
+int hg_utoprim_batch_learned(const hg_mlp *m, int64_t n, double gam, const double *U, const double *gcov,
+                             const double *gcon, const double *gdet, const double *gcovpert, double *pr,
+                             int32_t *pflag, int32_t *iters, int64_t counts[2], void *stream);
+
 /* Several grids advanced by one HIP graph, no counterpart in the reference
    (which runs one grid per process): the graph holds nsteps RK2 steps of
    every context, one concurrent branch per context, and each launch advances

@@ -115,6 +115,12 @@ int or_step(const or_grid *G, double *p, double *dt_io, int64_t *nfail, int64_t 
 int or_fluxcalc(const or_grid *G, const double *p, double *F1, double *F2, double *F3,
                 double *ndt);
 int or_bound_prim(const or_grid *G, double *p);
+/* post_fixup (fixup.c:37): ghost fill of the failure flags, the
+   check of good solutions against their neighbours and the neighbour
+   average of failed zones, on p (bounded) and pflag (full array of int,
+   layout of p without the pl index: inversion codes in the active zones) */
+int or_post_fixup(const or_grid *G, double *p, int *pflag, int finalstep);
+void or_bound_pflag(const or_grid *G, int *pflag);
 /* interface states of the reconstruction along dir for the whole grid */
 int or_reconstruct(const or_grid *G, const double *p, int dir, double *pleft, double *pright);
 

@@ -62,6 +62,9 @@ static inline double or_gdet(const or_grid *G, const or_dims *d, int i, int j, i
   return G->gdet[((ptrdiff_t)(i + d->nb1) * d->g2 + (j + d->nb2)) * OL_NLOC + loc];
 }
 
+/* phys.c:1023 gamma_calc (VELREL4, JONCHECKS): 0 ok, 1 failure */
+int or_gamma_calc(const double *pr, const or_geom *g, double *gamma);
+
 /* or_weno.c: WENO5BND interface states along one pencil (interpline.c:517) */
 void or_weno5bnd_line(const or_grid *G, const or_dims *d, const double *p, int dir, int i, int j,
                       int k, double *pleft, double *pright);

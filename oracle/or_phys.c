@@ -50,6 +50,8 @@ static int gamma_calc(const double *pr, const or_geom *g, double *gamma)
   return 0;
 }
 
+int or_gamma_calc(const double *pr, const or_geom *g, double *gamma) { return gamma_calc(pr, g, gamma); }
+
 /* phys.c:1006 ucon_calc_rel4vel -> phys.c:987 ucon_calc_rel4vel_fromuconrel */
 static int ucon_calc(const double *pr, const or_geom *g, double *ucon)
 {

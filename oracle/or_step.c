@@ -2,7 +2,7 @@
  * or_step.c -- grid-level restatement of one RK2 step of
  * pseudotensor/harm_finalwham (TIMEORDER 2, NOENOFLUX, FLUXCTTOTH):
  * ghost fill, reconstruction, Riemann fluxes, flux-CT, conserved update,
- * Noble 2D inversion and fixup1zone.  TEST INFRASTRUCTURE ONLY.
+ * Noble 2D inversion, fixup1zone and post_fixup.  TEST INFRASTRUCTURE ONLY.
  */
 #include "or_internal.h"
 #include <float.h>
@@ -354,7 +354,7 @@ int or_fluxcalc(const or_grid *G, const double *p, double *F1, double *F2, doubl
 /* ------------------------------------------------------------------ */
 static int advance(const or_grid *G, const double *pi, const double *pb, double *pf, double *uf,
                    double *ucum, const double CUf[4], const double Cunew[3], int timeorder,
-                   double dt, double *ndt_out, int64_t *nfail, int64_t *iters)
+                   double dt, double *ndt_out, int *pflag, int64_t *nfail, int64_t *iters)
 {
   or_dims d = or_get_dims(G);
   size_t nz = (size_t)d.m1 * d.m2 * d.m3;
@@ -398,6 +398,7 @@ static int advance(const or_grid *G, const double *pi, const double *pb, double 
         for (pl = 0; pl < OR_NPR; pl++) pf[z * 8 + pl] = pb[z * 8 + pl];
         if (finalstep) or_utoprimgen(&ucum[z * 8], &g, G->gam, &pf[z * 8], &fl, &it);
         else or_utoprimgen(Uf, &g, G->gam, &pf[z * 8], &fl, &it);
+        pflag[z] = fl;
         if (fl) (*nfail)++;
         *iters += it;
         if (finalstep) {
@@ -419,6 +420,7 @@ int or_step(const or_grid *G, double *p, double *dt_io, int64_t *nfail, int64_t 
   size_t nz = (size_t)d.m1 * d.m2 * d.m3;
   double *pk = malloc(nz * 8 * sizeof(double));
   double *uf = calloc(nz * 8, sizeof(double)), *ucum = calloc(nz * 8, sizeof(double));
+  int *pflag = calloc(nz, sizeof(int));
   /* step_ch.c:506-520, TIMEORDER 2 midpoint */
   const double CUf[2][4] = {{1.0, 0.0, 0.5, 0.0}, {1.0, 0.0, 1.0, 0.0}};
   const double Cunew[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}};
@@ -426,11 +428,12 @@ int or_step(const or_grid *G, double *p, double *dt_io, int64_t *nfail, int64_t 
   int rc = 0;
   memcpy(pk, p, nz * 8 * sizeof(double));
   /* substep 0: pi=p pb=p pf=pk */
-  if (advance(G, p, p, pk, uf, ucum, CUf[0], Cunew[0], 0, dt, &ndt, nfail, iters)) { rc = 1; goto done; }
-  if (or_bound_prim(G, pk) || or_bound_prim(G, pk)) { rc = 1; goto done; }
+  /* each substep: advance, bound_prim, post_fixup, bound_prim (step_ch.c:123-157) */
+  if (advance(G, p, p, pk, uf, ucum, CUf[0], Cunew[0], 0, dt, &ndt, pflag, nfail, iters)) { rc = 1; goto done; }
+  if (or_bound_prim(G, pk) || or_post_fixup(G, pk, pflag, 0) || or_bound_prim(G, pk)) { rc = 1; goto done; }
   /* substep 1: pi=p pb=pk pf=p */
-  if (advance(G, p, pk, p, uf, ucum, CUf[1], Cunew[1], 1, dt, &ndt, nfail, iters)) { rc = 1; goto done; }
-  if (or_bound_prim(G, p) || or_bound_prim(G, p)) { rc = 1; goto done; }
+  if (advance(G, p, pk, p, uf, ucum, CUf[1], Cunew[1], 1, dt, &ndt, pflag, nfail, iters)) { rc = 1; goto done; }
+  if (or_bound_prim(G, p) || or_post_fixup(G, p, pflag, 1) || or_bound_prim(G, p)) { rc = 1; goto done; }
   /* step_ch.c:243-247 */
   if (ndt > G->safe * dt) ndt = G->safe * dt;
   *dt_io = ndt;
@@ -438,5 +441,6 @@ done:
   free(pk);
   free(uf);
   free(ucum);
+  free(pflag);
   return rc;
 }

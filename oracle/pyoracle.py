@@ -39,6 +39,7 @@ def lib():
         _lib.or_bound_prim.argtypes = [C.POINTER(OrGrid), d]
         _lib.or_reconstruct.argtypes = [C.POINTER(OrGrid), d, C.c_int, d, d]
         _lib.or_floors.argtypes = [C.c_int64, d, C.c_double, C.c_double, d]
+        _lib.or_post_fixup.argtypes = [C.POINTER(OrGrid), d, i32, C.c_int]
     return _lib
 
 
@@ -146,3 +147,22 @@ class Grid:
         if lib().or_bound_prim(C.byref(self.g), _p(p)):
             raise RuntimeError("oracle bound failed")
         return p
+
+    def post_fixup_sequence(self, p, flags, finalstep):
+        """bound_prim, post_fixup, bound_prim (step_ch.c:132-157) on p with
+        the inversion codes `flags` [n1][n2][n3] of the active zones; returns
+        p and the full flag array [m1][m2][m3]."""
+        p = np.ascontiguousarray(p, dtype=np.float64).copy()
+        n1, n2, n3 = self.g.n1, self.g.n2, self.g.n3
+        m1, m2, m3 = self.shape[:3]
+        b1, b2, b3 = (m1 - n1) // 2, (m2 - n2) // 2, (m3 - n3) // 2
+        pf = np.zeros((m1, m2, m3), np.int32)
+        pf[b1:b1 + n1, b2:b2 + n2, b3:b3 + n3] = flags
+        L = lib()
+        if L.or_bound_prim(C.byref(self.g), _p(p)):
+            raise RuntimeError("oracle bound failed")
+        if L.or_post_fixup(C.byref(self.g), _p(p), pf.ctypes.data_as(C.POINTER(C.c_int32)), int(finalstep)):
+            raise RuntimeError("oracle post_fixup failed")
+        if L.or_bound_prim(C.byref(self.g), _p(p)):
+            raise RuntimeError("oracle bound failed")
+        return p, pf

@@ -12,10 +12,10 @@
  *   MAXBND 6      ghost width; WENO5BND needs interporder[WENO5BND]/2+1 = 6
  *                 (initbase.c:718 orders_set, initbase.c:792 get_num_bnd_zones_used)
  *   EOMTYPE       EOMGRMHD; init.fishmon.h:170 ships EOMFFDE by mistake
- *   CHECKSOLUTION 0: fixup_checksolution (fixup.c:460) is not part of the
- *                 path being pinned; fixup_utoprim still runs and is a no-op
- *                 whenever no zone fails its inversion (the harness counts
- *                 failed zones and records them in the fixture manifest).
+ *
+ * Everything else, CHECKSOLUTION 1 (fixup_checksolution, fixup.c:460) and
+ * UTOPRIMADJUST UTOPRIMAVG (fixup_utoprim, fixup.c:669) included, is the
+ * torus header's own setting.
  */
 #include "init.fishmon.h"
 
@@ -32,5 +32,3 @@
 #undef EOMTYPE
 #define EOMTYPE EOMGRMHD
 
-#undef CHECKSOLUTION
-#define CHECKSOLUTION 0

@@ -21,6 +21,9 @@
  *   time <weno 0|1> <nsteps>
  *       Times <nsteps> steps of the torus (IO excluded) and prints
  *       zone-updates/s; used as the reference CPU baseline.
+ *   fixup <infile> <outdir> <weno 0|1> <finalstep>
+ *       bound_prim + post_fixup + bound_prim on a given state and given
+ *       inversion codes (see mode_fixup).
  *
  * The initialisation below follows init() in initbase.c:5-176 call for call;
  * the only additions are the parameter overrides marked OVERRIDE.
@@ -403,11 +406,48 @@ static int mode_c2p(const char *infile, const char *outfile, double gamma_eos)
   return 0;
 }
 
+/*
+ * fixup mode: the primitives p[N1M][N2M][N3M][NPR] (ghosts included) and the
+ * inversion codes of the active zones (int32 [N1][N2][N3]) are read from
+ * <infile>; then bound_prim, post_fixup (fixup.c:37) and bound_prim run as
+ * in step_ch.c:132-157, and p and the full flag array are written.
+ */
+extern int post_fixup(int stageit, FTYPE (*pv)[N2M][N3M][NPR], FTYPE (*pbackup)[N2M][N3M][NPR], int finalstep);
+static int mode_fixup(const char *infile, const char *outdir, int use_weno, int finalstep)
+{
+  FILE *fi = fopen(infile, "rb");
+  int i, j, k, pl;
+  size_t np = (size_t)N1M * N2M * N3M * NPR;
+  int *fl = malloc(sizeof(int) * N1 * N2 * N3);
+  int *flout = malloc(sizeof(int) * N1M * N2M * N3M);
+  long shp_f[3] = {N1M, N2M, N3M};
+  if (!fi) { fprintf(stderr, "fixup: cannot open %s\n", infile); return 2; }
+  mkdir(outdir, 0755);
+  harness_init(use_weno);
+  if (fread(&p[-N1BND][-N2BND][-N3BND][0], sizeof(FTYPE), np, fi) != np) { fprintf(stderr, "fixup: short p\n"); return 2; }
+  if (fread(fl, sizeof(int), (size_t)N1 * N2 * N3, fi) != (size_t)N1 * N2 * N3) { fprintf(stderr, "fixup: short flags\n"); return 2; }
+  fclose(fi);
+  FULLLOOP pflag[i][j][k][FLAGUTOPRIMFAIL] = 0;
+  ZLOOP pflag[i][j][k][FLAGUTOPRIMFAIL] = fl[((long)i * N2 + j) * N3 + k];
+  if (bound_prim(STAGEM1, p) >= 1) { fprintf(stderr, "bound_prim failed\n"); return 3; }
+  dump_prims(outdir, "p_in_bounded", p);
+  if (post_fixup(STAGEM1, p, p, finalstep) >= 1) { fprintf(stderr, "post_fixup failed\n"); return 3; }
+  if (bound_prim(STAGEM1, p) >= 1) { fprintf(stderr, "bound_prim failed\n"); return 3; }
+  dump_prims(outdir, "p_out", p);
+  FULLLOOP flout[((long)(i + N1BND) * N2M + (j + N2BND)) * N3M + (k + N3BND)] = pflag[i][j][k][FLAGUTOPRIMFAIL];
+  write_npy(outdir, "pflag_out", "<i4", 3, shp_f, flout, sizeof(int));
+  (void)pl;
+  free(fl);
+  free(flout);
+  return 0;
+}
+
 int main(int argc, char **argv)
 {
   if (argc >= 5 && !strcmp(argv[1], "torus")) return mode_torus(argv[2], atoi(argv[3]), atoi(argv[4]));
   if (argc >= 5 && !strcmp(argv[1], "c2p")) return mode_c2p(argv[2], argv[3], atof(argv[4]));
   if (argc >= 4 && !strcmp(argv[1], "time")) return mode_time(atoi(argv[2]), atoi(argv[3]));
-  fprintf(stderr, "usage: %s torus <outdir> <weno> <nsteps> | c2p <in> <out> <gamma> | time <weno> <nsteps>\n", argv[0]);
+  if (argc >= 6 && !strcmp(argv[1], "fixup")) return mode_fixup(argv[2], argv[3], atoi(argv[4]), atoi(argv[5]));
+  fprintf(stderr, "usage: %s torus <outdir> <weno> <nsteps> | c2p <in> <out> <gamma> | time <weno> <nsteps> | fixup <in> <outdir> <weno> <finalstep>\n", argv[0]);
   return 1;
 }

@@ -205,9 +205,117 @@ def main_long(full):
         print(name, os.path.getsize(os.path.join(HERE, name)))
 
 
+FIXUP_CODES = np.array([101, 101, 101, 2, 3, 4, 9, 10, 11, 14, 15], np.int32)
+
+
+def plant_fixup(base, rng):
+    """A torus state with planted inversion failures and fast, hot zones for
+    post_fixup: returns (flat indices, new values) of the primitive entries
+    changed and the inversion codes of the active zones.  Planted: random
+    failures (4%) with every code the inversion or a fixup can leave, failed
+    clusters (2x2, and 3x3 whose centre has no good neighbour), failures on
+    the boundary rows, zones with gamma in [3, 8] (alone, in rows along j,
+    in 2x2 patches, next to failures), and hot zones (u x 30) among them."""
+    m3 = base.shape[2]
+    g = np.load(os.path.join(HERE, "t2d_mc.npz" if m3 == 1 else "t3d_weno.npz"))
+    n1 = int(g["params"][0]); n2 = int(g["params"][1]); n3 = int(g["params"][2]); nb = int(g["params"][3])
+    b3 = nb if n3 > 1 else 0
+    gcov11 = g["gcov"][:, :, 6, 1, 1]  # CENT
+    p = base.copy()
+    flags = np.zeros((n1, n2, n3), np.int32)
+    nz = n1 * n2 * n3
+    pick = rng.choice(nz, size=max(4, nz // 25), replace=False)
+    flags.flat[pick] = rng.choice(FIXUP_CODES, size=pick.size)
+    ks = [0, n3 - 1, n3 // 2] if n3 > 1 else [0]
+    q = lambda f, n: int(f * n)  # planted positions scale with the grid
+    for k in ks:
+        flags[q(.12, n1):q(.12, n1) + 2, q(.31, n2):q(.31, n2) + 2, k] = 101       # 2x2 cluster
+        flags[q(.31, n1):q(.31, n1) + 3, q(.62, n2):q(.62, n2) + 3, k] = 101       # 3x3 cluster
+        flags[0, q(.1, n2), k] = 101                                               # boundary rows
+        flags[n1 - 1, q(.16, n2), k] = 2
+        flags[q(.22, n1), 0, k] = 101
+        flags[q(.28, n1), n2 - 1, k] = 15
+    fast = []
+    for k in ks:
+        a, b = q(.62, n1), q(.19, n2)
+        fast += [(a, b, k), (a + 1, b, k), (a, b + 1, k), (a + 1, b + 1, k)]        # 2x2 patch
+        fast += [(q(.78, n1), j, k) for j in range(q(.37, n2), q(.37, n2) + 4)]     # row along j
+        fast += [(q(.47, n1), q(.81, n2), k), (q(.09, n1), q(.25, n2), k),          # alone, at the
+                 (n1 - 1, n2 - 1, k), (q(.12, n1) + 1, q(.31, n2) + 1, k)]          # edge, in a cluster
+    gam_t = rng.uniform(3.0, 8.0, size=len(fast))
+    for (i, j, k), G in zip(fast, gam_t):
+        ii, jj, kk = i + nb, j + nb, k + b3
+        p[ii, jj, kk, 2] = np.sqrt(G * G - 1.0) / np.sqrt(gcov11[ii, jj])
+        p[ii, jj, kk, 3] = 0.0
+        p[ii, jj, kk, 4] = 0.0
+        if (i, j) != (q(.12, n1) + 1, q(.31, n2) + 1):
+            flags[i, j, k] = 0
+    for (i, j, k) in fast[::3]:
+        p[i + nb, j + nb, k + b3, 1] *= 30.0
+    i0, j0 = q(.47, n1), q(.81, n2)
+    flags[i0 - 1, j0, ks[0]] = 101                           # a failed (i-1) neighbour of a fast zone
+    flags[i0, j0 + 1, ks[-1]] = 101                          # a failed (i, j+1) neighbour: no votes
+    # an order-dependent pair: A = (i, j) with gamma 8 is flagged by the
+    # check; B = (i, j+1) with gamma 3 then has exactly 4 votes, where
+    # counting A's vote (as if A were still good) would give 5 and flag it.
+    # C = (i-1, j), D = (i-1, j+1), E = (i+1, j+1) have gamma 3.
+    ic, jc = q(.5, n1), q(.5, n2)
+    for k in ks:
+        flags[ic - 3:ic + 4, jc - 3:jc + 5, k] = 0
+        for (i, j, G) in ((ic, jc, 8.0), (ic, jc + 1, 3.0), (ic - 1, jc, 3.0), (ic - 1, jc + 1, 3.0),
+                          (ic + 1, jc + 1, 3.0)):
+            ii, jj, kk = i + nb, j + nb, k + b3
+            p[ii, jj, kk, 2] = np.sqrt(G * G - 1.0) / np.sqrt(gcov11[ii, jj])
+            p[ii, jj, kk, 3] = 0.0
+            p[ii, jj, kk, 4] = 0.0
+    diff = np.flatnonzero(p.view(np.int64) != base.view(np.int64))
+    return diff, p.reshape(-1)[diff], flags
+
+
+def run_fixup(harness, p, flags, weno, finalstep):
+    with tempfile.TemporaryDirectory() as tmp:
+        fin = os.path.join(tmp, "in.bin")
+        with open(fin, "wb") as f:
+            f.write(np.ascontiguousarray(p, "<f8").tobytes())
+            f.write(np.ascontiguousarray(flags, "<i4").tobytes())
+        out = os.path.join(tmp, "out")
+        subprocess.run([harness, "fixup", fin, out, str(weno), str(finalstep)], cwd=tmp, check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        return np.load(os.path.join(out, "p_out.npy")), np.load(os.path.join(out, "pflag_out.npy"))
+
+
+def make_fixup(tag, base_name, seed):
+    """post_fixup on a planted state (see plant_fixup), both substeps: the
+    changes against the input state are stored sparsely."""
+    base = np.load(os.path.join(HERE, base_name))["p_init"]
+    rng = np.random.default_rng(seed)
+    idx, val, flags = plant_fixup(base, rng)
+    p_in = base.copy().reshape(-1)
+    p_in[idx] = val
+    p_in = p_in.reshape(base.shape)
+    out = dict(base=np.array(base_name), mod_idx=idx, mod_val=val, flags=flags, seed=np.int64(seed))
+    for fs in (0, 1):
+        po, fo = run_fixup(os.path.join(REFDIR, "harness_" + tag), p_in, flags, 0, fs)
+        ch = np.flatnonzero(po.view(np.int64) != p_in.view(np.int64))
+        out["fs%d_idx" % fs] = ch
+        out["fs%d_val" % fs] = po.reshape(-1)[ch]
+        out["fs%d_pflag" % fs] = fo.astype(np.int8)
+        print(tag, "finalstep", fs, "changed", ch.size, "flags", np.unique(fo, return_counts=True))
+    return out
+
+
+def main_fixup():
+    for tag, (n1, n2, n3) in {"t2d": (32, 32, 1), "t3d": (16, 16, 8)}.items():
+        subprocess.run([BUILD, tag, str(n1), str(n2), str(n3)], check=True, stdout=subprocess.DEVNULL)
+    np.savez_compressed(os.path.join(HERE, "fixup_t2d.npz"), **make_fixup("t2d", "t2d_mc.npz", 31))
+    np.savez_compressed(os.path.join(HERE, "fixup_t3d.npz"), **make_fixup("t3d", "t3d_weno.npz", 32))
+
+
 def main():
     if "--long" in sys.argv:
         return main_long("--full" in sys.argv)
+    if "--fixup" in sys.argv:
+        return main_fixup()
     builds = {"t2d": (32, 32, 1), "t3d": (16, 16, 8)}
     for tag, (n1, n2, n3) in builds.items():
         subprocess.run([BUILD, tag, str(n1), str(n2), str(n3)], check=True, stdout=subprocess.DEVNULL)
