@@ -51,6 +51,8 @@ def profile_summary(prof, zones):
         v["mean_ms"] = v["total_ms"] / v["launches"]
         v["achieved_GBs"] = v["bytes_per_zone"] * zones / (v["mean_ms"] * 1e-3) / 1e9
     out["bound"] = dict(total_ms=prof["bound"][0], launches=prof["bound"][1])
+    if "fixup" in prof:
+        out["fixup"] = dict(total_ms=prof["fixup"][0], launches=prof["fixup"][1])
     return out
 
 
@@ -131,7 +133,111 @@ def c2p_bench(n, reps=3):
     return res
 
 
-def c2p_cpu_baseline(n=1 << 16):
+MLP_FLOP = 2 * (8 * 64 + 64 * 64 + 64 * 1)  # multiply-adds of one forward pass, x2
+BF16_DENSE_TFLOPS = 2500.0                  # MI355X dense BF16 matrix peak
+
+
+def _timed(fn, reps):
+    import torch
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        r = fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    return r, best
+
+
+def mlp_bench(n_train, n_eval, epochs=8, reps=3):
+    """Learned Newton start: a network trained on n_train inversions of the
+    benchmark distribution (gam 4/3) that the plain FP64 path solved, then
+    n_eval held-out inversions solved plain (FP64), from the learned start,
+    and mixed precision; plus the matrix-core forward pass alone."""
+    import torch
+    from harm_finalwham_amd import harmgpu
+    gam = 4.0 / 3.0
+    res = {"n_train": n_train, "n_eval": n_eval, "epochs": epochs, "gam": gam}
+
+    def solve(b, mode=None, mlp=None):
+        pr = b["p_guess"].clone()
+        n = pr.shape[1]
+        pflag = torch.zeros(n, dtype=torch.int32, device="cuda")
+        iters = torch.zeros_like(pflag)
+
+        def run():
+            pr.copy_(b["p_guess"])
+            if mlp is None:
+                harmgpu.utoprim_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], b["gcovpert"], pr, pflag, iters,
+                                      gam=gam, mode=mode)
+                return None
+            return harmgpu.utoprim_batch_learned(mlp, b["U"], b["gcov"], b["gcon"], b["gdet"], b["gcovpert"], pr,
+                                                 pflag, iters, gam=gam)
+        cnt, ms = _timed(run, reps)
+        return pr, pflag, iters, cnt, ms
+
+    tr = harmgpu.make_c2p_batch(n_train, gam=gam, seed=4096)
+    p64, f64, _, _, _ = solve(tr, harmgpu.HG_C2P_FP64)
+    X, y, ok = harmgpu.mlp_features(tr["U"], tr["gcov"], tr["gcon"], tr["gdet"], tr["p_guess"], p_solved=p64,
+                                    gam=gam)
+    sel = (ok == 1) & (f64 == 0)
+    X, y = X[sel].contiguous(), y[sel].contiguous()
+    del tr, p64, f64
+    mu, sd = X.mean(0), X.std(0)
+    sd[sd < 1e-6] = 1.0
+    m = harmgpu.Mlp(seed=7)
+    m.set_normalisation(mu.cpu().numpy(), sd.cpu().numpy())
+    perm = torch.randperm(X.shape[0], device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+    X, y = X[perm].contiguous(), y[perm].contiguous()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = []
+    for ep in range(epochs):
+        hist.append(m.train(X, y, epochs=1, batch=2048, lr=2e-3 if ep < epochs // 2 else 5e-4))
+    torch.cuda.synchronize()
+    res["train_s"] = time.perf_counter() - t0
+    res["train_samples"] = int(X.shape[0])
+    res["loss_by_epoch"] = hist
+    del X, y
+
+    ev = harmgpu.make_c2p_batch(n_eval, gam=gam, seed=8192)
+    p64, f64, i64, _, t64 = solve(ev, harmgpu.HG_C2P_FP64)
+    pl, fl, il, cnt, tl = solve(ev, mlp=m)
+    _, fmx, imx, _, tmx = solve(ev, harmgpu.HG_C2P_MIXED)
+    Xe, ye, oke = harmgpu.mlp_features(ev["U"], ev["gcov"], ev["gcon"], ev["gdet"], ev["p_guess"], p_solved=p64,
+                                       gam=gam)
+    yp, tf = _timed(lambda: m.forward(Xe), reps)
+    s = (oke == 1) & (f64 == 0)
+    okz = f64 == 0
+    d = (pl - p64)[:, okz].abs()
+    ref = p64[:, okz].abs()
+    n = n_eval
+    res.update({
+        "fp64_ms": t64, "learned_ms": tl, "mixed_ms": tmx,
+        "fp64_solves_per_s": n / (t64 * 1e-3), "learned_solves_per_s": n / (tl * 1e-3),
+        "learned_speedup_vs_fp64": t64 / tl,
+        "flags_identical": bool(torch.equal(f64, fl)) and bool(torch.equal(f64, fmx)),
+        "learned_starts": cnt[0], "fallbacks": cnt[1],
+        "max_err_vs_fp64": [float((d[0] / ref[0]).max()), float((d[1] / (ref[0] + gam * ref[1])).max()),
+                            float((d[2:5].amax(dim=0) / ref[2:5].amax(dim=0)).max())],
+        "rms_logW_error_plain_guess": float(ye[s].pow(2).mean().sqrt()),
+        "rms_logW_error_learned": float((yp - ye)[s].pow(2).mean().sqrt()),
+        "evaluations_per_zone": {"fp64": float(i64.double().mean()), "learned": float(il.double().mean()),
+                                 "mixed": float(imx.double().mean())},
+        "wavefront_max_evaluations": {"fp64": float(i64.view(-1, 64).amax(1).double().mean()),
+                                      "learned": float(il.view(-1, 64).amax(1).double().mean())},
+    })
+    tflops = MLP_FLOP * n / (tf * 1e-3) / 1e12
+    res["forward"] = {"ms": tf, "inferences_per_s": n / (tf * 1e-3), "flop_per_inference": MLP_FLOP,
+                      "roofline": {"bound": "mfma", "achieved": tflops, "peak": BF16_DENSE_TFLOPS,
+                                   "unit": "TFLOP/s", "frac": tflops / BF16_DENSE_TFLOPS}}
+    return res
+
+
+def c2p_cpu_baseline(n=1 << 22):
     """The oracle's Noble 2D inversion (C, gcc -O2, one thread; bit-identical
     to the reference's Utoprimgen) on n inversions of the same distribution."""
     import numpy as np
@@ -227,6 +333,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--c2p-n", type=int, default=1 << 24, help="inversions per gam for the con2prim leg (0: skip)")
     ap.add_argument("--batch-tori", type=int, default=48, help="tori of the batched-graph leg (0: skip)")
+    ap.add_argument("--mlp-train-n", type=int, default=1 << 22, help="training inversions of the learned start")
+    ap.add_argument("--mlp-eval-n", type=int, default=1 << 24, help="held-out inversions (0: skip the leg)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -282,6 +390,12 @@ def main():
         if not args.no_cpu_baseline:
             c2p["cpu_baseline"] = c2p_cpu_baseline()
         print(f"# con2prim: {json.dumps(c2p)}", file=sys.stderr, flush=True)
+    learned = None
+    if args.mlp_eval_n > 0:
+        if args.c2p_n <= 0:
+            g.close()
+        learned = mlp_bench(args.mlp_train_n, args.mlp_eval_n)
+        print(f"# learned start: {json.dumps(learned)}", file=sys.stderr, flush=True)
     batched = None
     if args.batch_tori > 0:
         batched = batched_bench(args.batch_tori)
@@ -308,6 +422,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v["total_ms"] / args.steps for k, v in summ.items()},
         "con2prim": c2p,
+        "learned_start": learned,
         "batched_tori": batched,
         "failed_inversions": nf.value,
         "newton_iterations_per_zone_update": it.value / (2.0 * zones * args.steps),

@@ -85,6 +85,7 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
     if (e == hipSuccess) e = hipMemset(c->err, 0, sizeof(int));
     if (e != hipSuccess) { set_error("hg_ctx_create: %s", hipGetErrorString(e)); rc = 1; }
   }
+  if (!rc) rc = fixup_alloc(c);
   if (rc) { hg_ctx_destroy(c); return 1; }
   *out = c;
   return 0;
@@ -99,6 +100,7 @@ extern "C" int hg_ctx_destroy(hg_ctx *c) {
   if (c->ndt_bits) (void)hipFree(c->ndt_bits);
   if (c->counters) (void)hipFree(c->counters);
   if (c->err) (void)hipFree(c->err);
+  fixup_free(c);
   for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
   delete c;
   return 0;
@@ -200,7 +202,8 @@ static int check_err(hg_ctx *c) {
   HIPCHECK(hipMemcpy(&e, c->err, sizeof(int), hipMemcpyDeviceToHost));
   if (e) {
     HIPCHECK(hipMemset(c->err, 0, sizeof(int)));
-    set_error("device reported failure mask %d (1: ghost fill, 2: face state, 4: zone update)", e);
+    set_error("device reported failure mask %d (1: ghost fill, 2: face state, 4: zone update, 8: post_fixup "
+              "gamma, 16: post_fixup check did not settle, 32: unknown failure code)", e);
     return 1;
   }
   return 0;
@@ -215,6 +218,48 @@ static int need_geometry(hg_ctx *c, const char *who) {
 extern "C" int hg_bound_prim(hg_ctx *c, void *stream) {
   if (need_geometry(c, "hg_bound_prim")) return 1;
   hipStream_t s = (hipStream_t)stream;
+  launch_bound(c->p, c->geo, c->d, c->cfg.gammamax, c->err, s);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s));
+  return check_err(c);
+}
+
+// the failure codes of the active zones, host order [i][j][k] <-> device int8
+static int codes_to_device(hg_ctx *c, const int32_t *h) {
+  const Dims &d = c->d;
+  std::vector<int8_t> buf((size_t)d.spl, 0);
+  for (int i = 0; i < d.n1; i++)
+    for (int j = 0; j < d.n2; j++)
+      for (int k = 0; k < d.n3; k++) {
+        int32_t v = h[((size_t)i * d.n2 + j) * d.n3 + k];
+        if (v < -128 || v > 127) {
+          set_error("hg_post_fixup: failure code %d out of range", (int)v);
+          return 1;
+        }
+        buf[d.idx(i, j, k)] = (int8_t)v;
+      }
+  HIPCHECK(hipMemcpy(c->pflag, buf.data(), buf.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int hg_get_pflag(hg_ctx *c, int32_t *h) {
+  if (!c || !h) { set_error("hg_get_pflag: null argument"); return 1; }
+  const Dims &d = c->d;
+  std::vector<int8_t> buf((size_t)d.spl);
+  HIPCHECK(hipMemcpy(buf.data(), c->pflag, buf.size(), hipMemcpyDeviceToHost));
+  for (int i = 0; i < d.n1; i++)
+    for (int j = 0; j < d.n2; j++)
+      for (int k = 0; k < d.n3; k++) h[((size_t)i * d.n2 + j) * d.n3 + k] = buf[d.idx(i, j, k)];
+  return 0;
+}
+
+extern "C" int hg_post_fixup(hg_ctx *c, const int32_t *codes, int finalstep, void *stream) {
+  if (need_geometry(c, "hg_post_fixup")) return 1;
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHECK(hipStreamSynchronize(s));
+  if (codes && codes_to_device(c, codes)) return 1;
+  launch_bound(c->p, c->geo, c->d, c->cfg.gammamax, c->err, s);
+  launch_post_fixup(c, c->p, finalstep ? 1 : 0, s);
   launch_bound(c->p, c->geo, c->d, c->cfg.gammamax, c->err, s);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(s));
@@ -288,7 +333,7 @@ extern "C" int hg_profile_read(hg_ctx *c, double ms[HG_PROF_NCLASS], int64_t lau
 
 extern "C" const char *hg_profile_class_name(int k) {
   static const char *names[HG_PROF_NCLASS] = {"bound", "recon1", "recon2", "recon3", "flux1",
-                                              "flux2", "flux3", "emf",    "update"};
+                                              "flux2", "flux3", "emf",    "update", "fixup"};
   return (k >= 0 && k < HG_PROF_NCLASS) ? names[k] : "";
 }
 
@@ -390,21 +435,24 @@ extern "C" int hg_fluxcalc(hg_ctx *c, double *F1, double *F2, double *F3, double
   return 0;
 }
 
-// one substep of step_ch.c:94-166 (advance + bound_prim; post_fixup is a
-// no-op while no zone fails its inversion)
+// one substep of step_ch.c:94-166: advance, bound_prim, post_fixup, bound_prim
 static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, double cuf2, int finalstep,
                             hipStream_t s) {
   const hg_config &f = c->cfg;
   enqueue_fluxcalc(c, pb, s);
   prof_mark(c, HG_PROF_UPDATE, s);
   launch_update(pi, pb, pf, c->uf, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2], c->geo, c->conn,
-                c->floors, c->dt, cuf2, finalstep, c->d, f.gam, f.dx, f.gammamax, c->counters, c->err, c->tune, s);
+                c->floors, c->dt, cuf2, finalstep, c->d, f.gam, f.dx, f.gammamax, c->counters, c->pflag, c->err,
+                c->tune, s);
   if (finalstep) launch_dt(c->ndt_bits, c->dt, f.defcon, f.safe, s);
   // step_ch.c:127 and :166 call bound_prim twice per substep; the second call
   // is not a repeat of the first: POLEDEATH has zeroed U2 in the rows next to
   // the axis, which the radial inflow check then sees in the corner ghosts
   prof_mark(c, HG_PROF_BOUND, s);
   launch_bound(pf, c->geo, c->d, f.gammamax, c->err, s);
+  prof_mark(c, HG_PROF_FIXUP, s);
+  launch_post_fixup(c, pf, finalstep, s);
+  prof_mark(c, HG_PROF_BOUND, s);
   launch_bound(pf, c->geo, c->d, f.gammamax, c->err, s);
   prof_mark(c, -1, s);
 }

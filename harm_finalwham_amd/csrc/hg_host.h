@@ -22,6 +22,14 @@ struct hg_ctx {
   double *dt = nullptr;
   unsigned long long *ndt_bits = nullptr, *counters = nullptr;
   int *err = nullptr;
+  // post_fixup: inversion codes and checked codes (int8, layout of one
+  // primitive), candidate and fix lists (hg_fixup.hip)
+  int8_t *pflag = nullptr, *pchk = nullptr;
+  int *fx_cand_z = nullptr, *fx_fix_z = nullptr;
+  uint16_t *fx_cand_bits = nullptr;
+  int8_t *fx_cand_new = nullptr, *fx_fix_flag = nullptr;
+  double *fx_fix_v = nullptr;
+  unsigned *fx_count = nullptr;
   bool have_geometry = false;
   hg::Tune tune;                             // launch shapes (hg_autotune)
   // modified Kerr-Schild grid parameters when the geometry was built by
@@ -49,6 +57,11 @@ int copy_prims(hg_ctx *c, double *dev, const double *host);
 void enqueue_step(hg_ctx *c, hipStream_t s);
 int fetch_prims(hg_ctx *c, const double *dev, double *host);
 // returns 1 (with the error set) when the work counter cannot be set up
+// post_fixup buffers (hg_fixup.hip)
+int fixup_alloc(hg_ctx *c);
+void fixup_free(hg_ctx *c);
+// fixup_checksolution + fixup_utoprim on bounded primitives p (kernels only)
+void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s);
 int launch_c2p_batch_mixed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
                            const double *gdet, const double *gcovpert, double *pr, int *pflag, int *iters,
                            hipStream_t s);

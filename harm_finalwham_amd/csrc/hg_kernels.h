@@ -31,7 +31,7 @@ void launch_update(const double *pi, const double *pb, double *pf, double *ubuf,
                    const double *F2, const double *F3, const double *e1, const double *e2, const double *e3,
                    const double *geo, const double *conn, const double *floors, const double *dtp, double cuf2,
                    int finalstep, const Dims &d, double gam, const double *dx, double gammamax,
-                   unsigned long long *counters, int *err, const Tune &tu, hipStream_t s);
+                   unsigned long long *counters, int8_t *pflag, int *err, const Tune &tu, hipStream_t s);
 void launch_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe, hipStream_t s);
 void launch_reset_ndt(unsigned long long *ndt_bits, hipStream_t s);
 void launch_c2p_batch(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,

@@ -517,7 +517,8 @@ __global__ void __launch_bounds__(256) k_invert(const double *__restrict__ ubuf,
                                                 double *__restrict__ pf, const double *__restrict__ geo,
                                                 const double *__restrict__ floors, int finalstep, Dims d,
                                                 double gam, double gammamax,
-                                                unsigned long long *__restrict__ counters, int *__restrict__ err) {
+                                                unsigned long long *__restrict__ counters,
+                                                int8_t *__restrict__ pflag, int *__restrict__ err) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
   unsigned long long nfail = 0, nit = 0;
@@ -539,6 +540,7 @@ __global__ void __launch_bounds__(256) k_invert(const double *__restrict__ ubuf,
     bool bad = false;
     int iters = 0;
     int flag = utoprimgen_fp64(Uf, G, gam, pr, iters);
+    pflag[z] = (int8_t)flag;
     nit = (unsigned long long)iters;
     nfail = flag ? 1ull : 0ull;
     if (finalstep) {
@@ -660,12 +662,12 @@ void launch_update(const double *pi, const double *pb, double *pf, double *ubuf,
                    const double *F2, const double *F3, const double *e1, const double *e2, const double *e3,
                    const double *geo, const double *conn, const double *floors, const double *dtp, double cuf2,
                    int finalstep, const Dims &d, double gam, const double *dx, double gammamax,
-                   unsigned long long *counters, int *err, const Tune &tu, hipStream_t s) {
+                   unsigned long long *counters, int8_t *pflag, int *err, const Tune &tu, hipStream_t s) {
   int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
   hipLaunchKernelGGL(k_update, dim3(nblocks(n, tu.bs_update)), dim3(tu.bs_update), 0, s, pi, pb, ubuf, F1, F2, F3, e1, e2, e3, geo,
                      conn, dtp, cuf2, d, gam, 1.0 / dx[1], 1.0 / dx[2], 1.0 / dx[3], err);
   hipLaunchKernelGGL(k_invert, dim3(nblocks(n, tu.bs_invert)), dim3(tu.bs_invert), 0, s, ubuf, pb, pf, geo, floors, finalstep, d,
-                     gam, gammamax, counters, err);
+                     gam, gammamax, counters, pflag, err);
 }
 
 void launch_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe, hipStream_t s) {

@@ -45,17 +45,19 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // ---------------------------------------------------------------------------
 // features of one inversion from its FP64 set-up (scale free: ratios to D)
 // ---------------------------------------------------------------------------
+// (ratios formed in FP64, logarithms in FP32: the network sees ~16 bits)
 __device__ __forceinline__ void features(const C2PInv &c, double W_last, double alpha, double gam, float *x) {
   const double tiny = 1e-300;
   double D = c.D;
   double E = fmax(-c.Qdotn, tiny);
-  x[0] = (float)log(E / D);                                   // energy per rest mass
-  x[1] = (float)log1p(c.Qtsq / (E * E));                      // momentum^2 / energy^2
-  x[2] = (float)log1p(c.Bsq / D);                             // magnetisation
+  double iD = 1.0 / D;
+  x[0] = logf((float)(E * iD));                               // energy per rest mass
+  x[1] = log1pf((float)(c.Qtsq / (E * E)));                   // momentum^2 / energy^2
+  x[2] = log1pf((float)(c.Bsq * iD));                         // magnetisation
   x[3] = (float)(c.QdotBsq / (c.Bsq * c.Qtsq + tiny));        // cos^2 (S, B)
-  x[4] = (float)log10(D);                                     // scale
+  x[4] = log10f((float)D);                                    // scale
   x[5] = (float)alpha;                                        // lapse
-  x[6] = (float)log(fmax(W_last, tiny) / D);                  // the plain guess
+  x[6] = logf((float)(fmax(W_last, tiny) * iD));              // the plain guess
   x[7] = (float)gam;                                          // adiabatic index
 }
 
@@ -143,11 +145,11 @@ __global__ void __launch_bounds__(256) k_mlp_features(int64_t n, double gam, con
 //   A (16 x 32):  A[r][8q + j], j = 0..7
 //   B (32 x 16):  B[8q + j][r]
 //   C (16 x 16):  C[4q + i][r], i = 0..3
-// Each layer input v enters as a pair of BF16 numbers hi = bf16(v),
-// lo = bf16(v - hi) against the same BF16 weights in both halves of K, so
+// The network input x enters as a pair of BF16 numbers hi = bf16(x),
+// lo = bf16(x - hi) against the same BF16 weights in both halves of K, so
 // the product keeps ~16 bits of the input (the weights are BF16, the
-// accumulation FP32).  Layer 1: K = 32 = [x_hi(8) x_lo(8) 0(16)];
-// layer 2: K = 128 = [h_hi(64) h_lo(64)].
+// accumulation FP32): layer 1 has K = 32 = [x_hi(8) x_lo(8) 0(16)].  The
+// hidden activations enter layer 2 as BF16 (K = 64).
 // ---------------------------------------------------------------------------
 struct MlpDev {
   const float *w;       // FP32 parameters (biases, W3, normalisation)
@@ -156,6 +158,13 @@ struct MlpDev {
 };
 
 __device__ __forceinline__ __bf16 bf_hi(float v) { return (__bf16)v; }
+
+// tanh as 1 - 2 / (exp(2x) + 1) with the hardware exponential and
+// reciprocal (absolute error ~1e-7; saturates to +-1 without overflow)
+__device__ __forceinline__ float act(float x) {
+  float e = __expf(2.0f * x);
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
 __device__ __forceinline__ __bf16 bf_lo(float v) { return (__bf16)(v - (float)(__bf16)v); }
 
 // 16 samples: xin[16][8] (normalised) -> y[16]; h[16][64] float LDS scratch
@@ -176,22 +185,19 @@ __device__ __forceinline__ void forward16(const MlpDev &m, const float (*xin)[NI
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < 4; i++) h[4 * q + i][16 * nt + r] = tanhf(acc[i] + m.w[OFF_B1 + 16 * nt + r]);
+    for (int i = 0; i < 4; i++) h[4 * q + i][16 * nt + r] = act(acc[i] + m.w[OFF_B1 + 16 * nt + r]);
   }
   __syncthreads();
-  // layer 2: K steps 0,1 take h_hi, steps 2,3 h_lo
+  // layer 2: K = 64 in two steps, BF16 inputs
   f32x4 acc2[4];
 #pragma unroll
   for (int nt = 0; nt < 4; nt++) acc2[nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int ks = 0; ks < 4; ks++) {
-    const int kb = 32 * (ks & 1) + 8 * q;
+  for (int ks = 0; ks < 2; ks++) {
+    const int kb = 32 * ks + 8 * q;
     bf16x8 a2;
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      float v = h[r][kb + j];
-      a2[j] = (ks < 2) ? bf_hi(v) : bf_lo(v);
-    }
+    for (int j = 0; j < 8; j++) a2[j] = bf_hi(h[r][kb + j]);
 #pragma unroll
     for (int nt = 0; nt < 4; nt++) {
       bf16x8 b2;
@@ -204,7 +210,7 @@ __device__ __forceinline__ void forward16(const MlpDev &m, const float (*xin)[NI
 #pragma unroll
   for (int nt = 0; nt < 4; nt++)
 #pragma unroll
-    for (int i = 0; i < 4; i++) h[4 * q + i][16 * nt + r] = tanhf(acc2[nt][i] + m.w[OFF_B2 + 16 * nt + r]);
+    for (int i = 0; i < 4; i++) h[4 * q + i][16 * nt + r] = act(acc2[nt][i] + m.w[OFF_B2 + 16 * nt + r]);
   __syncthreads();
   // layer 3 on the vector units: lanes 0..15 one sample each
   if (l < 16) {
@@ -240,9 +246,9 @@ __global__ void __launch_bounds__(64) k_mlp_forward(int64_t n, const float *__re
 // ---------------------------------------------------------------------------
 // backward pass, 64 samples per block of one wavefront, FP32 on the vector
 // units.  The forward is recomputed from the same BF16-rounded weights and
-// hi + lo layer inputs the matrix-core forward uses (the weight rounding is
-// treated as the identity when differentiating: the gradient is that of the
-// FP32 master weights).  Activations are staged in LDS (row stride 65 floats, so both
+// layer inputs the matrix-core forward uses (hi + lo input, BF16 hidden
+// activations; the roundings are treated as the identity when
+// differentiating: the gradient is that of the FP32 master weights).  Activations are staged in LDS (row stride 65 floats, so both
 // per-sample rows and per-unit columns are free of bank conflicts):
 //   A  thread = sample: x, h1, h2, dy
 //   B  thread = unit j: dW3[j], dz2[s][j] in place of h2, db2[j]
@@ -267,30 +273,30 @@ __global__ void __launch_bounds__(64) k_mlp_backward(int64_t n, const float *__r
   __shared__ float dys[64];
   const int l = threadIdx.x;
   const int64_t t = (int64_t)blockIdx.x * 64 + l;
-  const bool act = t < n;
+  const bool live = t < n;
   // A: forward of sample l
   {
     float x[NIN];
 #pragma unroll
     for (int i = 0; i < NIN; i++) {
-      x[i] = act ? eff((X[t * NIN + i] - w[OFF_MU + i]) / w[OFF_SD + i]) : 0.f;
+      x[i] = live ? eff((X[t * NIN + i] - w[OFF_MU + i]) / w[OFF_SD + i]) : 0.f;
       xs[l][i] = x[i];
     }
     for (int j = 0; j < NH; j++) {
       float s = 0.f;
 #pragma unroll
       for (int i = 0; i < NIN; i++) s += x[i] * bfr(w[OFF_W1 + i * NH + j]);
-      h1s[l][j] = tanhf(s + w[OFF_B1 + j]);
+      h1s[l][j] = act(s + w[OFF_B1 + j]);
     }
     float yp = w[OFF_B3];
     for (int j = 0; j < NH; j++) {
       float s = 0.f;
-      for (int k = 0; k < NH; k++) s += eff(h1s[l][k]) * bfr(w[OFF_W2 + k * NH + j]);
-      float h2 = tanhf(s + w[OFF_B2 + j]);
+      for (int k = 0; k < NH; k++) s += bfr(h1s[l][k]) * bfr(w[OFF_W2 + k * NH + j]);
+      float h2 = act(s + w[OFF_B2 + j]);
       h2s[l][j] = h2;
       yp += h2 * w[OFF_W3 + j];
     }
-    float r = act ? yp - Y[t] : 0.f;
+    float r = live ? yp - Y[t] : 0.f;
     dys[l] = 2.f * r * inv_n;
     float lsum = r * r * loss_scale;
     for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
@@ -337,7 +343,7 @@ __global__ void __launch_bounds__(64) k_mlp_backward(int64_t n, const float *__r
     for (int i = 0; i < NIN; i++) g1[i] = 0.f;
     float gb1 = 0.f;
     for (int s = 0; s < 64; s++) {
-      float hk = eff(h1s[s][k]);
+      float hk = bfr(h1s[s][k]);
 #pragma unroll
       for (int j = 0; j < NH; j++) g2[j] += hk * h2s[s][j];
       float dz1 = d1s[s][k];
@@ -428,13 +434,13 @@ __global__ void __launch_bounds__(64) k_c2p_learned(int64_t n, double gam, const
   __shared__ float yo[64];
   const int l = threadIdx.x;
   const int64_t t = (int64_t)blockIdx.x * 64 + l;
-  const bool act = t < n;
+  const bool live = t < n;
   Geom G;
   double u[NPR], p[NPR], Ug[NPR];
   C2PInv c;
   double W_last = 1., alpha = 1.;
   int code = 1;
-  if (act) {
+  if (live) {
     load_zone(n, t, U, gcov, gcon, gdet, pr, G, u, p);
     G.gpert0 = gcovpert[t];
     code = setup_zone(u, G, gam, p, c, W_last, alpha, Ug);
@@ -451,7 +457,7 @@ __global__ void __launch_bounds__(64) k_c2p_learned(int64_t n, double gam, const
     forward16(m, (const float(*)[NIN]) & xin[16 * g16][0], h, &yo[16 * g16]);
     __syncthreads();
   }
-  if (!act) return;
+  if (!live) return;
   const double igr = (gam - 1.0) / gam;
   const double gsq_big = GAMMAFAIL * GAMMAFAIL;
   const double vsq_big = (gsq_big - 1.0) / (gsq_big);
@@ -465,7 +471,7 @@ __global__ void __launch_bounds__(64) k_c2p_learned(int64_t n, double gam, const
     double dp0, dp1, fp, dfp;
     newton_dx(c, igr, xp0, xp1, dp0, dp1, fp, dfp);
     // the learned start and its first step
-    double Wm = W_last * exp((double)yo[l]);
+    double Wm = W_last * (double)__expf(yo[l]);
     double xm0 = Wm, vm = vsq_of_W(c, Wm);
     double xm1 = (vm > 1.) ? vsq_big : vm;
     double dm0 = 0., dm1 = 0., fm = 0., dfm = 0.;

@@ -24,8 +24,8 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_graph_create", "hg_graph_launch", "hg_graph_destroy", "hg_get_tune", "hg_set_tune", "hg_autotune",
            "hg_mlp_create", "hg_mlp_destroy", "hg_mlp_nparams", "hg_mlp_get_params", "hg_mlp_set_params",
            "hg_mlp_features", "hg_mlp_forward", "hg_mlp_gradient", "hg_mlp_train",
-           "hg_utoprim_batch_learned"]
-HG_PROF_NCLASS = 9
+           "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag"]
+HG_PROF_NCLASS = 10
 
 
 class HarmError(RuntimeError):
@@ -92,6 +92,8 @@ def lib():
     L.hg_set_tune.argtypes = [vp, C.POINTER(HgTune)]
     L.hg_autotune.argtypes = [vp, C.c_int, C.POINTER(HgTune), vp]
     L.hg_profile.argtypes = [vp, C.c_int]
+    L.hg_post_fixup.argtypes = [vp, C.POINTER(C.c_int32), C.c_int, vp]
+    L.hg_get_pflag.argtypes = [vp, C.POINTER(C.c_int32)]
     fp = C.POINTER(C.c_float)
     L.hg_mlp_create.argtypes = [C.c_uint64, C.POINTER(vp)]
     L.hg_mlp_destroy.argtypes = [vp]
@@ -136,6 +138,7 @@ class Grid:
         h = C.c_void_p()
         _check(lib().hg_ctx_create(C.byref(cfg), C.byref(h)))
         self.h = h
+        self.n1, self.n2, self.n3 = n1, n2, n3
         nb1, nb2, nb3 = (nb if n > 1 else 0 for n in (n1, n2, n3))
         self.shape = (n1 + 2 * nb1, n2 + 2 * nb2, n3 + 2 * nb3, 8)
 
@@ -180,6 +183,24 @@ class Grid:
 
     def bound_prim(self):
         _check(lib().hg_bound_prim(self.h, None))
+
+    def post_fixup(self, codes=None, finalstep=1):
+        """bound_prim, post_fixup (fixup.c:37), bound_prim on the current
+        primitives; codes: int array [n1][n2][n3] of inversion failure codes
+        (None: those of the last inversion)."""
+        ptr = None
+        if codes is not None:
+            codes = np.ascontiguousarray(codes, dtype=np.int32)
+            if codes.shape != (self.n1, self.n2, self.n3):
+                raise ValueError("codes must have shape (n1, n2, n3)")
+            ptr = codes.ctypes.data_as(C.POINTER(C.c_int32))
+        _check(lib().hg_post_fixup(self.h, ptr, int(finalstep), None))
+
+    def get_pflag(self):
+        """Failure codes of the active zones [n1][n2][n3] (FIXED = -1)."""
+        out = np.zeros((self.n1, self.n2, self.n3), np.int32)
+        _check(lib().hg_get_pflag(self.h, out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
 
     def fluxcalc(self):
         F = [np.zeros(self.shape) for _ in range(3)]

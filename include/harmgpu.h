@@ -128,6 +128,18 @@ int hg_set_dt(hg_ctx *ctx, double dt);
 int hg_get_dt(hg_ctx *ctx, double *dt);
 int hg_get_counters(hg_ctx *ctx, int64_t *nfail, int64_t *iters);
 
+/* post_fixup (fixup.c:37) with the bound_prim calls around it
+   (step_ch.c:132-157), on the context's primitives: bound_prim, the
+   neighbour check of good inversions (fixup_checksolution, fixup.c:460),
+   the neighbour average of failed ones (fixup_utoprim, fixup.c:669),
+   bound_prim.  hg_step runs it in every substep.  codes (host, may be NULL:
+   the codes of the last inversion) are failure codes of the active zones,
+   [i][j][k] over n1 x n2 x n3; finalstep as in the reference (1 on the last
+   RK substep).  hg_get_pflag returns the active zones' codes after the
+   last post_fixup or inversion (FIXED = -1 for repaired zones). */
+int hg_post_fixup(hg_ctx *ctx, const int32_t *codes, int finalstep, void *stream);
+int hg_get_pflag(hg_ctx *ctx, int32_t *codes);
+
 /* Batched Noble-2D inversion on device memory (utoprimgen.c:23).  All
    arrays are structure-of-arrays over n zones:
      U[8][n] evolved conserved variables (with gdet), gcov[16][n], gcon[16][n],
@@ -244,8 +256,9 @@ const char *hg_last_error(void);
 #define HG_PROF_RECON1 1 /* WENO5BND interface states, direction 1 (2, 3 follow) */
 #define HG_PROF_FLUX1 4  /* Riemann flux, direction 1 (MC: fused reconstruction) */
 #define HG_PROF_EMF 7
-#define HG_PROF_UPDATE 8 /* update + source + inversion + fixup (+ dt) */
-#define HG_PROF_NCLASS 9
+#define HG_PROF_UPDATE 8 /* update + source + inversion + fixup1zone (+ dt) */
+#define HG_PROF_FIXUP 9  /* post_fixup: neighbour check and average of failed zones */
+#define HG_PROF_NCLASS 10
 int hg_profile(hg_ctx *ctx, int enable);
 int hg_profile_read(hg_ctx *ctx, double ms[HG_PROF_NCLASS], int64_t launches[HG_PROF_NCLASS]);
 const char *hg_profile_class_name(int k);

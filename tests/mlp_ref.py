@@ -3,8 +3,9 @@
 The network is the one the library runs (8 -> 64 -> 64 -> 1, tanh), with
 the same parameter block layout (W1[8][64] b1[64] W2[64][64] b2[64] W3[64]
 b3 mu[8] sd[8]).  With quant=True it applies the library's number formats:
-hidden-layer weights rounded to BF16 (round to nearest even), layer inputs
-carried as hi + lo BF16 pairs (hi = bf16(v), lo = bf16(v - hi)); the
+hidden-layer weights rounded to BF16 (round to nearest even), the network
+input carried as a hi + lo BF16 pair (hi = bf16(x), lo = bf16(x - hi)), the
+first layer's activations rounded to BF16 before the second layer; the
 gradient treats these roundings as the identity, as the library does.  With
 quant=False it is the plain FP64 network, which the finite-difference check
 pins.
@@ -52,7 +53,7 @@ def forward(w, X, quant=True):
         x = (np.asarray(X, np.float64) - p["mu"]) / p["sd"]
         W1, W2 = p["W1"], p["W2"]
     h1 = np.tanh(x @ W1 + p["b1"])
-    h1in = hilo(h1.astype(np.float32)).astype(np.float64) if quant else h1
+    h1in = bf16(h1.astype(np.float32)).astype(np.float64) if quant else h1
     h2 = np.tanh(h1in @ W2 + p["b2"])
     y = h2 @ p["W3"] + p["b3"]
     return y, dict(x=x, h1=h1, h1in=h1in, h2=h2, W2=W2)
