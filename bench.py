@@ -56,6 +56,34 @@ def profile_summary(prof, zones):
     return out
 
 
+# device kernels of each logical kernel (WENO5BND build)
+DEVICE_KERNELS = {"recon+flux1": ("hg::k_weno_x", "hg::k_flux_lr<1>"),
+                  "recon+flux2": ("hg::k_weno_m<2>", "hg::k_flux_lr<2>"),
+                  "recon+flux3": ("hg::k_weno_m<3>", "hg::k_flux_lr<3>"),
+                  "emf": ("hg::k_emf",), "update": ("hg::k_update", "hg::k_invert")}
+TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "hbm_traffic_torus3d_256x128x64_weno_hll.md")
+
+
+def traffic_per_launch(logical):
+    """HBM bytes per launch of a logical kernel at 256x128x64 from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against a
+    plain copy); None when the summary or a row is missing."""
+    if not os.path.exists(TRAFFIC_PROFILE) or logical not in DEVICE_KERNELS:
+        return None
+    rows = {}
+    for line in open(TRAFFIC_PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0].startswith("`"):
+            try:
+                rows[cells[0].strip("`")] = float(cells[4]) * 1e6
+            except ValueError:
+                pass
+    try:
+        return sum(rows[k] for k in DEVICE_KERNELS[logical])
+    except KeyError:
+        return None
+
+
 def cpu_baseline(timeout_s=240):
     """Reference C build (oracle/_ref/harness_b3d, gcc -O2, one core) on the
     same torus with phi reduced to 8 zones: 256x128x8 = 262144 zones, 1 step."""
@@ -335,7 +363,18 @@ def main():
     ap.add_argument("--batch-tori", type=int, default=48, help="tori of the batched-graph leg (0: skip)")
     ap.add_argument("--mlp-train-n", type=int, default=1 << 22, help="training inversions of the learned start")
     ap.add_argument("--mlp-eval-n", type=int, default=1 << 24, help="held-out inversions (0: skip the leg)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copied {a.numel() * 8} bytes three times", flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
@@ -417,7 +456,10 @@ def main():
         "config": {"workload": f"3-D magnetised torus {n1}x{n2}x{n3}, WENO5BND + HLL + flux-CT + Noble 2D, RK2",
                    "zones": zones, "global_batch": 1, "seq_len": 0, "parallelism": "none"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["achieved_GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dk["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": dk["achieved_GBs"] / HBM_PEAK_GBS,
+                     "traffic": traffic_per_launch(dom) if (n1, n2, n3) == (256, 128, 64) else None,
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, HERE),
+                     "algorithmic_bytes_per_launch": dk["bytes_per_zone"] * zones,
                      "bytes_per_zone": dk["bytes_per_zone"], "mean_launch_ms": dk["mean_ms"]},
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v["total_ms"] / args.steps for k, v in summ.items()},
