@@ -101,6 +101,7 @@ extern "C" int hg_ctx_destroy(hg_ctx *c) {
   if (c->counters) (void)hipFree(c->counters);
   if (c->err) (void)hipFree(c->err);
   fixup_free(c);
+  diag_free(c);
   for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
   delete c;
   return 0;
@@ -342,6 +343,8 @@ extern "C" const char *hg_profile_class_name(int k) {
 static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
   const Dims &d = c->d;
   const hg_config &f = c->cfg;
+  c->flux_state = pb;
+  c->dg_faces_valid = false;
   int nn[4] = {0, d.n1, d.n2, d.n3};
   launch_reset_ndt(c->ndt_bits, s);
   if (f.lim != HG_LIM_MC) {
@@ -440,6 +443,11 @@ static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, doubl
                             hipStream_t s) {
   const hg_config &f = c->cfg;
   enqueue_fluxcalc(c, pb, s);
+  if (finalstep && c->dg_record) {
+    // diag_flux of advance.c:130, where the reference's p is still pi
+    launch_diag_faces(c, pb, pi, s);
+    c->dg_faces_valid = true;
+  }
   prof_mark(c, HG_PROF_UPDATE, s);
   launch_update(pi, pb, pf, c->uf, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2], c->geo, c->conn,
                 c->floors, c->dt, cuf2, finalstep, c->d, f.gam, f.dx, f.gammamax, c->counters, c->pflag, c->err,

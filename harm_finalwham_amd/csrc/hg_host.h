@@ -30,6 +30,12 @@ struct hg_ctx {
   int8_t *fx_cand_new = nullptr, *fx_fix_flag = nullptr;
   double *fx_fix_v = nullptr;
   unsigned *fx_count = nullptr;
+  // diagnostics (hg_diag.hip): scratch on first use, time-integrated
+  // boundary fluxes (diag.c pcum) of the global region
+  double *dg_buf = nullptr;
+  double pcum[6][8] = {};
+  const double *flux_state = nullptr;        // the primitives the last fluxcalc read
+  bool dg_record = false, dg_faces_valid = false;
   bool have_geometry = false;
   hg::Tune tune;                             // launch shapes (hg_autotune)
   // modified Kerr-Schild grid parameters when the geometry was built by
@@ -60,6 +66,9 @@ int fetch_prims(hg_ctx *c, const double *dev, double *host);
 // post_fixup buffers (hg_fixup.hip)
 int fixup_alloc(hg_ctx *c);
 void fixup_free(hg_ctx *c);
+void diag_free(hg_ctx *c);
+int diag_alloc(hg_ctx *c);
+void launch_diag_faces(hg_ctx *c, const double *prim, const double *pglob, hipStream_t s);
 // fixup_checksolution + fixup_utoprim on bounded primitives p (kernels only)
 void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s);
 int launch_c2p_batch_mixed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,

@@ -24,7 +24,7 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_graph_create", "hg_graph_launch", "hg_graph_destroy", "hg_get_tune", "hg_set_tune", "hg_autotune",
            "hg_mlp_create", "hg_mlp_destroy", "hg_mlp_nparams", "hg_mlp_get_params", "hg_mlp_set_params",
            "hg_mlp_features", "hg_mlp_forward", "hg_mlp_gradient", "hg_mlp_train",
-           "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag"]
+           "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag", "hg_diagnostics", "hg_diag_record"]
 HG_PROF_NCLASS = 10
 
 
@@ -38,6 +38,22 @@ class HgTune(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class HgDiag(C.Structure):
+    _fields_ = [("cons", C.c_double * 8), ("divbmax", C.c_double), ("divbavg", C.c_double),
+                ("divbmax_ijk", C.c_int32 * 3), ("pdot", C.c_double * 8 * 6),
+                ("pdotterms", C.c_double * 8 * 7 * 6), ("pdottermsjet2", C.c_double * 8 * 7 * 6),
+                ("pcum", C.c_double * 8 * 6)]
+
+    def as_dict(self):
+        import numpy as np
+        return {"cons": np.array(self.cons[:]), "divbmax": self.divbmax, "divbavg": self.divbavg,
+                "divbmax_ijk": tuple(self.divbmax_ijk[:]),
+                "pdot": np.ctypeslib.as_array(self.pdot).copy(),
+                "pdotterms": np.ctypeslib.as_array(self.pdotterms).copy(),
+                "pdottermsjet2": np.ctypeslib.as_array(self.pdottermsjet2).copy(),
+                "pcum": np.ctypeslib.as_array(self.pcum).copy()}
 
 
 class HgConfig(C.Structure):
@@ -94,6 +110,8 @@ def lib():
     L.hg_profile.argtypes = [vp, C.c_int]
     L.hg_post_fixup.argtypes = [vp, C.POINTER(C.c_int32), C.c_int, vp]
     L.hg_get_pflag.argtypes = [vp, C.POINTER(C.c_int32)]
+    L.hg_diagnostics.argtypes = [vp, C.c_double, C.POINTER(HgDiag), vp]
+    L.hg_diag_record.argtypes = [vp, C.c_int]
     fp = C.POINTER(C.c_float)
     L.hg_mlp_create.argtypes = [C.c_uint64, C.POINTER(vp)]
     L.hg_mlp_destroy.argtypes = [vp]
@@ -195,6 +213,17 @@ class Grid:
                 raise ValueError("codes must have shape (n1, n2, n3)")
             ptr = codes.ctypes.data_as(C.POINTER(C.c_int32))
         _check(lib().hg_post_fixup(self.h, ptr, int(finalstep), None))
+
+    def diagnostics(self, Dt=0.0):
+        """constotal, divbmaxavg and diag_flux of the global region (see
+        hg_diagnostics); Dt is added to the running flux integral pcum."""
+        out = HgDiag()
+        _check(lib().hg_diagnostics(self.h, float(Dt), C.byref(out), None))
+        return out.as_dict()
+
+    def diag_record(self, on=True):
+        """Record diag_flux's boundary terms in every step (hg_diag_record)."""
+        _check(lib().hg_diag_record(self.h, int(on)))
 
     def get_pflag(self):
         """Failure codes of the active zones [n1][n2][n3] (FIXED = -1)."""

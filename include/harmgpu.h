@@ -140,6 +140,46 @@ int hg_get_counters(hg_ctx *ctx, int64_t *nfail, int64_t *iters);
 int hg_post_fixup(hg_ctx *ctx, const int32_t *codes, int finalstep, void *stream);
 int hg_get_pflag(hg_ctx *ctx, int32_t *codes);
 
+/* Diagnostics of the global energy region on the context's state, computed
+   on the device (dump_ener.c / diag.c of the reference):
+     cons[pl]        constotal (dump_ener.c:596): sum over the active zones of
+                     U[pl] dx1 dx2 dx3 (U in the evolved form, with gdet)
+     divbmax/avg     divbmaxavg (dump_ener.c:482): max and mean over the
+                     active zones of the normalised flux-CT |div B|, and the
+                     (i,j,k) of the first maximum
+     pdot[dir][pl]   diag_flux (diag.c:734) as advance.c:130 calls it: flux
+                     through each boundary (dir = X1UP, X1DN, X2UP, X2DN,
+                     X3UP, X3DN; zero where the dimension is flat) from the
+                     last fluxcalc (after hg_step: its final substep, on the
+                     RK midpoint state), field fluxes as flux-CT leaves them
+     pdotterms       its term-by-term split from the state that fluxcalc
+                     read, [dir][term 0..6][pl], and pdottermsjet2 the same
+                     over the jet boundary zones (b^2/rho > 3.162 at X1DN,
+                     outgoing unbound at X1UP)
+     pcum[dir][pl]   the context's running sum of pdot Dt (diag.c:1002).
+                     The reference adds Cunew[1] dt per substep, which is 0
+                     in both substeps of its RK2 midpoint scheme, so its pcum
+                     stays 0; pass Dt = 0 to follow it.
+   Sums are taken in the reference's loop order (bit-identical to its
+   double sums); div B carries a few ulp, the reference normalising it in
+   x87 extended precision. */
+typedef struct {
+  double cons[8];
+  double divbmax, divbavg;
+  int32_t divbmax_ijk[3];
+  double pdot[6][8];
+  double pdotterms[6][7][8];
+  double pdottermsjet2[6][7][8];
+  double pcum[6][8];
+} hg_diag;
+int hg_diagnostics(hg_ctx *ctx, double Dt, hg_diag *out, void *stream);
+/* Two of diag_flux's terms read the state at the start of the step, which a
+   step overwrites.  hg_diag_record(ctx, 1) makes every step record the
+   boundary terms where the reference computes them, so that hg_diagnostics
+   after hg_step reproduces diag_flux exactly; without it hg_diagnostics
+   takes those two terms from the current state. */
+int hg_diag_record(hg_ctx *ctx, int on);
+
 /* Batched Noble-2D inversion on device memory (utoprimgen.c:23).  All
    arrays are structure-of-arrays over n zones:
      U[8][n] evolved conserved variables (with gdet), gcov[16][n], gcon[16][n],

@@ -24,6 +24,8 @@
  *   fixup <infile> <outdir> <weno 0|1> <finalstep>
  *       bound_prim + post_fixup + bound_prim on a given state and given
  *       inversion codes (see mode_fixup).
+ *   diag <outdir> <weno 0|1> <nsteps>
+ *       constotal, divbmaxavg and diag_flux after <nsteps> steps (mode_diag).
  *
  * The initialisation below follows init() in initbase.c:5-176 call for call;
  * the only additions are the parameter overrides marked OVERRIDE.
@@ -442,12 +444,53 @@ static int mode_fixup(const char *infile, const char *outdir, int use_weno, int 
   return 0;
 }
 
+/*
+ * diag mode: the torus over <nsteps> RK2 steps (step_ch() calls diag_flux()
+ * after each step, step_ch.c:221), then constotal() and divbmaxavg() of the
+ * final state; writes them with the global region's pdot, pdotterms,
+ * pdottermsjet2 and pcum (diag.c:734) and the final state.
+ */
+extern int constotal(int enerregion, SFTYPE *vars);
+extern void divbmaxavg(FTYPE p[][N2M][N3M][NPR], SFTYPE *ptrdivbmax, SFTYPE *ptrdivbavg);
+static int mode_diag(const char *outdir, int use_weno, int nsteps)
+{
+  SFTYPE vars[NPR], dmax, davg, dv[2];
+  long s1[1] = {NPR}, s2[1] = {2}, sp[2] = {COMPDIM * 2, NPR}, st[3] = {COMPDIM * 2, NUMFLUXTERMS, NPR};
+  long shp1[1];
+  FTYPE *dts = malloc(sizeof(FTYPE) * (nsteps + 1));
+  int n;
+  mkdir(outdir, 0755);
+  harness_init(use_weno);
+  dts[0] = dt;
+  for (n = 1; n <= nsteps; n++) {
+    step_ch();
+    nstep++;
+    dts[n] = dt;
+  }
+  if (constotal(0, vars) >= 1) { fprintf(stderr, "constotal failed\n"); return 3; }
+  divbmaxavg(p, &dmax, &davg);
+  dv[0] = dmax;
+  dv[1] = davg;
+  write_npy(outdir, "cons", "<f8", 1, s1, vars, sizeof(SFTYPE));
+  write_npy(outdir, "divb", "<f8", 1, s2, dv, sizeof(SFTYPE));
+  write_npy(outdir, "pdot", "<f8", 2, sp, &pdotreg[0][0][0], sizeof(SFTYPE));
+  write_npy(outdir, "pdotterms", "<f8", 3, st, &pdottermsreg[0][0][0][0], sizeof(SFTYPE));
+  write_npy(outdir, "pdottermsjet2", "<f8", 3, st, &pdottermsjet2[0][0][0], sizeof(SFTYPE));
+  write_npy(outdir, "pcum", "<f8", 2, sp, &pcumreg[0][0][0], sizeof(SFTYPE));
+  shp1[0] = nsteps + 1;
+  write_npy(outdir, "dt_seq", "<f8", 1, shp1, dts, sizeof(FTYPE));
+  dump_prims(outdir, "p_final", p);
+  free(dts);
+  return 0;
+}
+
 int main(int argc, char **argv)
 {
   if (argc >= 5 && !strcmp(argv[1], "torus")) return mode_torus(argv[2], atoi(argv[3]), atoi(argv[4]));
   if (argc >= 5 && !strcmp(argv[1], "c2p")) return mode_c2p(argv[2], argv[3], atof(argv[4]));
   if (argc >= 4 && !strcmp(argv[1], "time")) return mode_time(atoi(argv[2]), atoi(argv[3]));
   if (argc >= 6 && !strcmp(argv[1], "fixup")) return mode_fixup(argv[2], argv[3], atoi(argv[4]), atoi(argv[5]));
+  if (argc >= 5 && !strcmp(argv[1], "diag")) return mode_diag(argv[2], atoi(argv[3]), atoi(argv[4]));
   fprintf(stderr, "usage: %s torus <outdir> <weno> <nsteps> | c2p <in> <out> <gamma> | time <weno> <nsteps> | fixup <in> <outdir> <weno> <finalstep>\n", argv[0]);
   return 1;
 }

@@ -311,7 +311,31 @@ def main_fixup():
     np.savez_compressed(os.path.join(HERE, "fixup_t3d.npz"), **make_fixup("t3d", "t3d_weno.npz", 32))
 
 
+def run_diag(harness, weno, nsteps):
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "out")
+        subprocess.run([harness, "diag", out, str(weno), str(nsteps)], cwd=tmp, check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        d = {f[:-4]: np.load(os.path.join(out, f)) for f in os.listdir(out) if f.endswith(".npy")}
+    d["digest_final"] = np.array(state_digest(d.pop("p_final")))
+    d["weno"] = np.int32(weno)
+    d["nsteps"] = np.int32(nsteps)
+    return d
+
+
+def main_diag():
+    """constotal / divbmaxavg / diag_flux of the reference after a few steps
+    (the torus fixtures' grids): diag_t2d (MC, 5 steps), diag_t3d (WENO5BND,
+    2 steps)."""
+    for tag, (n1, n2, n3) in {"t2d": (32, 32, 1), "t3d": (16, 16, 8)}.items():
+        subprocess.run([BUILD, tag, str(n1), str(n2), str(n3)], check=True, stdout=subprocess.DEVNULL)
+    np.savez_compressed(os.path.join(HERE, "diag_t2d.npz"), **run_diag(os.path.join(REFDIR, "harness_t2d"), 0, 5))
+    np.savez_compressed(os.path.join(HERE, "diag_t3d.npz"), **run_diag(os.path.join(REFDIR, "harness_t3d"), 1, 2))
+
+
 def main():
+    if "--diag" in sys.argv:
+        return main_diag()
     if "--long" in sys.argv:
         return main_long("--full" in sys.argv)
     if "--fixup" in sys.argv:
