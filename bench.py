@@ -359,6 +359,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--zones", default="256,128,64", help="n1,n2,n3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--c2p-n", type=int, default=1 << 24, help="inversions per gam for the con2prim leg (0: skip)")
     ap.add_argument("--batch-tori", type=int, default=48, help="tori of the batched-graph leg (0: skip)")
     ap.add_argument("--mlp-train-n", type=int, default=1 << 22, help="training inversions of the learned start")
@@ -392,6 +393,10 @@ def main():
     dt0 = g.init_torus(rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0, randfact=4e-2, seed=7)
     t_setup = time.perf_counter() - t_setup
     print(f"# set-up {t_setup:.1f} s, dt0 = {dt0:.6g}", file=sys.stderr, flush=True)
+    # launch shapes for this grid (hg_autotune times a few steps from the
+    # current state and restores it; shapes never change a result bit)
+    shapes = (g.tune() if args.no_autotune else g.autotune(2)).as_dict()
+    print(f"# launch shapes {json.dumps(shapes)}", file=sys.stderr, flush=True)
 
     g.step_async(args.warmup)
     torch.cuda.synchronize()
@@ -463,6 +468,7 @@ def main():
                      "bytes_per_zone": dk["bytes_per_zone"], "mean_launch_ms": dk["mean_ms"]},
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v["total_ms"] / args.steps for k, v in summ.items()},
+        "launch_shapes": shapes,
         "con2prim": c2p,
         "learned_start": learned,
         "batched_tori": batched,

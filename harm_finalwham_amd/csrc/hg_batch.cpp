@@ -142,8 +142,9 @@ static int check_tune(const hg_tune *t) {
     set_error("hg_set_tune: weno_xwaves must be 1, 2 or 4");
     return 1;
   }
-  if (t->weno_nseg < 0 || t->weno_nseg > 64) {
-    set_error("hg_set_tune: weno_nseg must be in [0, 64]");
+  if (t->weno_nseg < 0 || t->weno_nseg > 64 || t->weno_nseg3 < 0 || t->weno_nseg3 > 64 || t->weno_xmarch < 0 ||
+      t->weno_xmarch > 64) {
+    set_error("hg_set_tune: weno_nseg, weno_nseg3 and weno_xmarch must be in [0, 64]");
     return 1;
   }
   return 0;
@@ -156,6 +157,8 @@ static void to_c(const Tune &a, hg_tune *t) {
   t->bs_invert = a.bs_invert;
   t->weno_xwaves = a.weno_xwaves;
   t->weno_nseg = a.weno_nseg;
+  t->weno_nseg3 = a.weno_nseg3;
+  t->weno_xmarch = a.weno_xmarch;
 }
 
 static void from_c(const hg_tune *t, Tune &a) {
@@ -165,6 +168,8 @@ static void from_c(const hg_tune *t, Tune &a) {
   a.bs_invert = t->bs_invert;
   a.weno_xwaves = t->weno_xwaves;
   a.weno_nseg = t->weno_nseg;
+  a.weno_nseg3 = t->weno_nseg3;
+  a.weno_xmarch = t->weno_xmarch;
 }
 
 extern "C" int hg_get_tune(const hg_ctx *c, hg_tune *t) {
@@ -235,8 +240,10 @@ extern "C" int hg_autotune(hg_ctx *c, int nsteps, hg_tune *best, void *stream) {
     };
     std::vector<Axis> axes = {{&Tune::bs_flux, {64, 128, 256}},   {&Tune::bs_update, {64, 128, 256}},
                               {&Tune::bs_invert, {64, 128, 256}}, {&Tune::bs_emf, {128, 256}},
-                              {&Tune::weno_xwaves, {1, 2, 4}},    {&Tune::weno_nseg, {1, 2, 4}}};
+                              {&Tune::weno_xwaves, {1, 2, 4}},    {&Tune::weno_nseg, {1, 2, 3, 4, 6}},
+                              {&Tune::weno_xmarch, {0, 2, 4}}, {&Tune::weno_nseg3, {1, 2, 3, 4}}};
     if (c->cfg.lim == HG_LIM_MC) axes.resize(4);
+    else if (c->d.n3 == 1) axes.pop_back();  // no reconstruction along k
     for (const Axis &ax : axes) {
       if (rc) break;
       for (int v : ax.values) {

@@ -13,7 +13,8 @@ namespace hg {
 // reconstruction (0: chosen from the line length)
 struct Tune {
   int bs_flux = 256, bs_emf = 256, bs_update = 256, bs_invert = 256;
-  int weno_xwaves = 4, weno_nseg = 0;
+  int weno_xwaves = 4, weno_nseg = 0, weno_nseg3 = 0;  // segments per line along j, along k
+  int weno_xmarch = 0;  // along i: 0 = one lane per point (shuffles), n > 0 = marching, n segments
 };
 
 void launch_bound(double *p, const double *geo, const Dims &d, double gammamax, int *err, hipStream_t s);

@@ -399,26 +399,30 @@ __global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, co
 // ---------------------------------------------------------------------------
 // along j (DIR 2) or k (DIR 3): lane = (i, transverse index) flattened with i
 // fastest; each lane marches over its segment of the line: at step t it
-// evaluates C(t), D(t-2) and E(t-3), keeping the last values in registers
+// evaluates C(t), D(t-2) and E(t-3), keeping the last values in registers.
+// DIR 1 marches along i with lane = (j, k): no halo lanes, but every load of
+// a wavefront touches 64 rows (the rows stay in cache over the next steps).
 // ---------------------------------------------------------------------------
 template <int DIR>
 __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, const double *__restrict__ aux,
                                                 double *__restrict__ pleft, double *__restrict__ pright, Dims d,
                                                 int nseg, int seglen) {
-  const int n = (DIR == 2) ? d.n2 : d.n3;
-  const int s1 = d.n1 > 1;
-  const int so = (DIR == 2) ? (d.n3 > 1) : (d.n2 > 1);
-  const int ci = d.n1 + 2 * s1, co = ((DIR == 2) ? d.n3 : d.n2) + 2 * so;
+  const int n = (DIR == 1) ? d.n1 : (DIR == 2) ? d.n2 : d.n3;
+  // the lane's two transverse indices: a (fastest) and o
+  const int sa = (DIR == 1) ? (d.n2 > 1) : (d.n1 > 1);
+  const int so = (DIR == 2) ? (d.n3 > 1) : (DIR == 3) ? (d.n2 > 1) : (d.n3 > 1);
+  const int ca = ((DIR == 1) ? d.n2 : d.n1) + 2 * sa;
+  const int co = ((DIR == 2) ? d.n3 : (DIR == 3) ? d.n2 : d.n3) + 2 * so;
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= (int64_t)ci * co) return;
-  const int i = -s1 + (int)(g % ci);
-  const int o = -so + (int)(g / ci);
+  if (g >= (int64_t)ca * co) return;
+  const int a_ = -sa + (int)(g % ca);
+  const int o = -so + (int)(g / ca);
   const int seg = blockIdx.y % nseg;
   const int pl = blockIdx.y / nseg;
   const int e0 = -1 + seg * seglen;
   const int e1 = (e0 + seglen - 1 < n) ? e0 + seglen - 1 : n;
-  const int64_t zl = (DIR == 2) ? d.idx(i, 0, o) : d.idx(i, o, 0);
-  const int64_t sd = (DIR == 2) ? d.sj : d.sk;
+  const int64_t zl = (DIR == 1) ? d.idx(0, a_, o) : (DIR == 2) ? d.idx(a_, 0, o) : d.idx(a_, o, 0);
+  const int64_t sd = (DIR == 1) ? 1 : (DIR == 2) ? d.sj : d.sk;
   const double *yl = p + (int64_t)pl * d.spl + zl;
   const double *rl = p + (int64_t)RHO * d.spl + zl;
   const double *a0 = aux + zl, *a1 = aux + d.spl + zl;
@@ -486,22 +490,26 @@ void launch_weno_prep(const double *p, const double *geo, double *aux, const Dim
 void launch_weno(int dir, const double *p, const double *aux, double *pleft, double *pright, const Dims &d,
                  const Tune &tu, hipStream_t s) {
   int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
-  if (dir == 1) {
+  if (dir == 1 && tu.weno_xmarch == 0) {
     int nseg = (d.n1 + 2 + XOUT - 1) / XOUT;
     int64_t nw = (int64_t)nseg * NPR * (d.n2 + 2 * s2) * (d.n3 + 2 * s3);
     int wpb = tu.weno_xwaves;
     hipLaunchKernelGGL(k_weno_x, dim3((unsigned)((nw + wpb - 1) / wpb)), dim3(64 * wpb), 0, s, p, aux, pleft,
                        pright, d, nseg);
   } else {
-    int n = (dir == 2) ? d.n2 : d.n3;
-    int64_t lanes = (int64_t)(d.n1 + 2 * s1) * ((dir == 2) ? d.n3 + 2 * s3 : d.n2 + 2 * s2);
+    int n = (dir == 1) ? d.n1 : (dir == 2) ? d.n2 : d.n3;
+    int64_t lanes = (dir == 1) ? (int64_t)(d.n2 + 2 * s2) * (d.n3 + 2 * s3)
+                               : (int64_t)(d.n1 + 2 * s1) * ((dir == 2) ? d.n3 + 2 * s3 : d.n2 + 2 * s2);
     // split long lines so that the grid holds enough wavefronts
-    int nseg = tu.weno_nseg > 0 ? tu.weno_nseg : ((n + 2 >= 96) ? 2 : 1);
+    int want = (dir == 1) ? tu.weno_xmarch : (dir == 2) ? tu.weno_nseg : tu.weno_nseg3;
+    int nseg = want > 0 ? want : ((n + 2 >= 96) ? 2 : 1);
     if (nseg > n + 2) nseg = n + 2;
     int seglen = (n + 2 + nseg - 1) / nseg;
     nseg = (n + 2 + seglen - 1) / seglen;  // every segment starts at or before point n
     dim3 grid((unsigned)((lanes + 255) / 256), (unsigned)(NPR * nseg));
-    if (dir == 2)
+    if (dir == 1)
+      hipLaunchKernelGGL(k_weno_m<1>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen);
+    else if (dir == 2)
       hipLaunchKernelGGL(k_weno_m<2>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen);
     else
       hipLaunchKernelGGL(k_weno_m<3>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen);

@@ -34,7 +34,8 @@ class HarmError(RuntimeError):
 
 class HgTune(C.Structure):
     _fields_ = [("bs_flux", C.c_int), ("bs_emf", C.c_int), ("bs_update", C.c_int), ("bs_invert", C.c_int),
-                ("weno_xwaves", C.c_int), ("weno_nseg", C.c_int)]
+                ("weno_xwaves", C.c_int), ("weno_nseg", C.c_int), ("weno_nseg3", C.c_int),
+                ("weno_xmarch", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -74,6 +75,15 @@ def lib():
         return _lib
     if not os.path.exists(LIBPATH):
         raise HarmError("libharmgpu.so not built: run python -m harm_finalwham_amd.build")
+    # this binding hands torch device tensors to the library: torch's HIP
+    # runtime must come up before the library's first HIP call, or torch
+    # finds no device afterwards
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except ImportError:
+        pass
     L = C.CDLL(LIBPATH)
     d = C.POINTER(C.c_double)
     vp = C.c_void_p

@@ -271,14 +271,17 @@ int hg_graph_destroy(hg_graph *g);
 
 /* Launch shapes of the step (no effect on results): block sizes of the flux,
    EMF, update and inversion kernels (64, 128 or 256), wavefronts per block of
-   the i-direction reconstruction (1, 2 or 4) and segments per line of the j/k
-   reconstruction (0 = by line length).  hg_autotune times nsteps steps per
+   the i-direction reconstruction (1, 2 or 4), segments per line of the j
+   and of the k reconstruction (0 = by line length), and the scheme along i
+   (weno_xmarch 0: one lane per point with wavefront shuffles; n > 0: lanes
+   across j and k marching along i in n segments).  hg_autotune times nsteps steps per
    candidate (coordinate descent) from the current state, restores the state,
    dt and counters, keeps the fastest shapes and returns them in *best (may be
    NULL).  No counterpart in the reference. */
 typedef struct {
   int bs_flux, bs_emf, bs_update, bs_invert;
-  int weno_xwaves, weno_nseg;
+  int weno_xwaves, weno_nseg, weno_nseg3;
+  int weno_xmarch;
 } hg_tune;
 int hg_get_tune(const hg_ctx *ctx, hg_tune *t);
 int hg_set_tune(hg_ctx *ctx, const hg_tune *t);

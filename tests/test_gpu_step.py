@@ -62,3 +62,23 @@ def test_steps_match_reference(name):
             assert np.array_equal(p1, d["p_step1"]), np.abs(p1 - d["p_step1"]).max()
     p = g.get_prims()
     assert np.array_equal(p, d["p_final"]), np.abs(p - d["p_final"]).max()
+
+
+@pytest.mark.parametrize("name", ["t2d_weno", "t3d_weno"])
+@pytest.mark.parametrize("shape", [dict(weno_xmarch=1), dict(weno_xmarch=3, weno_nseg=3, weno_nseg3=2),
+                                   dict(weno_xwaves=1, weno_nseg=6, weno_nseg3=4, bs_flux=64, bs_update=128)])
+def test_launch_shapes_keep_every_bit(name, shape):
+    """Every launch shape of the tuner, the i-direction marching scheme
+    included, gives the reference's bits."""
+    d = load(name)
+    g = make_grid(d)
+    t = g.tune()
+    for k, v in shape.items():
+        setattr(t, k, v)
+    g.set_tune(t)
+    g.set_prims(d["p_init"])
+    dt = float(d["dt_seq"][0])
+    for n in range(1, int(d["nsteps"]) + 1):
+        dt, nf, it = g.step_ch(dt)
+        assert dt == d["dt_seq"][n]
+    assert np.array_equal(g.get_prims(), d["p_final"])
