@@ -73,7 +73,9 @@ extern "C" int hg_graph_create(hg_ctx *const *ctxs, int n, int nsteps, hg_graph 
       HIPTRY(hipEventRecord(fork, cap));
       for (int i = 0; i < n && !rc; i++) {
         HIPTRY(hipStreamWaitEvent(st[i], fork, 0));
+        ctxs[i]->capturing = true;
         for (int k = 0; k < nsteps; k++) enqueue_step(ctxs[i], st[i]);
+        ctxs[i]->capturing = false;
         HIPTRY(hipGetLastError());
         HIPTRY(hipEventRecord(join[i], st[i]));
         HIPTRY(hipStreamWaitEvent(cap, join[i], 0));
@@ -142,6 +144,10 @@ static int check_tune(const hg_tune *t) {
     set_error("hg_set_tune: weno_xwaves must be 1, 2 or 4");
     return 1;
   }
+  if (t->concurrent != 0 && t->concurrent != 1) {
+    set_error("hg_set_tune: concurrent must be 0 or 1");
+    return 1;
+  }
   if (t->weno_nseg < 0 || t->weno_nseg > 64 || t->weno_nseg3 < 0 || t->weno_nseg3 > 64 || t->weno_xmarch < 0 ||
       t->weno_xmarch > 64) {
     set_error("hg_set_tune: weno_nseg, weno_nseg3 and weno_xmarch must be in [0, 64]");
@@ -159,6 +165,7 @@ static void to_c(const Tune &a, hg_tune *t) {
   t->weno_nseg = a.weno_nseg;
   t->weno_nseg3 = a.weno_nseg3;
   t->weno_xmarch = a.weno_xmarch;
+  t->concurrent = a.concurrent;
 }
 
 static void from_c(const hg_tune *t, Tune &a) {
@@ -170,6 +177,7 @@ static void from_c(const hg_tune *t, Tune &a) {
   a.weno_nseg = t->weno_nseg;
   a.weno_nseg3 = t->weno_nseg3;
   a.weno_xmarch = t->weno_xmarch;
+  a.concurrent = t->concurrent;
 }
 
 extern "C" int hg_get_tune(const hg_ctx *c, hg_tune *t) {

@@ -70,6 +70,8 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
   if (cfg->lim == HG_LIM_WENO5BND) {
     rc |= dalloc(&c->pl, ns);
     rc |= dalloc(&c->pr, ns);
+    if (d.n2 > 1) { rc |= dalloc(&c->pl2, ns); rc |= dalloc(&c->pr2, ns); }
+    if (d.n3 > 1) { rc |= dalloc(&c->pl3, ns); rc |= dalloc(&c->pr3, ns); }
     rc |= dalloc(&c->aux, 2 * nz);
   }
   rc |= dalloc(&c->geo, (size_t)4 * GCOMP * d.gplane);
@@ -83,6 +85,11 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
     if (e == hipSuccess) e = hipMalloc((void **)&c->err, sizeof(int));
     if (e == hipSuccess) e = hipMemset(c->counters, 0, 2 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->err, 0, sizeof(int));
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {
+      e = hipStreamCreateWithFlags(&c->xs[k], hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join[k], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
     if (e != hipSuccess) { set_error("hg_ctx_create: %s", hipGetErrorString(e)); rc = 1; }
   }
   if (!rc) rc = fixup_alloc(c);
@@ -93,13 +100,19 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
 
 extern "C" int hg_ctx_destroy(hg_ctx *c) {
   if (!c) return 0;
-  double *bufs[] = {c->p, c->pk, c->uf, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2],
-                    c->pl, c->pr, c->aux, c->geo, c->corn, c->conn, c->floors, c->dt};
+  double *bufs[] = {c->p,      c->pk,     c->uf,     c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1],
+                    c->emf[2], c->pl,     c->pr,     c->pl2,  c->pr2,  c->pl3,  c->pr3,    c->aux,
+                    c->geo,    c->corn,   c->conn,   c->floors, c->dt};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (c->ndt_bits) (void)hipFree(c->ndt_bits);
   if (c->counters) (void)hipFree(c->counters);
   if (c->err) (void)hipFree(c->err);
+  for (int k = 0; k < 2; k++) {
+    if (c->xs[k]) (void)hipStreamDestroy(c->xs[k]);
+    if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   fixup_free(c);
   diag_free(c);
   for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
@@ -352,22 +365,48 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
     prof_mark(c, HG_PROF_RECON1, s);
     launch_weno_prep(pb, c->geo, c->aux, d, f.gam, s);
   }
-  for (int dir = 1; dir <= 3; dir++) {
+  // the directions are independent (own interface states, fluxes and ndt
+  // slot): j and k on the side streams, i on s, joined before the EMFs.
+  // A profile runs them one after another so that every interval holds one
+  // kernel class; so does a graph capture (the capture of this fork onto the
+  // side streams crashed the HIP runtime, so it is not used there).
+  const bool conc = c->tune.concurrent && !c->prof_on && !c->capturing;
+  hipStream_t ds[4] = {nullptr, s, s, s};
+  if (conc) {
+    (void)hipEventRecord(c->ev_fork, s);
+    for (int dir = 2; dir <= 3; dir++)
+      if (nn[dir] > 1) {
+        ds[dir] = c->xs[dir - 2];
+        (void)hipStreamWaitEvent(ds[dir], c->ev_fork, 0);
+      }
+  }
+  double *pls[4] = {nullptr, c->pl, c->pl2, c->pl3}, *prs[4] = {nullptr, c->pr, c->pr2, c->pr3};
+  // concurrent: the side streams' work first; a profile keeps the order 1, 2, 3
+  const int order[2][3] = {{1, 2, 3}, {3, 2, 1}};
+  for (int o = 0; o < 3; o++) {
+    const int dir = order[conc ? 1 : 0][o];
     if (nn[dir] <= 1) continue;
+    hipStream_t sd = ds[dir];
     if (f.lim == HG_LIM_MC) {
-      prof_mark(c, HG_PROF_FLUX1 + dir - 1, s);
+      prof_mark(c, HG_PROF_FLUX1 + dir - 1, sd);
       launch_flux_mc(dir, pb, c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,
-                     c->ndt_bits + dir - 1, c->err, c->tune, s);
-      prof_mark(c, -1, s);
+                     c->ndt_bits + dir - 1, c->err, c->tune, sd);
+      prof_mark(c, -1, sd);
     } else {
-      if (dir != 1) prof_mark(c, HG_PROF_RECON1 + dir - 1, s);
-      launch_weno(dir, pb, c->aux, c->pl, c->pr, d, c->tune, s);
-      prof_mark(c, HG_PROF_FLUX1 + dir - 1, s);
-      launch_flux_lr(dir, c->pl, c->pr, c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,
-                     c->ndt_bits + dir - 1, c->err, c->tune, s);
-      prof_mark(c, -1, s);
+      if (dir != 1) prof_mark(c, HG_PROF_RECON1 + dir - 1, sd);
+      launch_weno(dir, pb, c->aux, pls[dir], prs[dir], d, c->tune, sd);
+      prof_mark(c, HG_PROF_FLUX1 + dir - 1, sd);
+      launch_flux_lr(dir, pls[dir], prs[dir], c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,
+                     c->ndt_bits + dir - 1, c->err, c->tune, sd);
+      prof_mark(c, -1, sd);
     }
   }
+  if (conc)
+    for (int dir = 2; dir <= 3; dir++)
+      if (nn[dir] > 1) {
+        (void)hipEventRecord(c->ev_join[dir - 2], ds[dir]);
+        (void)hipStreamWaitEvent(s, c->ev_join[dir - 2], 0);
+      }
   prof_mark(c, HG_PROF_EMF, s);
   launch_emf(c->F[0], c->F[1], c->F[2], c->geo, c->corn, c->emf[0], c->emf[1], c->emf[2], d, c->tune, s);
   prof_mark(c, -1, s);

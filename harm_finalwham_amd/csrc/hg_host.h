@@ -15,7 +15,13 @@ struct hg_ctx {
   double *p = nullptr, *pk = nullptr;        // primitives and the RK2 midpoint state
   double *F[3] = {nullptr, nullptr, nullptr};
   double *emf[3] = {nullptr, nullptr, nullptr};
-  double *pl = nullptr, *pr = nullptr;       // WENO5BND interface states (per direction, reused)
+  double *pl = nullptr, *pr = nullptr;       // WENO5BND interface states along i
+  double *pl2 = nullptr, *pr2 = nullptr;     // ... along j
+  double *pl3 = nullptr, *pr3 = nullptr;     // ... along k
+  // the j and k reconstruction + flux run on these streams beside the i one
+  hipStream_t xs[2] = {nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+  bool capturing = false;                    // inside hg_graph_create: no side streams
   double *aux = nullptr;                     // WENO5BND per-zone dP/P inputs
   double *uf = nullptr;                      // updated conserved variables before the inversion
   double *geo = nullptr, *corn = nullptr, *conn = nullptr, *floors = nullptr;

@@ -15,6 +15,7 @@ struct Tune {
   int bs_flux = 256, bs_emf = 256, bs_update = 256, bs_invert = 256;
   int weno_xwaves = 4, weno_nseg = 0, weno_nseg3 = 0;  // segments per line along j, along k
   int weno_xmarch = 0;  // along i: 0 = one lane per point (shuffles), n > 0 = marching, n segments
+  int concurrent = 1;   // 1: the three directions' fluxes on three streams, 0: one after another
 };
 
 void launch_bound(double *p, const double *geo, const Dims &d, double gammamax, int *err, hipStream_t s);

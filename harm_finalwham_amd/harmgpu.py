@@ -35,7 +35,7 @@ class HarmError(RuntimeError):
 class HgTune(C.Structure):
     _fields_ = [("bs_flux", C.c_int), ("bs_emf", C.c_int), ("bs_update", C.c_int), ("bs_invert", C.c_int),
                 ("weno_xwaves", C.c_int), ("weno_nseg", C.c_int), ("weno_nseg3", C.c_int),
-                ("weno_xmarch", C.c_int)]
+                ("weno_xmarch", C.c_int), ("concurrent", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -282,6 +282,7 @@ typedef struct {
   int bs_flux, bs_emf, bs_update, bs_invert;
   int weno_xwaves, weno_nseg, weno_nseg3;
   int weno_xmarch;
+  int concurrent; /* 1: the three directions' fluxes on three streams, 0: in turn */
 } hg_tune;
 int hg_get_tune(const hg_ctx *ctx, hg_tune *t);
 int hg_set_tune(hg_ctx *ctx, const hg_tune *t);

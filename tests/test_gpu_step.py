@@ -65,7 +65,7 @@ def test_steps_match_reference(name):
 
 
 @pytest.mark.parametrize("name", ["t2d_weno", "t3d_weno"])
-@pytest.mark.parametrize("shape", [dict(weno_xmarch=1), dict(weno_xmarch=3, weno_nseg=3, weno_nseg3=2),
+@pytest.mark.parametrize("shape", [dict(concurrent=0), dict(weno_xmarch=1), dict(weno_xmarch=3, weno_nseg=3, weno_nseg3=2),
                                    dict(weno_xwaves=1, weno_nseg=6, weno_nseg3=4, bs_flux=64, bs_update=128)])
 def test_launch_shapes_keep_every_bit(name, shape):
     """Every launch shape of the tuner, the i-direction marching scheme
