@@ -24,7 +24,8 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_graph_create", "hg_graph_launch", "hg_graph_destroy", "hg_get_tune", "hg_set_tune", "hg_autotune",
            "hg_mlp_create", "hg_mlp_destroy", "hg_mlp_nparams", "hg_mlp_get_params", "hg_mlp_set_params",
            "hg_mlp_features", "hg_mlp_forward", "hg_mlp_gradient", "hg_mlp_train",
-           "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag", "hg_diagnostics", "hg_diag_record"]
+           "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag", "hg_diagnostics", "hg_diag_record", "hg_rdump_parse",
+           "hg_rdump_read"]
 HG_PROF_NCLASS = 10
 
 
@@ -55,6 +56,24 @@ class HgDiag(C.Structure):
                 "pdotterms": np.ctypeslib.as_array(self.pdotterms).copy(),
                 "pdottermsjet2": np.ctypeslib.as_array(self.pdottermsjet2).copy(),
                 "pcum": np.ctypeslib.as_array(self.pcum).copy()}
+
+
+class HgRdumpInfo(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("t", C.c_double), ("tf", C.c_double), ("nstep", C.c_long),
+                ("a", C.c_double), ("gam", C.c_double), ("cour", C.c_double), ("DT", C.c_double * 4),
+                ("DTr", C.c_long), ("DTdebug", C.c_double), ("counts", C.c_long * 5), ("dt", C.c_double),
+                ("lim", C.c_int), ("timeorder", C.c_int), ("fluxmethod", C.c_int), ("fluxb", C.c_int),
+                ("utoprimversion", C.c_int), ("failed", C.c_int), ("R0", C.c_double), ("Rin", C.c_double),
+                ("Rout", C.c_double), ("hslope", C.c_double), ("bctype", C.c_int * 6), ("realnstep", C.c_long),
+                ("defcon", C.c_double), ("safe", C.c_double), ("rhomin", C.c_double), ("uumin", C.c_double),
+                ("gammamax", C.c_double), ("gammafail", C.c_double)]
+
+    def as_dict(self):
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = tuple(v[:]) if hasattr(v, "_length_") else v
+        return out
 
 
 class HgConfig(C.Structure):
@@ -122,6 +141,8 @@ def lib():
     L.hg_get_pflag.argtypes = [vp, C.POINTER(C.c_int32)]
     L.hg_diagnostics.argtypes = [vp, C.c_double, C.POINTER(HgDiag), vp]
     L.hg_diag_record.argtypes = [vp, C.c_int]
+    L.hg_rdump_parse.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, d, d, C.POINTER(HgRdumpInfo)]
+    L.hg_rdump_read.argtypes = [vp, C.c_char_p, C.POINTER(HgRdumpInfo), d, vp]
     fp = C.POINTER(C.c_float)
     L.hg_mlp_create.argtypes = [C.c_uint64, C.POINTER(vp)]
     L.hg_mlp_destroy.argtypes = [vp]
@@ -235,6 +256,14 @@ class Grid:
         """Record diag_flux's boundary terms in every step (hg_diag_record)."""
         _check(lib().hg_diag_record(self.h, int(on)))
 
+    def rdump_read(self, path):
+        """Load the primitives of a reference restart dump (then bound_prim);
+        returns (unew [n1,n2,n3,8], header dict)."""
+        u = np.zeros((self.n1, self.n2, self.n3, 8))
+        info = HgRdumpInfo()
+        _check(lib().hg_rdump_read(self.h, os.fsencode(path), C.byref(info), _d(u), None))
+        return u, info.as_dict()
+
     def get_pflag(self):
         """Failure codes of the active zones [n1][n2][n3] (FIXED = -1)."""
         out = np.zeros((self.n1, self.n2, self.n3), np.int32)
@@ -288,6 +317,16 @@ class Grid:
         v = np.zeros(1)
         _check(lib().hg_get_dt(self.h, _d(v)))
         return float(v[0])
+
+
+def rdump_parse(path, n1, n2, n3):
+    """Read a restart dump of the reference on the host: (p [n1,n2,n3,8],
+    unew [n1,n2,n3,8], header dict)."""
+    p = np.zeros((n1, n2, n3, 8))
+    u = np.zeros((n1, n2, n3, 8))
+    info = HgRdumpInfo()
+    _check(lib().hg_rdump_parse(os.fsencode(path), n1, n2, n3, _d(p), _d(u), C.byref(info)))
+    return p, u, info.as_dict()
 
 
 def mks_geometry_tables(n1, n2, n3, nb=6, a=0.9375, Rin=1.3, Rout=40.0, R0=0.0, hslope=0.3):

@@ -180,6 +180,34 @@ int hg_diagnostics(hg_ctx *ctx, double Dt, hg_diag *out, void *stream);
    takes those two terms from the current state. */
 int hg_diag_record(hg_ctx *ctx, int on);
 
+/* Restart dumps of the reference (restart.c:206 restart_write, text output):
+   a header line, then one line per active zone (k outer, i fastest) with
+   p[0..7] and unew[0..7] at 15 significant digits.  hg_rdump_parse reads
+   one on the host (no GPU): the active zones' primitives into p_active and
+   the conserved state into unew (both [i][j][k][8] over n1 x n2 x n3, may be
+   NULL), and the header fields the hot path uses into *info (may be NULL);
+   the header's defcoord is skipped (the reference prints that double with
+   "%d").  hg_rdump_read puts the primitives into a context and runs
+   bound_prim.  Writing rdumps is not provided. */
+typedef struct {
+  int n[3];
+  double t, tf;
+  long nstep;
+  double a, gam, cour;
+  double DT[4];            /* DTd, DTavg, DTener, DTi */
+  long DTr;
+  double DTdebug;
+  long counts[5];          /* dump, image, rdump, avg, debug counters */
+  double dt;
+  int lim, timeorder, fluxmethod, fluxb, utoprimversion, failed;
+  double R0, Rin, Rout, hslope;
+  int bctype[6];           /* X1UP, X1DN, X2UP, X2DN, X3UP, X3DN */
+  long realnstep;
+  double defcon, safe, rhomin, uumin, gammamax, gammafail;
+} hg_rdump_info;
+int hg_rdump_parse(const char *path, int n1, int n2, int n3, double *p_active, double *unew, hg_rdump_info *info);
+int hg_rdump_read(hg_ctx *ctx, const char *path, hg_rdump_info *info, double *unew, void *stream);
+
 /* Batched Noble-2D inversion on device memory (utoprimgen.c:23).  All
    arrays are structure-of-arrays over n zones:
      U[8][n] evolved conserved variables (with gdet), gcov[16][n], gcon[16][n],

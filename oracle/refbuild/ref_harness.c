@@ -484,6 +484,65 @@ static int mode_diag(const char *outdir, int use_weno, int nsteps)
   return 0;
 }
 
+/*
+ * rdump mode: the torus over <nsteps> RK2 steps, then restart_write(0)
+ * (restart.c:206), which writes dumps/rdump-0 in the working directory, and
+ * the state as .npy (p with ghosts, unew of the active zones).
+ *
+ * rdread mode: restart_read(0) (restart.c:262) of dumps/rdump-0 in the
+ * working directory after the usual set-up; writes p, unew and the header
+ * values the hot path uses.
+ */
+extern int restart_write(long dump_cnt);
+extern int restart_read(long dump_cnt);
+static void dump_unew(const char *dir, const char *name)
+{
+  long shape[4] = {N1, N2, N3, NPR};
+  FTYPE *buf = malloc(sizeof(FTYPE) * N1 * N2 * N3 * NPR);
+  int i, j, k, pl;
+  ZLOOP PLOOP(pl) buf[(((long)i * N2 + j) * N3 + k) * NPR + pl] = unew[i][j][k][pl];
+  write_npy(dir, name, "<f8", 4, shape, buf, sizeof(FTYPE));
+  free(buf);
+}
+static void dump_header_values(const char *dir)
+{
+  FTYPE h[16];
+  long s[1] = {16};
+  h[0] = t; h[1] = tf; h[2] = (FTYPE)nstep; h[3] = a; h[4] = gam; h[5] = cour; h[6] = dt;
+  h[7] = lim; h[8] = fluxmethod; h[9] = R0; h[10] = Rin; h[11] = Rout; h[12] = hslope;
+  h[13] = RHOMIN; h[14] = UUMIN; h[15] = GAMMAMAX;
+  write_npy(dir, "header", "<f8", 1, s, h, sizeof(FTYPE));
+}
+static int mode_rdump(const char *outdir, int use_weno, int nsteps)
+{
+  int n;
+  mkdir(outdir, 0755);
+  harness_init(use_weno);
+  for (n = 1; n <= nsteps; n++) {
+    step_ch();
+    nstep++;
+    t += 0.0;
+  }
+  mkdir("dumps", 0755);
+  if (restart_write(0) >= 1) { fprintf(stderr, "restart_write failed\n"); return 3; }
+  dump_prims(outdir, "p", p);
+  dump_unew(outdir, "unew");
+  dump_header_values(outdir);
+  return 0;
+}
+static int mode_rdread(const char *outdir, int use_weno)
+{
+  int i, j, k, pl;
+  mkdir(outdir, 0755);
+  harness_init(use_weno);
+  FULLLOOP PLOOP(pl) p[i][j][k][pl] = 0.0;
+  if (restart_read(0) >= 1) { fprintf(stderr, "restart_read failed\n"); return 3; }
+  dump_prims(outdir, "p", p);
+  dump_unew(outdir, "unew");
+  dump_header_values(outdir);
+  return 0;
+}
+
 int main(int argc, char **argv)
 {
   if (argc >= 5 && !strcmp(argv[1], "torus")) return mode_torus(argv[2], atoi(argv[3]), atoi(argv[4]));
@@ -491,6 +550,8 @@ int main(int argc, char **argv)
   if (argc >= 4 && !strcmp(argv[1], "time")) return mode_time(atoi(argv[2]), atoi(argv[3]));
   if (argc >= 6 && !strcmp(argv[1], "fixup")) return mode_fixup(argv[2], argv[3], atoi(argv[4]), atoi(argv[5]));
   if (argc >= 5 && !strcmp(argv[1], "diag")) return mode_diag(argv[2], atoi(argv[3]), atoi(argv[4]));
+  if (argc >= 5 && !strcmp(argv[1], "rdump")) return mode_rdump(argv[2], atoi(argv[3]), atoi(argv[4]));
+  if (argc >= 4 && !strcmp(argv[1], "rdread")) return mode_rdread(argv[2], atoi(argv[3]));
   fprintf(stderr, "usage: %s torus <outdir> <weno> <nsteps> | c2p <in> <out> <gamma> | time <weno> <nsteps> | fixup <in> <outdir> <weno> <finalstep>\n", argv[0]);
   return 1;
 }
