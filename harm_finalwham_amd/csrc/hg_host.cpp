@@ -47,7 +47,14 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
   if (!cfg || !out) { set_error("hg_ctx_create: null argument"); return 1; }
   *out = nullptr;
   if (cfg->n1 < 1 || cfg->n2 < 1 || cfg->n3 < 1) { set_error("hg_ctx_create: grid sizes must be >= 1"); return 1; }
-  if (cfg->lim != HG_LIM_MC && cfg->lim != HG_LIM_WENO5BND) { set_error("hg_ctx_create: unsupported lim %d", cfg->lim); return 1; }
+  if (cfg->lim != HG_LIM_MC && cfg->lim != HG_LIM_PARA && cfg->lim != HG_LIM_WENO5BND) {
+    set_error("hg_ctx_create: unsupported lim %d", cfg->lim);
+    return 1;
+  }
+  if (cfg->timeorder != 0 && cfg->timeorder != 2 && cfg->timeorder != 4) {
+    set_error("hg_ctx_create: unsupported timeorder %d (2 or 4)", cfg->timeorder);
+    return 1;
+  }
   if (cfg->fluxmethod != HG_FLUX_LAXF && cfg->fluxmethod != HG_FLUX_HLL) {
     set_error("hg_ctx_create: unsupported fluxmethod %d", cfg->fluxmethod);
     return 1;
@@ -65,6 +72,10 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
   rc |= dalloc(&c->p, ns);
   rc |= dalloc(&c->pk, ns);
   rc |= dalloc(&c->uf, ns);
+  if (cfg->timeorder == 4) {
+    rc |= dalloc(&c->pk2, ns);
+    rc |= dalloc(&c->ucum, ns);
+  }
   for (int k = 0; k < 3; k++) rc |= dalloc(&c->F[k], ns);
   for (int k = 0; k < 3; k++) rc |= dalloc(&c->emf[k], nz);
   if (cfg->lim == HG_LIM_WENO5BND) {
@@ -100,9 +111,9 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
 
 extern "C" int hg_ctx_destroy(hg_ctx *c) {
   if (!c) return 0;
-  double *bufs[] = {c->p,      c->pk,     c->uf,     c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1],
-                    c->emf[2], c->pl,     c->pr,     c->pl2,  c->pr2,  c->pl3,  c->pr3,    c->aux,
-                    c->geo,    c->corn,   c->conn,   c->floors, c->dt};
+  double *bufs[] = {c->p,      c->pk,     c->pk2,    c->ucum, c->uf,  c->F[0], c->F[1],  c->F[2],
+                    c->emf[0], c->emf[1], c->emf[2], c->pl,   c->pr,  c->pl2,  c->pr2,   c->pl3,
+                    c->pr3,    c->aux,    c->geo,    c->corn, c->conn, c->floors, c->dt};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (c->ndt_bits) (void)hipFree(c->ndt_bits);
@@ -360,7 +371,7 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
   c->dg_faces_valid = false;
   int nn[4] = {0, d.n1, d.n2, d.n3};
   launch_reset_ndt(c->ndt_bits, s);
-  if (f.lim != HG_LIM_MC) {
+  if (f.lim == HG_LIM_WENO5BND) {
     // the dP/P inputs are shared by the three directions; timed with direction 1
     prof_mark(c, HG_PROF_RECON1, s);
     launch_weno_prep(pb, c->geo, c->aux, d, f.gam, s);
@@ -387,10 +398,10 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
     const int dir = order[conc ? 1 : 0][o];
     if (nn[dir] <= 1) continue;
     hipStream_t sd = ds[dir];
-    if (f.lim == HG_LIM_MC) {
+    if (f.lim != HG_LIM_WENO5BND) {
       prof_mark(c, HG_PROF_FLUX1 + dir - 1, sd);
-      launch_flux_mc(dir, pb, c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,
-                     c->ndt_bits + dir - 1, c->err, c->tune, sd);
+      launch_flux_point(dir, f.lim == HG_LIM_PARA, pb, c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir],
+                        f.fluxmethod, c->ndt_bits + dir - 1, c->err, c->tune, sd);
       prof_mark(c, -1, sd);
     } else {
       if (dir != 1) prof_mark(c, HG_PROF_RECON1 + dir - 1, sd);
@@ -478,7 +489,7 @@ extern "C" int hg_fluxcalc(hg_ctx *c, double *F1, double *F2, double *F3, double
 }
 
 // one substep of step_ch.c:94-166: advance, bound_prim, post_fixup, bound_prim
-static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, double cuf2, int finalstep,
+static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, const RkStage &rk, int finalstep,
                             hipStream_t s) {
   const hg_config &f = c->cfg;
   enqueue_fluxcalc(c, pb, s);
@@ -489,7 +500,7 @@ static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, doubl
   }
   prof_mark(c, HG_PROF_UPDATE, s);
   launch_update(pi, pb, pf, c->uf, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2], c->geo, c->conn,
-                c->floors, c->dt, cuf2, finalstep, c->d, f.gam, f.dx, f.gammamax, c->counters, c->pflag, c->err,
+                c->floors, c->dt, rk, finalstep, c->d, f.gam, f.dx, f.gammamax, c->counters, c->pflag, c->err,
                 c->tune, s);
   if (finalstep) launch_dt(c->ndt_bits, c->dt, f.defcon, f.safe, s);
   // step_ch.c:127 and :166 call bound_prim twice per substep; the second call
@@ -504,10 +515,25 @@ static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, doubl
   prof_mark(c, -1, s);
 }
 
-// step_ch.c:442 setup_rktimestep, TIMEORDER 2: CUf[0][2]=0.5, CUf[1][2]=1
+// step_ch.c:35 step_ch_simplempi over the substeps of step_ch.c:442
+// setup_rktimestep; pi is p on every substep
 void hg::enqueue_step(hg_ctx *c, hipStream_t s) {
-  enqueue_substep(c, c->p, c->p, c->pk, 0.5, 0, s);
-  enqueue_substep(c, c->p, c->pk, c->p, 1.0, 1, s);
+  if (c->cfg.timeorder != 4) {
+    // TIMEORDER 2 midpoint: CUf[.][2] = 0.5, 1; pb = p, pk; pf = pk, p
+    RkStage a{0.5, 0.0, 0.0, 0.0, nullptr, 1, 0}, b{1.0, 0.0, 0.0, 1.0, nullptr, 0, 1};
+    enqueue_substep(c, c->p, c->p, c->pk, a, 0, s);
+    enqueue_substep(c, c->p, c->pk, c->p, b, 1, s);
+    return;
+  }
+  // TIMEORDER 4: CUf[.][2] = 1/2, 1/2, 1, 1; Cunew = {1, 1/6, 0}, {0, 1/3, 0},
+  // {0, 1/3, 0}, {0, 1/6, 0}; pb = p, pk, pk2, pk; pf = pk, pk2, pk, p
+  const double cuf2[4] = {0.5, 0.5, 1.0, 1.0};
+  const double cn[4][3] = {{1.0, 1.0 / 6.0, 0.0}, {0.0, 1.0 / 3.0, 0.0}, {0.0, 1.0 / 3.0, 0.0}, {0.0, 1.0 / 6.0, 0.0}};
+  double *pb[4] = {c->p, c->pk, c->pk2, c->pk}, *pf[4] = {c->pk, c->pk2, c->pk, c->p};
+  for (int t = 0; t < 4; t++) {
+    RkStage rk{cuf2[t], cn[t][0], cn[t][1], cn[t][2], c->ucum, t == 0, t == 3};
+    enqueue_substep(c, c->p, pb[t], pf[t], rk, t == 3, s);
+  }
 }
 
 extern "C" int hg_set_dt(hg_ctx *c, double dt) {

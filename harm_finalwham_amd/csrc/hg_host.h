@@ -12,7 +12,8 @@
 struct hg_ctx {
   hg_config cfg;
   hg::Dims d;
-  double *p = nullptr, *pk = nullptr;        // primitives and the RK2 midpoint state
+  double *p = nullptr, *pk = nullptr;        // primitives and the first intermediate state
+  double *pk2 = nullptr, *ucum = nullptr;    // RK4 only: second intermediate state, ucum
   double *F[3] = {nullptr, nullptr, nullptr};
   double *emf[3] = {nullptr, nullptr, nullptr};
   double *pl = nullptr, *pr = nullptr;       // WENO5BND interface states along i

@@ -19,9 +19,10 @@ struct Tune {
 };
 
 void launch_bound(double *p, const double *geo, const Dims &d, double gammamax, int *err, hipStream_t s);
-void launch_flux_mc(int dir, const double *p, const double *geo, double *F, const Dims &d, double gam,
-                    double cour, double dxdir, int fluxmethod, unsigned long long *ndt_bits, int *err,
-                    const Tune &tu, hipStream_t s);
+// MC (para = false) or PARA (para = true) reconstruction fused with the flux
+void launch_flux_point(int dir, bool para, const double *p, const double *geo, double *F, const Dims &d,
+                       double gam, double cour, double dxdir, int fluxmethod, unsigned long long *ndt_bits,
+                       int *err, const Tune &tu, hipStream_t s);
 void launch_flux_lr(int dir, const double *pleft, const double *pright, const double *geo, double *F,
                     const Dims &d, double gam, double cour, double dxdir, int fluxmethod,
                     unsigned long long *ndt_bits, int *err, const Tune &tu, hipStream_t s);
@@ -29,10 +30,18 @@ void launch_emf(const double *F1, const double *F2, const double *F3, const doub
                 double *e1, double *e2, double *e3, const Dims &d, const Tune &tu, hipStream_t s);
 // per-zone update (-> ubuf) then inversion + fixup (-> pf); ubuf has the
 // layout of the primitives
+// coefficients of one Runge-Kutta substep (step_ch.c:442 setup_rktimestep):
+// Uf = Ui + cuf2 dt dU; with ucum set (RK4), ucum += cn0 Ui + cn1 dt dU +
+// cn2 Uf, zeroed first on the first substep and inverted on the last
+struct RkStage {
+  double cuf2, cn0, cn1, cn2;
+  double *ucum;  // null: midpoint scheme, the last substep inverts Uf
+  int first, last;
+};
 void launch_update(const double *pi, const double *pb, double *pf, double *ubuf, const double *F1,
                    const double *F2, const double *F3, const double *e1, const double *e2, const double *e3,
-                   const double *geo, const double *conn, const double *floors, const double *dtp, double cuf2,
-                   int finalstep, const Dims &d, double gam, const double *dx, double gammamax,
+                   const double *geo, const double *conn, const double *floors, const double *dtp,
+                   const RkStage &rk, int finalstep, const Dims &d, double gam, const double *dx, double gammamax,
                    unsigned long long *counters, int8_t *pflag, int *err, const Tune &tu, hipStream_t s);
 void launch_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe, hipStream_t s);
 void launch_reset_ndt(unsigned long long *ndt_bits, hipStream_t s);

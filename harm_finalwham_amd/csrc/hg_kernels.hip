@@ -2,6 +2,7 @@
 //
 //   k_bound_x1/x2/x3   ghost fill (bounds.fishmon.c:29)
 //   k_flux_mc<dir>     MC reconstruction fused with the Riemann flux (flux.c:469)
+//   k_flux_para<dir>   PARA (para4) reconstruction fused with the Riemann flux
 //   k_flux_lr<dir>     Riemann flux from precomputed interface states (WENO5BND)
 //   k_emf              flux-CT corner EMFs (fluxct.c:10)
 //   k_update           conserved update + source + Noble 2D + fixup1zone (advance.c:58)
@@ -255,11 +256,55 @@ __device__ __forceinline__ double mc_slope(double yl, double yc, double yr) {
   return minmod(Dqc, inner);
 }
 
-template <int DIR>
-__global__ void __launch_bounds__(256) k_flux_mc(const double *__restrict__ p, const double *__restrict__ geo,
-                                                 double *__restrict__ F, Dims d, double gam, double cour,
-                                                 double dxdir, int fluxmethod,
-                                                 unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
+// phys.c:1797 max, phys.c:1804 min, phys.c:1778 sign: plain comparisons, so
+// NaN and signed-zero operands resolve as in the reference
+__device__ __forceinline__ double ref_max(double a, double b) { return (a > b) ? a : b; }
+__device__ __forceinline__ double ref_min(double a, double b) { return (a > b) ? b : a; }
+__device__ __forceinline__ double ref_sign(double a) { return (a > 0.0) ? 1.0 : ((a < 0.0) ? -1.0 : 0.0); }
+
+// interppoint.c:738 para4 with PARA2LIM MC (global.h:426): y[0..4] holds the
+// zone's five-point stencil (offsets -2..2); returns the zone's left and
+// right interface values
+__device__ __forceinline__ void para4(const double y[5], double &lout, double &rout) {
+  double dq[3];  // CW1.7 limited slopes at offsets -1, 0, 1
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    double Dqm = 2.0 * (y[m + 1] - y[m]);
+    double Dqp = 2.0 * (y[m + 2] - y[m + 1]);
+    double Dqc = 0.5 * (y[m + 2] - y[m]);
+    double s = Dqm * Dqp;
+    dq[m] = (s <= 0.) ? 0. : ref_min(fabs(Dqc), ref_min(fabs(Dqm), fabs(Dqp))) * ref_sign(Dqc);
+  }
+  // CW1.6 with 1/8 in place of 1/6, bounded by the neighbouring values
+  double l = 0.5 * (y[2] + y[1]) - (dq[1] - dq[0]) / 8.0;
+  double r = 0.5 * (y[3] + y[2]) - (dq[2] - dq[1]) / 8.0;
+  l = ref_max(ref_min(y[2], y[1]), l);
+  l = ref_min(ref_max(y[2], y[1]), l);
+  r = ref_max(ref_min(y[2], y[3]), r);
+  r = ref_min(ref_max(y[2], y[3]), r);
+  double qa = (r - y[2]) * (y[2] - l);
+  double qd = (r - l);
+  double qe = 6.0 * (y[2] - 0.5 * (l + r));
+  if (qa <= 0.) {
+    l = y[2];
+    r = y[2];
+  } else {
+    // steepening at the top or bottom of a gradient; r sees the new l
+    if (qd * (qd - qe) < 0.0) l = 3.0 * y[2] - 2.0 * r;
+    if (qd * (qd + qe) < 0.0) r = 3.0 * y[2] - 2.0 * l;
+  }
+  lout = l;
+  rout = r;
+}
+
+// point-type reconstruction fused with the Riemann flux (flux.c:469
+// fluxcalc_standard): MC slopes (interppoint.c:387) or PARA interface values
+// (interppoint.c:289), turned into the face states by flux.c:753 getp2interplr
+template <int DIR, bool PARA>
+__device__ __forceinline__ void flux_point_body(const double *__restrict__ p, const double *__restrict__ geo,
+                                                double *__restrict__ F, const Dims &d, double gam, double cour,
+                                                double dxdir, int fluxmethod,
+                                                unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
   FaceRange fr = face_range(d, DIR);
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)fr.ni * fr.nj * fr.nk;
@@ -277,10 +322,20 @@ __global__ void __launch_bounds__(256) k_flux_mc(const double *__restrict__ p, c
 #pragma unroll
     for (int pl = 0; pl < NPR; pl++) {
       const double *c = p + z + pl * d.spl;
-      double ym2 = c[-2 * s], ym1 = c[-s], y0 = c[0], yp1 = c[s];
-      // flux.c:800-801 getp2interplr
-      p_l[pl] = ym1 + 0.5 * mc_slope(ym2, ym1, y0);
-      p_r[pl] = y0 - 0.5 * mc_slope(ym1, y0, yp1);
+      if (PARA) {
+        // p_l = right value of zone z-s (stencil -3..1), p_r = left value
+        // of zone z (stencil -2..2)
+        double y[6] = {c[-3 * s], c[-2 * s], c[-s], c[0], c[s], c[2 * s]};
+        double lm, rm, l0, r0;
+        para4(y, lm, rm);
+        para4(y + 1, l0, r0);
+        p_l[pl] = rm;
+        p_r[pl] = l0;
+      } else {
+        double ym2 = c[-2 * s], ym1 = c[-s], y0 = c[0], yp1 = c[s];
+        p_l[pl] = ym1 + 0.5 * mc_slope(ym2, ym1, y0);
+        p_r[pl] = y0 - 0.5 * mc_slope(ym1, y0, yp1);
+      }
     }
     Geom G;
     load_geom(geo, DIR, d.gplane, d.gidx(i, j), G);
@@ -294,6 +349,23 @@ __global__ void __launch_bounds__(256) k_flux_mc(const double *__restrict__ p, c
     }
   }
   block_min_to(dtij, ndt_bits);
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) k_flux_mc(const double *__restrict__ p, const double *__restrict__ geo,
+                                                 double *__restrict__ F, Dims d, double gam, double cour,
+                                                 double dxdir, int fluxmethod,
+                                                 unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
+  flux_point_body<DIR, false>(p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) k_flux_para(const double *__restrict__ p, const double *__restrict__ geo,
+                                                   double *__restrict__ F, Dims d, double gam, double cour,
+                                                   double dxdir, int fluxmethod,
+                                                   unsigned long long *__restrict__ ndt_bits,
+                                                   int *__restrict__ err) {
+  flux_point_body<DIR, true>(p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
 template <int DIR>
@@ -396,13 +468,14 @@ __global__ void k_emf(const double *__restrict__ F1, const double *__restrict__ 
 // (one kernel holding both needs more registers than either, and the Uf
 // round trip through HBM costs less than the occupancy it loses)
 // ---------------------------------------------------------------------------
+template <bool RK4>
 __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, const double *__restrict__ pb,
                                                 double *__restrict__ ubuf, const double *__restrict__ F1,
                                                 const double *__restrict__ F2, const double *__restrict__ F3,
                                                 const double *__restrict__ emf1, const double *__restrict__ emf2,
                                                 const double *__restrict__ emf3, const double *__restrict__ geo,
                                                 const double *__restrict__ conn, const double *__restrict__ dtp,
-                                                double cuf2, Dims d, double gam, double idx1, double idx2,
+                                                RkStage rk, Dims d, double gam, double idx1, double idx2,
                                                 double idx3, int *__restrict__ err) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
@@ -502,10 +575,18 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
         }
       }
       double dUr = dU1 + dU2 + dU3;
-      // advance.c:922 dUtoU with CUf = {1, 0, cuf2}; uf carries no memory
-      // across substeps in the midpoint scheme (CUf[1] = 0) and ucum equals
-      // Uf on the final substep (Cunew = {0,0,1})
-      Uf[pl] = 1.0 * Ui[pl] + 0.0 * 0.0 + cuf2 * dt * (dUr + dUg[pl]);
+      // advance.c:922 dUtoU with CUf = {1, 0, cuf2}: uf carries no memory
+      // across substeps (CUf[1] = 0 in both schemes)
+      Uf[pl] = 1.0 * Ui[pl] + 0.0 * 0.0 + rk.cuf2 * dt * (dUr + dUg[pl]);
+      if (RK4) {
+        // ucum, zeroed on the first substep (advance.c:141), collects
+        // Cunew = {cn0, cn1, cn2}; the last substep inverts ucum
+        double uc = rk.first ? 0.0 : rk.ucum[z + pl * P];
+        uc += rk.cn0 * Ui[pl] + rk.cn1 * dt * (dUr + dUg[pl]) + rk.cn2 * Uf[pl];
+        if (rk.last) Uf[pl] = uc;
+        else rk.ucum[z + pl * P] = uc;
+      }
+      // midpoint scheme: ucum equals Uf on the final substep (Cunew = {0,0,1})
     }
     if (bad) atomicOr(err, 4);
 #pragma unroll
@@ -629,12 +710,18 @@ void launch_bound(double *p, const double *geo, const Dims &d, double gammamax, 
   }
 }
 
-void launch_flux_mc(int dir, const double *p, const double *geo, double *F, const Dims &d, double gam,
-                    double cour, double dxdir, int fluxmethod, unsigned long long *ndt_bits, int *err,
-                    const Tune &tu, hipStream_t s) {
+void launch_flux_point(int dir, bool para, const double *p, const double *geo, double *F, const Dims &d,
+                       double gam, double cour, double dxdir, int fluxmethod, unsigned long long *ndt_bits,
+                       int *err, const Tune &tu, hipStream_t s) {
   FaceRange fr = face_range(d, dir);
   int64_t n = (int64_t)fr.ni * fr.nj * fr.nk;
   dim3 g(nblocks(n, tu.bs_flux)), b(tu.bs_flux);
+  if (para) {
+    if (dir == 1) hipLaunchKernelGGL(k_flux_para<1>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 2) hipLaunchKernelGGL(k_flux_para<2>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 3) hipLaunchKernelGGL(k_flux_para<3>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    return;
+  }
   if (dir == 1) hipLaunchKernelGGL(k_flux_mc<1>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
   if (dir == 2) hipLaunchKernelGGL(k_flux_mc<2>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
   if (dir == 3) hipLaunchKernelGGL(k_flux_mc<3>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
@@ -660,12 +747,16 @@ void launch_emf(const double *F1, const double *F2, const double *F3, const doub
 
 void launch_update(const double *pi, const double *pb, double *pf, double *ubuf, const double *F1,
                    const double *F2, const double *F3, const double *e1, const double *e2, const double *e3,
-                   const double *geo, const double *conn, const double *floors, const double *dtp, double cuf2,
-                   int finalstep, const Dims &d, double gam, const double *dx, double gammamax,
+                   const double *geo, const double *conn, const double *floors, const double *dtp,
+                   const RkStage &rk, int finalstep, const Dims &d, double gam, const double *dx, double gammamax,
                    unsigned long long *counters, int8_t *pflag, int *err, const Tune &tu, hipStream_t s) {
   int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
-  hipLaunchKernelGGL(k_update, dim3(nblocks(n, tu.bs_update)), dim3(tu.bs_update), 0, s, pi, pb, ubuf, F1, F2, F3, e1, e2, e3, geo,
-                     conn, dtp, cuf2, d, gam, 1.0 / dx[1], 1.0 / dx[2], 1.0 / dx[3], err);
+  if (rk.ucum)
+    hipLaunchKernelGGL(k_update<true>, dim3(nblocks(n, tu.bs_update)), dim3(tu.bs_update), 0, s, pi, pb, ubuf, F1, F2, F3, e1, e2,
+                       e3, geo, conn, dtp, rk, d, gam, 1.0 / dx[1], 1.0 / dx[2], 1.0 / dx[3], err);
+  else
+    hipLaunchKernelGGL(k_update<false>, dim3(nblocks(n, tu.bs_update)), dim3(tu.bs_update), 0, s, pi, pb, ubuf, F1, F2, F3, e1,
+                       e2, e3, geo, conn, dtp, rk, d, gam, 1.0 / dx[1], 1.0 / dx[2], 1.0 / dx[3], err);
   hipLaunchKernelGGL(k_invert, dim3(nblocks(n, tu.bs_invert)), dim3(tu.bs_invert), 0, s, ubuf, pb, pf, geo, floors, finalstep, d,
                      gam, gammamax, counters, pflag, err);
 }

@@ -11,7 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libharmgpu.so")
 
-HG_LIM_MC, HG_LIM_WENO5BND = 3, 17
+HG_LIM_MC, HG_LIM_PARA, HG_LIM_WENO5BND = 3, 4, 17
 HG_FLUX_LAXF, HG_FLUX_HLL = 0, 1
 HG_C2P_FP64, HG_C2P_MIXED = 0, 1
 
@@ -81,7 +81,8 @@ class HgConfig(C.Structure):
                 ("lim", C.c_int), ("fluxmethod", C.c_int),
                 ("gam", C.c_double), ("cour", C.c_double), ("safe", C.c_double),
                 ("defcon", C.c_double), ("dx", C.c_double * 4),
-                ("rhomin", C.c_double), ("uumin", C.c_double), ("gammamax", C.c_double)]
+                ("rhomin", C.c_double), ("uumin", C.c_double), ("gammamax", C.c_double),
+                ("timeorder", C.c_int)]
 
 
 _lib = None
@@ -176,9 +177,10 @@ class Grid:
 
     def __init__(self, n1, n2, n3, nb=6, lim=HG_LIM_WENO5BND, fluxmethod=HG_FLUX_HLL, gam=4.0 / 3.0,
                  cour=0.8, dx=(0.0, 1.0, 1.0, 1.0), safe=1.3, defcon=1.0, rhomin=1e-4, uumin=1e-6,
-                 gammamax=25.0):
+                 gammamax=25.0, timeorder=2):
         cfg = HgConfig()
         cfg.n1, cfg.n2, cfg.n3, cfg.nb, cfg.lim, cfg.fluxmethod = n1, n2, n3, nb, lim, fluxmethod
+        cfg.timeorder = timeorder
         cfg.gam, cfg.cour, cfg.safe, cfg.defcon = gam, cour, safe, defcon
         for k in range(4):
             cfg.dx[k] = dx[k]

@@ -49,6 +49,7 @@ extern "C" {
 
 /* reconstruction and flux choices (reference values of lim/fluxmethod, global.h:247-314) */
 #define HG_LIM_MC 3
+#define HG_LIM_PARA 4 /* para4 (interppoint.c:738) with PARA2LIM MC, WHICHPARA PARA4 */
 #define HG_LIM_WENO5BND 17
 #define HG_FLUX_LAXF 0
 #define HG_FLUX_HLL 1
@@ -58,7 +59,7 @@ typedef struct hg_ctx hg_ctx;
 typedef struct {
   int n1, n2, n3;      /* active zones */
   int nb;              /* ghost layers per side (>= 6 for WENO5BND, initbase.c:792) */
-  int lim;             /* HG_LIM_MC or HG_LIM_WENO5BND */
+  int lim;             /* HG_LIM_MC, HG_LIM_PARA or HG_LIM_WENO5BND */
   int fluxmethod;      /* HG_FLUX_LAXF or HG_FLUX_HLL */
   double gam;          /* adiabatic index */
   double cour;         /* Courant factor */
@@ -67,6 +68,7 @@ typedef struct {
   double dx[4];        /* coordinate spacings dx[1..3] */
   double rhomin, uumin;/* floor coefficients RHOMIN, UUMIN (initbase.c:459-460) */
   double gammamax;     /* GAMMAMAX (initbase.c:471) */
+  int timeorder;       /* TIMEORDER: 2 (or 0) midpoint, 4 classical RK4 (step_ch.c:442 setup_rktimestep) */
 } hg_config;
 
 /* Context with device-resident state for one grid.  The caller owns the

@@ -97,7 +97,7 @@ void or_floors(int64_t n, const double *r, double rhomin, double uumin, double *
 /* ---- grid-level step (advance.c, flux.c, fluxct.c, step_ch.c, bounds.fishmon.c) ---- */
 typedef struct {
   int n1, n2, n3, nb;       /* active zones per direction and ghost width */
-  int lim;                  /* 0 = MC, 1 = WENO5BND */
+  int lim;                  /* 0 = MC, 1 = WENO5BND, 2 = PARA (para4) */
   int flux;                 /* 0 = LAXF, 1 = HLL */
   double gam, cour, dx[4];
   double rhomin, uumin, gammamax;
@@ -106,9 +106,10 @@ typedef struct {
      gcov [(n1+2nb+1)*(n2+2nb+1)][8][16], gcon same, gcovpert [..][8][4], gdet [..][8]
      conn [(n1+2nb)*(n2+2nb)][64], rfloor [(n1+2nb)*(n2+2nb)][2] (rho, u floors at CENT) */
   const double *gcov, *gcon, *gcovpert, *gdet, *conn, *floors;
+  int timeorder;            /* 2 (or 0) = midpoint RK2, 4 = RK4 (step_ch.c:442) */
 } or_grid;
 
-/* one RK2 step on p (full array with ghosts, layout [i][j][k][pl] with
+/* one step on p (full array with ghosts, layout [i][j][k][pl] with
    dims (n1+2nb)x(n2+2nb)x(n3+2nb3)); returns the next dt in *dt_io. */
 int or_step(const or_grid *G, double *p, double *dt_io, int64_t *nfail, int64_t *iters);
 /* fluxcalc + flux_ct on p for diagnostics; F arrays same layout as p */

@@ -14,6 +14,7 @@
 
 static double rmax(double a, double b) { if (a > b) return a; else return b; } /* phys.c:1797 */
 static double rmin(double a, double b) { if (a > b) return b; else return a; } /* phys.c:1804 */
+static double rsign(double a) { if (a > 0.0) return 1.0; else if (a < 0.0) return -1.0; else return 0.0; } /* phys.c:1778 */
 
 /* global.h:361-362 */
 #define MINMODB(a, b) ((fabs(a) < fabs(b)) ? (a) : (b))
@@ -128,11 +129,72 @@ static void mc_slopes(const or_dims *d, const double *p, int dir, double *dq)
         }
 }
 
+/* interppoint.c:738 para4, PARA2LIM MC (global.h:426 for WHICHPARA PARA4);
+   y points at the zone, y[-2..2] its stencil */
+static void para4(const double *y, double *lout, double *rout)
+{
+  double dq0[3], *dq = dq0 + 1;
+  double Dqm, Dqp, Dqc, s, l, r, qa, qd, qe;
+  int mm;
+  for (mm = -1; mm <= 1; mm++) {
+    Dqm = 2.0 * (y[mm] - y[mm - 1]);
+    Dqp = 2.0 * (y[mm + 1] - y[mm]);
+    Dqc = 0.5 * (y[mm + 1] - y[mm - 1]);
+    s = Dqm * Dqp;
+    if (s <= 0.) dq[mm] = 0.;
+    else dq[mm] = rmin(fabs(Dqc), rmin(fabs(Dqm), fabs(Dqp))) * rsign(Dqc);
+  }
+  l = 0.5 * (y[0] + y[-1]) - (dq[0] - dq[-1]) / 8.0;
+  r = 0.5 * (y[1] + y[0]) - (dq[1] - dq[0]) / 8.0;
+  l = rmax(rmin(y[0], y[-1]), l);
+  l = rmin(rmax(y[0], y[-1]), l);
+  r = rmax(rmin(y[0], y[1]), r);
+  r = rmin(rmax(y[0], y[1]), r);
+  qa = (r - y[0]) * (y[0] - l);
+  qd = (r - l);
+  qe = 6.0 * (y[0] - 0.5 * (l + r));
+  if (qa <= 0.) {
+    l = y[0];
+    r = y[0];
+  } else {
+    if (qd * (qd - qe) < 0.0) l = 3.0 * y[0] - 2.0 * r;
+    if (qd * (qd + qe) < 0.0) r = 3.0 * y[0] - 2.0 * l;
+  }
+  *lout = l;
+  *rout = r;
+}
+
+/* interppoint.c:24 slope_lim_pointtype + interppoint.c:271 slope_lim_point
+   (PARA): pleft/pright of every zone of set_interppoint_loop (interppoint.c:159) */
+static void para_states(const or_dims *d, const double *p, int dir, double *pleft, double *pright)
+{
+  int i, j, k, pl, l;
+  int is = -d->nb1, ie = d->n1 + d->nb1 - 1, js = -d->nb2, je = d->n2 + d->nb2 - 1;
+  int ks = -d->nb3, ke = d->n3 + d->nb3 - 1;
+  int di = (dir == 1), dj = (dir == 2), dk = (dir == 3);
+  double y[5];
+  if (dir == 1) { is = -1; ie = d->n1; }
+  if (dir == 2) { js = -1; je = d->n2; }
+  if (dir == 3) { ks = -1; ke = d->n3; }
+  for (i = is; i <= ie; i++)
+    for (j = js; j <= je; j++)
+      for (k = ks; k <= ke; k++)
+        for (pl = 0; pl < OR_NPR; pl++) {
+          ptrdiff_t z = or_zidx(d, i, j, k);
+          for (l = -2; l <= 2; l++) y[l + 2] = p[or_zidx(d, i + l * di, j + l * dj, k + l * dk) * 8 + pl];
+          para4(y + 2, &pleft[z * 8 + pl], &pright[z * 8 + pl]);
+        }
+}
+
 int or_reconstruct(const or_grid *G, const double *p, int dir, double *pleft, double *pright)
 {
   or_dims d = or_get_dims(G);
   if (G->lim == 0) {
     mc_slopes(&d, p, dir, pleft);
+    return 0;
+  }
+  if (G->lim == 2) {
+    para_states(&d, p, dir, pleft, pright);
     return 0;
   }
   /* WENO5BND: one call per pencil, pencils over the full transverse extent
@@ -362,7 +424,7 @@ static int advance(const or_grid *G, const double *pi, const double *pb, double 
          *F3 = calloc(nz * 8, sizeof(double));
   double ndt[3], idx1 = 1.0 / G->dx[1], idx2 = 1.0 / G->dx[2], idx3 = 1.0 / G->dx[3];
   int i, j, k, pl, rc = 0;
-  int finalstep = (timeorder == 1);
+  int finalstep = (timeorder == ((G->timeorder == 4) ? 3 : 1));
 
   if (or_fluxcalc(G, pb, F1, F2, F3, ndt)) { rc = 1; goto done; }
   for (i = 0; i < d.n1; i++)
@@ -421,24 +483,45 @@ int or_step(const or_grid *G, double *p, double *dt_io, int64_t *nfail, int64_t 
   double *pk = malloc(nz * 8 * sizeof(double));
   double *uf = calloc(nz * 8, sizeof(double)), *ucum = calloc(nz * 8, sizeof(double));
   int *pflag = calloc(nz, sizeof(int));
+  double *pk2 = NULL;
   /* step_ch.c:506-520, TIMEORDER 2 midpoint */
-  const double CUf[2][4] = {{1.0, 0.0, 0.5, 0.0}, {1.0, 0.0, 1.0, 0.0}};
-  const double Cunew[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}};
+  const double CUf2[2][4] = {{1.0, 0.0, 0.5, 0.0}, {1.0, 0.0, 1.0, 0.0}};
+  const double Cunew2[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}};
+  /* step_ch.c:451-474, TIMEORDER 4 */
+  const double CUf4[4][4] = {{1.0, 0.0, 0.5, 0.0}, {1.0, 0.0, 0.5, 0.0}, {1.0, 0.0, 1.0, 0.0}, {1.0, 0.0, 1.0, 0.0}};
+  const double Cunew4[4][3] = {{1.0, 1.0 / 6.0, 0.0}, {0.0, 1.0 / 3.0, 0.0}, {0.0, 1.0 / 3.0, 0.0}, {0.0, 1.0 / 6.0, 0.0}};
   double dt = *dt_io, ndt = BIG;
-  int rc = 0;
+  int rc = 0, t;
   memcpy(pk, p, nz * 8 * sizeof(double));
-  /* substep 0: pi=p pb=p pf=pk */
-  /* each substep: advance, bound_prim, post_fixup, bound_prim (step_ch.c:123-157) */
-  if (advance(G, p, p, pk, uf, ucum, CUf[0], Cunew[0], 0, dt, &ndt, pflag, nfail, iters)) { rc = 1; goto done; }
-  if (or_bound_prim(G, pk) || or_post_fixup(G, pk, pflag, 0) || or_bound_prim(G, pk)) { rc = 1; goto done; }
-  /* substep 1: pi=p pb=pk pf=p */
-  if (advance(G, p, pk, p, uf, ucum, CUf[1], Cunew[1], 1, dt, &ndt, pflag, nfail, iters)) { rc = 1; goto done; }
-  if (or_bound_prim(G, p) || or_post_fixup(G, p, pflag, 1) || or_bound_prim(G, p)) { rc = 1; goto done; }
+  if (G->timeorder == 4) {
+    /* pi = p; pb = p, pk, pk2, pk; pf = pk, pk2, pk, p */
+    double *pbb[4], *pff[4];
+    pk2 = malloc(nz * 8 * sizeof(double));
+    memcpy(pk2, p, nz * 8 * sizeof(double));
+    pbb[0] = p; pbb[1] = pk; pbb[2] = pk2; pbb[3] = pk;
+    pff[0] = pk; pff[1] = pk2; pff[2] = pk; pff[3] = p;
+    for (t = 0; t < 4; t++) {
+      if (advance(G, p, pbb[t], pff[t], uf, ucum, CUf4[t], Cunew4[t], t, dt, &ndt, pflag, nfail, iters) ||
+          or_bound_prim(G, pff[t]) || or_post_fixup(G, pff[t], pflag, t == 3) || or_bound_prim(G, pff[t])) {
+        rc = 1;
+        goto done;
+      }
+    }
+  } else {
+    /* substep 0: pi=p pb=p pf=pk */
+    /* each substep: advance, bound_prim, post_fixup, bound_prim (step_ch.c:123-157) */
+    if (advance(G, p, p, pk, uf, ucum, CUf2[0], Cunew2[0], 0, dt, &ndt, pflag, nfail, iters)) { rc = 1; goto done; }
+    if (or_bound_prim(G, pk) || or_post_fixup(G, pk, pflag, 0) || or_bound_prim(G, pk)) { rc = 1; goto done; }
+    /* substep 1: pi=p pb=pk pf=p */
+    if (advance(G, p, pk, p, uf, ucum, CUf2[1], Cunew2[1], 1, dt, &ndt, pflag, nfail, iters)) { rc = 1; goto done; }
+    if (or_bound_prim(G, p) || or_post_fixup(G, p, pflag, 1) || or_bound_prim(G, p)) { rc = 1; goto done; }
+  }
   /* step_ch.c:243-247 */
   if (ndt > G->safe * dt) ndt = G->safe * dt;
   *dt_io = ndt;
 done:
   free(pk);
+  free(pk2);
   free(uf);
   free(ucum);
   free(pflag);

@@ -47,6 +47,12 @@ def _p(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+# scheme number of the fixtures ("weno" entry, the golden harness's second
+# argument) -> (oracle lim, flux, timeorder): MC + LaxF + RK2, WENO5BND +
+# HLL + RK2, PARA + HLL + RK4
+SCHEMES = {0: (0, 0, 2), 1: (1, 1, 2), 2: (2, 1, 4)}
+
+
 class OrGrid(C.Structure):
     _fields_ = [("n1", C.c_int), ("n2", C.c_int), ("n3", C.c_int), ("nb", C.c_int),
                 ("lim", C.c_int), ("flux", C.c_int),
@@ -55,7 +61,8 @@ class OrGrid(C.Structure):
                 ("defcon", C.c_double), ("safe", C.c_double),
                 ("gcov", C.POINTER(C.c_double)), ("gcon", C.POINTER(C.c_double)),
                 ("gcovpert", C.POINTER(C.c_double)), ("gdet", C.POINTER(C.c_double)),
-                ("conn", C.POINTER(C.c_double)), ("floors", C.POINTER(C.c_double))]
+                ("conn", C.POINTER(C.c_double)), ("floors", C.POINTER(C.c_double)),
+                ("timeorder", C.c_int)]
 
 
 def c2p_batch(gcov, gcon, gdet, gcovpert, U, gam, guess):
@@ -92,7 +99,7 @@ class Grid:
     (FACE1, FACE2, FACE3, CORN1, CORN2, CORN3, CENT)."""
 
     def __init__(self, n1, n2, n3, nb, lim, flux, gam, cour, dx, gcov, gcon, gcovpert, gdet, conn,
-                 r_cent, defcon=1.0, safe=1.3):
+                 r_cent, defcon=1.0, safe=1.3, timeorder=2):
         self.keep = [np.ascontiguousarray(x, dtype=np.float64)
                      for x in (gcov, gcon, gcovpert, gdet, conn, floors_from_r(r_cent))]
         g = OrGrid()
@@ -102,6 +109,7 @@ class Grid:
             g.dx[i] = dx[i]
         g.rhomin, g.uumin, g.gammamax, g.defcon, g.safe = RHOMIN, UUMIN, GAMMAMAX, defcon, safe
         g.gcov, g.gcon, g.gcovpert, g.gdet, g.conn, g.floors = [_p(a) for a in self.keep]
+        g.timeorder = timeorder
         self.g = g
         self.shape = (n1 + 2 * nb * (n1 > 1), n2 + 2 * nb * (n2 > 1), n3 + 2 * nb * (n3 > 1), 8)
 
@@ -109,11 +117,11 @@ class Grid:
     def from_fixture(cls, d):
         par = d["params"]
         n1, n2, n3, nb = (int(x) for x in par[:4])
-        weno = int(d["weno"])
+        lim, flux, timeorder = SCHEMES[int(d["weno"])]
         dx = [0.0, par[12], par[13], par[14]]
         r = d["xv_cent"][:, :, 1, 1]
-        return cls(n1, n2, n3, nb, weno, weno, par[5], par[10], dx, d["gcov"], d["gcon"],
-                   d["gcovpert"], d["gdet"], d["conn"], r)
+        return cls(n1, n2, n3, nb, lim, flux, par[5], par[10], dx, d["gcov"], d["gcon"],
+                   d["gcovpert"], d["gdet"], d["conn"], r, timeorder=timeorder)
 
     def step(self, p, dt):
         p = np.ascontiguousarray(p, dtype=np.float64).copy()

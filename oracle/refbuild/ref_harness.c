@@ -6,13 +6,17 @@
  * project's oracle and its GPU kernels.  It never ships in the product.
  *
  * Modes
- *   torus <outdir> <weno 0|1> <nsteps>
+ *   torus <outdir> <scheme 0|1|2> <nsteps>
  *       Fishbone-Moncrief torus in MKS (LOGRSINTH) coordinates with the
- *       BASELINE overrides of the problem's init_global/init_grid, then
+ *       overrides of the problem's init_global/init_grid marked below, then
  *       writes geometry tables, the initial primitives, one call of
  *       fluxcalc() (which ends in flux_ct()), the slope_lim() interface
  *       states per direction, and the primitives after step 1 and after
- *       <nsteps> RK2 steps of step_ch().
+ *       <nsteps> steps of step_ch().  Schemes: 0 MC + LaxF + RK2,
+ *       1 WENO5BND + HLL + RK2, 2 PARA + HLL + RK4 (the torus file's own
+ *       lim, fluxmethod and TIMEORDER, init.fishmon.c:110-114, without the
+ *       finite-volume ENO step).  Every mode's <weno> argument below is
+ *       this scheme number.
  *   c2p <infile> <outfile> <gamma>
  *       Batched inversion: each input record holds a metric (gcov, gcon,
  *       gdet, gcovpert), the true primitives and a guess.  U is built with
@@ -111,11 +115,12 @@ static void dump_flux(const char *dir, const char *name, FTYPE (*F)[N2M][N3M][NP
 }
 
 /* ------------------------------------------------------------------ */
-static void harness_init(int use_weno)
+static void harness_init(int scheme)
 {
+  int use_weno = (scheme == 1);
   char *fake_argv[] = {"harness", NULL};
 
-  fprintf(stderr, "harness init: N=%dx%dx%d MAXBND=%d weno=%d\n", N1, N2, N3, MAXBND, use_weno);
+  fprintf(stderr, "harness init: N=%dx%dx%d MAXBND=%d scheme=%d\n", N1, N2, N3, MAXBND, scheme);
   pre_init(1, fake_argv);
   pre_init_specific_init();
 
@@ -123,10 +128,10 @@ static void harness_init(int use_weno)
   init_defconsts();
   init_global();
   /* OVERRIDE: the hot-path configurations (init.fishmon.c:96-118 picks PARA/RK4/FV) */
-  lim = use_weno ? WENO5BND : MC;
-  fluxmethod = use_weno ? HLLFLUX : LAXFFLUX;
+  lim = (scheme == 2) ? PARA : use_weno ? WENO5BND : MC;
+  fluxmethod = (scheme == 0) ? LAXFFLUX : HLLFLUX;
   FLUXB = FLUXCTTOTH;
-  TIMEORDER = 2;
+  TIMEORDER = (scheme == 2) ? 4 : 2;
   DOENOFLUX = NOENOFLUX;
   cour = 0.8;
   rescaletype = 0; /* floors rho>=RHOMIN r^-1.5, u>=UUMIN r^-2.5 (fixup.c:1562) */
@@ -286,7 +291,7 @@ static int mode_torus(const char *outdir, int use_weno, int nsteps)
     if ((dir == 1 && N1 == 1) || (dir == 2 && N2 == 1) || (dir == 3 && N3 == 1)) continue;
     fprintf(stderr, "slope_lim dir %d\n", dir);
     slope_lim(NPR2INTERP, dir, idel, jdel, kdel, p, p, dqd, pleft, pright);
-    if (lim >= WENO3) {
+    if (lim >= WENO3 || lim == PARA) {
       snprintf(nm, sizeof nm, "pleft_dir%d", dir);
       dump_flux(outdir, nm, (FTYPE (*)[N2M][N3M][NPR])pleft);
       snprintf(nm, sizeof nm, "pright_dir%d", dir);

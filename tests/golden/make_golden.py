@@ -14,6 +14,9 @@ Fixtures
   t2d_mc.npz     2-D torus 32x32, MC + LaxF + flux-CT, 5 RK2 steps
   t2d_weno.npz   2-D torus 32x32, WENO5BND + HLL + flux-CT, 5 RK2 steps
   t3d_weno.npz   3-D torus 16x16x8, WENO5BND + HLL + flux-CT, 1 RK2 step
+  t2d_para.npz   2-D torus 32x32, PARA + HLL + flux-CT, 5 RK4 steps
+  t3d_para.npz   3-D torus 16x16x8, PARA + HLL + flux-CT, 1 RK4 step
+        (written alone by --para)
 Each torus fixture carries the reference geometry tables, the initial
 primitives, the reconstruction states (or MC slopes) per direction, the
 fluxes after one fluxcalc()+flux_ct() and the primitives after steps.
@@ -355,7 +358,18 @@ def main_rdump():
                         p_active=pr[nb:nb + 32, nb:nb + 32], unew=un, header=hd)
 
 
+def main_para():
+    """The PARA + HLL + RK4 tori (scheme 2 of the harness)."""
+    builds = {"t2d": (32, 32, 1), "t3d": (16, 16, 8)}
+    for tag, (n1, n2, n3) in builds.items():
+        subprocess.run([BUILD, tag, str(n1), str(n2), str(n3)], check=True, stdout=subprocess.DEVNULL)
+    np.savez_compressed(os.path.join(HERE, "t2d_para.npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), 2, 5))
+    np.savez_compressed(os.path.join(HERE, "t3d_para.npz"), **run_torus(os.path.join(REFDIR, "harness_t3d"), 2, 1))
+
+
 def main():
+    if "--para" in sys.argv:
+        return main_para()
     if "--rdump" in sys.argv:
         return main_rdump()
     if "--diag" in sys.argv:
@@ -374,6 +388,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "t2d_mc.npz"), **run_torus(h2, 0, 5))
     np.savez_compressed(os.path.join(HERE, "t2d_weno.npz"), **run_torus(h2, 1, 5))
     np.savez_compressed(os.path.join(HERE, "t3d_weno.npz"), **run_torus(h3, 1, 1))
+    main_para()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
