@@ -66,6 +66,12 @@ extern "C" int hg_graph_create(hg_ctx *const *ctxs, int n, int nsteps, hg_graph 
     HIPTRY(hipEventCreateWithFlags(&join[i], hipEventDisableTiming));
   }
   hg_graph *g = nullptr;
+  // a pending conserved state of the finite-volume scheme is derived now,
+  // outside the graph, so that replays do not repeat it
+  for (int i = 0; i < n && !rc; i++) {
+    ensure_conserved(ctxs[i], nullptr);
+    HIPTRY(hipDeviceSynchronize());
+  }
   if (!rc) {
     g = new hg_graph();
     HIPTRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));

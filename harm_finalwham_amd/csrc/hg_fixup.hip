@@ -231,8 +231,8 @@ __device__ __forceinline__ int avg_flag(const Dims &d, const int8_t *flag0, cons
 }
 
 __global__ void __launch_bounds__(256) k_fx_average(const double *__restrict__ p, const double *__restrict__ geo,
-                                                    Dims d, FxBufs b, int finalstep, double gammamax,
-                                                    int *__restrict__ err) {
+                                                    Dims d, FxBufs b, int finalstep, double gammamax, double gam,
+                                                    double *__restrict__ ucons, int *__restrict__ err) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
   if (t >= total) return;
@@ -254,8 +254,8 @@ __global__ void __launch_bounds__(256) k_fx_average(const double *__restrict__ p
   for (int pl = 0; pl < NPR; pl++) pv[pl] = p[z + pl * P];
   // STEPOVERNEG* == -1: negative densities are left as they are
   const bool fixed = (f == FX_UNEG || f == FX_RHONEG || f == FX_RHOUNEG);
-  if (!fixed) {
-    Geom G;
+  Geom G;
+  {
     const double *gb = geo + (int64_t)GL_CENT * GCOMP * d.gplane + d.gidx(i, j);
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -266,6 +266,8 @@ __global__ void __launch_bounds__(256) k_fx_average(const double *__restrict__ p
       }
     G.gpert0 = gb[32 * d.gplane];
     G.g = gb[33 * d.gplane];
+  }
+  if (!fixed) {
     double gamma;
     if (gamma_calc(pv, G, gamma)) {
       atomicOr(err, 8);
@@ -322,6 +324,24 @@ __global__ void __launch_bounds__(256) k_fx_average(const double *__restrict__ p
     if (limit_gamma(gamma, pv, G) >= 1) {
       atomicOr(err, 8);
       return;
+    }
+  }
+  if (ucons && finalstep && cnt == 1) {
+    // fixup.c:1428 diag_fixup -> fixup.c:240-248 (finite-volume scheme):
+    // the carried conserved state of this zone follows the fix; U of the
+    // pre-fix and fixed states each passed through UtoU (UDIAG -> UEVOLVE),
+    // once per energy region (the zone is correctable in each of them)
+    double pr0[NPR], U0[NPR], U1[NPR];
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) pr0[pl] = p[z + pl * P];
+    primtoU(pr0, G, gam, U0);
+    primtoU(pv, G, gam, U1);
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) {
+      const double du = (U1[pl] / G.g) * G.g - (U0[pl] / G.g) * G.g;
+      double u = ucons[z + pl * P];
+      for (int region = 0; region < NENERREGIONS; region++) u = u + du;
+      ucons[z + pl * P] = u;
     }
   }
   unsigned slot = atomicAdd(&b.count[1], 1u);
@@ -407,7 +427,8 @@ void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s) {
   hipLaunchKernelGGL(k_fx_reset, dim3(1), dim3(64), 0, s, b.count);
   hipLaunchKernelGGL(k_fx_check, dim3(nb), dim3(256), 0, s, p, c->geo, d, b, c->err);
   hipLaunchKernelGGL(k_fx_resolve, dim3(1), dim3(1024), 0, s, d, b, c->err);
-  hipLaunchKernelGGL(k_fx_average, dim3(nb), dim3(256), 0, s, p, c->geo, d, b, finalstep, c->cfg.gammamax, c->err);
+  hipLaunchKernelGGL(k_fx_average, dim3(nb), dim3(256), 0, s, p, c->geo, d, b, finalstep, c->cfg.gammamax,
+                     c->cfg.gam, c->ucons, c->err);
   hipLaunchKernelGGL(k_fx_apply, dim3(256), dim3(256), 0, s, p, d, b);
 }
 

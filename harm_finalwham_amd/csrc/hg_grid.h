@@ -19,6 +19,9 @@ namespace hg {
 
 // primitive indices (global.h:916-923)
 constexpr int RHO = 0, UU = 1, U1 = 2, U2 = 3, U3 = 4, B1 = 5, B2 = 6, B3 = 7, NPR = 8;
+// energy regions of the diagnostics (global.h:436, DOJETDIAG 1 in the torus
+// file): the global region and two jet regions
+constexpr int NENERREGIONS = 3;
 
 constexpr int GCOMP = 34;
 constexpr int GL_CENT = 0, GL_FACE1 = 1, GL_FACE2 = 2, GL_FACE3 = 3;

@@ -55,6 +55,14 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
     set_error("hg_ctx_create: unsupported timeorder %d (2 or 4)", cfg->timeorder);
     return 1;
   }
+  if (cfg->doenoflux != HG_NOENOFLUX && cfg->doenoflux != HG_ENOFINITEVOLUME) {
+    set_error("hg_ctx_create: unsupported doenoflux %d", cfg->doenoflux);
+    return 1;
+  }
+  if (cfg->doenoflux == HG_ENOFINITEVOLUME && cfg->lim == HG_LIM_WENO5BND) {
+    set_error("hg_ctx_create: the finite-volume scheme is built for MC and PARA reconstruction only");
+    return 1;
+  }
   if (cfg->fluxmethod != HG_FLUX_LAXF && cfg->fluxmethod != HG_FLUX_HLL) {
     set_error("hg_ctx_create: unsupported fluxmethod %d", cfg->fluxmethod);
     return 1;
@@ -72,10 +80,10 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
   rc |= dalloc(&c->p, ns);
   rc |= dalloc(&c->pk, ns);
   rc |= dalloc(&c->uf, ns);
-  if (cfg->timeorder == 4) {
-    rc |= dalloc(&c->pk2, ns);
-    rc |= dalloc(&c->ucum, ns);
-  }
+  const bool fv = cfg->doenoflux == HG_ENOFINITEVOLUME;
+  if (cfg->timeorder == 4) rc |= dalloc(&c->pk2, ns);
+  if (cfg->timeorder == 4 || fv) rc |= dalloc(&c->ucum, ns);
+  if (fv) rc |= dalloc(&c->ucons, ns);
   for (int k = 0; k < 3; k++) rc |= dalloc(&c->F[k], ns);
   for (int k = 0; k < 3; k++) rc |= dalloc(&c->emf[k], nz);
   if (cfg->lim == HG_LIM_WENO5BND) {
@@ -111,9 +119,9 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
 
 extern "C" int hg_ctx_destroy(hg_ctx *c) {
   if (!c) return 0;
-  double *bufs[] = {c->p,      c->pk,     c->pk2,    c->ucum, c->uf,  c->F[0], c->F[1],  c->F[2],
-                    c->emf[0], c->emf[1], c->emf[2], c->pl,   c->pr,  c->pl2,  c->pr2,   c->pl3,
-                    c->pr3,    c->aux,    c->geo,    c->corn, c->conn, c->floors, c->dt};
+  double *bufs[] = {c->p,      c->pk,     c->pk2,    c->ucum,  c->ucons, c->uf,  c->F[0],  c->F[1],
+                    c->F[2],   c->emf[0], c->emf[1], c->emf[2], c->pl,    c->pr,  c->pl2,   c->pr2,
+                    c->pl3,    c->pr3,    c->aux,    c->geo,    c->corn,  c->conn, c->floors, c->dt};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (c->ndt_bits) (void)hipFree(c->ndt_bits);
@@ -213,7 +221,41 @@ int hg::fetch_prims(hg_ctx *c, const double *dev, double *host) {
 
 extern "C" int hg_set_prims(hg_ctx *c, const double *p) {
   if (!c || !p) { set_error("hg_set_prims: null argument"); return 1; }
+  c->ucons_stale = true;
   return copy_prims(c, c->p, p);
+}
+
+static int need_fv(hg_ctx *c, const char *who) {
+  if (!c) { set_error("%s: null argument", who); return 1; }
+  if (!c->ucons) { set_error("%s: the context does not run the finite-volume scheme", who); return 1; }
+  return 0;
+}
+
+extern "C" int hg_set_conserved(hg_ctx *c, const double *U) {
+  if (need_fv(c, "hg_set_conserved")) return 1;
+  if (!U) { set_error("hg_set_conserved: null argument"); return 1; }
+  const Dims &d = c->d;
+  std::vector<double> act((size_t)d.m1 * d.m2 * d.m3 * NPR, 0.0);
+  for (int i = 0; i < d.n1; i++)
+    for (int j = 0; j < d.n2; j++)
+      for (int k = 0; k < d.n3; k++) {
+        size_t h = ((((size_t)(i + d.nb1)) * d.m2 + (j + d.nb2)) * d.m3 + (k + d.nb3)) * NPR;
+        for (int pl = 0; pl < NPR; pl++) act[h + pl] = U[h + pl];
+      }
+  if (copy_prims(c, c->ucons, act.data())) return 1;
+  c->ucons_stale = false;
+  return 0;
+}
+
+extern "C" int hg_get_conserved(hg_ctx *c, double *U) {
+  if (need_fv(c, "hg_get_conserved")) return 1;
+  if (!U) { set_error("hg_get_conserved: null argument"); return 1; }
+  if (c->ucons_stale) {
+    if (!c->have_geometry) { set_error("hg_get_conserved: geometry not set"); return 1; }
+    hg::ensure_conserved(c, nullptr);
+  }
+  HIPCHECK(hipDeviceSynchronize());
+  return fetch_prims(c, c->ucons, U);
 }
 
 extern "C" int hg_get_prims(hg_ctx *c, double *p) {
@@ -283,6 +325,7 @@ extern "C" int hg_post_fixup(hg_ctx *c, const int32_t *codes, int finalstep, voi
   hipStream_t s = (hipStream_t)stream;
   HIPCHECK(hipStreamSynchronize(s));
   if (codes && codes_to_device(c, codes)) return 1;
+  ensure_conserved(c, s);
   launch_bound(c->p, c->geo, c->d, c->cfg.gammamax, c->err, s);
   launch_post_fixup(c, c->p, finalstep ? 1 : 0, s);
   launch_bound(c->p, c->geo, c->d, c->cfg.gammamax, c->err, s);
@@ -400,8 +443,8 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
     hipStream_t sd = ds[dir];
     if (f.lim != HG_LIM_WENO5BND) {
       prof_mark(c, HG_PROF_FLUX1 + dir - 1, sd);
-      launch_flux_point(dir, f.lim == HG_LIM_PARA, pb, c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir],
-                        f.fluxmethod, c->ndt_bits + dir - 1, c->err, c->tune, sd);
+      launch_flux_point(dir, f.lim == HG_LIM_PARA, f.doenoflux == HG_ENOFINITEVOLUME, pb, c->geo, c->F[dir - 1], d,
+                        f.gam, f.cour, f.dx[dir], f.fluxmethod, c->ndt_bits + dir - 1, c->err, c->tune, sd);
       prof_mark(c, -1, sd);
     } else {
       if (dir != 1) prof_mark(c, HG_PROF_RECON1 + dir - 1, sd);
@@ -515,12 +558,24 @@ static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, const
   prof_mark(c, -1, s);
 }
 
+// the finite-volume scheme's conserved state from p when p was set since
+// (initbase.c:147 init_conservatives); kernels only
+void hg::ensure_conserved(hg_ctx *c, hipStream_t s) {
+  if (!c->ucons || !c->ucons_stale) return;
+  launch_cons_from_prims(c->p, c->geo, c->ucons, c->d, c->cfg.gam, c->err, s);
+  c->ucons_stale = false;
+}
+
 // step_ch.c:35 step_ch_simplempi over the substeps of step_ch.c:442
 // setup_rktimestep; pi is p on every substep
 void hg::enqueue_step(hg_ctx *c, hipStream_t s) {
+  ensure_conserved(c, s);
   if (c->cfg.timeorder != 4) {
-    // TIMEORDER 2 midpoint: CUf[.][2] = 0.5, 1; pb = p, pk; pf = pk, p
-    RkStage a{0.5, 0.0, 0.0, 0.0, nullptr, 1, 0}, b{1.0, 0.0, 0.0, 1.0, nullptr, 0, 1};
+    // TIMEORDER 2 midpoint: CUf[.][2] = 0.5, 1; Cunew = {0, 0, 0}, {0, 0, 1};
+    // pb = p, pk; pf = pk, p.  The standard scheme inverts Uf, which equals
+    // that ucum; the finite-volume one keeps ucum itself
+    RkStage a{0.5, 0.0, 0.0, 0.0, c->ucons ? c->ucum : nullptr, 1, 0, c->ucons};
+    RkStage b{1.0, 0.0, 0.0, 1.0, c->ucons ? c->ucum : nullptr, 0, 1, c->ucons};
     enqueue_substep(c, c->p, c->p, c->pk, a, 0, s);
     enqueue_substep(c, c->p, c->pk, c->p, b, 1, s);
     return;
@@ -531,7 +586,7 @@ void hg::enqueue_step(hg_ctx *c, hipStream_t s) {
   const double cn[4][3] = {{1.0, 1.0 / 6.0, 0.0}, {0.0, 1.0 / 3.0, 0.0}, {0.0, 1.0 / 3.0, 0.0}, {0.0, 1.0 / 6.0, 0.0}};
   double *pb[4] = {c->p, c->pk, c->pk2, c->pk}, *pf[4] = {c->pk, c->pk2, c->pk, c->p};
   for (int t = 0; t < 4; t++) {
-    RkStage rk{cuf2[t], cn[t][0], cn[t][1], cn[t][2], c->ucum, t == 0, t == 3};
+    RkStage rk{cuf2[t], cn[t][0], cn[t][1], cn[t][2], c->ucum, t == 0, t == 3, c->ucons};
     enqueue_substep(c, c->p, pb[t], pf[t], rk, t == 3, s);
   }
 }

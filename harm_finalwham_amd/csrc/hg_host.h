@@ -14,6 +14,10 @@ struct hg_ctx {
   hg::Dims d;
   double *p = nullptr, *pk = nullptr;        // primitives and the first intermediate state
   double *pk2 = nullptr, *ucum = nullptr;    // RK4 only: second intermediate state, ucum
+  // finite-volume scheme: carried conserved state; stale until derived from
+  // p after hg_set_prims / hg_init_torus
+  double *ucons = nullptr;
+  bool ucons_stale = true;
   double *F[3] = {nullptr, nullptr, nullptr};
   double *emf[3] = {nullptr, nullptr, nullptr};
   double *pl = nullptr, *pr = nullptr;       // WENO5BND interface states along i
@@ -66,8 +70,11 @@ void set_error(const char *fmt, ...);
 int upload_geometry_tables(hg_ctx *c, const double *gcov, const double *gcon, const double *gcovpert,
                            const double *gdet, const double *conn, const double *r_cent);
 int copy_prims(hg_ctx *c, double *dev, const double *host);
-// one RK2 step of the context on stream s (kernels only: capturable)
+// one step of the context on stream s (kernels only: capturable)
 void enqueue_step(hg_ctx *c, hipStream_t s);
+// finite-volume scheme: derive the carried conserved state from p if p was
+// set since (kernels only)
+void ensure_conserved(hg_ctx *c, hipStream_t s);
 int fetch_prims(hg_ctx *c, const double *dev, double *host);
 // returns 1 (with the error set) when the work counter cannot be set up
 // post_fixup buffers (hg_fixup.hip)

@@ -19,10 +19,15 @@ struct Tune {
 };
 
 void launch_bound(double *p, const double *geo, const Dims &d, double gammamax, int *err, hipStream_t s);
-// MC (para = false) or PARA (para = true) reconstruction fused with the flux
-void launch_flux_point(int dir, bool para, const double *p, const double *geo, double *F, const Dims &d,
+// MC (para = false) or PARA (para = true) reconstruction fused with the
+// flux, over the faces of the standard (fv = false) or the finite-volume
+// scheme's loops
+void launch_flux_point(int dir, bool para, bool fv, const double *p, const double *geo, double *F, const Dims &d,
                        double gam, double cour, double dxdir, int fluxmethod, unsigned long long *ndt_bits,
                        int *err, const Tune &tu, hipStream_t s);
+// primtoU of every active zone of p into U (same layout)
+void launch_cons_from_prims(const double *p, const double *geo, double *U, const Dims &d, double gam, int *err,
+                            hipStream_t s);
 void launch_flux_lr(int dir, const double *pleft, const double *pright, const double *geo, double *F,
                     const Dims &d, double gam, double cour, double dxdir, int fluxmethod,
                     unsigned long long *ndt_bits, int *err, const Tune &tu, hipStream_t s);
@@ -35,8 +40,10 @@ void launch_emf(const double *F1, const double *F2, const double *F3, const doub
 // cn2 Uf, zeroed first on the first substep and inverted on the last
 struct RkStage {
   double cuf2, cn0, cn1, cn2;
-  double *ucum;  // null: midpoint scheme, the last substep inverts Uf
+  double *ucum;   // null: midpoint scheme, the last substep inverts Uf
   int first, last;
+  double *ucons;  // finite-volume scheme: the carried conserved state (Ui),
+                  // replaced by ucum on the last substep; null otherwise
 };
 void launch_update(const double *pi, const double *pb, double *pf, double *ubuf, const double *F1,
                    const double *F2, const double *F3, const double *e1, const double *e2, const double *e3,

@@ -242,6 +242,19 @@ __host__ __device__ inline FaceRange face_range(const Dims &d, int dir) {
   if (dir == 3) { f.ks = 0; f.nk = d.n3 + s3; }
   return f;
 }
+// the finite-volume scheme's flux loops (initbase.c:949-993 with avgscheme
+// DONOR): faces 0..n along dir, every ghost layer transverse to it
+__host__ __device__ inline FaceRange face_range_fv(const Dims &d, int dir) {
+  int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  FaceRange f;
+  f.is = -d.nb1; f.ni = d.m1;
+  f.js = -d.nb2; f.nj = d.m2;
+  f.ks = -d.nb3; f.nk = d.m3;
+  if (dir == 1) { f.is = 0; f.ni = d.n1 + s1; }
+  if (dir == 2) { f.js = 0; f.nj = d.n2 + s2; }
+  if (dir == 3) { f.ks = 0; f.nk = d.n3 + s3; }
+  return f;
+}
 
 // global.h:361 MINMOD
 __device__ __forceinline__ double minmod(double a, double b) {
@@ -302,10 +315,9 @@ __device__ __forceinline__ void para4(const double y[5], double &lout, double &r
 // (interppoint.c:289), turned into the face states by flux.c:753 getp2interplr
 template <int DIR, bool PARA>
 __device__ __forceinline__ void flux_point_body(const double *__restrict__ p, const double *__restrict__ geo,
-                                                double *__restrict__ F, const Dims &d, double gam, double cour,
-                                                double dxdir, int fluxmethod,
+                                                double *__restrict__ F, const Dims &d, const FaceRange &fr,
+                                                double gam, double cour, double dxdir, int fluxmethod,
                                                 unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
-  FaceRange fr = face_range(d, DIR);
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)fr.ni * fr.nj * fr.nk;
   double dtij = 1.e+50;
@@ -353,19 +365,19 @@ __device__ __forceinline__ void flux_point_body(const double *__restrict__ p, co
 
 template <int DIR>
 __global__ void __launch_bounds__(256) k_flux_mc(const double *__restrict__ p, const double *__restrict__ geo,
-                                                 double *__restrict__ F, Dims d, double gam, double cour,
-                                                 double dxdir, int fluxmethod,
+                                                 double *__restrict__ F, Dims d, FaceRange fr, double gam,
+                                                 double cour, double dxdir, int fluxmethod,
                                                  unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
-  flux_point_body<DIR, false>(p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  flux_point_body<DIR, false>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
 template <int DIR>
 __global__ void __launch_bounds__(256) k_flux_para(const double *__restrict__ p, const double *__restrict__ geo,
-                                                   double *__restrict__ F, Dims d, double gam, double cour,
-                                                   double dxdir, int fluxmethod,
+                                                   double *__restrict__ F, Dims d, FaceRange fr, double gam,
+                                                   double cour, double dxdir, int fluxmethod,
                                                    unsigned long long *__restrict__ ndt_bits,
                                                    int *__restrict__ err) {
-  flux_point_body<DIR, true>(p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  flux_point_body<DIR, true>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
 template <int DIR>
@@ -468,7 +480,7 @@ __global__ void k_emf(const double *__restrict__ F1, const double *__restrict__ 
 // (one kernel holding both needs more registers than either, and the Uf
 // round trip through HBM costs less than the occupancy it loses)
 // ---------------------------------------------------------------------------
-template <bool RK4>
+template <bool UCUM, bool FV>
 __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, const double *__restrict__ pb,
                                                 double *__restrict__ ubuf, const double *__restrict__ F1,
                                                 const double *__restrict__ F2, const double *__restrict__ F3,
@@ -491,8 +503,13 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
     Geom G;
     load_geom(geo, GL_CENT, d.gplane, d.gidx(i, j), G);
     double Ui[NPR], dUg[NPR], prb[NPR];
-    bool bad;
-    {
+    bool bad = false;
+    if (FV) {
+      // advance.c:321-334: Ui is the conserved state carried from the last
+      // step's ucum (or set from the primitives at the start)
+#pragma unroll
+      for (int pl = 0; pl < NPR; pl++) Ui[pl] = rk.ucons[z + pl * P];
+    } else {
       double pri[NPR];
 #pragma unroll
       for (int pl = 0; pl < NPR; pl++) pri[pl] = pi[z + pl * P];
@@ -578,13 +595,18 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
       // advance.c:922 dUtoU with CUf = {1, 0, cuf2}: uf carries no memory
       // across substeps (CUf[1] = 0 in both schemes)
       Uf[pl] = 1.0 * Ui[pl] + 0.0 * 0.0 + rk.cuf2 * dt * (dUr + dUg[pl]);
-      if (RK4) {
+      if (UCUM) {
         // ucum, zeroed on the first substep (advance.c:141), collects
-        // Cunew = {cn0, cn1, cn2}; the last substep inverts ucum
+        // Cunew = {cn0, cn1, cn2}; the last substep inverts ucum, which the
+        // finite-volume scheme also carries into the next step
         double uc = rk.first ? 0.0 : rk.ucum[z + pl * P];
         uc += rk.cn0 * Ui[pl] + rk.cn1 * dt * (dUr + dUg[pl]) + rk.cn2 * Uf[pl];
-        if (rk.last) Uf[pl] = uc;
-        else rk.ucum[z + pl * P] = uc;
+        if (rk.last) {
+          Uf[pl] = uc;
+          if (FV) rk.ucons[z + pl * P] = uc;
+        } else {
+          rk.ucum[z + pl * P] = uc;
+        }
       }
       // midpoint scheme: ucum equals Uf on the final substep (Cunew = {0,0,1})
     }
@@ -594,12 +616,14 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
   }
 }
 
+template <bool FV>
 __global__ void __launch_bounds__(256) k_invert(const double *__restrict__ ubuf, const double *__restrict__ pb,
                                                 double *__restrict__ pf, const double *__restrict__ geo,
                                                 const double *__restrict__ floors, int finalstep, Dims d,
                                                 double gam, double gammamax,
                                                 unsigned long long *__restrict__ counters,
-                                                int8_t *__restrict__ pflag, int *__restrict__ err) {
+                                                int8_t *__restrict__ pflag, double *__restrict__ ucons,
+                                                int *__restrict__ err) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
   unsigned long long nfail = 0, nit = 0;
@@ -626,12 +650,35 @@ __global__ void __launch_bounds__(256) k_invert(const double *__restrict__ ubuf,
     nfail = flag ? 1ull : 0ull;
     if (finalstep) {
       // fixup.c:341 fixup1zone with set_density_floors rescaletype 0
+      double pr0[NPR];
+      if (FV)
+#pragma unroll
+        for (int pl = 0; pl < NPR; pl++) pr0[pl] = pr[pl];
+      bool changed = false;
       double f0 = floors[d.cidx(i, j)], f1 = floors[d.cplane + d.cidx(i, j)];
       if (f0 < 1.e-20) f0 = 1.e-20;
       if (f1 < 1.e-20) f1 = 1.e-20;
-      if (f0 > pr[RHO]) pr[RHO] = f0;
-      if (f1 > pr[UU]) pr[UU] = f1;
-      if (limit_gamma(gammamax, pr, G) > 0) bad = true;
+      if (f0 > pr[RHO]) { pr[RHO] = f0; changed = true; }
+      if (f1 > pr[UU]) { pr[UU] = f1; changed = true; }
+      int lg = limit_gamma(gammamax, pr, G);
+      if (lg > 0) bad = true;
+      if (lg == -1) changed = true;
+      if (FV && changed) {
+        // fixup.c:443 diag_fixup -> fixup.c:240-248: the carried conserved
+        // state takes the change the floors made, each U passed through
+        // UtoU (UDIAG -> UEVOLVE: divided, then multiplied by gdet), once
+        // per energy region (three with DOJETDIAG, global.h:436)
+        double U0[NPR], U1[NPR];
+        primtoU(pr0, G, gam, U0);
+        primtoU(pr, G, gam, U1);
+#pragma unroll
+        for (int pl = 0; pl < NPR; pl++) {
+          const double du = (U1[pl] / G.g) * G.g - (U0[pl] / G.g) * G.g;
+          double u = ucons[z + pl * P];
+          for (int region = 0; region < NENERREGIONS; region++) u = u + du;
+          ucons[z + pl * P] = u;
+        }
+      }
     }
     if (bad) atomicOr(err, 4);
 #pragma unroll
@@ -710,21 +757,50 @@ void launch_bound(double *p, const double *geo, const Dims &d, double gammamax, 
   }
 }
 
-void launch_flux_point(int dir, bool para, const double *p, const double *geo, double *F, const Dims &d,
+void launch_flux_point(int dir, bool para, bool fv, const double *p, const double *geo, double *F, const Dims &d,
                        double gam, double cour, double dxdir, int fluxmethod, unsigned long long *ndt_bits,
                        int *err, const Tune &tu, hipStream_t s) {
-  FaceRange fr = face_range(d, dir);
+  FaceRange fr = fv ? face_range_fv(d, dir) : face_range(d, dir);
   int64_t n = (int64_t)fr.ni * fr.nj * fr.nk;
   dim3 g(nblocks(n, tu.bs_flux)), b(tu.bs_flux);
   if (para) {
-    if (dir == 1) hipLaunchKernelGGL(k_flux_para<1>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-    if (dir == 2) hipLaunchKernelGGL(k_flux_para<2>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-    if (dir == 3) hipLaunchKernelGGL(k_flux_para<3>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 1) hipLaunchKernelGGL(k_flux_para<1>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 2) hipLaunchKernelGGL(k_flux_para<2>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 3) hipLaunchKernelGGL(k_flux_para<3>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
     return;
   }
-  if (dir == 1) hipLaunchKernelGGL(k_flux_mc<1>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-  if (dir == 2) hipLaunchKernelGGL(k_flux_mc<2>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-  if (dir == 3) hipLaunchKernelGGL(k_flux_mc<3>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 1) hipLaunchKernelGGL(k_flux_mc<1>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 2) hipLaunchKernelGGL(k_flux_mc<2>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 3) hipLaunchKernelGGL(k_flux_mc<3>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+}
+
+// initbase.c:18 init_conservatives (primtoU of every active zone) into the
+// carried conserved state of the finite-volume scheme
+__global__ void __launch_bounds__(256) k_cons_from_prims(const double *__restrict__ p, const double *__restrict__ geo,
+                                                         double *__restrict__ U, Dims d, double gam,
+                                                         int *__restrict__ err) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
+  if (t >= total) return;
+  int i = (int)(t % d.n1);
+  int64_t r = t / d.n1;
+  int k = (int)(r % d.n3);
+  int j = (int)(r / d.n3);
+  int64_t z = d.idx(i, j, k);
+  Geom G;
+  load_geom(geo, GL_CENT, d.gplane, d.gidx(i, j), G);
+  double pr[NPR], Uz[NPR];
+#pragma unroll
+  for (int pl = 0; pl < NPR; pl++) pr[pl] = p[z + pl * d.spl];
+  if (primtoU(pr, G, gam, Uz)) atomicOr(err, 4);
+#pragma unroll
+  for (int pl = 0; pl < NPR; pl++) U[z + pl * d.spl] = Uz[pl];
+}
+
+void launch_cons_from_prims(const double *p, const double *geo, double *U, const Dims &d, double gam, int *err,
+                            hipStream_t s) {
+  int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
+  hipLaunchKernelGGL(k_cons_from_prims, dim3(nblocks(n, 256)), dim3(256), 0, s, p, geo, U, d, gam, err);
 }
 
 void launch_flux_lr(int dir, const double *pleft, const double *pright, const double *geo, double *F,
@@ -751,14 +827,24 @@ void launch_update(const double *pi, const double *pb, double *pf, double *ubuf,
                    const RkStage &rk, int finalstep, const Dims &d, double gam, const double *dx, double gammamax,
                    unsigned long long *counters, int8_t *pflag, int *err, const Tune &tu, hipStream_t s) {
   int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
-  if (rk.ucum)
-    hipLaunchKernelGGL(k_update<true>, dim3(nblocks(n, tu.bs_update)), dim3(tu.bs_update), 0, s, pi, pb, ubuf, F1, F2, F3, e1, e2,
-                       e3, geo, conn, dtp, rk, d, gam, 1.0 / dx[1], 1.0 / dx[2], 1.0 / dx[3], err);
+  dim3 g(nblocks(n, tu.bs_update)), b(tu.bs_update);
+  const double i1 = 1.0 / dx[1], i2 = 1.0 / dx[2], i3 = 1.0 / dx[3];
+  if (rk.ucons)
+    hipLaunchKernelGGL((k_update<true, true>), g, b, 0, s, pi, pb, ubuf, F1, F2, F3, e1, e2, e3, geo, conn, dtp, rk, d,
+                       gam, i1, i2, i3, err);
+  else if (rk.ucum)
+    hipLaunchKernelGGL((k_update<true, false>), g, b, 0, s, pi, pb, ubuf, F1, F2, F3, e1, e2, e3, geo, conn, dtp, rk,
+                       d, gam, i1, i2, i3, err);
   else
-    hipLaunchKernelGGL(k_update<false>, dim3(nblocks(n, tu.bs_update)), dim3(tu.bs_update), 0, s, pi, pb, ubuf, F1, F2, F3, e1,
-                       e2, e3, geo, conn, dtp, rk, d, gam, 1.0 / dx[1], 1.0 / dx[2], 1.0 / dx[3], err);
-  hipLaunchKernelGGL(k_invert, dim3(nblocks(n, tu.bs_invert)), dim3(tu.bs_invert), 0, s, ubuf, pb, pf, geo, floors, finalstep, d,
-                     gam, gammamax, counters, pflag, err);
+    hipLaunchKernelGGL((k_update<false, false>), g, b, 0, s, pi, pb, ubuf, F1, F2, F3, e1, e2, e3, geo, conn, dtp, rk,
+                       d, gam, i1, i2, i3, err);
+  dim3 gi(nblocks(n, tu.bs_invert)), bi(tu.bs_invert);
+  if (rk.ucons && finalstep)
+    hipLaunchKernelGGL(k_invert<true>, gi, bi, 0, s, ubuf, pb, pf, geo, floors, finalstep, d, gam, gammamax, counters,
+                       pflag, rk.ucons, err);
+  else
+    hipLaunchKernelGGL(k_invert<false>, gi, bi, 0, s, ubuf, pb, pf, geo, floors, finalstep, d, gam, gammamax, counters,
+                       pflag, nullptr, err);
 }
 
 void launch_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe, hipStream_t s) {

@@ -10,6 +10,7 @@
 // arguments every field sits in its own 8-byte stack slot, so the token is
 // the low 32 bits of defcoord's bits (0 for the integral values it takes) and
 // the fields after it are not displaced.  The reader skips that token.
+#include <algorithm>
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
@@ -161,8 +162,12 @@ extern "C" int hg_rdump_read(hg_ctx *c, const char *path, hg_rdump_info *info, d
   }
   const Dims &d = c->d;
   size_t nact = (size_t)d.n1 * d.n2 * d.n3;
-  std::vector<double> pa(nact * NPR);
-  if (hg_rdump_parse(path, d.n1, d.n2, d.n3, pa.data(), unew, info)) return 1;
+  std::vector<double> pa(nact * NPR), ua;
+  // the finite-volume scheme resumes from the dump's unew (initbase.c:155-160)
+  if (c->ucons) ua.resize(nact * NPR);
+  double *uout = c->ucons ? ua.data() : unew;
+  if (hg_rdump_parse(path, d.n1, d.n2, d.n3, pa.data(), uout, info)) return 1;
+  if (c->ucons && unew) memcpy(unew, ua.data(), ua.size() * sizeof(double));
   // the active zones into the state (ghost zones refilled by bound_prim)
   std::vector<double> full((size_t)d.m1 * d.m2 * d.m3 * NPR);
   if (hg_get_prims(c, full.data())) return 1;
@@ -174,5 +179,16 @@ extern "C" int hg_rdump_read(hg_ctx *c, const char *path, hg_rdump_info *info, d
         for (int pl = 0; pl < NPR; pl++) full[dst + pl] = pa[src + pl];
       }
   if (hg_set_prims(c, full.data())) return 1;
+  if (c->ucons) {
+    std::fill(full.begin(), full.end(), 0.0);
+    for (int i = 0; i < d.n1; i++)
+      for (int j = 0; j < d.n2; j++)
+        for (int k = 0; k < d.n3; k++) {
+          size_t src = (((size_t)i * d.n2 + j) * d.n3 + k) * NPR;
+          size_t dst = ((((size_t)(i + d.nb1)) * d.m2 + (j + d.nb2)) * d.m3 + (k + d.nb3)) * NPR;
+          for (int pl = 0; pl < NPR; pl++) full[dst + pl] = ua[src + pl];
+        }
+    if (hg_set_conserved(c, full.data())) return 1;
+  }
   return hg_bound_prim(c, stream);
 }

@@ -718,6 +718,7 @@ extern "C" int hg_init_torus(hg_ctx *c, double rin, double rmax, double kappa, d
     return 1;
   }
   if (copy_prims(c, c->p, P.data())) return 1;
+  c->ucons_stale = true;
   launch_bound(c->p, c->geo, c->d, f.gammamax, c->err, 0);
   launch_reset_ndt(c->ndt_bits, 0);
   launch_set_dt(c->p, c->geo, c->d, f.gam, f.cour, f.dx, c->ndt_bits, c->err, 0);

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libharmgpu.so")
 
 HG_LIM_MC, HG_LIM_PARA, HG_LIM_WENO5BND = 3, 4, 17
+HG_NOENOFLUX, HG_ENOFINITEVOLUME = 0, 3
 HG_FLUX_LAXF, HG_FLUX_HLL = 0, 1
 HG_C2P_FP64, HG_C2P_MIXED = 0, 1
 
@@ -25,7 +26,7 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_mlp_create", "hg_mlp_destroy", "hg_mlp_nparams", "hg_mlp_get_params", "hg_mlp_set_params",
            "hg_mlp_features", "hg_mlp_forward", "hg_mlp_gradient", "hg_mlp_train",
            "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag", "hg_diagnostics", "hg_diag_record", "hg_rdump_parse",
-           "hg_rdump_read"]
+           "hg_rdump_read", "hg_set_conserved", "hg_get_conserved"]
 HG_PROF_NCLASS = 10
 
 
@@ -82,7 +83,7 @@ class HgConfig(C.Structure):
                 ("gam", C.c_double), ("cour", C.c_double), ("safe", C.c_double),
                 ("defcon", C.c_double), ("dx", C.c_double * 4),
                 ("rhomin", C.c_double), ("uumin", C.c_double), ("gammamax", C.c_double),
-                ("timeorder", C.c_int)]
+                ("timeorder", C.c_int), ("doenoflux", C.c_int)]
 
 
 _lib = None
@@ -119,6 +120,8 @@ def lib():
                                 C.c_uint64, d]
     L.hg_set_prims.argtypes = [vp, d]
     L.hg_get_prims.argtypes = [vp, d]
+    L.hg_set_conserved.argtypes = [vp, d]
+    L.hg_get_conserved.argtypes = [vp, d]
     L.hg_bound_prim.argtypes = [vp, vp]
     L.hg_fluxcalc.argtypes = [vp, d, d, d, d, vp]
     L.hg_step.argtypes = [vp, d, i64p, i64p, vp]
@@ -177,10 +180,10 @@ class Grid:
 
     def __init__(self, n1, n2, n3, nb=6, lim=HG_LIM_WENO5BND, fluxmethod=HG_FLUX_HLL, gam=4.0 / 3.0,
                  cour=0.8, dx=(0.0, 1.0, 1.0, 1.0), safe=1.3, defcon=1.0, rhomin=1e-4, uumin=1e-6,
-                 gammamax=25.0, timeorder=2):
+                 gammamax=25.0, timeorder=2, doenoflux=HG_NOENOFLUX):
         cfg = HgConfig()
         cfg.n1, cfg.n2, cfg.n3, cfg.nb, cfg.lim, cfg.fluxmethod = n1, n2, n3, nb, lim, fluxmethod
-        cfg.timeorder = timeorder
+        cfg.timeorder, cfg.doenoflux = timeorder, doenoflux
         cfg.gam, cfg.cour, cfg.safe, cfg.defcon = gam, cour, safe, defcon
         for k in range(4):
             cfg.dx[k] = dx[k]
@@ -231,6 +234,17 @@ class Grid:
         p = np.zeros(self.shape)
         _check(lib().hg_get_prims(self.h, _d(p)))
         return p
+
+    def set_conserved(self, U):
+        """The finite-volume scheme's carried conserved state (layout of p)."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == self.shape, (U.shape, self.shape)
+        _check(lib().hg_set_conserved(self.h, _d(U)))
+
+    def get_conserved(self):
+        U = np.zeros(self.shape)
+        _check(lib().hg_get_conserved(self.h, _d(U)))
+        return U
 
     def bound_prim(self):
         _check(lib().hg_bound_prim(self.h, None))

@@ -53,6 +53,9 @@ extern "C" {
 #define HG_LIM_WENO5BND 17
 #define HG_FLUX_LAXF 0
 #define HG_FLUX_HLL 1
+/* DOENOFLUX (global.h:188-191) */
+#define HG_NOENOFLUX 0
+#define HG_ENOFINITEVOLUME 3
 
 typedef struct hg_ctx hg_ctx;
 
@@ -69,6 +72,10 @@ typedef struct {
   double rhomin, uumin;/* floor coefficients RHOMIN, UUMIN (initbase.c:459-460) */
   double gammamax;     /* GAMMAMAX (initbase.c:471) */
   int timeorder;       /* TIMEORDER: 2 (or 0) midpoint, 4 classical RK4 (step_ch.c:442 setup_rktimestep) */
+  int doenoflux;       /* HG_NOENOFLUX (advance.c:58 advance_standard) or HG_ENOFINITEVOLUME
+                          (advance.c:245 advance_finitevolume with the torus file's avgscheme
+                          DONOR: the conserved state is carried from step to step and the
+                          flux loops span every transverse ghost layer); MC or PARA only */
 } hg_config;
 
 /* Context with device-resident state for one grid.  The caller owns the
@@ -109,6 +116,15 @@ int hg_init_torus(hg_ctx *ctx, double rin, double rmax, double kappa, double bet
 
 int hg_set_prims(hg_ctx *ctx, const double *p_host);
 int hg_get_prims(hg_ctx *ctx, double *p_host);
+
+/* The conserved state the finite-volume scheme carries between steps (unew
+   of the reference, restart.c:244), in the layout of hg_set_prims (ghost
+   entries ignored on input, zero on output).  hg_set_prims and
+   hg_init_torus reset it to U(p) (initbase.c:147 init_conservatives) at the
+   next step; hg_set_conserved overrides that, as a restart does
+   (initbase.c:155-160).  Errors unless doenoflux is HG_ENOFINITEVOLUME. */
+int hg_set_conserved(hg_ctx *ctx, const double *U_host);
+int hg_get_conserved(hg_ctx *ctx, double *U_host);
 
 /* bound_prim (step_ch.c:568) on the current primitives. */
 int hg_bound_prim(hg_ctx *ctx, void *stream);
@@ -190,7 +206,8 @@ int hg_diag_record(hg_ctx *ctx, int on);
    NULL), and the header fields the hot path uses into *info (may be NULL);
    the header's defcoord is skipped (the reference prints that double with
    "%d").  hg_rdump_read puts the primitives into a context and runs
-   bound_prim.  Writing rdumps is not provided. */
+   bound_prim; a context running the finite-volume scheme also takes unew
+   as its carried conserved state.  Writing rdumps is not provided. */
 typedef struct {
   int n[3];
   double t, tf;

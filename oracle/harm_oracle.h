@@ -82,7 +82,10 @@ int or_utoprimgen(const double *U, const or_geom *g, double gam, double *pr,
                   int *pflag, int *iters);
 
 /* ---- fixup1zone (fixup.c:341) ---- */
-void or_fixup1zone(double *pr, const or_geom *g, double rhofloor, double ufloor, double gammamax);
+/* returns 1 when a floor or the gamma limit changed pr */
+int or_fixup1zone(double *pr, const or_geom *g, double rhofloor, double ufloor, double gammamax);
+/* fixup.c:181 diag_fixup's update of the finite-volume conserved state */
+void or_diag_fixup_fv(const double *pr0, const double *pr, const or_geom *g, double gam, double *u);
 int or_limit_gamma(double gammamax, double *pr, const or_geom *g);
 
 /* ---- batch helpers exported for tests ---- */
@@ -107,11 +110,18 @@ typedef struct {
      conn [(n1+2nb)*(n2+2nb)][64], rfloor [(n1+2nb)*(n2+2nb)][2] (rho, u floors at CENT) */
   const double *gcov, *gcon, *gcovpert, *gdet, *conn, *floors;
   int timeorder;            /* 2 (or 0) = midpoint RK2, 4 = RK4 (step_ch.c:442) */
+  int fv;                   /* 1 = ENOFINITEVOLUME with avgscheme DONOR (advance.c:245) */
 } or_grid;
 
 /* one step on p (full array with ghosts, layout [i][j][k][pl] with
-   dims (n1+2nb)x(n2+2nb)x(n3+2nb3)); returns the next dt in *dt_io. */
+   dims (n1+2nb)x(n2+2nb)x(n3+2nb3)); returns the next dt in *dt_io.
+   Finite-volume grids (fv) use or_step_fv. */
 int or_step(const or_grid *G, double *p, double *dt_io, int64_t *nfail, int64_t *iters);
+/* one finite-volume step: ucons (layout of p, active zones used) is the
+   conserved state carried between steps, replaced by the step's ucum */
+int or_step_fv(const or_grid *G, double *p, double *ucons, double *dt_io, int64_t *nfail, int64_t *iters);
+/* initbase.c:18 init_conservatives: U(p) of the active zones into ucons */
+void or_cons_from_prims(const or_grid *G, const double *p, double *ucons);
 /* fluxcalc + flux_ct on p for diagnostics; F arrays same layout as p */
 int or_fluxcalc(const or_grid *G, const double *p, double *F1, double *F2, double *F3,
                 double *ndt);
@@ -121,6 +131,8 @@ int or_bound_prim(const or_grid *G, double *p);
    average of failed zones, on p (bounded) and pflag (full array of int,
    layout of p without the pl index: inversion codes in the active zones) */
 int or_post_fixup(const or_grid *G, double *p, int *pflag, int finalstep);
+/* the same on a finite-volume grid: fixed zones also move ucons (fixup.c:1428) */
+int or_post_fixup_fv(const or_grid *G, double *p, int *pflag, int finalstep, double *ucons);
 void or_bound_pflag(const or_grid *G, int *pflag);
 /* interface states of the reconstruction along dir for the whole grid */
 int or_reconstruct(const or_grid *G, const double *p, int dir, double *pleft, double *pright);

@@ -127,7 +127,7 @@ static int counted(int f, int finalstep)
 }
 
 /* fixup.c:669 fixup_utoprim */
-static int fixup_utoprim(const or_grid *G, const or_dims *d, double *pv, int *pf, int finalstep)
+static int fixup_utoprim(const or_grid *G, const or_dims *d, double *pv, int *pf, int finalstep, double *ucons)
 {
   int s1 = d->n1 > 1, s2 = d->n2 > 1, s3 = d->n3 > 1;
   size_t nz = (size_t)d->m1 * d->m2 * d->m3;
@@ -203,6 +203,9 @@ static int fixup_utoprim(const or_grid *G, const or_dims *d, double *pv, int *pf
             int c = counted(f, finalstep);
             if (c == 2) { free(pa); return 1; }
             if (c == 1) pf[z] = FAILFIXED;
+            /* fixup.c:1428 diag_fixup, which moves the finite-volume
+               conserved state on the final substep */
+            if (c == 1 && ucons && finalstep) or_diag_fixup_fv(&pa[z * 8], &pv[z * 8], &g, G->gam, &ucons[z * 8]);
           }
         }
       }
@@ -216,5 +219,13 @@ int or_post_fixup(const or_grid *G, double *p, int *pflag, int finalstep)
   or_dims d = or_get_dims(G);
   or_bound_pflag(G, pflag);
   if (checksolution(G, &d, p, pflag)) return 1;
-  return fixup_utoprim(G, &d, p, pflag, finalstep);
+  return fixup_utoprim(G, &d, p, pflag, finalstep, NULL);
+}
+
+int or_post_fixup_fv(const or_grid *G, double *p, int *pflag, int finalstep, double *ucons)
+{
+  or_dims d = or_get_dims(G);
+  or_bound_pflag(G, pflag);
+  if (checksolution(G, &d, p, pflag)) return 1;
+  return fixup_utoprim(G, &d, p, pflag, finalstep, ucons);
 }

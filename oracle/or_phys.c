@@ -556,16 +556,38 @@ int or_limit_gamma(double gammamax, double *pr, const or_geom *g)
 
 /* fixup.c:341 fixup1zone with set_density_floors rescaletype 0 (fixup.c:1562);
    rhofloor/ufloor are RHOMIN*r^-1.5 and UUMIN*r^-2.5 at the zone centre */
-void or_fixup1zone(double *pr, const or_geom *g, double rhofloor, double ufloor, double gammamax)
+int or_fixup1zone(double *pr, const or_geom *g, double rhofloor, double ufloor, double gammamax)
 {
   double prf[2];
+  int changed = 0;
   prf[0] = rhofloor;
   prf[1] = ufloor;
   if (prf[0] < 1.e-20) prf[0] = 1.e-20; /* RHOMINLIMIT, initbase.c:461 */
   if (prf[1] < 1.e-20) prf[1] = 1.e-20; /* UUMINLIMIT */
-  if (prf[0] > pr[OR_RHO]) pr[OR_RHO] = prf[0];
-  if (prf[1] > pr[OR_UU]) pr[OR_UU] = prf[1];
-  or_limit_gamma(gammamax, pr, g);
+  if (prf[0] > pr[OR_RHO]) { pr[OR_RHO] = prf[0]; changed = 1; }
+  if (prf[1] > pr[OR_UU]) { pr[OR_UU] = prf[1]; changed = 1; }
+  if (or_limit_gamma(gammamax, pr, g) == -1) changed = 1;
+  return changed;
+}
+
+/* fixup.c:181 diag_fixup, finite-volume branch (fixup.c:240-248): the
+   carried conserved state u takes U(pr) - U(pr0), each U passed through
+   UtoU from UDIAG to UEVOLVE (divided, then multiplied by gdet).  The
+   update sits inside ENERREGIONLOOP (fixup.c:208) and an active zone is
+   "correctable" in every region, so it is applied once per energy region:
+   NUMENERREGIONS is 3 with the torus file's DOJETDIAG 1 (global.h:436) */
+void or_diag_fixup_fv(const double *pr0, const double *pr, const or_geom *g, double gam, double *u)
+{
+  double Ui[OR_NPR], Uf[OR_NPR], Upre[OR_NPR], Upost[OR_NPR];
+  int pl, region;
+  or_primtoU(pr0, g, gam, Ui);
+  or_primtoU(pr, g, gam, Uf);
+  for (pl = 0; pl < OR_NPR; pl++) {
+    Upre[pl] = (Ui[pl] / g->g) * g->g;
+    Upost[pl] = (Uf[pl] / g->g) * g->g;
+  }
+  for (region = 0; region < 3; region++)
+    for (pl = 0; pl < OR_NPR; pl++) u[pl] += Upost[pl] - Upre[pl];
 }
 
 /* fixup.c:2271 inflow_check_rel4vel for the radial direction at zone centres;

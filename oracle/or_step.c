@@ -272,6 +272,12 @@ static int fluxcalc_dir(const or_grid *G, const or_dims *d, const double *p, int
   or_geom g;
   double p_l[OR_NPR], p_r[OR_NPR], ctop, dtij;
 
+  if (G->fv) {
+    /* initbase.c:949-993 with avgo = 0: every transverse ghost layer */
+    is = -d->nb1; ie = d->n1 + d->nb1 - 1;
+    js = -d->nb2; je = d->n2 + d->nb2 - 1;
+    ks = -d->nb3; ke = d->n3 + d->nb3 - 1;
+  }
   if (dir == 1) { is = 0; ie = d->n1 - 1 + s1; }
   if (dir == 2) { js = 0; je = d->n2 - 1 + s2; }
   if (dir == 3) { ks = 0; ke = d->n3 - 1 + s3; }
@@ -414,8 +420,11 @@ int or_fluxcalc(const or_grid *G, const double *p, double *F1, double *F2, doubl
 /* ------------------------------------------------------------------ */
 /* advance.c:58 advance_standard + step_ch.c:35 step_ch_simplempi       */
 /* ------------------------------------------------------------------ */
+/* advance.c:245 advance_finitevolume with avgscheme DONOR differs in
+   taking Ui from the carried conserved state ui (ucons here) instead of
+   U(pi), in its flux loops, and in handing ucum on to the next step */
 static int advance(const or_grid *G, const double *pi, const double *pb, double *pf, double *uf,
-                   double *ucum, const double CUf[4], const double Cunew[3], int timeorder,
+                   double *ucum, double *ucons, const double CUf[4], const double Cunew[3], int timeorder,
                    double dt, double *ndt_out, int *pflag, int64_t *nfail, int64_t *iters)
 {
   or_dims d = or_get_dims(G);
@@ -438,7 +447,8 @@ static int advance(const or_grid *G, const double *pi, const double *pb, double 
         if (timeorder == 0)
           for (pl = 0; pl < OR_NPR; pl++) uf[z * 8 + pl] = ucum[z * 8 + pl] = 0.0;
         or_load_geom(G, &d, i, j, OL_CENT, &g);
-        or_primtoU(&pi[z * 8], &g, G->gam, Ui);
+        if (ucons) memcpy(Ui, &ucons[z * 8], sizeof Ui);
+        else or_primtoU(&pi[z * 8], &g, G->gam, Ui);
         memcpy(conn, &G->conn[((ptrdiff_t)(i + d.nb1) * d.m2 + (j + d.nb2)) * 64], sizeof(conn));
         or_source(&pb[z * 8], &g, (const double(*)[4][4])conn, G->gam, dUgeom);
         /* advance.c:741 flux2dUavg (FLUXB != FLUXCD) */
@@ -465,7 +475,13 @@ static int advance(const or_grid *G, const double *pi, const double *pb, double 
         *iters += it;
         if (finalstep) {
           const double *fz = &G->floors[((ptrdiff_t)(i + d.nb1) * d.m2 + (j + d.nb2)) * 2];
-          or_fixup1zone(&pf[z * 8], &g, fz[0], fz[1], G->gammamax);
+          double pr0[OR_NPR];
+          memcpy(pr0, &pf[z * 8], sizeof pr0);
+          /* advance.c:322-330: the next step's ui is this step's ucum,
+             which fixup1zone's diag_fixup (fixup.c:443) moves with the floors */
+          if (ucons) memcpy(&ucons[z * 8], &ucum[z * 8], 8 * sizeof(double));
+          if (or_fixup1zone(&pf[z * 8], &g, fz[0], fz[1], G->gammamax) && ucons)
+            or_diag_fixup_fv(pr0, &pf[z * 8], &g, G->gam, &ucons[z * 8]);
         }
       }
   *ndt_out = G->defcon * 1. / (1. / ndt[0] + 1. / ndt[1] + 1. / ndt[2]);
@@ -476,7 +492,7 @@ done:
   return rc;
 }
 
-int or_step(const or_grid *G, double *p, double *dt_io, int64_t *nfail, int64_t *iters)
+static int step_impl(const or_grid *G, double *p, double *ucons, double *dt_io, int64_t *nfail, int64_t *iters)
 {
   or_dims d = or_get_dims(G);
   size_t nz = (size_t)d.m1 * d.m2 * d.m3;
@@ -501,8 +517,8 @@ int or_step(const or_grid *G, double *p, double *dt_io, int64_t *nfail, int64_t 
     pbb[0] = p; pbb[1] = pk; pbb[2] = pk2; pbb[3] = pk;
     pff[0] = pk; pff[1] = pk2; pff[2] = pk; pff[3] = p;
     for (t = 0; t < 4; t++) {
-      if (advance(G, p, pbb[t], pff[t], uf, ucum, CUf4[t], Cunew4[t], t, dt, &ndt, pflag, nfail, iters) ||
-          or_bound_prim(G, pff[t]) || or_post_fixup(G, pff[t], pflag, t == 3) || or_bound_prim(G, pff[t])) {
+      if (advance(G, p, pbb[t], pff[t], uf, ucum, ucons, CUf4[t], Cunew4[t], t, dt, &ndt, pflag, nfail, iters) ||
+          or_bound_prim(G, pff[t]) || or_post_fixup_fv(G, pff[t], pflag, t == 3, ucons) || or_bound_prim(G, pff[t])) {
         rc = 1;
         goto done;
       }
@@ -510,11 +526,11 @@ int or_step(const or_grid *G, double *p, double *dt_io, int64_t *nfail, int64_t 
   } else {
     /* substep 0: pi=p pb=p pf=pk */
     /* each substep: advance, bound_prim, post_fixup, bound_prim (step_ch.c:123-157) */
-    if (advance(G, p, p, pk, uf, ucum, CUf2[0], Cunew2[0], 0, dt, &ndt, pflag, nfail, iters)) { rc = 1; goto done; }
-    if (or_bound_prim(G, pk) || or_post_fixup(G, pk, pflag, 0) || or_bound_prim(G, pk)) { rc = 1; goto done; }
+    if (advance(G, p, p, pk, uf, ucum, ucons, CUf2[0], Cunew2[0], 0, dt, &ndt, pflag, nfail, iters)) { rc = 1; goto done; }
+    if (or_bound_prim(G, pk) || or_post_fixup_fv(G, pk, pflag, 0, ucons) || or_bound_prim(G, pk)) { rc = 1; goto done; }
     /* substep 1: pi=p pb=pk pf=p */
-    if (advance(G, p, pk, p, uf, ucum, CUf2[1], Cunew2[1], 1, dt, &ndt, pflag, nfail, iters)) { rc = 1; goto done; }
-    if (or_bound_prim(G, p) || or_post_fixup(G, p, pflag, 1) || or_bound_prim(G, p)) { rc = 1; goto done; }
+    if (advance(G, p, pk, p, uf, ucum, ucons, CUf2[1], Cunew2[1], 1, dt, &ndt, pflag, nfail, iters)) { rc = 1; goto done; }
+    if (or_bound_prim(G, p) || or_post_fixup_fv(G, p, pflag, 1, ucons) || or_bound_prim(G, p)) { rc = 1; goto done; }
   }
   /* step_ch.c:243-247 */
   if (ndt > G->safe * dt) ndt = G->safe * dt;
@@ -526,4 +542,30 @@ done:
   free(ucum);
   free(pflag);
   return rc;
+}
+
+int or_step(const or_grid *G, double *p, double *dt_io, int64_t *nfail, int64_t *iters)
+{
+  if (G->fv) return 1;
+  return step_impl(G, p, NULL, dt_io, nfail, iters);
+}
+
+int or_step_fv(const or_grid *G, double *p, double *ucons, double *dt_io, int64_t *nfail, int64_t *iters)
+{
+  if (!G->fv || !ucons) return 1;
+  return step_impl(G, p, ucons, dt_io, nfail, iters);
+}
+
+void or_cons_from_prims(const or_grid *G, const double *p, double *ucons)
+{
+  or_dims d = or_get_dims(G);
+  int i, j, k;
+  for (i = 0; i < d.n1; i++)
+    for (j = 0; j < d.n2; j++)
+      for (k = 0; k < d.n3; k++) {
+        ptrdiff_t z = or_zidx(&d, i, j, k);
+        or_geom g;
+        or_load_geom(G, &d, i, j, OL_CENT, &g);
+        or_primtoU(&p[z * 8], &g, G->gam, &ucons[z * 8]);
+      }
 }

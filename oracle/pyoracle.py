@@ -40,6 +40,8 @@ def lib():
         _lib.or_reconstruct.argtypes = [C.POINTER(OrGrid), d, C.c_int, d, d]
         _lib.or_floors.argtypes = [C.c_int64, d, C.c_double, C.c_double, d]
         _lib.or_post_fixup.argtypes = [C.POINTER(OrGrid), d, i32, C.c_int]
+        _lib.or_step_fv.argtypes = [C.POINTER(OrGrid), d, d, d, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        _lib.or_cons_from_prims.argtypes = [C.POINTER(OrGrid), d, d]
     return _lib
 
 
@@ -48,9 +50,10 @@ def _p(a):
 
 
 # scheme number of the fixtures ("weno" entry, the golden harness's second
-# argument) -> (oracle lim, flux, timeorder): MC + LaxF + RK2, WENO5BND +
-# HLL + RK2, PARA + HLL + RK4
-SCHEMES = {0: (0, 0, 2), 1: (1, 1, 2), 2: (2, 1, 4)}
+# argument) -> (oracle lim, flux, timeorder, finite volume): MC + LaxF + RK2,
+# WENO5BND + HLL + RK2, PARA + HLL + RK4, and the last with the
+# finite-volume step (the torus file's own choice)
+SCHEMES = {0: (0, 0, 2, 0), 1: (1, 1, 2, 0), 2: (2, 1, 4, 0), 3: (2, 1, 4, 1)}
 
 
 class OrGrid(C.Structure):
@@ -62,7 +65,7 @@ class OrGrid(C.Structure):
                 ("gcov", C.POINTER(C.c_double)), ("gcon", C.POINTER(C.c_double)),
                 ("gcovpert", C.POINTER(C.c_double)), ("gdet", C.POINTER(C.c_double)),
                 ("conn", C.POINTER(C.c_double)), ("floors", C.POINTER(C.c_double)),
-                ("timeorder", C.c_int)]
+                ("timeorder", C.c_int), ("fv", C.c_int)]
 
 
 def c2p_batch(gcov, gcon, gdet, gcovpert, U, gam, guess):
@@ -99,7 +102,7 @@ class Grid:
     (FACE1, FACE2, FACE3, CORN1, CORN2, CORN3, CENT)."""
 
     def __init__(self, n1, n2, n3, nb, lim, flux, gam, cour, dx, gcov, gcon, gcovpert, gdet, conn,
-                 r_cent, defcon=1.0, safe=1.3, timeorder=2):
+                 r_cent, defcon=1.0, safe=1.3, timeorder=2, fv=0):
         self.keep = [np.ascontiguousarray(x, dtype=np.float64)
                      for x in (gcov, gcon, gcovpert, gdet, conn, floors_from_r(r_cent))]
         g = OrGrid()
@@ -109,7 +112,7 @@ class Grid:
             g.dx[i] = dx[i]
         g.rhomin, g.uumin, g.gammamax, g.defcon, g.safe = RHOMIN, UUMIN, GAMMAMAX, defcon, safe
         g.gcov, g.gcon, g.gcovpert, g.gdet, g.conn, g.floors = [_p(a) for a in self.keep]
-        g.timeorder = timeorder
+        g.timeorder, g.fv = timeorder, fv
         self.g = g
         self.shape = (n1 + 2 * nb * (n1 > 1), n2 + 2 * nb * (n2 > 1), n3 + 2 * nb * (n3 > 1), 8)
 
@@ -117,11 +120,11 @@ class Grid:
     def from_fixture(cls, d):
         par = d["params"]
         n1, n2, n3, nb = (int(x) for x in par[:4])
-        lim, flux, timeorder = SCHEMES[int(d["weno"])]
+        lim, flux, timeorder, fv = SCHEMES[int(d["weno"])]
         dx = [0.0, par[12], par[13], par[14]]
         r = d["xv_cent"][:, :, 1, 1]
         return cls(n1, n2, n3, nb, lim, flux, par[5], par[10], dx, d["gcov"], d["gcon"],
-                   d["gcovpert"], d["gdet"], d["conn"], r, timeorder=timeorder)
+                   d["gcovpert"], d["gdet"], d["conn"], r, timeorder=timeorder, fv=fv)
 
     def step(self, p, dt):
         p = np.ascontiguousarray(p, dtype=np.float64).copy()
@@ -133,6 +136,26 @@ class Grid:
         if rc:
             raise RuntimeError("oracle step failed")
         return p, float(dtv[0]), int(nf[0]), int(it[0])
+
+    def cons_from_prims(self, p):
+        """init_conservatives: U(p) of the active zones (zero elsewhere)."""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        U = np.zeros(self.shape)
+        lib().or_cons_from_prims(C.byref(self.g), _p(p), _p(U))
+        return U
+
+    def step_fv(self, p, U, dt):
+        """One finite-volume step; U is the carried conserved state."""
+        p = np.ascontiguousarray(p, dtype=np.float64).copy()
+        U = np.ascontiguousarray(U, dtype=np.float64).copy()
+        dtv = np.array([dt])
+        nf = np.zeros(1, np.int64)
+        it = np.zeros(1, np.int64)
+        rc = lib().or_step_fv(C.byref(self.g), _p(p), _p(U), _p(dtv), nf.ctypes.data_as(C.POINTER(C.c_int64)),
+                              it.ctypes.data_as(C.POINTER(C.c_int64)))
+        if rc:
+            raise RuntimeError("oracle step failed")
+        return p, U, float(dtv[0]), int(nf[0]), int(it[0])
 
     def fluxcalc(self, p):
         p = np.ascontiguousarray(p, dtype=np.float64)

@@ -14,9 +14,10 @@
  *       states per direction, and the primitives after step 1 and after
  *       <nsteps> steps of step_ch().  Schemes: 0 MC + LaxF + RK2,
  *       1 WENO5BND + HLL + RK2, 2 PARA + HLL + RK4 (the torus file's own
- *       lim, fluxmethod and TIMEORDER, init.fishmon.c:110-114, without the
- *       finite-volume ENO step).  Every mode's <weno> argument below is
- *       this scheme number.
+ *       lim, fluxmethod and TIMEORDER, init.fishmon.c:110-114, with
+ *       DOENOFLUX NOENOFLUX), 3 the same with its DOENOFLUX ENOFINITEVOLUME
+ *       (and its avgscheme DONOR, init.fishmon.c:97): the torus file's own
+ *       scheme.  Every mode's <weno> argument below is this scheme number.
  *   c2p <infile> <outfile> <gamma>
  *       Batched inversion: each input record holds a metric (gcov, gcon,
  *       gdet, gcovpert), the true primitives and a guess.  U is built with
@@ -128,11 +129,11 @@ static void harness_init(int scheme)
   init_defconsts();
   init_global();
   /* OVERRIDE: the hot-path configurations (init.fishmon.c:96-118 picks PARA/RK4/FV) */
-  lim = (scheme == 2) ? PARA : use_weno ? WENO5BND : MC;
+  lim = (scheme >= 2) ? PARA : use_weno ? WENO5BND : MC;
   fluxmethod = (scheme == 0) ? LAXFFLUX : HLLFLUX;
   FLUXB = FLUXCTTOTH;
-  TIMEORDER = (scheme == 2) ? 4 : 2;
-  DOENOFLUX = NOENOFLUX;
+  TIMEORDER = (scheme >= 2) ? 4 : 2;
+  DOENOFLUX = (scheme == 3) ? ENOFINITEVOLUME : NOENOFLUX;
   cour = 0.8;
   rescaletype = 0; /* floors rho>=RHOMIN r^-1.5, u>=UUMIN r^-2.5 (fixup.c:1562) */
   DODIAGS = 0;

@@ -16,7 +16,10 @@ Fixtures
   t3d_weno.npz   3-D torus 16x16x8, WENO5BND + HLL + flux-CT, 1 RK2 step
   t2d_para.npz   2-D torus 32x32, PARA + HLL + flux-CT, 5 RK4 steps
   t3d_para.npz   3-D torus 16x16x8, PARA + HLL + flux-CT, 1 RK4 step
-        (written alone by --para)
+  t2d_fv.npz     2-D torus 32x32, the same with the finite-volume step
+        (ENOFINITEVOLUME, avgscheme DONOR), 5 RK4 steps
+  t3d_fv.npz     3-D torus 16x16x8, finite-volume PARA + HLL, 2 RK4 steps
+        (these four written alone by --para)
 Each torus fixture carries the reference geometry tables, the initial
 primitives, the reconstruction states (or MC slopes) per direction, the
 fluxes after one fluxcalc()+flux_ct() and the primitives after steps.
@@ -359,12 +362,14 @@ def main_rdump():
 
 
 def main_para():
-    """The PARA + HLL + RK4 tori (scheme 2 of the harness)."""
+    """The PARA + HLL + RK4 tori (schemes 2 and 3 of the harness)."""
     builds = {"t2d": (32, 32, 1), "t3d": (16, 16, 8)}
     for tag, (n1, n2, n3) in builds.items():
         subprocess.run([BUILD, tag, str(n1), str(n2), str(n3)], check=True, stdout=subprocess.DEVNULL)
     np.savez_compressed(os.path.join(HERE, "t2d_para.npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), 2, 5))
     np.savez_compressed(os.path.join(HERE, "t3d_para.npz"), **run_torus(os.path.join(REFDIR, "harness_t3d"), 2, 1))
+    np.savez_compressed(os.path.join(HERE, "t2d_fv.npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), 3, 5))
+    np.savez_compressed(os.path.join(HERE, "t3d_fv.npz"), **run_torus(os.path.join(REFDIR, "harness_t3d"), 3, 2))
 
 
 def main():

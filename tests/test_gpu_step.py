@@ -1,7 +1,8 @@
 """Full torus steps on the GPU (ghost fill, reconstruction, fluxes,
 flux-CT, update, Noble 2D, fixup1zone, post_fixup) against the reference
 golden vectors: bit-for-bit primitives and time steps.  Schemes: MC + LaxF
-and WENO5BND + HLL with the midpoint RK2, PARA + HLL with RK4."""
+and WENO5BND + HLL with the midpoint RK2, PARA + HLL with RK4, and the
+last with the finite-volume step (the torus file's own scheme)."""
 import numpy as np
 import pytest
 
@@ -9,23 +10,25 @@ from golden_util import load
 
 pytestmark = pytest.mark.gpu
 
-TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para"]
+TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para", "t2d_fv", "t3d_fv"]
 
 
 def scheme(d):
-    """(lim, fluxmethod, timeorder) of a torus fixture's scheme number."""
+    """(lim, fluxmethod, timeorder, doenoflux) of a torus fixture's scheme number."""
     from harm_finalwham_amd import harmgpu as h
-    return {0: (h.HG_LIM_MC, h.HG_FLUX_LAXF, 2), 1: (h.HG_LIM_WENO5BND, h.HG_FLUX_HLL, 2),
-            2: (h.HG_LIM_PARA, h.HG_FLUX_HLL, 4)}[int(d["weno"])]
+    return {0: (h.HG_LIM_MC, h.HG_FLUX_LAXF, 2, h.HG_NOENOFLUX),
+            1: (h.HG_LIM_WENO5BND, h.HG_FLUX_HLL, 2, h.HG_NOENOFLUX),
+            2: (h.HG_LIM_PARA, h.HG_FLUX_HLL, 4, h.HG_NOENOFLUX),
+            3: (h.HG_LIM_PARA, h.HG_FLUX_HLL, 4, h.HG_ENOFINITEVOLUME)}[int(d["weno"])]
 
 
 def make_grid(d):
     from harm_finalwham_amd import harmgpu
     par = d["params"]
     n1, n2, n3, nb = (int(x) for x in par[:4])
-    lim, flux, timeorder = scheme(d)
+    lim, flux, timeorder, doenoflux = scheme(d)
     g = harmgpu.Grid(n1, n2, n3, nb=nb, lim=lim, fluxmethod=flux, gam=par[5], cour=par[10],
-                     dx=(0.0, par[12], par[13], par[14]), timeorder=timeorder)
+                     dx=(0.0, par[12], par[13], par[14]), timeorder=timeorder, doenoflux=doenoflux)
     g.upload_geometry(d["gcov"], d["gcon"], d["gcovpert"], d["gdet"], d["conn"], d["xv_cent"][:, :, 1, 1])
     return g
 
@@ -64,9 +67,13 @@ def test_steps_match_reference(name):
     g.set_prims(d["p_init"])
     G = pyoracle.Grid.from_fixture(d)
     po = d["p_init"]
+    Uo = G.cons_from_prims(po) if G.g.fv else None
     dt = float(d["dt_seq"][0])
     for n in range(1, int(d["nsteps"]) + 1):
-        po, _, nf_oracle, _ = G.step(po, dt)
+        if Uo is None:
+            po, _, nf_oracle, _ = G.step(po, dt)
+        else:
+            po, Uo, _, nf_oracle, _ = G.step_fv(po, Uo, dt)
         dt, nf, it = g.step_ch(dt)
         print(name, "step", n, "failed inversions", nf, flush=True)
         # failed inversions over the step's substeps, as the oracle counts them
@@ -77,6 +84,29 @@ def test_steps_match_reference(name):
             assert np.array_equal(p1, d["p_step1"]), np.abs(p1 - d["p_step1"]).max()
     p = g.get_prims()
     assert np.array_equal(p, d["p_final"]), np.abs(p - d["p_final"]).max()
+    if Uo is not None:
+        # the carried conserved state, floors' and fixes' corrections included
+        assert np.array_equal(g.get_conserved(), Uo)
+
+
+def test_finite_volume_restart_from_conserved_state():
+    """Two steps, then the primitives and the carried conserved state moved
+    to a fresh context (as a restart does) and three more steps: the
+    reference's final state."""
+    d = load("t2d_fv")
+    g = make_grid(d)
+    g.set_prims(d["p_init"])
+    dt = float(d["dt_seq"][0])
+    for n in (1, 2):
+        dt, _, _ = g.step_ch(dt)
+    p, U = g.get_prims(), g.get_conserved()
+    h = make_grid(d)
+    h.set_prims(p)
+    h.set_conserved(U)
+    for n in (3, 4, 5):
+        dt, _, _ = h.step_ch(dt)
+        assert dt == d["dt_seq"][n]
+    assert np.array_equal(h.get_prims(), d["p_final"])
 
 
 @pytest.mark.parametrize("name", ["t2d_weno", "t3d_weno"])

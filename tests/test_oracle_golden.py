@@ -27,7 +27,7 @@ def test_c2p_failure_codes_present():
     assert set(np.unique(d["pflag"])) - {0}
 
 
-TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para"]
+TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para", "t2d_fv", "t3d_fv"]
 
 
 @pytest.mark.parametrize("name", TORI)
@@ -36,8 +36,12 @@ def test_torus_steps_match_reference(name):
     G = po.Grid.from_fixture(d)
     nb, n3 = int(d["params"][3]), int(d["params"][2])
     p, dt = d["p_init"], float(d["dt_seq"][0])
+    U = G.cons_from_prims(p) if G.g.fv else None
     for n in range(1, int(d["nsteps"]) + 1):
-        p, dt, nf, _ = G.step(p, dt)
+        if U is None:
+            p, dt, nf, _ = G.step(p, dt)
+        else:
+            p, U, dt, nf, _ = G.step_fv(p, U, dt)
         # a step whose final flags hold failures failed in some substep
         if d["fail_seq"][n]:
             assert nf > 0
@@ -71,7 +75,7 @@ def test_ghost_fill_is_identity_on_reference_state():
     assert np.array_equal(G.bound(d["p_step1"]), d["p_step1"])
 
 
-@pytest.mark.parametrize("name", ["t2d_weno", "t3d_weno", "t2d_para", "t3d_para"])
+@pytest.mark.parametrize("name", ["t2d_weno", "t3d_weno", "t2d_para", "t3d_para", "t3d_fv"])
 def test_interface_states_match_reference(name):
     """WENO5BND (slope_lim_linetype_c2e, interpline.c:517) and PARA (para4,
     interppoint.c:738) states of cells -1..n along each direction."""
@@ -87,3 +91,16 @@ def test_interface_states_match_reference(name):
         sl = tuple(sl)
         assert np.array_equal(pl[sl], d["pleft_dir%d" % dr][sl])
         assert np.array_equal(pr[sl], d["pright_dir%d" % dr][sl])
+
+
+def test_finite_volume_step_carries_the_conserved_state():
+    """Planted mistake: deriving Ui from the primitives each step (the
+    standard scheme) instead of carrying ucum departs from the reference's
+    finite-volume run within its five steps."""
+    d = load("t2d_fv")
+    G = po.Grid.from_fixture(d)
+    G.g.fv = 0
+    p, dt = d["p_init"], float(d["dt_seq"][0])
+    for n in range(1, int(d["nsteps"]) + 1):
+        p, dt, _, _ = G.step(p, dt)
+    assert not np.array_equal(p, d["p_final"])
