@@ -35,6 +35,7 @@ struct DiagView {
   double *divb;    // [nact]
   double *face;    // [nface][FREC]
   double *out;     // results: cons 8, divbmax, divbavg, imax (as double), pdot 6x8, terms 6x7x8, jet2 6x7x8
+  double *stage;   // [4][6][8] pdot of each RK4 substep (for pcum)
   int nface[6];    // faces per direction (X1UP, X1DN, X2UP, X2DN, X3UP, X3DN), 0 = no flux there
   int face0[6];    // first record of each direction
 };
@@ -72,9 +73,9 @@ struct DivbCtx {
 };
 
 __global__ void __launch_bounds__(256) k_dg_zone(const double *__restrict__ p, const double *__restrict__ geo,
-                                                 const double *__restrict__ corn, Dims d, double gam, double dVF,
-                                                 double dx1, double dx2, double dx3, DiagView v,
-                                                 int *__restrict__ err) {
+                                                 const double *__restrict__ corn, const double *__restrict__ ucons,
+                                                 Dims d, double gam, double dVF, double dx1, double dx2,
+                                                 double dx3, DiagView v, int *__restrict__ err) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t nact = (int64_t)d.n1 * d.n2 * d.n3;
   if (t >= nact) return;
@@ -87,9 +88,15 @@ __global__ void __launch_bounds__(256) k_dg_zone(const double *__restrict__ p, c
   Geom G;
   load_full_geom(geo, GL_CENT, d, i, j, G);
   double pr[NPR], U[NPR];
+  if (ucons) {
+    // dump_ener.c:611-618: the finite-volume scheme sums its carried unew
 #pragma unroll
-  for (int pl = 0; pl < NPR; pl++) pr[pl] = p[z + pl * d.spl];
-  if (primtoU(pr, G, gam, U)) atomicOr(err, 64);
+    for (int pl = 0; pl < NPR; pl++) U[pl] = ucons[z + pl * d.spl];
+  } else {
+#pragma unroll
+    for (int pl = 0; pl < NPR; pl++) pr[pl] = p[z + pl * d.spl];
+    if (primtoU(pr, G, gam, U)) atomicOr(err, 64);
+  }
 #pragma unroll
   for (int pl = 0; pl < NPR; pl++) v.ud[(int64_t)pl * nact + t] = U[pl] * dVF;
   // SETFDIVBFLUXCT: every active zone satisfies DIVBCONDITION
@@ -218,6 +225,19 @@ __global__ void __launch_bounds__(256) k_dg_face(const double *__restrict__ p, c
   rec[FREC - 1] = condjet2 ? 1.0 : 0.0;
 }
 
+// pdot alone (diag.c:760-1007 for pl < NPR), one thread per sum, into the
+// record of RK4 substep `stage`
+__global__ void __launch_bounds__(64) k_dg_pdot(DiagView v, int stage) {
+  int t = threadIdx.x;
+  if (t >= 6 * NPR) return;
+  int dir = t / NPR, pl = t % NPR;
+  int n = v.nface[dir];
+  const double *rec = v.face + (int64_t)v.face0[dir] * FREC;
+  double s = 0.0;
+  for (int f = 0; f < n; f++) s += rec[(int64_t)f * FREC + pl];
+  v.stage[stage * 6 * NPR + t] = s;
+}
+
 // every sum by one thread in loop order
 __global__ void __launch_bounds__(1024) k_dg_sum(DiagView v, int64_t nact) {
   int t = threadIdx.x;
@@ -284,12 +304,13 @@ static DiagView view(hg_ctx *c) {
   v.divb = v.ud ? v.ud + (size_t)NPR * nact : nullptr;
   v.face = v.ud ? v.divb + nact : nullptr;
   v.out = v.ud ? v.face + (size_t)nf * FREC : nullptr;
+  v.stage = v.ud ? v.out + OUT_N : nullptr;
   return v;
 }
 
 static size_t buf_doubles(const Dims &d, const DiagView &v) {
   int nf = v.face0[5] + v.nface[5];
-  return (size_t)(NPR + 1) * ((int64_t)d.n1 * d.n2 * d.n3) + (size_t)nf * FREC + OUT_N;
+  return (size_t)(NPR + 1) * ((int64_t)d.n1 * d.n2 * d.n3) + (size_t)nf * FREC + OUT_N + 4 * 6 * NPR;
 }
 
 namespace hg {
@@ -325,6 +346,11 @@ void launch_diag_faces(hg_ctx *c, const double *prim, const double *pglob, hipSt
                      v, c->err);
 }
 
+// the pdot of RK4 substep `stage` from the faces just recorded
+void launch_diag_stage(hg_ctx *c, int stage, hipStream_t s) {
+  hipLaunchKernelGGL(k_dg_pdot, dim3(1), dim3(64), 0, s, view(c), stage);
+}
+
 }  // namespace hg
 
 extern "C" int hg_diag_record(hg_ctx *c, int on) {
@@ -354,15 +380,20 @@ extern "C" int hg_diagnostics(hg_ctx *c, double Dt, hg_diag *out, void *stream) 
   hipStream_t s = (hipStream_t)stream;
   const double *dx = c->cfg.dx;
   const double dVF = dx[1] * dx[2] * dx[3];  // set_grid.c:21-22
-  hipLaunchKernelGGL(k_dg_zone, dim3((unsigned)((nact + 255) / 256)), dim3(256), 0, s, c->p, c->geo, c->corn, d,
-                     c->cfg.gam, dVF, dx[1], dx[2], dx[3], v, c->err);
+  ensure_conserved(c, s);
+  hipLaunchKernelGGL(k_dg_zone, dim3((unsigned)((nact + 255) / 256)), dim3(256), 0, s, c->p, c->geo, c->corn,
+                     c->ucons, d, c->cfg.gam, dVF, dx[1], dx[2], dx[3], v, c->err);
   // the faces recorded by the last step, else those of the last fluxcalc on p
   if (!(c->dg_record && c->dg_faces_valid))
     launch_diag_faces(c, c->flux_state ? c->flux_state : c->p, c->p, s);
   hipLaunchKernelGGL(k_dg_sum, dim3(1), dim3(1024), 0, s, v, nact);
-  double h[OUT_N];
+  double h[OUT_N], hs[4 * 6 * NPR];
+  // RK4 with recording: every substep's pdot enters pcum (advance.c:130
+  // calls diag_flux with Dt = Cunew[1] dt on each of them)
+  const bool stages = c->cfg.timeorder == 4 && c->dg_record && c->dg_faces_valid;
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(h, v.out, sizeof h, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && stages) e = hipMemcpyAsync(hs, v.stage, sizeof hs, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     set_error("hg_diagnostics: %s", hipGetErrorString(e));
@@ -390,10 +421,16 @@ extern "C" int hg_diagnostics(hg_ctx *c, double Dt, hg_diag *out, void *stream) 
         out->pdottermsjet2[dir][fl][pl] = h[OUT_JET2 + (dir * NFLT + fl) * NPR + pl];
       }
     }
-  // diag.c:1002 pcum += pdot Dt, for the boundaries that carry flux
-  for (int dir = 0; dir < 6; dir++) {
-    if (v.nface[dir] == 0) continue;
-    for (int pl = 0; pl < NPR; pl++) c->pcum[dir][pl] += out->pdot[dir][pl] * Dt;
+  // diag.c:1007 pcum += pdot Dt, for the boundaries that carry flux; with
+  // RK4 recording, once per substep with Dt = Cunew[t][1] Dt (step_ch.c:464-467)
+  static const double cn1[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+  for (int t = 0; t < (stages ? 4 : 1); t++) {
+    const double Dts = stages ? cn1[t] * Dt : Dt;
+    for (int dir = 0; dir < 6; dir++) {
+      if (v.nface[dir] == 0) continue;
+      for (int pl = 0; pl < NPR; pl++)
+        c->pcum[dir][pl] += (stages ? hs[(t * 6 + dir) * NPR + pl] : out->pdot[dir][pl]) * Dts;
+    }
   }
   for (int dir = 0; dir < 6; dir++)
     for (int pl = 0; pl < NPR; pl++) out->pcum[dir][pl] = c->pcum[dir][pl];

@@ -532,14 +532,17 @@ extern "C" int hg_fluxcalc(hg_ctx *c, double *F1, double *F2, double *F3, double
 }
 
 // one substep of step_ch.c:94-166: advance, bound_prim, post_fixup, bound_prim
-static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, const RkStage &rk, int finalstep,
-                            hipStream_t s) {
+static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, const RkStage &rk, int stage,
+                            int finalstep, hipStream_t s) {
   const hg_config &f = c->cfg;
   enqueue_fluxcalc(c, pb, s);
-  if (finalstep && c->dg_record) {
-    // diag_flux of advance.c:130, where the reference's p is still pi
+  const bool rk4 = f.timeorder == 4;
+  if (c->dg_record && (finalstep || rk4)) {
+    // diag_flux of advance.c:130, where the reference's p is still pi; the
+    // midpoint scheme's first substep has Dt = 0, RK4 needs every substep
     launch_diag_faces(c, pb, pi, s);
-    c->dg_faces_valid = true;
+    if (rk4) launch_diag_stage(c, stage, s);
+    c->dg_faces_valid = finalstep != 0;
   }
   prof_mark(c, HG_PROF_UPDATE, s);
   launch_update(pi, pb, pf, c->uf, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2], c->geo, c->conn,
@@ -576,8 +579,8 @@ void hg::enqueue_step(hg_ctx *c, hipStream_t s) {
     // that ucum; the finite-volume one keeps ucum itself
     RkStage a{0.5, 0.0, 0.0, 0.0, c->ucons ? c->ucum : nullptr, 1, 0, c->ucons};
     RkStage b{1.0, 0.0, 0.0, 1.0, c->ucons ? c->ucum : nullptr, 0, 1, c->ucons};
-    enqueue_substep(c, c->p, c->p, c->pk, a, 0, s);
-    enqueue_substep(c, c->p, c->pk, c->p, b, 1, s);
+    enqueue_substep(c, c->p, c->p, c->pk, a, 0, 0, s);
+    enqueue_substep(c, c->p, c->pk, c->p, b, 1, 1, s);
     return;
   }
   // TIMEORDER 4: CUf[.][2] = 1/2, 1/2, 1, 1; Cunew = {1, 1/6, 0}, {0, 1/3, 0},
@@ -587,7 +590,7 @@ void hg::enqueue_step(hg_ctx *c, hipStream_t s) {
   double *pb[4] = {c->p, c->pk, c->pk2, c->pk}, *pf[4] = {c->pk, c->pk2, c->pk, c->p};
   for (int t = 0; t < 4; t++) {
     RkStage rk{cuf2[t], cn[t][0], cn[t][1], cn[t][2], c->ucum, t == 0, t == 3, c->ucons};
-    enqueue_substep(c, c->p, pb[t], pf[t], rk, t == 3, s);
+    enqueue_substep(c, c->p, pb[t], pf[t], rk, t, t == 3, s);
   }
 }
 

@@ -83,6 +83,7 @@ void fixup_free(hg_ctx *c);
 void diag_free(hg_ctx *c);
 int diag_alloc(hg_ctx *c);
 void launch_diag_faces(hg_ctx *c, const double *prim, const double *pglob, hipStream_t s);
+void launch_diag_stage(hg_ctx *c, int stage, hipStream_t s);
 // fixup_checksolution + fixup_utoprim on bounded primitives p (kernels only)
 void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s);
 int launch_c2p_batch_mixed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,

@@ -195,7 +195,11 @@ int hg_diagnostics(hg_ctx *ctx, double Dt, hg_diag *out, void *stream);
    step overwrites.  hg_diag_record(ctx, 1) makes every step record the
    boundary terms where the reference computes them, so that hg_diagnostics
    after hg_step reproduces diag_flux exactly; without it hg_diagnostics
-   takes those two terms from the current state. */
+   takes those two terms from the current state.  With RK4 and recording,
+   hg_diagnostics adds every substep's pdot to pcum with Dt = Cunew[t][1] Dt
+   (pass the step's dt as Dt), as the reference's per-substep diag_flux does;
+   otherwise pcum += pdot Dt.  The finite-volume scheme's conserved totals
+   are those of its carried state (dump_ener.c:611). */
 int hg_diag_record(hg_ctx *ctx, int on);
 
 /* Restart dumps of the reference (restart.c:206 restart_write, text output):

@@ -332,11 +332,16 @@ def run_diag(harness, weno, nsteps):
 def main_diag():
     """constotal / divbmaxavg / diag_flux of the reference after a few steps
     (the torus fixtures' grids): diag_t2d (MC, 5 steps), diag_t3d (WENO5BND,
-    2 steps)."""
+    2 steps), diag_t2d_para (PARA + RK4, 3 steps: pcum over every
+    substep), diag_t3d_fv (the finite-volume scheme, 2 steps: constotal of
+    the carried state)."""
     for tag, (n1, n2, n3) in {"t2d": (32, 32, 1), "t3d": (16, 16, 8)}.items():
         subprocess.run([BUILD, tag, str(n1), str(n2), str(n3)], check=True, stdout=subprocess.DEVNULL)
     np.savez_compressed(os.path.join(HERE, "diag_t2d.npz"), **run_diag(os.path.join(REFDIR, "harness_t2d"), 0, 5))
     np.savez_compressed(os.path.join(HERE, "diag_t3d.npz"), **run_diag(os.path.join(REFDIR, "harness_t3d"), 1, 2))
+    np.savez_compressed(os.path.join(HERE, "diag_t2d_para.npz"),
+                        **run_diag(os.path.join(REFDIR, "harness_t2d"), 2, 3))
+    np.savez_compressed(os.path.join(HERE, "diag_t3d_fv.npz"), **run_diag(os.path.join(REFDIR, "harness_t3d"), 3, 2))
 
 
 def main_rdump():

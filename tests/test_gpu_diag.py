@@ -11,7 +11,7 @@ from test_gpu_step import make_grid
 
 pytestmark = pytest.mark.gpu
 
-CASES = [("diag_t2d", "t2d_mc"), ("diag_t3d", "t3d_weno")]
+CASES = [("diag_t2d", "t2d_mc"), ("diag_t3d", "t3d_weno"), ("diag_t2d_para", "t2d_para"), ("diag_t3d_fv", "t3d_fv")]
 
 
 @pytest.mark.parametrize("name,base", CASES)
@@ -23,10 +23,16 @@ def test_diagnostics_match_reference(name, base):
     g.set_prims(d["p_init"])
     g.diag_record(True)
     dt = float(dg["dt_seq"][0])
+    rk4 = g.cfg.timeorder == 4
     for n in range(1, int(dg["nsteps"]) + 1):
+        dt_step = dt
         dt, nf, it = g.step_ch(dt)
         assert dt == dg["dt_seq"][n]
+        if rk4:
+            # every substep's boundary flux enters pcum with Cunew[t][1] dt
+            g.diagnostics(Dt=dt_step)
     assert state_digest(g.get_prims()) == str(dg["digest_final"])
+    # the midpoint scheme's diag_flux has Dt = Cunew[1][1] dt = 0
     r = g.diagnostics(Dt=0.0)
     assert np.array_equal(r["cons"], dg["cons"])
     assert np.array_equal(r["pdot"], dg["pdot"])
