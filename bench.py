@@ -101,6 +101,50 @@ def cpu_baseline(timeout_s=240):
                       f"zones, {res['steps']} RK2 step, {res['seconds']:.1f} s timed"}
 
 
+def torus_scheme_bench(n1, n2, n3, steps, warmup=2):
+    """The torus file's own scheme (init.fishmon.c:96-118): PARA + HLL + RK4
+    with the finite-volume step (avgscheme DONOR), on the benchmark grid."""
+    import torch
+    from harm_finalwham_amd import harmgpu
+    g = harmgpu.Grid(n1, n2, n3, nb=3, lim=harmgpu.HG_LIM_PARA, fluxmethod=harmgpu.HG_FLUX_HLL, gam=4.0 / 3.0,
+                     cour=0.8, timeorder=4, doenoflux=harmgpu.HG_ENOFINITEVOLUME)
+    g.build_mks_geometry(a=0.9375, Rin=1.3, Rout=40.0, R0=0.0, hslope=0.3)
+    g.init_torus(rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0, randfact=4e-2, seed=7)
+    shapes = g.autotune(2).as_dict()
+    g.step_async(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.step_async(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    g.profile(True)
+    g.step_async(steps)
+    torch.cuda.synchronize()
+    prof = profile_summary(g.profile_read(), n1 * n2 * n3)
+    g.profile(False)
+    g.close()
+    ms = (t1 - t0) * 1e3 / steps
+    return {"workload": f"3-D magnetised torus {n1}x{n2}x{n3}, PARA + HLL + flux-CT + Noble 2D, RK4, finite volume",
+            "steps": steps, "ms_per_step": ms, "zone_updates_per_s": n1 * n2 * n3 / (ms * 1e-3),
+            "launch_shapes": shapes,
+            "kernels_ms_per_step": {k: v["total_ms"] / steps for k, v in prof.items()}}
+
+
+def torus_scheme_cpu_baseline(timeout_s=240):
+    """The reference build on the same scheme, 256x128x8 zones, one step."""
+    exe = os.path.join(HERE, "oracle", "_ref", "harness_b3d")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        r = subprocess.run([exe, "time", "3", "1"], cwd=tmp, capture_output=True, text=True, timeout=timeout_s)
+    if r.returncode != 0:
+        return None
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": res["zone_updates_per_s"], "unit": "zone-updates/s", "cores": 1, "kind": "reference",
+            "sample": "reference harness (gcc -O2 FP64, single thread), PARA + HLL + RK4 finite volume at "
+                      f"256x128x8 zones, {res['steps']} step, {res['seconds']:.1f} s timed"}
+
+
 def c2p_bench(n, reps=3):
     """Batched inversions (the second half of the headline metric): the
     benchmark distribution for gam = 4/3 and 5/3 generated on the GPU,
@@ -365,6 +409,8 @@ def main():
     ap.add_argument("--batch-tori", type=int, default=48, help="tori of the batched-graph leg (0: skip)")
     ap.add_argument("--mlp-train-n", type=int, default=1 << 22, help="training inversions of the learned start")
     ap.add_argument("--mlp-eval-n", type=int, default=1 << 24, help="held-out inversions (0: skip the leg)")
+    ap.add_argument("--torus-scheme-steps", type=int, default=10,
+                    help="steps of the torus file's own scheme (PARA + RK4 + finite volume) leg (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -448,6 +494,13 @@ def main():
             g.close()
         learned = mlp_bench(args.mlp_train_n, args.mlp_eval_n)
         print(f"# learned start: {json.dumps(learned)}", file=sys.stderr, flush=True)
+    torus_scheme = None
+    if args.torus_scheme_steps > 0:
+        g.close()
+        torus_scheme = torus_scheme_bench(n1, n2, n3, args.torus_scheme_steps)
+        if not args.no_cpu_baseline:
+            torus_scheme["cpu_baseline"] = torus_scheme_cpu_baseline()
+        print(f"# torus scheme: {json.dumps(torus_scheme)}", file=sys.stderr, flush=True)
     batched = None
     if args.batch_tori > 0:
         batched = batched_bench(args.batch_tori)
@@ -481,6 +534,7 @@ def main():
         "con2prim": c2p,
         "learned_start": learned,
         "batched_tori": batched,
+        "torus_scheme": torus_scheme,
         "failed_inversions": nf.value,
         "newton_iterations_per_zone_update": it.value / (2.0 * zones * args.steps),
     }

@@ -222,6 +222,7 @@ int hg::fetch_prims(hg_ctx *c, const double *dev, double *host) {
 extern "C" int hg_set_prims(hg_ctx *c, const double *p) {
   if (!c || !p) { set_error("hg_set_prims: null argument"); return 1; }
   c->ucons_stale = true;
+  c->unew_in_uf = false;
   return copy_prims(c, c->p, p);
 }
 
@@ -573,6 +574,7 @@ void hg::ensure_conserved(hg_ctx *c, hipStream_t s) {
 // setup_rktimestep; pi is p on every substep
 void hg::enqueue_step(hg_ctx *c, hipStream_t s) {
   ensure_conserved(c, s);
+  c->unew_in_uf = true;
   if (c->cfg.timeorder != 4) {
     // TIMEORDER 2 midpoint: CUf[.][2] = 0.5, 1; Cunew = {0, 0, 0}, {0, 0, 1};
     // pb = p, pk; pf = pk, p.  The standard scheme inverts Uf, which equals

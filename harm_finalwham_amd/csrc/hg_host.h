@@ -18,6 +18,9 @@ struct hg_ctx {
   // p after hg_set_prims / hg_init_torus
   double *ucons = nullptr;
   bool ucons_stale = true;
+  // the standard scheme's unew (the last substep's ucum) sits in uf after a
+  // step until primitives are set again
+  bool unew_in_uf = false;
   double *F[3] = {nullptr, nullptr, nullptr};
   double *emf[3] = {nullptr, nullptr, nullptr};
   double *pl = nullptr, *pr = nullptr;       // WENO5BND interface states along i

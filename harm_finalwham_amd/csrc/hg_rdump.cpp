@@ -1,5 +1,6 @@
-// Reading the reference's restart dumps (restart.c:206 restart_write through
-// dumpgen.c:5 dump_gen, text output as initbase.c:237 selects without MPI).
+// Reading and writing the reference's restart dumps (restart.c:206
+// restart_write through dumpgen.c:5 dump_gen, text output as initbase.c:237
+// selects without MPI).
 //
 // Layout of dumps/rdump-N: one header line written by write_restart_header
 // (restart.c:641, text branch, format WRITERESTARTHEADER of global.h:1214),
@@ -191,4 +192,63 @@ extern "C" int hg_rdump_read(hg_ctx *c, const char *path, hg_rdump_info *info, d
     if (hg_set_conserved(c, full.data())) return 1;
   }
   return hg_bound_prim(c, stream);
+}
+
+// restart.c:206 restart_write: the caller's header line (the reference's
+// write_restart_header, restart.c:641), then rdump_content (restart.c:240),
+// p and unew of every active zone, each "%21.15g " (global.h:1214), one line
+// per zone in DUMPLOOP order
+extern "C" int hg_rdump_write(hg_ctx *c, const char *path, const char *header, void *stream) {
+  if (!c || !path || !header) {
+    set_error("hg_rdump_write: null argument");
+    return 1;
+  }
+  if (!c->have_geometry) {
+    set_error("hg_rdump_write: geometry not set");
+    return 1;
+  }
+  if (!c->ucons && !c->unew_in_uf) {
+    set_error("hg_rdump_write: no step since the primitives were set, so there is no unew");
+    return 1;
+  }
+  const Dims &d = c->d;
+  const size_t nfull = (size_t)d.m1 * d.m2 * d.m3 * NPR;
+  std::vector<double> p(nfull), u(nfull);
+  if (hg_get_prims(c, p.data())) return 1;
+  if (c->ucons) {
+    if (hg_get_conserved(c, u.data())) return 1;
+  } else {
+    hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) {
+      set_error("hg_rdump_write: %s", hipGetErrorString(e));
+      return 1;
+    }
+    if (fetch_prims(c, c->uf, u.data())) return 1;
+    // the midpoint scheme inverted Uf; its ucum is 0 + (0 Ui + 0 dt dU + Uf),
+    // which is Uf with -0 made +0
+    if (c->cfg.timeorder != 4)
+      for (double &x : u) x = x + 0.0;
+  }
+  FILE *f = fopen(path, "wb");
+  if (!f) {
+    set_error("hg_rdump_write: cannot open %s", path);
+    return 1;
+  }
+  std::vector<char> buf(1 << 20);
+  setvbuf(f, buf.data(), _IOFBF, buf.size());
+  bool ok = fprintf(f, "%s\n", header) >= 0;
+  for (int k = 0; k < d.n3 && ok; k++)
+    for (int j = 0; j < d.n2 && ok; j++)
+      for (int i = 0; i < d.n1 && ok; i++) {
+        size_t h = ((((size_t)(i + d.nb1)) * d.m2 + (j + d.nb2)) * d.m3 + (k + d.nb3)) * NPR;
+        for (int pl = 0; pl < NPR && ok; pl++) ok = fprintf(f, "%21.15g ", p[h + pl]) >= 0;
+        for (int pl = 0; pl < NPR && ok; pl++) ok = fprintf(f, "%21.15g ", u[h + pl]) >= 0;
+        if (ok) ok = fputc('\n', f) != EOF;
+      }
+  if (fclose(f) != 0) ok = false;
+  if (!ok) {
+    set_error("hg_rdump_write: write to %s failed", path);
+    return 1;
+  }
+  return 0;
 }

@@ -719,6 +719,7 @@ extern "C" int hg_init_torus(hg_ctx *c, double rin, double rmax, double kappa, d
   }
   if (copy_prims(c, c->p, P.data())) return 1;
   c->ucons_stale = true;
+  c->unew_in_uf = false;
   launch_bound(c->p, c->geo, c->d, f.gammamax, c->err, 0);
   launch_reset_ndt(c->ndt_bits, 0);
   launch_set_dt(c->p, c->geo, c->d, f.gam, f.cour, f.dx, c->ndt_bits, c->err, 0);

@@ -26,7 +26,7 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_mlp_create", "hg_mlp_destroy", "hg_mlp_nparams", "hg_mlp_get_params", "hg_mlp_set_params",
            "hg_mlp_features", "hg_mlp_forward", "hg_mlp_gradient", "hg_mlp_train",
            "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag", "hg_diagnostics", "hg_diag_record", "hg_rdump_parse",
-           "hg_rdump_read", "hg_set_conserved", "hg_get_conserved"]
+           "hg_rdump_read", "hg_set_conserved", "hg_get_conserved", "hg_rdump_write"]
 HG_PROF_NCLASS = 10
 
 
@@ -147,6 +147,7 @@ def lib():
     L.hg_diag_record.argtypes = [vp, C.c_int]
     L.hg_rdump_parse.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, d, d, C.POINTER(HgRdumpInfo)]
     L.hg_rdump_read.argtypes = [vp, C.c_char_p, C.POINTER(HgRdumpInfo), d, vp]
+    L.hg_rdump_write.argtypes = [vp, C.c_char_p, C.c_char_p, vp]
     fp = C.POINTER(C.c_float)
     L.hg_mlp_create.argtypes = [C.c_uint64, C.POINTER(vp)]
     L.hg_mlp_destroy.argtypes = [vp]
@@ -279,6 +280,12 @@ class Grid:
         info = HgRdumpInfo()
         _check(lib().hg_rdump_read(self.h, os.fsencode(path), C.byref(info), _d(u), None))
         return u, info.as_dict()
+
+    def rdump_write(self, path, header):
+        """Write a restart dump in the reference's text format (the header
+        line given, then p and unew of every active zone)."""
+        _check(lib().hg_rdump_write(self.h, os.fsencode(path), header.encode() if isinstance(header, str) else header,
+                                    None))
 
     def get_pflag(self):
         """Failure codes of the active zones [n1][n2][n3] (FIXED = -1)."""

@@ -211,7 +211,12 @@ int hg_diag_record(hg_ctx *ctx, int on);
    the header's defcoord is skipped (the reference prints that double with
    "%d").  hg_rdump_read puts the primitives into a context and runs
    bound_prim; a context running the finite-volume scheme also takes unew
-   as its carried conserved state.  Writing rdumps is not provided. */
+   as its carried conserved state.  hg_rdump_write (restart.c:206
+   restart_write) writes header (the text of the reference's header line,
+   restart.c:641, which the caller formats from its global state) and then
+   p and unew of every active zone in the reference's format; unew is the
+   last step's ucum (the carried state for the finite-volume scheme), so
+   the standard scheme needs a step since the primitives were set. */
 typedef struct {
   int n[3];
   double t, tf;
@@ -230,6 +235,7 @@ typedef struct {
 } hg_rdump_info;
 int hg_rdump_parse(const char *path, int n1, int n2, int n3, double *p_active, double *unew, hg_rdump_info *info);
 int hg_rdump_read(hg_ctx *ctx, const char *path, hg_rdump_info *info, double *unew, void *stream);
+int hg_rdump_write(hg_ctx *ctx, const char *path, const char *header, void *stream);
 
 /* Batched Noble-2D inversion on device memory (utoprimgen.c:23).  All
    arrays are structure-of-arrays over n zones:

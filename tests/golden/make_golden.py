@@ -344,26 +344,31 @@ def main_diag():
     np.savez_compressed(os.path.join(HERE, "diag_t3d_fv.npz"), **run_diag(os.path.join(REFDIR, "harness_t3d"), 3, 2))
 
 
-def main_rdump():
-    """A restart dump written by the reference (restart_write after 2 MC steps
-    of the 32x32 torus) and what the reference's own restart_read makes of
-    it: the file's bytes, the parsed p and unew of the active zones and the
-    header values (t, tf, nstep, a, gam, cour, dt, lim, fluxmethod, R0, Rin,
-    Rout, hslope, RHOMIN, UUMIN, GAMMAMAX)."""
-    subprocess.run([BUILD, "t2d", "32", "32", "1"], check=True, stdout=subprocess.DEVNULL)
-    h = os.path.join(REFDIR, "harness_t2d")
+def rdump_fixture(h, scheme):
     with tempfile.TemporaryDirectory() as tmp:
-        subprocess.run([h, "rdump", "w", "0", "2"], cwd=tmp, check=True, stdout=subprocess.DEVNULL,
+        subprocess.run([h, "rdump", "w", str(scheme), "2"], cwd=tmp, check=True, stdout=subprocess.DEVNULL,
                        stderr=subprocess.DEVNULL)
         text = open(os.path.join(tmp, "dumps", "rdump-0"), "rb").read()
-        subprocess.run([h, "rdread", "r", "0"], cwd=tmp, check=True, stdout=subprocess.DEVNULL,
+        subprocess.run([h, "rdread", "r", str(scheme)], cwd=tmp, check=True, stdout=subprocess.DEVNULL,
                        stderr=subprocess.DEVNULL)
         pr = np.load(os.path.join(tmp, "r", "p.npy"))
         un = np.load(os.path.join(tmp, "r", "unew.npy"))
         hd = np.load(os.path.join(tmp, "r", "header.npy"))
     nb = (pr.shape[0] - 32) // 2
-    np.savez_compressed(os.path.join(HERE, "rdump_t2d.npz"), text=np.frombuffer(text, np.uint8),
-                        p_active=pr[nb:nb + 32, nb:nb + 32], unew=un, header=hd)
+    return dict(text=np.frombuffer(text, np.uint8), p_active=pr[nb:nb + 32, nb:nb + 32], unew=un, header=hd)
+
+
+def main_rdump():
+    """Restart dumps written by the reference (restart_write after 2 steps of
+    the 32x32 torus) and what the reference's own restart_read makes of
+    them: the file's bytes, the parsed p and unew of the active zones and the
+    header values (t, tf, nstep, a, gam, cour, dt, lim, fluxmethod, R0, Rin,
+    Rout, hslope, RHOMIN, UUMIN, GAMMAMAX).  rdump_t2d: MC + LaxF + RK2;
+    rdump_t2d_fv: the torus file's finite-volume PARA + HLL + RK4 scheme."""
+    subprocess.run([BUILD, "t2d", "32", "32", "1"], check=True, stdout=subprocess.DEVNULL)
+    h = os.path.join(REFDIR, "harness_t2d")
+    np.savez_compressed(os.path.join(HERE, "rdump_t2d.npz"), **rdump_fixture(h, 0))
+    np.savez_compressed(os.path.join(HERE, "rdump_t2d_fv.npz"), **rdump_fixture(h, 3))
 
 
 def main_para():

@@ -1,6 +1,8 @@
-"""Reading the reference's restart dumps (restart.c:206, text output): the
-file restart_write produced after two MC steps of the 32x32 torus is parsed
-to exactly the values the reference's own restart_read gets from it."""
+"""The reference's restart dumps (restart.c:206, text output): the files
+restart_write produced after two steps of the 32x32 torus (MC + RK2, and the
+torus file's finite-volume PARA + RK4 scheme) are parsed to exactly the
+values the reference's own restart_read gets from them, and the library
+writes the same bytes after the same steps."""
 import os
 
 import numpy as np
@@ -9,16 +11,17 @@ import pytest
 HERE = os.path.join(os.path.dirname(__file__), "golden")
 
 
-def _file(tmp_path):
-    d = np.load(os.path.join(HERE, "rdump_t2d.npz"))
+def _file(tmp_path, name="rdump_t2d"):
+    d = np.load(os.path.join(HERE, name + ".npz"))
     path = tmp_path / "rdump-0"
     path.write_bytes(d["text"].tobytes())
     return d, str(path)
 
 
-def test_parse_matches_reference_reader(tmp_path):
+@pytest.mark.parametrize("name", ["rdump_t2d", "rdump_t2d_fv"])
+def test_parse_matches_reference_reader(tmp_path, name):
     from harm_finalwham_amd import harmgpu
-    d, path = _file(tmp_path)
+    d, path = _file(tmp_path, name)
     p, u, info = harmgpu.rdump_parse(path, 32, 32, 1)
     assert np.array_equal(p, d["p_active"])
     assert np.array_equal(u, d["unew"])
@@ -29,7 +32,7 @@ def test_parse_matches_reference_reader(tmp_path):
     assert np.array_equal(np.array(got, dtype=np.float64), h)
     assert info["n"] == (32, 32, 1)
     # the header as the torus set-up writes it: outflow in r, polar axis, periodic phi
-    assert info["timeorder"] == 2 and info["gammafail"] == 2500.0
+    assert info["timeorder"] == (4 if name.endswith("_fv") else 2) and info["gammafail"] == 2500.0
 
 
 def test_parse_rejects_other_grids_and_truncation(tmp_path):
@@ -66,3 +69,52 @@ def test_read_into_context(tmp_path):
     ref.set_prims(full)
     ref.bound_prim()
     assert np.array_equal(p, ref.get_prims())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,base_name", [("rdump_t2d", "t2d_mc"), ("rdump_t2d_fv", "t2d_fv")])
+def test_write_matches_reference_bytes(tmp_path, name, base_name):
+    """hg_rdump_write after the same two steps the reference took before its
+    restart_write: the same bytes (the header line passed through).  The
+    finite-volume dump's unew is the carried conserved state."""
+    from golden_util import load
+    from test_gpu_step import make_grid
+    print("case rdump write", name, flush=True)
+    d = load(name)
+    text = bytes(d["text"])
+    header = text.split(b"\n", 1)[0]
+    base = load(base_name)
+    g = make_grid(base)
+    g.set_prims(base["p_init"])
+    dt = float(base["dt_seq"][0])
+    for n in (1, 2):
+        dt, _, _ = g.step_ch(dt)
+    out = tmp_path / "rdump-0"
+    g.rdump_write(str(out), header)
+    assert out.read_bytes() == text
+
+
+@pytest.mark.gpu
+def test_finite_volume_read_takes_unew(tmp_path):
+    from golden_util import load
+    from test_gpu_step import make_grid
+    d, path = _file(tmp_path, "rdump_t2d_fv")
+    base = load("t2d_fv")
+    g = make_grid(base)
+    u, info = g.rdump_read(path)
+    assert np.array_equal(u, d["unew"]) and info["timeorder"] == 4
+    nb = int(base["params"][3])
+    U = g.get_conserved()
+    assert np.array_equal(U[nb:nb + 32, nb:nb + 32], d["unew"])
+
+
+@pytest.mark.gpu
+def test_write_needs_a_step(tmp_path):
+    from harm_finalwham_amd import harmgpu
+    from golden_util import load
+    from test_gpu_step import make_grid
+    base = load("t2d_mc")
+    g = make_grid(base)
+    g.set_prims(base["p_init"])
+    with pytest.raises(harmgpu.HarmError):
+        g.rdump_write(str(tmp_path / "x"), "header")
