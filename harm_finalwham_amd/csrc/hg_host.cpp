@@ -51,8 +51,8 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
     set_error("hg_ctx_create: unsupported lim %d", cfg->lim);
     return 1;
   }
-  if (cfg->timeorder != 0 && cfg->timeorder != 2 && cfg->timeorder != 4) {
-    set_error("hg_ctx_create: unsupported timeorder %d (2 or 4)", cfg->timeorder);
+  if (cfg->timeorder < 0 || cfg->timeorder > 4) {
+    set_error("hg_ctx_create: unsupported timeorder %d (1 to 4)", cfg->timeorder);
     return 1;
   }
   if (cfg->doenoflux != HG_NOENOFLUX && cfg->doenoflux != HG_ENOFINITEVOLUME) {
@@ -80,9 +80,10 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
   rc |= dalloc(&c->p, ns);
   rc |= dalloc(&c->pk, ns);
   rc |= dalloc(&c->uf, ns);
+  // pk2 and ucum also serve hg_advance_substep with any coefficients
   const bool fv = cfg->doenoflux == HG_ENOFINITEVOLUME;
-  if (cfg->timeorder == 4) rc |= dalloc(&c->pk2, ns);
-  if (cfg->timeorder == 4 || fv) rc |= dalloc(&c->ucum, ns);
+  rc |= dalloc(&c->pk2, ns);
+  rc |= dalloc(&c->ucum, ns);
   if (fv) rc |= dalloc(&c->ucons, ns);
   for (int k = 0; k < 3; k++) rc |= dalloc(&c->F[k], ns);
   for (int k = 0; k < 3; k++) rc |= dalloc(&c->emf[k], nz);
@@ -575,7 +576,23 @@ void hg::ensure_conserved(hg_ctx *c, hipStream_t s) {
 void hg::enqueue_step(hg_ctx *c, hipStream_t s) {
   ensure_conserved(c, s);
   c->unew_in_uf = true;
-  if (c->cfg.timeorder != 4) {
+  const int to = c->cfg.timeorder;
+  if (to == 1 || to == 3) {
+    // TIMEORDER 1 Euler: CUf {1, 0, 1}, Cunew {0, 0, 1}, pb = pf = p.
+    // TIMEORDER 3 TVD RK3: CUf {1, 0, 1}, {3/4, 1/4, 1/4}, {1/3, 2/3, 2/3},
+    // Cunew {0, 0, 0}, {0, 0, 0}, {0, 0, 1}; pb = p, pk, pk2; pf = pk, pk2, p
+    const int n = to;
+    const double cuf[3][3] = {{1.0, 0.0, 1.0}, {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    const double cn3[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}};
+    double *pb[3] = {c->p, c->pk, c->pk2}, *pf[3] = {c->pk, c->pk2, c->p};
+    for (int t = 0; t < n; t++) {
+      const double *cn = (n == 1) ? cn3[2] : cn3[t];
+      RkStage rk{cuf[t][2], cn[0], cn[1], cn[2], c->ucum, t == 0, t == n - 1, c->ucons, cuf[t][0], cuf[t][1]};
+      enqueue_substep(c, c->p, (n == 1) ? c->p : pb[t], (n == 1) ? c->p : pf[t], rk, t, t == n - 1, s);
+    }
+    return;
+  }
+  if (to != 4) {
     // TIMEORDER 2 midpoint: CUf[.][2] = 0.5, 1; Cunew = {0, 0, 0}, {0, 0, 1};
     // pb = p, pk; pf = pk, p.  The standard scheme inverts Uf, which equals
     // that ucum; the finite-volume one keeps ucum itself
@@ -624,6 +641,39 @@ extern "C" int hg_step_async(hg_ctx *c, int nsteps, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   for (int n = 0; n < nsteps; n++) enqueue_step(c, s);
   HIPCHECK(hipGetLastError());
+  return 0;
+}
+
+// one pass of step_ch.c:97-166 with the caller's substep: advance
+// (advance.c:16) from pi/pb into pf, bound_prim, post_fixup, bound_prim
+extern "C" int hg_advance_substep(hg_ctx *c, int pi, int pb, int pf, const double CUf[4], const double Cunew[3],
+                                  int timeorder, int numtimeorders, double dt, double *ndt, void *stream) {
+  if (need_geometry(c, "hg_advance_substep")) return 1;
+  if (!CUf || !Cunew || pi < 0 || pi > 2 || pb < 0 || pb > 2 || pf < 0 || pf > 2 || numtimeorders < 1 ||
+      numtimeorders > 4 || timeorder < 0 || timeorder >= numtimeorders) {
+    set_error("hg_advance_substep: bad arguments");
+    return 1;
+  }
+  double *slot[3] = {c->p, c->pk, c->pk2};
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHECK(hipStreamSynchronize(s));
+  if (hg_set_dt(c, dt)) return 1;
+  ensure_conserved(c, s);
+  const int last = timeorder == numtimeorders - 1;
+  if (last) c->unew_in_uf = true;
+  RkStage rk{CUf[2], Cunew[0], Cunew[1], Cunew[2], c->ucum, timeorder == 0, last, c->ucons, CUf[0], CUf[1]};
+  enqueue_substep(c, slot[pi], slot[pb], slot[pf], rk, timeorder, last, s);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s));
+  if (check_err(c)) return 1;
+  if (ndt) {
+    // advance.c:231
+    unsigned long long b[3];
+    double n[3];
+    HIPCHECK(hipMemcpy(b, c->ndt_bits, sizeof b, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; k++) memcpy(&n[k], &b[k], sizeof(double));
+    *ndt = c->cfg.defcon * 1. / (1. / n[0] + 1. / n[1] + 1. / n[2]);
+  }
   return 0;
 }
 

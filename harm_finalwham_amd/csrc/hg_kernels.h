@@ -36,14 +36,18 @@ void launch_emf(const double *F1, const double *F2, const double *F3, const doub
 // per-zone update (-> ubuf) then inversion + fixup (-> pf); ubuf has the
 // layout of the primitives
 // coefficients of one Runge-Kutta substep (step_ch.c:442 setup_rktimestep):
-// Uf = Ui + cuf2 dt dU; with ucum set (RK4), ucum += cn0 Ui + cn1 dt dU +
-// cn2 Uf, zeroed first on the first substep and inverted on the last
+// without ucum (the midpoint scheme's fast path) Uf = Ui + cuf2 dt dU and the
+// last substep inverts Uf; with ucum (every other case) Uf = cuf0 Ui +
+// cuf1 uf + cuf2 dt dU with uf the previous substep's Uf, and ucum +=
+// cn0 Ui + cn1 dt dU + cn2 Uf, both zeroed first on the first substep; the
+// last substep inverts ucum
 struct RkStage {
   double cuf2, cn0, cn1, cn2;
-  double *ucum;   // null: midpoint scheme, the last substep inverts Uf
+  double *ucum;
   int first, last;
   double *ucons;  // finite-volume scheme: the carried conserved state (Ui),
                   // replaced by ucum on the last substep; null otherwise
+  double cuf0 = 1.0, cuf1 = 0.0;
 };
 void launch_update(const double *pi, const double *pb, double *pf, double *ubuf, const double *F1,
                    const double *F2, const double *F3, const double *e1, const double *e2, const double *e3,

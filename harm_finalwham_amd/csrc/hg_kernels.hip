@@ -594,7 +594,14 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
       double dUr = dU1 + dU2 + dU3;
       // advance.c:922 dUtoU with CUf = {1, 0, cuf2}: uf carries no memory
       // across substeps (CUf[1] = 0 in both schemes)
-      Uf[pl] = 1.0 * Ui[pl] + 0.0 * 0.0 + rk.cuf2 * dt * (dUr + dUg[pl]);
+      if (UCUM) {
+        // general substep: CUf = {cuf0, cuf1, cuf2}; uf is the previous
+        // substep's Uf (ubuf), zeroed on the first substep (advance.c:141)
+        const double ufp = rk.first ? 0.0 : ubuf[z + pl * P];
+        Uf[pl] = rk.cuf0 * Ui[pl] + rk.cuf1 * ufp + rk.cuf2 * dt * (dUr + dUg[pl]);
+      } else {
+        Uf[pl] = 1.0 * Ui[pl] + 0.0 * 0.0 + rk.cuf2 * dt * (dUr + dUg[pl]);
+      }
       if (UCUM) {
         // ucum, zeroed on the first substep (advance.c:141), collects
         // Cunew = {cn0, cn1, cn2}; the last substep inverts ucum, which the

@@ -26,7 +26,7 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_mlp_create", "hg_mlp_destroy", "hg_mlp_nparams", "hg_mlp_get_params", "hg_mlp_set_params",
            "hg_mlp_features", "hg_mlp_forward", "hg_mlp_gradient", "hg_mlp_train",
            "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag", "hg_diagnostics", "hg_diag_record", "hg_rdump_parse",
-           "hg_rdump_read", "hg_set_conserved", "hg_get_conserved", "hg_rdump_write"]
+           "hg_rdump_read", "hg_set_conserved", "hg_get_conserved", "hg_rdump_write", "hg_advance_substep"]
 HG_PROF_NCLASS = 10
 
 
@@ -148,6 +148,7 @@ def lib():
     L.hg_rdump_parse.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, d, d, C.POINTER(HgRdumpInfo)]
     L.hg_rdump_read.argtypes = [vp, C.c_char_p, C.POINTER(HgRdumpInfo), d, vp]
     L.hg_rdump_write.argtypes = [vp, C.c_char_p, C.c_char_p, vp]
+    L.hg_advance_substep.argtypes = [vp, C.c_int, C.c_int, C.c_int, d, d, C.c_int, C.c_int, C.c_double, d, vp]
     fp = C.POINTER(C.c_float)
     L.hg_mlp_create.argtypes = [C.c_uint64, C.POINTER(vp)]
     L.hg_mlp_destroy.argtypes = [vp]
@@ -280,6 +281,16 @@ class Grid:
         info = HgRdumpInfo()
         _check(lib().hg_rdump_read(self.h, os.fsencode(path), C.byref(info), _d(u), None))
         return u, info.as_dict()
+
+    def advance_substep(self, pi, pb, pf, CUf, Cunew, timeorder, numtimeorders, dt):
+        """One substep of the reference's step loop (hg_advance_substep);
+        returns advance's ndt."""
+        cuf = np.array(list(CUf) + [0.0] * (4 - len(CUf)), dtype=np.float64)
+        cun = np.array(Cunew, dtype=np.float64)
+        ndt = np.zeros(1)
+        _check(lib().hg_advance_substep(self.h, pi, pb, pf, _d(cuf), _d(cun), timeorder, numtimeorders, float(dt),
+                                        _d(ndt), None))
+        return float(ndt[0])
 
     def rdump_write(self, path, header):
         """Write a restart dump in the reference's text format (the header

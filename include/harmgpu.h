@@ -71,7 +71,8 @@ typedef struct {
   double dx[4];        /* coordinate spacings dx[1..3] */
   double rhomin, uumin;/* floor coefficients RHOMIN, UUMIN (initbase.c:459-460) */
   double gammamax;     /* GAMMAMAX (initbase.c:471) */
-  int timeorder;       /* TIMEORDER: 2 (or 0) midpoint, 4 classical RK4 (step_ch.c:442 setup_rktimestep) */
+  int timeorder;       /* TIMEORDER (step_ch.c:442 setup_rktimestep): 1 Euler, 2 (or 0) midpoint,
+                          3 TVD RK3, 4 classical RK4 */
   int doenoflux;       /* HG_NOENOFLUX (advance.c:58 advance_standard) or HG_ENOFINITEVOLUME
                           (advance.c:245 advance_finitevolume with the torus file's avgscheme
                           DONOR: the conserved state is carried from step to step and the
@@ -138,6 +139,18 @@ int hg_fluxcalc(hg_ctx *ctx, double *F1, double *F2, double *F3, double ndt[3], 
    out: the next step size.  nfail/iters (may be NULL) receive the number of
    failed inversions and the total Newton iterations of the step. */
 int hg_step(hg_ctx *ctx, double *dt, int64_t *nfail, int64_t *iters, void *stream);
+
+/* One substep of the reference's step loop with the caller's coefficients
+   (step_ch.c:97-166: advance (advance.c:16), bound_prim, post_fixup,
+   bound_prim), for a caller that keeps its own setup_rktimestep.  pi, pb, pf
+   name the context's state arrays: 0 the primitives (hg_set_prims), 1 and 2
+   the two intermediate states (pk[0], pk[1]).  CUf[0..2] and Cunew[0..2] as
+   in dUtoU (advance.c:922); timeorder 0 zeroes uf and ucum, the last
+   (numtimeorders - 1) inverts ucum and applies fixup1zone.  dt is the step's
+   dt; *ndt (may be NULL) receives advance's ndt (advance.c:231).  The
+   context's timeorder does not matter here. */
+int hg_advance_substep(hg_ctx *ctx, int pi, int pb, int pf, const double CUf[4], const double Cunew[3],
+                       int timeorder, int numtimeorders, double dt, double *ndt, void *stream);
 
 /* Same as hg_step but leaves dt on the device and does not synchronise:
    n steps are enqueued on `stream`; hg_get_dt reads the device dt. */

@@ -109,7 +109,7 @@ typedef struct {
      gcov [(n1+2nb+1)*(n2+2nb+1)][8][16], gcon same, gcovpert [..][8][4], gdet [..][8]
      conn [(n1+2nb)*(n2+2nb)][64], rfloor [(n1+2nb)*(n2+2nb)][2] (rho, u floors at CENT) */
   const double *gcov, *gcon, *gcovpert, *gdet, *conn, *floors;
-  int timeorder;            /* 2 (or 0) = midpoint RK2, 4 = RK4 (step_ch.c:442) */
+  int timeorder;            /* 1 Euler, 2 (or 0) midpoint RK2, 3 TVD RK3, 4 RK4 (step_ch.c:442) */
   int fv;                   /* 1 = ENOFINITEVOLUME with avgscheme DONOR (advance.c:245) */
 } or_grid;
 

@@ -433,7 +433,8 @@ static int advance(const or_grid *G, const double *pi, const double *pb, double 
          *F3 = calloc(nz * 8, sizeof(double));
   double ndt[3], idx1 = 1.0 / G->dx[1], idx2 = 1.0 / G->dx[2], idx3 = 1.0 / G->dx[3];
   int i, j, k, pl, rc = 0;
-  int finalstep = (timeorder == ((G->timeorder == 4) ? 3 : 1));
+  int nto = (G->timeorder == 1 || G->timeorder == 3 || G->timeorder == 4) ? G->timeorder : 2;
+  int finalstep = (timeorder == nto - 1);
 
   if (or_fluxcalc(G, pb, F1, F2, F3, ndt)) { rc = 1; goto done; }
   for (i = 0; i < d.n1; i++)
@@ -509,7 +510,29 @@ static int step_impl(const or_grid *G, double *p, double *ucons, double *dt_io, 
   double dt = *dt_io, ndt = BIG;
   int rc = 0, t;
   memcpy(pk, p, nz * 8 * sizeof(double));
-  if (G->timeorder == 4) {
+  if (G->timeorder == 1 || G->timeorder == 3) {
+    /* step_ch.c:476-494 TVD RK3: pb = p, pk, pk2; pf = pk, pk2, p;
+       step_ch.c:546-557 Euler: pb = pf = p */
+    const double CUf3[3][4] = {{1.0, 0.0, 1.0, 0.0}, {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, 0.0},
+                               {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, 0.0}};
+    const double Cunew3[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}};
+    double *pbb[3], *pff[3];
+    int n = G->timeorder;
+    pk2 = malloc(nz * 8 * sizeof(double));
+    memcpy(pk2, p, nz * 8 * sizeof(double));
+    pbb[0] = p; pbb[1] = pk; pbb[2] = pk2;
+    pff[0] = pk; pff[1] = pk2; pff[2] = p;
+    if (n == 1) pff[0] = p;
+    for (t = 0; t < n; t++) {
+      const double *cn = (n == 1) ? Cunew3[2] : Cunew3[t];
+      if (advance(G, p, pbb[t], pff[t], uf, ucum, ucons, CUf3[t], cn, t, dt, &ndt, pflag, nfail, iters) ||
+          or_bound_prim(G, pff[t]) || or_post_fixup_fv(G, pff[t], pflag, t == n - 1, ucons) ||
+          or_bound_prim(G, pff[t])) {
+        rc = 1;
+        goto done;
+      }
+    }
+  } else if (G->timeorder == 4) {
     /* pi = p; pb = p, pk, pk2, pk; pf = pk, pk2, pk, p */
     double *pbb[4], *pff[4];
     pk2 = malloc(nz * 8 * sizeof(double));

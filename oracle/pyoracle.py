@@ -51,9 +51,9 @@ def _p(a):
 
 # scheme number of the fixtures ("weno" entry, the golden harness's second
 # argument) -> (oracle lim, flux, timeorder, finite volume): MC + LaxF + RK2,
-# WENO5BND + HLL + RK2, PARA + HLL + RK4, and the last with the
-# finite-volume step (the torus file's own choice)
-SCHEMES = {0: (0, 0, 2, 0), 1: (1, 1, 2, 0), 2: (2, 1, 4, 0), 3: (2, 1, 4, 1)}
+# WENO5BND + HLL + RK2, PARA + HLL + RK4, the last with the finite-volume
+# step (the torus file's own choice), MC + LaxF + TVD RK3, MC + LaxF + Euler
+SCHEMES = {0: (0, 0, 2, 0), 1: (1, 1, 2, 0), 2: (2, 1, 4, 0), 3: (2, 1, 4, 1), 4: (0, 0, 3, 0), 5: (0, 0, 1, 0)}
 
 
 class OrGrid(C.Structure):

@@ -17,7 +17,9 @@
  *       lim, fluxmethod and TIMEORDER, init.fishmon.c:110-114, with
  *       DOENOFLUX NOENOFLUX), 3 the same with its DOENOFLUX ENOFINITEVOLUME
  *       (and its avgscheme DONOR, init.fishmon.c:97): the torus file's own
- *       scheme.  Every mode's <weno> argument below is this scheme number.
+ *       scheme; 4 MC + LaxF + TVD RK3, 5 MC + LaxF + Euler (the other
+ *       TIMEORDERs of step_ch.c:442).  Every mode's <weno> argument below is
+ *       this scheme number.
  *   c2p <infile> <outfile> <gamma>
  *       Batched inversion: each input record holds a metric (gcov, gcon,
  *       gdet, gcovpert), the true primitives and a guess.  U is built with
@@ -129,10 +131,10 @@ static void harness_init(int scheme)
   init_defconsts();
   init_global();
   /* OVERRIDE: the hot-path configurations (init.fishmon.c:96-118 picks PARA/RK4/FV) */
-  lim = (scheme >= 2) ? PARA : use_weno ? WENO5BND : MC;
-  fluxmethod = (scheme == 0) ? LAXFFLUX : HLLFLUX;
+  lim = (scheme == 2 || scheme == 3) ? PARA : use_weno ? WENO5BND : MC;
+  fluxmethod = (scheme == 0 || scheme >= 4) ? LAXFFLUX : HLLFLUX;
   FLUXB = FLUXCTTOTH;
-  TIMEORDER = (scheme >= 2) ? 4 : 2;
+  TIMEORDER = (scheme == 2 || scheme == 3) ? 4 : (scheme == 4) ? 3 : (scheme == 5) ? 1 : 2;
   DOENOFLUX = (scheme == 3) ? ENOFINITEVOLUME : NOENOFLUX;
   cour = 0.8;
   rescaletype = 0; /* floors rho>=RHOMIN r^-1.5, u>=UUMIN r^-2.5 (fixup.c:1562) */

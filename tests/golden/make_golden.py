@@ -19,7 +19,9 @@ Fixtures
   t2d_fv.npz     2-D torus 32x32, the same with the finite-volume step
         (ENOFINITEVOLUME, avgscheme DONOR), 5 RK4 steps
   t3d_fv.npz     3-D torus 16x16x8, finite-volume PARA + HLL, 2 RK4 steps
-        (these four written alone by --para)
+  t2d_rk3.npz    2-D torus 32x32, MC + LaxF, 5 TVD RK3 steps
+  t2d_euler.npz  2-D torus 32x32, MC + LaxF, 5 Euler steps
+        (these six written alone by --para)
 Each torus fixture carries the reference geometry tables, the initial
 primitives, the reconstruction states (or MC slopes) per direction, the
 fluxes after one fluxcalc()+flux_ct() and the primitives after steps.
@@ -380,6 +382,8 @@ def main_para():
     np.savez_compressed(os.path.join(HERE, "t3d_para.npz"), **run_torus(os.path.join(REFDIR, "harness_t3d"), 2, 1))
     np.savez_compressed(os.path.join(HERE, "t2d_fv.npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), 3, 5))
     np.savez_compressed(os.path.join(HERE, "t3d_fv.npz"), **run_torus(os.path.join(REFDIR, "harness_t3d"), 3, 2))
+    np.savez_compressed(os.path.join(HERE, "t2d_rk3.npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), 4, 5))
+    np.savez_compressed(os.path.join(HERE, "t2d_euler.npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), 5, 5))
 
 
 def main():

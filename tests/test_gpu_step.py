@@ -1,8 +1,10 @@
 """Full torus steps on the GPU (ghost fill, reconstruction, fluxes,
 flux-CT, update, Noble 2D, fixup1zone, post_fixup) against the reference
 golden vectors: bit-for-bit primitives and time steps.  Schemes: MC + LaxF
-and WENO5BND + HLL with the midpoint RK2, PARA + HLL with RK4, and the
-last with the finite-volume step (the torus file's own scheme)."""
+and WENO5BND + HLL with the midpoint RK2, PARA + HLL with RK4, the last with
+the finite-volume step (the torus file's own scheme), MC + LaxF with TVD RK3
+and with Euler; and the reference's step loop driven substep by substep
+through hg_advance_substep."""
 import numpy as np
 import pytest
 
@@ -10,7 +12,7 @@ from golden_util import load
 
 pytestmark = pytest.mark.gpu
 
-TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para", "t2d_fv", "t3d_fv"]
+TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para", "t2d_fv", "t3d_fv", "t2d_rk3", "t2d_euler"]
 
 
 def scheme(d):
@@ -19,7 +21,9 @@ def scheme(d):
     return {0: (h.HG_LIM_MC, h.HG_FLUX_LAXF, 2, h.HG_NOENOFLUX),
             1: (h.HG_LIM_WENO5BND, h.HG_FLUX_HLL, 2, h.HG_NOENOFLUX),
             2: (h.HG_LIM_PARA, h.HG_FLUX_HLL, 4, h.HG_NOENOFLUX),
-            3: (h.HG_LIM_PARA, h.HG_FLUX_HLL, 4, h.HG_ENOFINITEVOLUME)}[int(d["weno"])]
+            3: (h.HG_LIM_PARA, h.HG_FLUX_HLL, 4, h.HG_ENOFINITEVOLUME),
+            4: (h.HG_LIM_MC, h.HG_FLUX_LAXF, 3, h.HG_NOENOFLUX),
+            5: (h.HG_LIM_MC, h.HG_FLUX_LAXF, 1, h.HG_NOENOFLUX)}[int(d["weno"])]
 
 
 def make_grid(d):
@@ -126,4 +130,35 @@ def test_launch_shapes_keep_every_bit(name, shape):
     for n in range(1, int(d["nsteps"]) + 1):
         dt, nf, it = g.step_ch(dt)
         assert dt == d["dt_seq"][n]
+    assert np.array_equal(g.get_prims(), d["p_final"])
+
+
+SUBSTEP_TABLES = {
+    # step_ch.c:442 setup_rktimestep: (CUf, Cunew, pb, pf) per substep
+    2: ([(1.0, 0.0, 0.5, 0.0), (1.0, 0.0, 1.0, 0.0)], [(0.0, 0.0, 0.0), (0.0, 0.0, 1.0)], [0, 1], [1, 0]),
+    3: ([(1.0, 0.0, 1.0, 0.0), (3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, 0.0), (1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, 0.0)],
+        [(0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0)], [0, 1, 2], [1, 2, 0]),
+    4: ([(1.0, 0.0, 0.5, 0.0), (1.0, 0.0, 0.5, 0.0), (1.0, 0.0, 1.0, 0.0), (1.0, 0.0, 1.0, 0.0)],
+        [(1.0, 1.0 / 6.0, 0.0), (0.0, 1.0 / 3.0, 0.0), (0.0, 1.0 / 3.0, 0.0), (0.0, 1.0 / 6.0, 0.0)],
+        [0, 1, 2, 1], [1, 2, 1, 0]),
+}
+
+
+@pytest.mark.parametrize("name", ["t2d_mc", "t2d_rk3", "t2d_para", "t3d_fv"])
+def test_substeps_driven_by_caller_match_reference(name):
+    """The reference's step loop (step_ch.c:97-166) with its own RK tables,
+    one hg_advance_substep per substep; dt from advance's ndt as step_ch.c
+    sets it (min over the substeps' last, SAFE limit)."""
+    d = load(name)
+    g = make_grid(d)
+    g.set_prims(d["p_init"])
+    n = int(g.cfg.timeorder) or 2
+    cuf, cun, pb, pf = SUBSTEP_TABLES[n]
+    dt = float(d["dt_seq"][0])
+    for step in range(1, int(d["nsteps"]) + 1):
+        for t in range(n):
+            ndt = g.advance_substep(0, pb[t], pf[t], cuf[t], cun[t], t, n, dt)
+        ndt = min(ndt, 1.3 * dt)  # step_ch.c:243-245, SAFE 1.3
+        dt = ndt
+        assert dt == d["dt_seq"][step]
     assert np.array_equal(g.get_prims(), d["p_final"])
