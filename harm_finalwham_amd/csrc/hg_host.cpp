@@ -63,7 +63,7 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
     set_error("hg_ctx_create: the finite-volume scheme is built for MC and PARA reconstruction only");
     return 1;
   }
-  if (cfg->fluxmethod != HG_FLUX_LAXF && cfg->fluxmethod != HG_FLUX_HLL) {
+  if (cfg->fluxmethod < HG_FLUX_LAXF || cfg->fluxmethod > HG_FLUX_HLLLAXF1) {
     set_error("hg_ctx_create: unsupported fluxmethod %d", cfg->fluxmethod);
     return 1;
   }

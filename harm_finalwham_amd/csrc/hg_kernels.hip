@@ -226,6 +226,132 @@ __device__ __forceinline__ bool face_flux(int dir, Geom &G, double *p_l, double 
   return false;
 }
 
+// ---------------------------------------------------------------------------
+// the other flux methods of fluxcompute.c:100 flux_compute: FORCE, MUSTA and
+// HLLLAXF1 (fluxmethod 2, 3, 4).  FORCE and MUSTA invert intermediate
+// conserved states at the face, so they live apart from the fast path.
+// ---------------------------------------------------------------------------
+// fluxcompute.c:57 p2SFUevolve + vchar of one side
+__device__ __forceinline__ bool side_flux(int dir, const Geom &G, const double *p, double gam, double *F,
+                                          double *U, double &cmax, double &cmin) {
+  State q;
+  if (get_state(p, G, q)) return true;
+  Maxwell M;
+  maxwell(p, q, G, M);
+  double p1 = one_plus_ud0(q, G);
+  primtoflux(p, q, M, p1, dir, G, gam, F);
+  if (U) primtoflux(p, q, M, p1, 0, G, gam, U);
+  return vchar(p, q, dir, G, gam, cmax, cmin) != 0;
+}
+
+// fluxcompute.c:232 hllflux_compute (LaxF when cmax + cmin = 0)
+__device__ __forceinline__ void hll_or_laxf(double cmin, double cmax, double ctop, const double *U_l,
+                                            const double *U_r, const double *F_l, const double *F_r, double *F) {
+  if (cmax + cmin != 0.0) {
+    for (int pl = 0; pl < NPR; pl++)
+      F[pl] = ((cmax * F_l[pl] + cmin * F_r[pl]) - (cmax * cmin) * (U_r[pl] - U_l[pl])) / (cmax + cmin);
+  } else {
+    for (int pl = 0; pl < NPR; pl++) F[pl] = 0.5 * ((F_l[pl] + F_r[pl]) - ctop * (U_r[pl] - U_l[pl]));
+  }
+}
+
+__device__ __noinline__ bool face_flux_ext(int dir, const Geom &G, double *p_l, double *p_r, double gam,
+                                           int fluxmethod, double *F, double &ctop) {
+  double bav = 0.5 * (p_l[B1 + dir - 1] + p_r[B1 + dir - 1]);  // flux.c:629 BDIRCONT
+  p_l[B1 + dir - 1] = bav;
+  p_r[B1 + dir - 1] = bav;
+  double F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
+  double cmax_l, cmin_l, cmax_r, cmin_r;
+  if (side_flux(dir, G, p_l, gam, F_l, U_l, cmax_l, cmin_l)) return true;
+  if (side_flux(dir, G, p_r, gam, F_r, U_r, cmax_r, cmin_r)) return true;
+  double cmin = fabs(rmax(0., -rmin(cmin_l, cmin_r)));
+  double cmax = fabs(rmax(0., rmax(cmax_l, cmax_r)));
+  ctop = rmax(cmax, cmin);
+  int iters;
+  if (fluxmethod == 4) {
+    // fluxcompute.c:189 HLLLAXF1: HLL and LaxF blended by the jump in u
+    double dPoP = fabs((p_r[UU] - p_l[UU]) / (0.5 * (p_r[UU] + p_l[UU])));
+    double f_s = (1.0 - 0.0) / (0.3 - 0.0) * (dPoP - 0.0) + 0.0;  // MINDPOP 0, MAXDPOP 0.3
+    if (f_s > 1.0) f_s = 1.0;
+    if (f_s < 0.0) f_s = 0.0;
+    if (cmax + cmin != 0.0) {
+      for (int pl = 0; pl < NPR; pl++) {
+        double hll = ((cmax * F_l[pl] + cmin * F_r[pl]) - (cmax * cmin) * (U_r[pl] - U_l[pl])) / (cmax + cmin);
+        double lax = 0.5 * ((F_l[pl] + F_r[pl]) - ctop * (U_r[pl] - U_l[pl]));
+        F[pl] = hll * f_s + lax * (1.0 - f_s);
+      }
+    } else {
+      for (int pl = 0; pl < NPR; pl++) F[pl] = 0.5 * ((F_l[pl] + F_r[pl]) - ctop * (U_r[pl] - U_l[pl]));
+    }
+    return false;
+  }
+  if (fluxmethod == 2) {
+    // fluxcompute.c:298 forceflux_compute with cforce = ctop: the two-step
+    // Lax-Wendroff state inverted (guesses: the mean, then each side),
+    // else HLL; GFORCE with MUSTACOEF 1 (fluxcompute.c:95)
+    const double cforce = ctop;
+    double umid[NPR], pmid[NPR];
+    for (int pl = 0; pl < NPR; pl++) umid[pl] = 0.5 * ((U_l[pl] + U_r[pl]) - (F_r[pl] - F_l[pl]) / cforce);
+    for (int pl = 0; pl < NPR; pl++) pmid[pl] = 0.5 * (p_l[pl] + p_r[pl]);
+    int fl = utoprimgen_fp64(umid, G, gam, pmid, iters);
+    if (fl) {
+      for (int pl = 0; pl < NPR; pl++) pmid[pl] = p_l[pl];
+      fl = utoprimgen_fp64(umid, G, gam, pmid, iters);
+      if (fl) {
+        for (int pl = 0; pl < NPR; pl++) pmid[pl] = p_r[pl];
+        fl = utoprimgen_fp64(umid, G, gam, pmid, iters);
+      }
+    }
+    if (fl) {
+      hll_or_laxf(cmin, cmax, ctop, U_l, U_r, F_l, F_r, F);
+      return false;
+    }
+    double fmid[NPR];
+    {
+      State q;
+      if (get_state(pmid, G, q)) return true;
+      Maxwell M;
+      maxwell(pmid, q, G, M);
+      double p1 = one_plus_ud0(q, G);
+      primtoflux(pmid, q, M, p1, dir, G, gam, fmid);
+    }
+    for (int pl = 0; pl < NPR; pl++)
+      F[pl] = ((fmid[pl]) + (1.0) * (0.5 * ((F_l[pl] + F_r[pl]) - cforce * (U_r[pl] - U_l[pl])))) / (1.0 + (1.0));
+    return false;
+  }
+  // fluxcompute.c:398 mustaflux_compute -> :538 musta1flux_compute (MUSTAHLL,
+  // one single-cell iteration, MUSTAVERSION 0, MUSTACOEF 1): HLL predictor,
+  // both sides' U corrected by it and inverted (guess: that side's state),
+  // then HLL of the corrected states; any failed inversion falls back to
+  // HLL of the original states.  The energy-momentum fluxes are always the
+  // original states' HLL (fluxcompute.c:527-536)
+  double Fh[NPR];
+  hll_or_laxf(cmin, cmax, ctop, U_l, U_r, F_l, F_r, Fh);
+  double Ul2[NPR], Ur2[NPR], pl2[NPR], pr2[NPR], Fl2[NPR], Fr2[NPR];
+  const double cmusta = ctop;
+  for (int pl = 0; pl < NPR; pl++) {
+    Ul2[pl] = U_l[pl] - (1.0) * (Fh[pl] - F_l[pl]) / cmusta;
+    Ur2[pl] = U_r[pl] - (1.0) * (F_r[pl] - Fh[pl]) / cmusta;
+    pl2[pl] = p_l[pl];
+    pr2[pl] = p_r[pl];
+  }
+  bool musta = utoprimgen_fp64(Ul2, G, gam, pl2, iters) == 0;
+  if (musta) musta = utoprimgen_fp64(Ur2, G, gam, pr2, iters) == 0;
+  if (musta) {
+    double cmax_l2, cmin_l2, cmax_r2, cmin_r2;
+    if (side_flux(dir, G, pl2, gam, Fl2, nullptr, cmax_l2, cmin_l2)) return true;
+    if (side_flux(dir, G, pr2, gam, Fr2, nullptr, cmax_r2, cmin_r2)) return true;
+    double cmin2 = fabs(rmax(0., -rmin(cmin_l2, cmin_r2)));
+    double cmax2 = fabs(rmax(0., rmax(cmax_l2, cmax_r2)));
+    double ctop2 = rmax(cmax2, cmin2);
+    hll_or_laxf(cmin2, cmax2, ctop2, Ul2, Ur2, Fl2, Fr2, F);
+  } else {
+    for (int pl = 0; pl < NPR; pl++) F[pl] = Fh[pl];
+  }
+  for (int pl = UU; pl <= U3; pl++) F[pl] = Fh[pl];
+  return false;
+}
+
 // face range of interp_loop_set (initbase.c:1027-1054): faces 0..n along dir,
 // one extra layer on each side transverse to it
 struct FaceRange {
@@ -313,7 +439,7 @@ __device__ __forceinline__ void para4(const double y[5], double &lout, double &r
 // point-type reconstruction fused with the Riemann flux (flux.c:469
 // fluxcalc_standard): MC slopes (interppoint.c:387) or PARA interface values
 // (interppoint.c:289), turned into the face states by flux.c:753 getp2interplr
-template <int DIR, bool PARA>
+template <int DIR, bool PARA, bool EXT = false>
 __device__ __forceinline__ void flux_point_body(const double *__restrict__ p, const double *__restrict__ geo,
                                                 double *__restrict__ F, const Dims &d, const FaceRange &fr,
                                                 double gam, double cour, double dxdir, int fluxmethod,
@@ -352,7 +478,8 @@ __device__ __forceinline__ void flux_point_body(const double *__restrict__ p, co
     Geom G;
     load_geom(geo, DIR, d.gplane, d.gidx(i, j), G);
     double Fz[NPR], ctop;
-    if (face_flux(DIR, G, p_l, p_r, gam, fluxmethod, Fz, ctop)) {
+    if (EXT ? face_flux_ext(DIR, G, p_l, p_r, gam, fluxmethod, Fz, ctop)
+            : face_flux(DIR, G, p_l, p_r, gam, fluxmethod, Fz, ctop)) {
       atomicOr(err, 2);
     } else {
 #pragma unroll
@@ -380,8 +507,18 @@ __global__ void __launch_bounds__(256) k_flux_para(const double *__restrict__ p,
   flux_point_body<DIR, true>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
-template <int DIR>
-__global__ void __launch_bounds__(256) k_flux_lr(const double *__restrict__ pleft,
+// MC or PARA with the FORCE / MUSTA / HLLLAXF1 fluxes
+template <int DIR, bool PARA>
+__global__ void __launch_bounds__(256) k_flux_pt_ext(const double *__restrict__ p, const double *__restrict__ geo,
+                                                     double *__restrict__ F, Dims d, FaceRange fr, double gam,
+                                                     double cour, double dxdir, int fluxmethod,
+                                                     unsigned long long *__restrict__ ndt_bits,
+                                                     int *__restrict__ err) {
+  flux_point_body<DIR, PARA, true>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+}
+
+template <int DIR, bool EXT>
+__device__ __forceinline__ void flux_lr_body(const double *__restrict__ pleft,
                                                  const double *__restrict__ pright,
                                                  const double *__restrict__ geo, double *__restrict__ F, Dims d,
                                                  double gam, double cour, double dxdir, int fluxmethod,
@@ -408,7 +545,8 @@ __global__ void __launch_bounds__(256) k_flux_lr(const double *__restrict__ plef
     Geom G;
     load_geom(geo, DIR, d.gplane, d.gidx(i, j), G);
     double Fz[NPR], ctop;
-    if (face_flux(DIR, G, p_l, p_r, gam, fluxmethod, Fz, ctop)) {
+    if (EXT ? face_flux_ext(DIR, G, p_l, p_r, gam, fluxmethod, Fz, ctop)
+            : face_flux(DIR, G, p_l, p_r, gam, fluxmethod, Fz, ctop)) {
       atomicOr(err, 2);
     } else {
 #pragma unroll
@@ -417,6 +555,26 @@ __global__ void __launch_bounds__(256) k_flux_lr(const double *__restrict__ plef
     }
   }
   block_min_to(dtij, ndt_bits);
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) k_flux_lr(const double *__restrict__ pleft,
+                                                 const double *__restrict__ pright,
+                                                 const double *__restrict__ geo, double *__restrict__ F, Dims d,
+                                                 double gam, double cour, double dxdir, int fluxmethod,
+                                                 unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
+  flux_lr_body<DIR, false>(pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+}
+
+// WENO5BND states with the FORCE / MUSTA / HLLLAXF1 fluxes
+template <int DIR>
+__global__ void __launch_bounds__(256) k_flux_lr_ext(const double *__restrict__ pleft,
+                                                     const double *__restrict__ pright,
+                                                     const double *__restrict__ geo, double *__restrict__ F, Dims d,
+                                                     double gam, double cour, double dxdir, int fluxmethod,
+                                                     unsigned long long *__restrict__ ndt_bits,
+                                                     int *__restrict__ err) {
+  flux_lr_body<DIR, true>(pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -770,6 +928,18 @@ void launch_flux_point(int dir, bool para, bool fv, const double *p, const doubl
   FaceRange fr = fv ? face_range_fv(d, dir) : face_range(d, dir);
   int64_t n = (int64_t)fr.ni * fr.nj * fr.nk;
   dim3 g(nblocks(n, tu.bs_flux)), b(tu.bs_flux);
+  if (fluxmethod >= 2) {
+    if (para) {
+      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_ext<1, true>), g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_ext<2, true>), g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_ext<3, true>), g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    } else {
+      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_ext<1, false>), g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_ext<2, false>), g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_ext<3, false>), g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    }
+    return;
+  }
   if (para) {
     if (dir == 1) hipLaunchKernelGGL(k_flux_para<1>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
     if (dir == 2) hipLaunchKernelGGL(k_flux_para<2>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
@@ -816,6 +986,12 @@ void launch_flux_lr(int dir, const double *pleft, const double *pright, const do
   FaceRange fr = face_range(d, dir);
   int64_t n = (int64_t)fr.ni * fr.nj * fr.nk;
   dim3 g(nblocks(n, tu.bs_flux)), b(tu.bs_flux);
+  if (fluxmethod >= 2) {
+    if (dir == 1) hipLaunchKernelGGL(k_flux_lr_ext<1>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 2) hipLaunchKernelGGL(k_flux_lr_ext<2>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 3) hipLaunchKernelGGL(k_flux_lr_ext<3>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    return;
+  }
   if (dir == 1) hipLaunchKernelGGL(k_flux_lr<1>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
   if (dir == 2) hipLaunchKernelGGL(k_flux_lr<2>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
   if (dir == 3) hipLaunchKernelGGL(k_flux_lr<3>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);

@@ -13,6 +13,7 @@ LIBPATH = os.path.join(HERE, "libharmgpu.so")
 
 HG_LIM_MC, HG_LIM_PARA, HG_LIM_WENO5BND = 3, 4, 17
 HG_NOENOFLUX, HG_ENOFINITEVOLUME = 0, 3
+HG_FLUX_FORCE, HG_FLUX_MUSTA, HG_FLUX_HLLLAXF1 = 2, 3, 4
 HG_FLUX_LAXF, HG_FLUX_HLL = 0, 1
 HG_C2P_FP64, HG_C2P_MIXED = 0, 1
 

@@ -53,6 +53,9 @@ extern "C" {
 #define HG_LIM_WENO5BND 17
 #define HG_FLUX_LAXF 0
 #define HG_FLUX_HLL 1
+#define HG_FLUX_FORCE 2    /* GFORCE with the Lax-Wendroff state inverted at the face (fluxcompute.c:298) */
+#define HG_FLUX_MUSTA 3    /* single-cell MUSTA on HLL (fluxcompute.c:398, 538) */
+#define HG_FLUX_HLLLAXF1 4 /* HLL / LaxF blended by the jump in u (fluxcompute.c:189) */
 /* DOENOFLUX (global.h:188-191) */
 #define HG_NOENOFLUX 0
 #define HG_ENOFINITEVOLUME 3
@@ -63,7 +66,7 @@ typedef struct {
   int n1, n2, n3;      /* active zones */
   int nb;              /* ghost layers per side (>= 6 for WENO5BND, initbase.c:792) */
   int lim;             /* HG_LIM_MC, HG_LIM_PARA or HG_LIM_WENO5BND */
-  int fluxmethod;      /* HG_FLUX_LAXF or HG_FLUX_HLL */
+  int fluxmethod;      /* HG_FLUX_LAXF, HG_FLUX_HLL, HG_FLUX_FORCE, HG_FLUX_MUSTA or HG_FLUX_HLLLAXF1 */
   double gam;          /* adiabatic index */
   double cour;         /* Courant factor */
   double safe;         /* max dt growth per step (SAFE, step_ch.c:245) */

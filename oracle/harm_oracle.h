@@ -101,7 +101,7 @@ void or_floors(int64_t n, const double *r, double rhomin, double uumin, double *
 typedef struct {
   int n1, n2, n3, nb;       /* active zones per direction and ghost width */
   int lim;                  /* 0 = MC, 1 = WENO5BND, 2 = PARA (para4) */
-  int flux;                 /* 0 = LAXF, 1 = HLL */
+  int flux;                 /* 0 = LAXF, 1 = HLL, 2 = FORCE, 3 = MUSTA, 4 = HLLLAXF1 */
   double gam, cour, dx[4];
   double rhomin, uumin, gammamax;
   double defcon, safe;

@@ -219,6 +219,110 @@ int or_reconstruct(const or_grid *G, const double *p, int dir, double *pleft, do
 /* fluxes                                                              */
 /* ------------------------------------------------------------------ */
 
+/* fluxcompute.c:232 hllflux_compute, LaxF when cmax + cmin = 0 */
+static void hllflux(double cmin, double cmax, double ctop, const double *U_l, const double *U_r, const double *F_l,
+                    const double *F_r, double *F)
+{
+  int pl;
+  if (cmax + cmin != 0.0) {
+    for (pl = 0; pl < OR_NPR; pl++)
+      F[pl] = ((cmax * F_l[pl] + cmin * F_r[pl]) - (cmax * cmin) * (U_r[pl] - U_l[pl])) / (cmax + cmin);
+  } else {
+    for (pl = 0; pl < OR_NPR; pl++) F[pl] = 0.5 * ((F_l[pl] + F_r[pl]) - ctop * (U_r[pl] - U_l[pl]));
+  }
+}
+
+/* fluxcompute.c:100 flux_compute for FORCEFLUX (2), MUSTAFLUX (3) and
+   HLLLAXF1FLUX (4).  FORCE: forceflux_compute (fluxcompute.c:298) with
+   cforce = ctop and GFORCE, MUSTACOEF 1.  MUSTA: mustaflux_compute
+   (fluxcompute.c:398) -> musta1flux_compute (fluxcompute.c:538) with
+   WHICHFLUX MUSTAHLL, one iteration, MUSTAVERSION 0; the energy-momentum
+   fluxes are then the original HLL ones (fluxcompute.c:527-536). */
+static int other_flux(const or_grid *G, int dir, const or_geom *g, const double *p_l, const double *p_r,
+                      const double *U_l, const double *U_r, const double *F_l, const double *F_r, double cmin,
+                      double cmax, double ctop, double *F)
+{
+  int pl, fl, it;
+  if (G->flux == 4) {
+    double dPoP = fabs((p_r[OR_UU] - p_l[OR_UU]) / (0.5 * (p_r[OR_UU] + p_l[OR_UU])));
+    double f_s = (1.0 - 0.0) / (0.3 - 0.0) * (dPoP - 0.0) + 0.0;
+    if (f_s > 1.0) f_s = 1.0;
+    if (f_s < 0.0) f_s = 0.0;
+    if (cmax + cmin != 0.0) {
+      for (pl = 0; pl < OR_NPR; pl++)
+        F[pl] = ((cmax * F_l[pl] + cmin * F_r[pl]) - (cmax * cmin) * (U_r[pl] - U_l[pl])) / (cmax + cmin) * f_s +
+                (0.5 * ((F_l[pl] + F_r[pl]) - ctop * (U_r[pl] - U_l[pl]))) * (1.0 - f_s);
+    } else {
+      for (pl = 0; pl < OR_NPR; pl++) F[pl] = 0.5 * ((F_l[pl] + F_r[pl]) - ctop * (U_r[pl] - U_l[pl]));
+    }
+    return 0;
+  }
+  if (G->flux == 2) {
+    double cforce = ctop, umid[OR_NPR], pmid[OR_NPR], fmid[OR_NPR];
+    or_state q;
+    for (pl = 0; pl < OR_NPR; pl++) umid[pl] = 0.5 * ((U_l[pl] + U_r[pl]) - (F_r[pl] - F_l[pl]) / cforce);
+    for (pl = 0; pl < OR_NPR; pl++) pmid[pl] = 0.5 * (p_l[pl] + p_r[pl]);
+    or_utoprimgen(umid, g, G->gam, pmid, &fl, &it);
+    if (fl) {
+      for (pl = 0; pl < OR_NPR; pl++) pmid[pl] = p_l[pl];
+      or_utoprimgen(umid, g, G->gam, pmid, &fl, &it);
+      if (fl) {
+        for (pl = 0; pl < OR_NPR; pl++) pmid[pl] = p_r[pl];
+        or_utoprimgen(umid, g, G->gam, pmid, &fl, &it);
+      }
+    }
+    if (fl) {
+      hllflux(cmin, cmax, ctop, U_l, U_r, F_l, F_r, F);
+      return 0;
+    }
+    if (or_get_state(pmid, g, &q)) return 1;
+    or_primtoflux(pmid, &q, dir, g, G->gam, fmid);
+    for (pl = 0; pl < OR_NPR; pl++)
+      F[pl] = ((fmid[pl]) + (1.0) * (0.5 * ((F_l[pl] + F_r[pl]) - cforce * (U_r[pl] - U_l[pl])))) / (1.0 + (1.0));
+    return 0;
+  }
+  {
+    double Fh[OR_NPR], Ul[OR_NPR], Ur[OR_NPR], pl_[OR_NPR], pr_[OR_NPR], Fl[OR_NPR], Fr[OR_NPR];
+    double cmusta = ctop, cmax_l, cmin_l, cmax_r, cmin_r, lmin, lmax;
+    or_state ql, qr;
+    int domusta = 1;
+    hllflux(cmin, cmax, ctop, U_l, U_r, F_l, F_r, Fh);
+    for (pl = 0; pl < OR_NPR; pl++) {
+      Ul[pl] = U_l[pl];
+      Ur[pl] = U_r[pl];
+      pl_[pl] = p_l[pl];
+      pr_[pl] = p_r[pl];
+    }
+    for (pl = 0; pl < OR_NPR; pl++) {
+      Ul[pl] -= (1.0) * (Fh[pl] - F_l[pl]) / cmusta;
+      Ur[pl] -= (1.0) * (F_r[pl] - Fh[pl]) / cmusta;
+    }
+    or_utoprimgen(Ul, g, G->gam, pl_, &fl, &it);
+    if (fl) domusta = 0;
+    if (domusta) {
+      or_utoprimgen(Ur, g, G->gam, pr_, &fl, &it);
+      if (fl) domusta = 0;
+    }
+    if (domusta) {
+      if (or_get_state(pl_, g, &ql)) return 1;
+      or_primtoflux(pl_, &ql, dir, g, G->gam, Fl);
+      if (or_get_state(pr_, g, &qr)) return 1;
+      or_primtoflux(pr_, &qr, dir, g, G->gam, Fr);
+      if (or_vchar(pl_, &ql, dir, g, G->gam, &cmax_l, &cmin_l)) return 1;
+      if (or_vchar(pr_, &qr, dir, g, G->gam, &cmax_r, &cmin_r)) return 1;
+      lmin = rmin(cmin_l, cmin_r);
+      lmax = rmax(cmax_l, cmax_r);
+      lmin = fabs(rmax(0., -lmin));
+      lmax = fabs(rmax(0., lmax));
+      hllflux(lmin, lmax, rmax(lmax, lmin), Ul, Ur, Fl, Fr, F);
+    } else {
+      for (pl = 0; pl < OR_NPR; pl++) F[pl] = Fh[pl];
+    }
+    for (pl = OR_UU; pl <= OR_U3; pl++) F[pl] = Fh[pl];
+  }
+  return 0;
+}
+
 /* fluxcompute.c:12 flux_compute_general (p2SFUevolve, get_wavespeeds, HLL/LaxF) */
 static int flux_compute_general(const or_grid *G, int dir, const or_geom *g, const double *p_l,
                                 const double *p_r, double *F, double *ctop)
@@ -246,6 +350,8 @@ static int flux_compute_general(const or_grid *G, int dir, const or_geom *g, con
   if (G->flux == 0) {
     /* fluxcompute.c:78 LAXFCOMPUTE */
     for (pl = 0; pl < OR_NPR; pl++) F[pl] = 0.5 * ((F_l[pl] + F_r[pl]) - *ctop * (U_r[pl] - U_l[pl]));
+  } else if (G->flux >= 2) {
+    return other_flux(G, dir, g, p_l, p_r, U_l, U_r, F_l, F_r, cmin, cmax, *ctop, F);
   } else {
     /* fluxcompute.c:223 hllflux_compute */
     if (cmax + cmin != 0.0) {

@@ -21,7 +21,10 @@ Fixtures
   t3d_fv.npz     3-D torus 16x16x8, finite-volume PARA + HLL, 2 RK4 steps
   t2d_rk3.npz    2-D torus 32x32, MC + LaxF, 5 TVD RK3 steps
   t2d_euler.npz  2-D torus 32x32, MC + LaxF, 5 Euler steps
-        (these six written alone by --para)
+  t2d_force.npz  2-D torus 32x32, MC + FORCE, 5 RK2 steps
+  t2d_musta.npz  2-D torus 32x32, PARA + MUSTA, 5 RK2 steps
+  t2d_hlllaxf1.npz 2-D torus 32x32, WENO5BND + HLLLAXF1, 5 RK2 steps
+        (these nine written alone by --para)
 Each torus fixture carries the reference geometry tables, the initial
 primitives, the reconstruction states (or MC slopes) per direction, the
 fluxes after one fluxcalc()+flux_ct() and the primitives after steps.
@@ -384,6 +387,8 @@ def main_para():
     np.savez_compressed(os.path.join(HERE, "t3d_fv.npz"), **run_torus(os.path.join(REFDIR, "harness_t3d"), 3, 2))
     np.savez_compressed(os.path.join(HERE, "t2d_rk3.npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), 4, 5))
     np.savez_compressed(os.path.join(HERE, "t2d_euler.npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), 5, 5))
+    for scheme, name in ((6, "t2d_force"), (7, "t2d_musta"), (8, "t2d_hlllaxf1")):
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), scheme, 5))
 
 
 def main():

@@ -3,8 +3,8 @@ flux-CT, update, Noble 2D, fixup1zone, post_fixup) against the reference
 golden vectors: bit-for-bit primitives and time steps.  Schemes: MC + LaxF
 and WENO5BND + HLL with the midpoint RK2, PARA + HLL with RK4, the last with
 the finite-volume step (the torus file's own scheme), MC + LaxF with TVD RK3
-and with Euler; and the reference's step loop driven substep by substep
-through hg_advance_substep."""
+and with Euler, the FORCE, MUSTA and HLLLAXF1 fluxes; and the reference's
+step loop driven substep by substep through hg_advance_substep."""
 import numpy as np
 import pytest
 
@@ -12,7 +12,8 @@ from golden_util import load
 
 pytestmark = pytest.mark.gpu
 
-TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para", "t2d_fv", "t3d_fv", "t2d_rk3", "t2d_euler"]
+TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para", "t2d_fv", "t3d_fv", "t2d_rk3", "t2d_euler",
+        "t2d_force", "t2d_musta", "t2d_hlllaxf1"]
 
 
 def scheme(d):
@@ -23,7 +24,10 @@ def scheme(d):
             2: (h.HG_LIM_PARA, h.HG_FLUX_HLL, 4, h.HG_NOENOFLUX),
             3: (h.HG_LIM_PARA, h.HG_FLUX_HLL, 4, h.HG_ENOFINITEVOLUME),
             4: (h.HG_LIM_MC, h.HG_FLUX_LAXF, 3, h.HG_NOENOFLUX),
-            5: (h.HG_LIM_MC, h.HG_FLUX_LAXF, 1, h.HG_NOENOFLUX)}[int(d["weno"])]
+            5: (h.HG_LIM_MC, h.HG_FLUX_LAXF, 1, h.HG_NOENOFLUX),
+            6: (h.HG_LIM_MC, h.HG_FLUX_FORCE, 2, h.HG_NOENOFLUX),
+            7: (h.HG_LIM_PARA, h.HG_FLUX_MUSTA, 2, h.HG_NOENOFLUX),
+            8: (h.HG_LIM_WENO5BND, h.HG_FLUX_HLLLAXF1, 2, h.HG_NOENOFLUX)}[int(d["weno"])]
 
 
 def make_grid(d):

@@ -27,7 +27,8 @@ def test_c2p_failure_codes_present():
     assert set(np.unique(d["pflag"])) - {0}
 
 
-TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para", "t2d_fv", "t3d_fv", "t2d_rk3", "t2d_euler"]
+TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para", "t2d_fv", "t3d_fv", "t2d_rk3", "t2d_euler",
+        "t2d_force", "t2d_musta", "t2d_hlllaxf1"]
 
 
 @pytest.mark.parametrize("name", TORI)
