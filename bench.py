@@ -61,8 +61,29 @@ def profile_summary(prof, zones):
 DEVICE_KERNELS = {"recon+flux1": ("hg::k_weno_x", "hg::k_flux_lr<1>"),
                   "recon+flux2": ("hg::k_weno_m<2>", "hg::k_flux_lr<2>"),
                   "recon+flux3": ("hg::k_weno_m<3>", "hg::k_flux_lr<3>"),
-                  "emf": ("hg::k_emf",), "update": ("hg::k_update", "hg::k_invert")}
+                  "emf": ("hg::k_emf",), "update": ("hg::k_update<false, false>", "hg::k_invert<false>")}
 TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "hbm_traffic_torus3d_256x128x64_weno_hll.md")
+VALU_PROFILE = os.path.join(HERE, "profiles", "valu_issue_torus3d_256x128x64_weno_hll.md")
+FP64_PEAK_TFLOPS = 78.6
+
+
+def fp64_gflop_per_launch(logical):
+    """FP64 GFLOP per launch of a logical kernel at 256x128x64 from the
+    committed rocprofv3 FP64-instruction summary; None when missing."""
+    if not os.path.exists(VALU_PROFILE) or logical not in DEVICE_KERNELS:
+        return None
+    rows = {}
+    for line in open(VALU_PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 9 and cells[0].startswith("`"):
+            try:
+                rows[cells[0].strip("`")] = float(cells[7])
+            except ValueError:
+                pass
+    try:
+        return sum(rows[k] for k in DEVICE_KERNELS[logical])
+    except KeyError:
+        return None
 
 
 def traffic_per_launch(logical):
@@ -83,6 +104,15 @@ def traffic_per_launch(logical):
         return sum(rows[k] for k in DEVICE_KERNELS[logical])
     except KeyError:
         return None
+
+
+def fp64_rate(logical, mean_ms):
+    gf = fp64_gflop_per_launch(logical)
+    if gf is None:
+        return None
+    tf = gf / (mean_ms * 1e-3) / 1e3
+    return {"achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS,
+            "gflop_per_launch": gf, "source": os.path.relpath(VALU_PROFILE, HERE)}
 
 
 def cpu_baseline(timeout_s=240):
@@ -405,6 +435,8 @@ def main():
     ap.add_argument("--zones", default="256,128,64", help="n1,n2,n3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--serial", action="store_true",
+                    help="run the three directions one after another (for kernel traces)")
     ap.add_argument("--c2p-n", type=int, default=1 << 24, help="inversions per gam for the con2prim leg (0: skip)")
     ap.add_argument("--batch-tori", type=int, default=48, help="tori of the batched-graph leg (0: skip)")
     ap.add_argument("--mlp-train-n", type=int, default=1 << 22, help="training inversions of the learned start")
@@ -442,7 +474,13 @@ def main():
     print(f"# set-up {t_setup:.1f} s, dt0 = {dt0:.6g}", file=sys.stderr, flush=True)
     # launch shapes for this grid (hg_autotune times a few steps from the
     # current state and restores it; shapes never change a result bit)
-    shapes = (g.tune() if args.no_autotune else g.autotune(2)).as_dict()
+    t = g.tune() if args.no_autotune else g.autotune(2)
+    if args.serial:
+        # the three directions one after another, as the HIP-event profile
+        # pass below runs them (a kernel trace then sees each kernel alone)
+        t.concurrent = 0
+        g.set_tune(t)
+    shapes = t.as_dict()
     print(f"# launch shapes {json.dumps(shapes)}", file=sys.stderr, flush=True)
 
     g.step_async(args.warmup)
@@ -527,6 +565,8 @@ def main():
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, HERE),
                      "algorithmic_bytes_per_launch": dk["bytes_per_zone"] * zones,
                      "bytes_per_zone": dk["bytes_per_zone"], "mean_launch_ms": dk["mean_ms"]},
+        # the same kernel against the FP64 issue rate that actually bounds it
+        "fp64_valu": fp64_rate(dom, dk["mean_ms"]) if (n1, n2, n3) == (256, 128, 64) else None,
         "cpu_baseline": cpu,
         "kernels_ms_per_step": {k: v["total_ms"] / args.steps for k, v in summ.items()},
         "profiled_ms_per_step": (tp1 - tp0) * 1e3 / args.steps,
