@@ -205,10 +205,14 @@ def run_torus_long(harness, weno, nsteps, keep_final):
 
 def main_long(full):
     """128x128 MC/LaxF torus over 100 steps; with full, also the 256x128x64
-    WENO5BND/HLL torus over one step (about ten minutes and 14 GB here)."""
+    WENO5BND/HLL torus over one step (about ten minutes and 14 GB here);
+    with --fv alone, the 256x128x64 torus in the torus file's own scheme
+    (PARA + HLL + RK4, finite volume) over one step."""
     runs = [("c1", (128, 128, 1), 0, 100, True, "c1_mc_100.npz")]
     if full:
         runs.append(("f3d", (256, 128, 64), 1, 1, False, "f3d_weno_1.npz"))
+    if "--fv" in sys.argv:
+        runs = [("f3d", (256, 128, 64), 3, 1, False, "f3d_fv_1.npz")]
     for tag, (n1, n2, n3), weno, nsteps, keep, name in runs:
         subprocess.run([BUILD, tag, str(n1), str(n2), str(n3)], check=True, stdout=subprocess.DEVNULL)
         d = run_torus_long(os.path.join(REFDIR, "harness_" + tag), weno, nsteps, keep)

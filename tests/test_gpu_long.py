@@ -16,11 +16,12 @@ pytestmark = pytest.mark.gpu
 
 def _own(d, nb):
     from harm_finalwham_amd import harmgpu
+    from test_gpu_step import scheme
     par = d["params"]
     n1, n2, n3 = (int(x) for x in par[:3])
-    weno = int(d["weno"])
-    g = harmgpu.Grid(n1, n2, n3, nb=nb, lim=harmgpu.HG_LIM_WENO5BND if weno else harmgpu.HG_LIM_MC,
-                     fluxmethod=harmgpu.HG_FLUX_HLL if weno else harmgpu.HG_FLUX_LAXF, gam=par[5], cour=par[10])
+    lim, flux, timeorder, doenoflux = scheme(d)
+    g = harmgpu.Grid(n1, n2, n3, nb=nb, lim=lim, fluxmethod=flux, gam=par[5], cour=par[10], timeorder=timeorder,
+                     doenoflux=doenoflux)
     g.build_mks_geometry(a=par[4], Rin=par[6], Rout=par[7], R0=par[9], hslope=par[8])
     dt = g.init_torus(rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0, randfact=4e-2, seed=7)
     return g, dt
@@ -58,6 +59,26 @@ def test_benchmark_grid_one_step_matches_reference():
     assert state_digest(g.get_prims()) == str(d["digest_init"])
     dt, nfail, _ = g.step_ch(dt)
     assert nfail == d["fail_seq"][1]
+    assert dt == d["dt_seq"][1]
+    p = g.get_prims()
+    assert np.array_equal(p.reshape(-1, 8).sum(axis=0), d["sum_final"])
+    assert state_digest(p) == str(d["digest_final"])
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "f3d_fv_1.npz")),
+                    reason="full-size fixture not generated (make_golden.py --long --fv)")
+def test_benchmark_grid_torus_scheme_one_step_matches_reference():
+    """The torus file's own scheme (PARA + HLL + RK4, finite volume) on the
+    benchmark grid: one step, digests of the whole state."""
+    d = load("f3d_fv_1")
+    nb = int(d["params"][3])
+    print("case 256x128x64 PARA/HLL/RK4 finite volume, 1 step", flush=True)
+    g, dt = _own(d, nb)
+    assert dt == d["dt_seq"][0]
+    assert state_digest(g.get_prims()) == str(d["digest_init"])
+    dt, nfail, _ = g.step_ch(dt)
+    if d["fail_seq"][1]:
+        assert nfail > 0
     assert dt == d["dt_seq"][1]
     p = g.get_prims()
     assert np.array_equal(p.reshape(-1, 8).sum(axis=0), d["sum_final"])
