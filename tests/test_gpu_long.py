@@ -1,9 +1,11 @@
 """Longer and larger runs of the library against the reference:
-  the 128x128 MC/LaxF torus (configuration 1) over 100 RK2 steps, from the
-  library's own initial data: every dt, no failed inversion, the final state
-  bit for bit, div B at round-off;
-  the 256x128x64 WENO5BND/HLL torus (the benchmark grid) over one RK2 step:
-  digests of the initial and final states and the dt sequence."""
+  the 128x128 MC/LaxF torus over 100 RK2 steps, from the library's own
+  initial data: every dt, no failed inversion, the final state bit for bit,
+  div B at round-off;
+  the 256x128x64 WENO5BND/HLL torus (the benchmark grid) over one RK2 step,
+  and the same grid in the torus file's own scheme (PARA + HLL + RK4,
+  finite volume) over one step: digests of the initial and final states and
+  the dt sequence."""
 import os
 
 import numpy as np
