@@ -13,21 +13,27 @@ SPECS = [(32, 32, 0.0, 10.0, 1), (48, 64, 0.5, 11.0, 2), (64, 64, 0.9375, 12.0, 
          (64, 32, 0.3, 13.0, 5)]
 
 
-def torus(n1, n2, a, rmax, seed):
+def torus(n1, n2, a, rmax, seed, scheme="weno"):
     from harm_finalwham_amd import harmgpu
-    g = harmgpu.Grid(n1, n2, 1, nb=6, lim=harmgpu.HG_LIM_WENO5BND, fluxmethod=harmgpu.HG_FLUX_HLL)
+    if scheme == "weno":
+        g = harmgpu.Grid(n1, n2, 1, nb=6, lim=harmgpu.HG_LIM_WENO5BND, fluxmethod=harmgpu.HG_FLUX_HLL)
+    else:
+        # the torus file's own scheme: PARA + HLL + RK4, finite volume
+        g = harmgpu.Grid(n1, n2, 1, nb=3, lim=harmgpu.HG_LIM_PARA, fluxmethod=harmgpu.HG_FLUX_HLL, timeorder=4,
+                         doenoflux=harmgpu.HG_ENOFINITEVOLUME)
     rh = 1.0 + math.sqrt(1.0 - a * a)
     g.build_mks_geometry(a=a, Rin=0.9 * rh, Rout=40.0, R0=0.0, hslope=0.3)
     g.init_torus(rin=6.0, rmax=rmax, kappa=1e-3, beta=100.0, randfact=4e-2, seed=seed)
     return g
 
 
-def test_graph_matches_single_runs_on_every_replay():
+@pytest.mark.parametrize("scheme", ["weno", "fv"])
+def test_graph_matches_single_runs_on_every_replay(scheme):
     import torch
     from harm_finalwham_amd import harmgpu
-    print("case graph of", len(SPECS), "tori", flush=True)
-    batch = [torus(*s) for s in SPECS]
-    alone = [torus(*s) for s in SPECS]
+    print("case graph of", len(SPECS), "tori,", scheme, flush=True)
+    batch = [torus(*s, scheme=scheme) for s in SPECS]
+    alone = [torus(*s, scheme=scheme) for s in SPECS]
     graph = harmgpu.Graph(batch, nsteps=3)
     x = torch.ones(1 << 20, device="cuda", dtype=torch.float64)
     for replay in range(3):
@@ -38,6 +44,8 @@ def test_graph_matches_single_runs_on_every_replay():
         for gb, ga in zip(batch, alone):
             assert gb.get_dt() == ga.get_dt(), replay
             assert np.array_equal(gb.get_prims(), ga.get_prims()), replay
+            if scheme == "fv":
+                assert np.array_equal(gb.get_conserved(), ga.get_conserved()), replay
         # unrelated eager work between replays
         x = x * 1.0000001 + 1.0
         torch.cuda.synchronize()
