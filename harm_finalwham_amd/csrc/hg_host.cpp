@@ -59,10 +59,6 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
     set_error("hg_ctx_create: unsupported doenoflux %d", cfg->doenoflux);
     return 1;
   }
-  if (cfg->doenoflux == HG_ENOFINITEVOLUME && cfg->lim == HG_LIM_WENO5BND) {
-    set_error("hg_ctx_create: the finite-volume scheme is built for MC and PARA reconstruction only");
-    return 1;
-  }
   if (cfg->fluxmethod < HG_FLUX_LAXF || cfg->fluxmethod > HG_FLUX_HLLLAXF1) {
     set_error("hg_ctx_create: unsupported fluxmethod %d", cfg->fluxmethod);
     return 1;
@@ -419,7 +415,7 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
   if (f.lim == HG_LIM_WENO5BND) {
     // the dP/P inputs are shared by the three directions; timed with direction 1
     prof_mark(c, HG_PROF_RECON1, s);
-    launch_weno_prep(pb, c->geo, c->aux, d, f.gam, s);
+    launch_weno_prep(pb, c->geo, c->aux, d, f.gam, f.doenoflux == HG_ENOFINITEVOLUME, s);
   }
   // the directions are independent (own interface states, fluxes and ndt
   // slot): j and k on the side streams, i on s, joined before the EMFs.
@@ -450,10 +446,10 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
       prof_mark(c, -1, sd);
     } else {
       if (dir != 1) prof_mark(c, HG_PROF_RECON1 + dir - 1, sd);
-      launch_weno(dir, pb, c->aux, pls[dir], prs[dir], d, c->tune, sd);
+      launch_weno(dir, pb, c->aux, pls[dir], prs[dir], d, c->tune, f.doenoflux == HG_ENOFINITEVOLUME, sd);
       prof_mark(c, HG_PROF_FLUX1 + dir - 1, sd);
       launch_flux_lr(dir, pls[dir], prs[dir], c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,
-                     c->ndt_bits + dir - 1, c->err, c->tune, sd);
+                     c->ndt_bits + dir - 1, c->err, c->tune, f.doenoflux == HG_ENOFINITEVOLUME, sd);
       prof_mark(c, -1, sd);
     }
   }

@@ -30,7 +30,7 @@ void launch_cons_from_prims(const double *p, const double *geo, double *U, const
                             hipStream_t s);
 void launch_flux_lr(int dir, const double *pleft, const double *pright, const double *geo, double *F,
                     const Dims &d, double gam, double cour, double dxdir, int fluxmethod,
-                    unsigned long long *ndt_bits, int *err, const Tune &tu, hipStream_t s);
+                    unsigned long long *ndt_bits, int *err, const Tune &tu, bool fv, hipStream_t s);
 void launch_emf(const double *F1, const double *F2, const double *F3, const double *geo, const double *corn,
                 double *e1, double *e2, double *e3, const Dims &d, const Tune &tu, hipStream_t s);
 // per-zone update (-> ubuf) then inversion + fixup (-> pf); ubuf has the
@@ -63,9 +63,11 @@ void launch_c2p_batch(int64_t n, double gam, const double *U, const double *gcov
 // WENO5BND (hg_weno.hip): the per-zone dP/P inputs (2 x spl doubles in aux),
 // then the interface states along dir for every zone; pleft/pright have the
 // layout of the primitives
-void launch_weno_prep(const double *p, const double *geo, double *aux, const Dims &d, double gam, hipStream_t s);
+// wide: the finite-volume flux loops, which span every transverse ghost row
+void launch_weno_prep(const double *p, const double *geo, double *aux, const Dims &d, double gam, int wide,
+                      hipStream_t s);
 void launch_weno(int dir, const double *p, const double *aux, double *pleft, double *pright, const Dims &d,
-                 const Tune &tu, hipStream_t s);
+                 const Tune &tu, int wide, hipStream_t s);
 
 // initial time step over every zone (advance.c:957 set_dt), folded into *out
 void launch_set_dt(const double *p, const double *geo, const Dims &d, double gam, double cour, const double *dx,

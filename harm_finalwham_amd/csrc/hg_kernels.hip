@@ -521,9 +521,9 @@ template <int DIR, bool EXT>
 __device__ __forceinline__ void flux_lr_body(const double *__restrict__ pleft,
                                                  const double *__restrict__ pright,
                                                  const double *__restrict__ geo, double *__restrict__ F, Dims d,
-                                                 double gam, double cour, double dxdir, int fluxmethod,
-                                                 unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
-  FaceRange fr = face_range(d, DIR);
+                                                 FaceRange fr, double gam, double cour, double dxdir,
+                                                 int fluxmethod, unsigned long long *__restrict__ ndt_bits,
+                                                 int *__restrict__ err) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)fr.ni * fr.nj * fr.nk;
   double dtij = 1.e+50;
@@ -561,9 +561,10 @@ template <int DIR>
 __global__ void __launch_bounds__(256) k_flux_lr(const double *__restrict__ pleft,
                                                  const double *__restrict__ pright,
                                                  const double *__restrict__ geo, double *__restrict__ F, Dims d,
-                                                 double gam, double cour, double dxdir, int fluxmethod,
-                                                 unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
-  flux_lr_body<DIR, false>(pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+                                                 FaceRange fr, double gam, double cour, double dxdir,
+                                                 int fluxmethod, unsigned long long *__restrict__ ndt_bits,
+                                                 int *__restrict__ err) {
+  flux_lr_body<DIR, false>(pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
 // WENO5BND states with the FORCE / MUSTA / HLLLAXF1 fluxes
@@ -571,10 +572,10 @@ template <int DIR>
 __global__ void __launch_bounds__(256) k_flux_lr_ext(const double *__restrict__ pleft,
                                                      const double *__restrict__ pright,
                                                      const double *__restrict__ geo, double *__restrict__ F, Dims d,
-                                                     double gam, double cour, double dxdir, int fluxmethod,
-                                                     unsigned long long *__restrict__ ndt_bits,
+                                                     FaceRange fr, double gam, double cour, double dxdir,
+                                                     int fluxmethod, unsigned long long *__restrict__ ndt_bits,
                                                      int *__restrict__ err) {
-  flux_lr_body<DIR, true>(pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  flux_lr_body<DIR, true>(pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -982,19 +983,19 @@ void launch_cons_from_prims(const double *p, const double *geo, double *U, const
 
 void launch_flux_lr(int dir, const double *pleft, const double *pright, const double *geo, double *F,
                     const Dims &d, double gam, double cour, double dxdir, int fluxmethod,
-                    unsigned long long *ndt_bits, int *err, const Tune &tu, hipStream_t s) {
-  FaceRange fr = face_range(d, dir);
+                    unsigned long long *ndt_bits, int *err, const Tune &tu, bool fv, hipStream_t s) {
+  FaceRange fr = fv ? face_range_fv(d, dir) : face_range(d, dir);
   int64_t n = (int64_t)fr.ni * fr.nj * fr.nk;
   dim3 g(nblocks(n, tu.bs_flux)), b(tu.bs_flux);
   if (fluxmethod >= 2) {
-    if (dir == 1) hipLaunchKernelGGL(k_flux_lr_ext<1>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-    if (dir == 2) hipLaunchKernelGGL(k_flux_lr_ext<2>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-    if (dir == 3) hipLaunchKernelGGL(k_flux_lr_ext<3>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 1) hipLaunchKernelGGL(k_flux_lr_ext<1>, g, b, 0, s, pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 2) hipLaunchKernelGGL(k_flux_lr_ext<2>, g, b, 0, s, pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 3) hipLaunchKernelGGL(k_flux_lr_ext<3>, g, b, 0, s, pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
     return;
   }
-  if (dir == 1) hipLaunchKernelGGL(k_flux_lr<1>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-  if (dir == 2) hipLaunchKernelGGL(k_flux_lr<2>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-  if (dir == 3) hipLaunchKernelGGL(k_flux_lr<3>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 1) hipLaunchKernelGGL(k_flux_lr<1>, g, b, 0, s, pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 2) hipLaunchKernelGGL(k_flux_lr<2>, g, b, 0, s, pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 3) hipLaunchKernelGGL(k_flux_lr<3>, g, b, 0, s, pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
 void launch_emf(const double *F1, const double *F2, const double *F3, const double *geo, const double *corn,

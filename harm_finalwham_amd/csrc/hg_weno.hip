@@ -24,7 +24,8 @@
 //     marches along the line keeping the stage values of the last points
 //     in registers
 // Only the pencils and points the fluxes read are computed: transverse
-// indices -1..n and points -1..n along the line (weights at -4..n+3).
+// indices -1..n (every ghost row for the finite-volume flux loops) and
+// points -1..n along the line (weights at -4..n+3).
 // All arithmetic is FP64 in the reference's operation order.
 #include <hip/hip_runtime.h>
 
@@ -317,8 +318,10 @@ __device__ __forceinline__ void stage_e(const double *y, double m0, double w0, d
 // over the box [-2, n+1] of every direction with more than one zone
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_weno_prep(const double *__restrict__ p, const double *__restrict__ geo,
-                                                   double *__restrict__ aux, Dims d, double gam) {
-  const int e1 = d.n1 > 1 ? 2 : 0, e2 = d.n2 > 1 ? 2 : 0, e3 = d.n3 > 1 ? 2 : 0;
+                                                   double *__restrict__ aux, Dims d, double gam, int wide) {
+  // wide (the finite-volume flux loops): every ghost layer
+  const int e1 = d.n1 > 1 ? (wide ? d.nb1 : 2) : 0, e2 = d.n2 > 1 ? (wide ? d.nb2 : 2) : 0;
+  const int e3 = d.n3 > 1 ? (wide ? d.nb3 : 2) : 0;
   const int b1 = d.n1 + 2 * e1, b2 = d.n2 + 2 * e2, b3 = d.n3 + 2 * e3;
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= (int64_t)b1 * b2 * b3) return;
@@ -357,11 +360,12 @@ constexpr int XOUT = 58;
 
 __global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, const double *__restrict__ aux,
                                                 double *__restrict__ pleft, double *__restrict__ pright, Dims d,
-                                                int nseg) {
+                                                int nseg, int wide) {
   const int lane = threadIdx.x & 63;
   const int64_t wid = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int n = d.n1;
-  const int s2 = d.n2 > 1, s3 = d.n3 > 1;
+  // transverse rows: -1..n, or every ghost row for the finite-volume loops
+  const int s2 = d.n2 > 1 ? (wide ? d.nb2 : 1) : 0, s3 = d.n3 > 1 ? (wide ? d.nb3 : 1) : 0;
   const int c1 = d.n2 + 2 * s2, c2 = d.n3 + 2 * s3;
   if (wid >= (int64_t)nseg * NPR * c1 * c2) return;  // uniform per wavefront
   const int seg = (int)(wid % nseg);
@@ -406,11 +410,14 @@ __global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, co
 template <int DIR>
 __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, const double *__restrict__ aux,
                                                 double *__restrict__ pleft, double *__restrict__ pright, Dims d,
-                                                int nseg, int seglen) {
+                                                int nseg, int seglen, int wide) {
   const int n = (DIR == 1) ? d.n1 : (DIR == 2) ? d.n2 : d.n3;
-  // the lane's two transverse indices: a (fastest) and o
-  const int sa = (DIR == 1) ? (d.n2 > 1) : (d.n1 > 1);
-  const int so = (DIR == 2) ? (d.n3 > 1) : (DIR == 3) ? (d.n2 > 1) : (d.n3 > 1);
+  // the lane's two transverse indices: a (fastest) and o, over -1..n or,
+  // for the finite-volume loops, every ghost row
+  const int h1 = d.n1 > 1 ? (wide ? d.nb1 : 1) : 0, h2 = d.n2 > 1 ? (wide ? d.nb2 : 1) : 0;
+  const int h3 = d.n3 > 1 ? (wide ? d.nb3 : 1) : 0;
+  const int sa = (DIR == 1) ? h2 : h1;
+  const int so = (DIR == 2) ? h3 : (DIR == 3) ? h2 : h3;
   const int ca = ((DIR == 1) ? d.n2 : d.n1) + 2 * sa;
   const int co = ((DIR == 2) ? d.n3 : (DIR == 3) ? d.n2 : d.n3) + 2 * so;
   const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -482,20 +489,25 @@ __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, co
   }
 }
 
-void launch_weno_prep(const double *p, const double *geo, double *aux, const Dims &d, double gam, hipStream_t s) {
-  int64_t n = (int64_t)(d.n1 + (d.n1 > 1 ? 4 : 0)) * (d.n2 + (d.n2 > 1 ? 4 : 0)) * (d.n3 + (d.n3 > 1 ? 4 : 0));
-  hipLaunchKernelGGL(k_weno_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, geo, aux, d, gam);
+void launch_weno_prep(const double *p, const double *geo, double *aux, const Dims &d, double gam, int wide,
+                      hipStream_t s) {
+  const int e1 = d.n1 > 1 ? (wide ? d.nb1 : 2) : 0, e2 = d.n2 > 1 ? (wide ? d.nb2 : 2) : 0;
+  const int e3 = d.n3 > 1 ? (wide ? d.nb3 : 2) : 0;
+  int64_t n = (int64_t)(d.n1 + 2 * e1) * (d.n2 + 2 * e2) * (d.n3 + 2 * e3);
+  hipLaunchKernelGGL(k_weno_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, geo, aux, d, gam, wide);
 }
 
 void launch_weno(int dir, const double *p, const double *aux, double *pleft, double *pright, const Dims &d,
-                 const Tune &tu, hipStream_t s) {
-  int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+                 const Tune &tu, int wide, hipStream_t s) {
+  // transverse halo rows (the kernels use the same widths)
+  int s1 = d.n1 > 1 ? (wide ? d.nb1 : 1) : 0, s2 = d.n2 > 1 ? (wide ? d.nb2 : 1) : 0;
+  int s3 = d.n3 > 1 ? (wide ? d.nb3 : 1) : 0;
   if (dir == 1 && tu.weno_xmarch == 0) {
     int nseg = (d.n1 + 2 + XOUT - 1) / XOUT;
     int64_t nw = (int64_t)nseg * NPR * (d.n2 + 2 * s2) * (d.n3 + 2 * s3);
     int wpb = tu.weno_xwaves;
     hipLaunchKernelGGL(k_weno_x, dim3((unsigned)((nw + wpb - 1) / wpb)), dim3(64 * wpb), 0, s, p, aux, pleft,
-                       pright, d, nseg);
+                       pright, d, nseg, wide);
   } else {
     int n = (dir == 1) ? d.n1 : (dir == 2) ? d.n2 : d.n3;
     int64_t lanes = (dir == 1) ? (int64_t)(d.n2 + 2 * s2) * (d.n3 + 2 * s3)
@@ -508,11 +520,11 @@ void launch_weno(int dir, const double *p, const double *aux, double *pleft, dou
     nseg = (n + 2 + seglen - 1) / seglen;  // every segment starts at or before point n
     dim3 grid((unsigned)((lanes + 255) / 256), (unsigned)(NPR * nseg));
     if (dir == 1)
-      hipLaunchKernelGGL(k_weno_m<1>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen);
+      hipLaunchKernelGGL(k_weno_m<1>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen, wide);
     else if (dir == 2)
-      hipLaunchKernelGGL(k_weno_m<2>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen);
+      hipLaunchKernelGGL(k_weno_m<2>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen, wide);
     else
-      hipLaunchKernelGGL(k_weno_m<3>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen);
+      hipLaunchKernelGGL(k_weno_m<3>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen, wide);
   }
 }
 

@@ -79,7 +79,7 @@ typedef struct {
   int doenoflux;       /* HG_NOENOFLUX (advance.c:58 advance_standard) or HG_ENOFINITEVOLUME
                           (advance.c:245 advance_finitevolume with the torus file's avgscheme
                           DONOR: the conserved state is carried from step to step and the
-                          flux loops span every transverse ghost layer); MC or PARA only */
+                          flux loops span every transverse ghost layer) */
 } hg_config;
 
 /* Context with device-resident state for one grid.  The caller owns the

@@ -55,7 +55,7 @@ def _p(a):
 # step (the torus file's own choice), MC + LaxF + TVD RK3, MC + LaxF + Euler,
 # MC + FORCE, PARA + MUSTA, WENO5BND + HLLLAXF1 (midpoint RK2)
 SCHEMES = {0: (0, 0, 2, 0), 1: (1, 1, 2, 0), 2: (2, 1, 4, 0), 3: (2, 1, 4, 1), 4: (0, 0, 3, 0), 5: (0, 0, 1, 0),
-           6: (0, 2, 2, 0), 7: (2, 3, 2, 0), 8: (1, 4, 2, 0)}
+           6: (0, 2, 2, 0), 7: (2, 3, 2, 0), 8: (1, 4, 2, 0), 9: (1, 1, 2, 1)}
 
 
 class OrGrid(C.Structure):

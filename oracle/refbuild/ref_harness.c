@@ -20,8 +20,8 @@
  *       scheme; 4 MC + LaxF + TVD RK3, 5 MC + LaxF + Euler (the other
  *       TIMEORDERs of step_ch.c:442); 6 MC + FORCE, 7 PARA + MUSTA,
  *       8 WENO5BND + HLLLAXF1, each with RK2 (the other fluxmethods of
- *       fluxcompute.c:100).  Every mode's <weno> argument below is this
- *       scheme number.
+ *       fluxcompute.c:100); 9 WENO5BND + HLL + RK2 with the finite-volume
+ *       step.  Every mode's <weno> argument below is this scheme number.
  *   c2p <infile> <outfile> <gamma>
  *       Batched inversion: each input record holds a metric (gcov, gcon,
  *       gdet, gcovpert), the true primitives and a guess.  U is built with
@@ -133,12 +133,12 @@ static void harness_init(int scheme)
   init_defconsts();
   init_global();
   /* OVERRIDE: the hot-path configurations (init.fishmon.c:96-118 picks PARA/RK4/FV) */
-  lim = (scheme == 2 || scheme == 3 || scheme == 7) ? PARA : (use_weno || scheme == 8) ? WENO5BND : MC;
+  lim = (scheme == 2 || scheme == 3 || scheme == 7) ? PARA : (use_weno || scheme == 8 || scheme == 9) ? WENO5BND : MC;
   fluxmethod = (scheme == 6) ? FORCEFLUX : (scheme == 7) ? MUSTAFLUX : (scheme == 8) ? HLLLAXF1FLUX
              : (scheme == 0 || scheme == 4 || scheme == 5) ? LAXFFLUX : HLLFLUX;
   FLUXB = FLUXCTTOTH;
   TIMEORDER = (scheme == 2 || scheme == 3) ? 4 : (scheme == 4) ? 3 : (scheme == 5) ? 1 : 2;
-  DOENOFLUX = (scheme == 3) ? ENOFINITEVOLUME : NOENOFLUX;
+  DOENOFLUX = (scheme == 3 || scheme == 9) ? ENOFINITEVOLUME : NOENOFLUX;
   cour = 0.8;
   rescaletype = 0; /* floors rho>=RHOMIN r^-1.5, u>=UUMIN r^-2.5 (fixup.c:1562) */
   DODIAGS = 0;

@@ -13,7 +13,7 @@ from golden_util import load
 pytestmark = pytest.mark.gpu
 
 TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para", "t2d_fv", "t3d_fv", "t2d_rk3", "t2d_euler",
-        "t2d_force", "t2d_musta", "t2d_hlllaxf1"]
+        "t2d_force", "t2d_musta", "t2d_hlllaxf1", "t2d_fvweno", "t3d_fvweno"]
 
 
 def scheme(d):
@@ -27,7 +27,8 @@ def scheme(d):
             5: (h.HG_LIM_MC, h.HG_FLUX_LAXF, 1, h.HG_NOENOFLUX),
             6: (h.HG_LIM_MC, h.HG_FLUX_FORCE, 2, h.HG_NOENOFLUX),
             7: (h.HG_LIM_PARA, h.HG_FLUX_MUSTA, 2, h.HG_NOENOFLUX),
-            8: (h.HG_LIM_WENO5BND, h.HG_FLUX_HLLLAXF1, 2, h.HG_NOENOFLUX)}[int(d["weno"])]
+            8: (h.HG_LIM_WENO5BND, h.HG_FLUX_HLLLAXF1, 2, h.HG_NOENOFLUX),
+            9: (h.HG_LIM_WENO5BND, h.HG_FLUX_HLL, 2, h.HG_ENOFINITEVOLUME)}[int(d["weno"])]
 
 
 def make_grid(d):
@@ -117,7 +118,7 @@ def test_finite_volume_restart_from_conserved_state():
     assert np.array_equal(h.get_prims(), d["p_final"])
 
 
-@pytest.mark.parametrize("name", ["t2d_weno", "t3d_weno"])
+@pytest.mark.parametrize("name", ["t2d_weno", "t3d_weno", "t3d_fvweno"])
 @pytest.mark.parametrize("shape", [dict(concurrent=0), dict(weno_xmarch=1), dict(weno_xmarch=3, weno_nseg=3, weno_nseg3=2),
                                    dict(weno_xwaves=1, weno_nseg=6, weno_nseg3=4, bs_flux=64, bs_update=128)])
 def test_launch_shapes_keep_every_bit(name, shape):
