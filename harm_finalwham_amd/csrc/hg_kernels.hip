@@ -492,28 +492,42 @@ __device__ __forceinline__ void flux_point_body(const double *__restrict__ p, co
 
 template <int DIR>
 __global__ void __launch_bounds__(256) k_flux_mc(const double *__restrict__ p, const double *__restrict__ geo,
-                                                 double *__restrict__ F, Dims d, FaceRange fr, double gam,
-                                                 double cour, double dxdir, int fluxmethod,
+                                                 double *__restrict__ F, Dims d, double gam, double cour,
+                                                 double dxdir, int fluxmethod,
                                                  unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
+  FaceRange fr = face_range(d, DIR);
   flux_point_body<DIR, false>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
 template <int DIR>
 __global__ void __launch_bounds__(256) k_flux_para(const double *__restrict__ p, const double *__restrict__ geo,
-                                                   double *__restrict__ F, Dims d, FaceRange fr, double gam,
-                                                   double cour, double dxdir, int fluxmethod,
+                                                   double *__restrict__ F, Dims d, double gam, double cour,
+                                                   double dxdir, int fluxmethod,
                                                    unsigned long long *__restrict__ ndt_bits,
                                                    int *__restrict__ err) {
+  FaceRange fr = face_range(d, DIR);
   flux_point_body<DIR, true>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+}
+
+// the finite-volume flux loops (every transverse ghost row)
+template <int DIR, bool PARA>
+__global__ void __launch_bounds__(256) k_flux_pt_fv(const double *__restrict__ p, const double *__restrict__ geo,
+                                                    double *__restrict__ F, Dims d, double gam, double cour,
+                                                    double dxdir, int fluxmethod,
+                                                    unsigned long long *__restrict__ ndt_bits,
+                                                    int *__restrict__ err) {
+  FaceRange fr = face_range_fv(d, DIR);
+  flux_point_body<DIR, PARA>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
 // MC or PARA with the FORCE / MUSTA / HLLLAXF1 fluxes
 template <int DIR, bool PARA>
 __global__ void __launch_bounds__(256) k_flux_pt_ext(const double *__restrict__ p, const double *__restrict__ geo,
-                                                     double *__restrict__ F, Dims d, FaceRange fr, double gam,
+                                                     double *__restrict__ F, Dims d, int fv, double gam,
                                                      double cour, double dxdir, int fluxmethod,
                                                      unsigned long long *__restrict__ ndt_bits,
                                                      int *__restrict__ err) {
+  FaceRange fr = fv ? face_range_fv(d, DIR) : face_range(d, DIR);
   flux_point_body<DIR, PARA, true>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
@@ -561,9 +575,21 @@ template <int DIR>
 __global__ void __launch_bounds__(256) k_flux_lr(const double *__restrict__ pleft,
                                                  const double *__restrict__ pright,
                                                  const double *__restrict__ geo, double *__restrict__ F, Dims d,
-                                                 FaceRange fr, double gam, double cour, double dxdir,
-                                                 int fluxmethod, unsigned long long *__restrict__ ndt_bits,
-                                                 int *__restrict__ err) {
+                                                 double gam, double cour, double dxdir, int fluxmethod,
+                                                 unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
+  FaceRange fr = face_range(d, DIR);
+  flux_lr_body<DIR, false>(pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+}
+
+// the finite-volume flux loops (every transverse ghost row)
+template <int DIR>
+__global__ void __launch_bounds__(256) k_flux_lr_fv(const double *__restrict__ pleft,
+                                                    const double *__restrict__ pright,
+                                                    const double *__restrict__ geo, double *__restrict__ F, Dims d,
+                                                    double gam, double cour, double dxdir, int fluxmethod,
+                                                    unsigned long long *__restrict__ ndt_bits,
+                                                    int *__restrict__ err) {
+  FaceRange fr = face_range_fv(d, DIR);
   flux_lr_body<DIR, false>(pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
@@ -572,9 +598,10 @@ template <int DIR>
 __global__ void __launch_bounds__(256) k_flux_lr_ext(const double *__restrict__ pleft,
                                                      const double *__restrict__ pright,
                                                      const double *__restrict__ geo, double *__restrict__ F, Dims d,
-                                                     FaceRange fr, double gam, double cour, double dxdir,
+                                                     int fv, double gam, double cour, double dxdir,
                                                      int fluxmethod, unsigned long long *__restrict__ ndt_bits,
                                                      int *__restrict__ err) {
+  FaceRange fr = fv ? face_range_fv(d, DIR) : face_range(d, DIR);
   flux_lr_body<DIR, true>(pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
@@ -931,25 +958,37 @@ void launch_flux_point(int dir, bool para, bool fv, const double *p, const doubl
   dim3 g(nblocks(n, tu.bs_flux)), b(tu.bs_flux);
   if (fluxmethod >= 2) {
     if (para) {
-      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_ext<1, true>), g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_ext<2, true>), g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_ext<3, true>), g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_ext<1, true>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_ext<2, true>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_ext<3, true>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
     } else {
-      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_ext<1, false>), g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_ext<2, false>), g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_ext<3, false>), g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_ext<1, false>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_ext<2, false>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_ext<3, false>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    }
+    return;
+  }
+  if (fv) {
+    if (para) {
+      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_fv<1, true>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_fv<2, true>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_fv<3, true>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    } else {
+      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_fv<1, false>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_fv<2, false>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_fv<3, false>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
     }
     return;
   }
   if (para) {
-    if (dir == 1) hipLaunchKernelGGL(k_flux_para<1>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-    if (dir == 2) hipLaunchKernelGGL(k_flux_para<2>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-    if (dir == 3) hipLaunchKernelGGL(k_flux_para<3>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 1) hipLaunchKernelGGL(k_flux_para<1>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 2) hipLaunchKernelGGL(k_flux_para<2>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 3) hipLaunchKernelGGL(k_flux_para<3>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
     return;
   }
-  if (dir == 1) hipLaunchKernelGGL(k_flux_mc<1>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-  if (dir == 2) hipLaunchKernelGGL(k_flux_mc<2>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-  if (dir == 3) hipLaunchKernelGGL(k_flux_mc<3>, g, b, 0, s, p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 1) hipLaunchKernelGGL(k_flux_mc<1>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 2) hipLaunchKernelGGL(k_flux_mc<2>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 3) hipLaunchKernelGGL(k_flux_mc<3>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
 // initbase.c:18 init_conservatives (primtoU of every active zone) into the
@@ -988,14 +1027,20 @@ void launch_flux_lr(int dir, const double *pleft, const double *pright, const do
   int64_t n = (int64_t)fr.ni * fr.nj * fr.nk;
   dim3 g(nblocks(n, tu.bs_flux)), b(tu.bs_flux);
   if (fluxmethod >= 2) {
-    if (dir == 1) hipLaunchKernelGGL(k_flux_lr_ext<1>, g, b, 0, s, pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-    if (dir == 2) hipLaunchKernelGGL(k_flux_lr_ext<2>, g, b, 0, s, pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-    if (dir == 3) hipLaunchKernelGGL(k_flux_lr_ext<3>, g, b, 0, s, pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 1) hipLaunchKernelGGL(k_flux_lr_ext<1>, g, b, 0, s, pleft, pright, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 2) hipLaunchKernelGGL(k_flux_lr_ext<2>, g, b, 0, s, pleft, pright, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 3) hipLaunchKernelGGL(k_flux_lr_ext<3>, g, b, 0, s, pleft, pright, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
     return;
   }
-  if (dir == 1) hipLaunchKernelGGL(k_flux_lr<1>, g, b, 0, s, pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-  if (dir == 2) hipLaunchKernelGGL(k_flux_lr<2>, g, b, 0, s, pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-  if (dir == 3) hipLaunchKernelGGL(k_flux_lr<3>, g, b, 0, s, pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (fv) {
+    if (dir == 1) hipLaunchKernelGGL(k_flux_lr_fv<1>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 2) hipLaunchKernelGGL(k_flux_lr_fv<2>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 3) hipLaunchKernelGGL(k_flux_lr_fv<3>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    return;
+  }
+  if (dir == 1) hipLaunchKernelGGL(k_flux_lr<1>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 2) hipLaunchKernelGGL(k_flux_lr<2>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 3) hipLaunchKernelGGL(k_flux_lr<3>, g, b, 0, s, pleft, pright, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
 }
 
 void launch_emf(const double *F1, const double *F2, const double *F3, const double *geo, const double *corn,
