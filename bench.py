@@ -474,7 +474,7 @@ def main():
     print(f"# set-up {t_setup:.1f} s, dt0 = {dt0:.6g}", file=sys.stderr, flush=True)
     # launch shapes for this grid (hg_autotune times a few steps from the
     # current state and restores it; shapes never change a result bit)
-    t = g.tune() if args.no_autotune else g.autotune(2)
+    t = g.tune() if args.no_autotune else g.autotune(4)
     if args.serial:
         # the three directions one after another, as the HIP-event profile
         # pass below runs them (a kernel trace then sees each kernel alone)

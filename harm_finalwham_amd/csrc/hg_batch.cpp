@@ -267,7 +267,9 @@ extern "C" int hg_autotune(hg_ctx *c, int nsteps, hg_tune *best, void *stream) {
         c->tune = trial;
         rc = time_steps(c, p0, dt0, nsteps, s, e0, e1, ms);
         if (rc) break;
-        if (ms < best_ms) {
+        // a shape must win by 1% to replace the current one: timing noise
+        // of a few steps is about that size
+        if (ms < 0.99f * best_ms) {
           best_ms = ms;
           cur = trial;
         }
