@@ -26,7 +26,8 @@ Fixtures
   t2d_hlllaxf1.npz 2-D torus 32x32, WENO5BND + HLLLAXF1, 5 RK2 steps
   t2d_fvweno.npz 2-D torus 32x32, WENO5BND + HLL + RK2, finite volume, 3 steps
   t3d_fvweno.npz 3-D torus 16x16x8, the same, 1 step
-        (these eleven written alone by --para)
+  t2d_fv_30.npz  2-D torus 32x32, the torus file's own scheme, 30 steps
+        (these twelve written alone by --para)
 Each torus fixture carries the reference geometry tables, the initial
 primitives, the reconstruction states (or MC slopes) per direction, the
 fluxes after one fluxcalc()+flux_ct() and the primitives after steps.
@@ -397,6 +398,7 @@ def main_para():
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), scheme, 5))
     np.savez_compressed(os.path.join(HERE, "t2d_fvweno.npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), 9, 3))
     np.savez_compressed(os.path.join(HERE, "t3d_fvweno.npz"), **run_torus(os.path.join(REFDIR, "harness_t3d"), 9, 1))
+    np.savez_compressed(os.path.join(HERE, "t2d_fv_30.npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), 3, 30))
 
 
 def main():
