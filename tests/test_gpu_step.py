@@ -13,7 +13,7 @@ from golden_util import load
 pytestmark = pytest.mark.gpu
 
 TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para", "t2d_fv", "t3d_fv", "t2d_rk3", "t2d_euler",
-        "t2d_force", "t2d_musta", "t2d_hlllaxf1", "t2d_fvweno", "t3d_fvweno", "t2d_fv_30"]
+        "t2d_force", "t2d_musta", "t2d_hlllaxf1", "t2d_fvweno", "t3d_fvweno", "t2d_fv_30", "t3d_weno_10", "t3d_fv_10"]
 
 
 def scheme(d):
