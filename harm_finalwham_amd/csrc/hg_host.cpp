@@ -47,7 +47,8 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
   if (!cfg || !out) { set_error("hg_ctx_create: null argument"); return 1; }
   *out = nullptr;
   if (cfg->n1 < 1 || cfg->n2 < 1 || cfg->n3 < 1) { set_error("hg_ctx_create: grid sizes must be >= 1"); return 1; }
-  if (cfg->lim != HG_LIM_MC && cfg->lim != HG_LIM_PARA && cfg->lim != HG_LIM_WENO5BND) {
+  if (cfg->lim != HG_LIM_DONOR && cfg->lim != HG_LIM_VANL && cfg->lim != HG_LIM_MINM && cfg->lim != HG_LIM_MC &&
+      cfg->lim != HG_LIM_PARA && cfg->lim != HG_LIM_WENO5BND) {
     set_error("hg_ctx_create: unsupported lim %d", cfg->lim);
     return 1;
   }
@@ -441,7 +442,7 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
     hipStream_t sd = ds[dir];
     if (f.lim != HG_LIM_WENO5BND) {
       prof_mark(c, HG_PROF_FLUX1 + dir - 1, sd);
-      launch_flux_point(dir, f.lim == HG_LIM_PARA, f.doenoflux == HG_ENOFINITEVOLUME, pb, c->geo, c->F[dir - 1], d,
+      launch_flux_point(dir, f.lim, f.doenoflux == HG_ENOFINITEVOLUME, pb, c->geo, c->F[dir - 1], d,
                         f.gam, f.cour, f.dx[dir], f.fluxmethod, c->ndt_bits + dir - 1, c->err, c->tune, sd);
       prof_mark(c, -1, sd);
     } else {

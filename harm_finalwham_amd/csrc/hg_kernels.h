@@ -19,10 +19,10 @@ struct Tune {
 };
 
 void launch_bound(double *p, const double *geo, const Dims &d, double gammamax, int *err, hipStream_t s);
-// MC (para = false) or PARA (para = true) reconstruction fused with the
-// flux, over the faces of the standard (fv = false) or the finite-volume
-// scheme's loops
-void launch_flux_point(int dir, bool para, bool fv, const double *p, const double *geo, double *F, const Dims &d,
+// point-type reconstruction (lim: HG_LIM_PARA, HG_LIM_MC or another
+// three-point limiter) fused with the flux, over the faces of the standard
+// (fv = false) or the finite-volume scheme's loops
+void launch_flux_point(int dir, int lim, bool fv, const double *p, const double *geo, double *F, const Dims &d,
                        double gam, double cour, double dxdir, int fluxmethod, unsigned long long *ndt_bits,
                        int *err, const Tune &tu, hipStream_t s);
 // primtoU of every active zone of p into U (same layout)

@@ -11,6 +11,7 @@
 //   k_set_dt           initial time step of the set-up (advance.c:957)
 #include <hip/hip_runtime.h>
 
+#include "harmgpu.h"
 #include "hg_c2p.h"
 #include "hg_grid.h"
 #include "hg_kernels.h"
@@ -386,6 +387,25 @@ __host__ __device__ inline FaceRange face_range_fv(const Dims &d, int dir) {
 __device__ __forceinline__ double minmod(double a, double b) {
   return ((a) * (b) <= 0) ? 0.0 : ((fabs(a) < fabs(b)) ? (a) : (b));
 }
+// interppoint.c:387 slope_lim_3points for the other point limiters: DONOR
+// (0), VANL (1), MINM (2, theta 1); MC (3) is mc_slope below
+__device__ __forceinline__ double slope3_other(int lim, double yl, double yc, double yr) {
+  if (lim == 1) {
+    double Dqm = (yc - yl);
+    double Dqp = (yr - yc);
+    double s = Dqm * Dqp;
+    return (s <= 0.) ? 0. : (2.0 * s / (Dqm + Dqp));
+  }
+  if (lim == 2) {
+    double Dqm = 1.0 * (yc - yl);
+    double Dqp = 1.0 * (yr - yc);
+    double Dqc = 0.5 * (yr - yl);
+    double inner = ((Dqm) * (Dqc) <= 0) ? 0.0 : ((fabs(Dqm) < fabs(Dqc)) ? (Dqm) : (Dqc));
+    return ((inner) * (Dqp) <= 0) ? 0.0 : ((fabs(inner) < fabs(Dqp)) ? (inner) : (Dqp));
+  }
+  return 0.0;
+}
+
 // interppoint.c:387 slope_lim_3points, MC
 __device__ __forceinline__ double mc_slope(double yl, double yc, double yr) {
   double Dqm = 2.0 * (yc - yl);
@@ -442,7 +462,7 @@ __device__ __forceinline__ void para4(const double y[5], double &lout, double &r
 template <int DIR, bool PARA, bool EXT = false>
 __device__ __forceinline__ void flux_point_body(const double *__restrict__ p, const double *__restrict__ geo,
                                                 double *__restrict__ F, const Dims &d, const FaceRange &fr,
-                                                double gam, double cour, double dxdir, int fluxmethod,
+                                                double gam, double cour, double dxdir, int fluxmethod, int slim,
                                                 unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)fr.ni * fr.nj * fr.nk;
@@ -471,8 +491,13 @@ __device__ __forceinline__ void flux_point_body(const double *__restrict__ p, co
         p_r[pl] = l0;
       } else {
         double ym2 = c[-2 * s], ym1 = c[-s], y0 = c[0], yp1 = c[s];
-        p_l[pl] = ym1 + 0.5 * mc_slope(ym2, ym1, y0);
-        p_r[pl] = y0 - 0.5 * mc_slope(ym1, y0, yp1);
+        if (slim == HG_LIM_MC) {
+          p_l[pl] = ym1 + 0.5 * mc_slope(ym2, ym1, y0);
+          p_r[pl] = y0 - 0.5 * mc_slope(ym1, y0, yp1);
+        } else {
+          p_l[pl] = ym1 + 0.5 * slope3_other(slim, ym2, ym1, y0);
+          p_r[pl] = y0 - 0.5 * slope3_other(slim, ym1, y0, yp1);
+        }
       }
     }
     Geom G;
@@ -493,42 +518,42 @@ __device__ __forceinline__ void flux_point_body(const double *__restrict__ p, co
 template <int DIR>
 __global__ void __launch_bounds__(256) k_flux_mc(const double *__restrict__ p, const double *__restrict__ geo,
                                                  double *__restrict__ F, Dims d, double gam, double cour,
-                                                 double dxdir, int fluxmethod,
+                                                 double dxdir, int fluxmethod, int slim,
                                                  unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
   FaceRange fr = face_range(d, DIR);
-  flux_point_body<DIR, false>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  flux_point_body<DIR, false>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, slim, ndt_bits, err);
 }
 
 template <int DIR>
 __global__ void __launch_bounds__(256) k_flux_para(const double *__restrict__ p, const double *__restrict__ geo,
                                                    double *__restrict__ F, Dims d, double gam, double cour,
-                                                   double dxdir, int fluxmethod,
+                                                   double dxdir, int fluxmethod, int slim,
                                                    unsigned long long *__restrict__ ndt_bits,
                                                    int *__restrict__ err) {
   FaceRange fr = face_range(d, DIR);
-  flux_point_body<DIR, true>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  flux_point_body<DIR, true>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, slim, ndt_bits, err);
 }
 
 // the finite-volume flux loops (every transverse ghost row)
 template <int DIR, bool PARA>
 __global__ void __launch_bounds__(256) k_flux_pt_fv(const double *__restrict__ p, const double *__restrict__ geo,
                                                     double *__restrict__ F, Dims d, double gam, double cour,
-                                                    double dxdir, int fluxmethod,
+                                                    double dxdir, int fluxmethod, int slim,
                                                     unsigned long long *__restrict__ ndt_bits,
                                                     int *__restrict__ err) {
   FaceRange fr = face_range_fv(d, DIR);
-  flux_point_body<DIR, PARA>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  flux_point_body<DIR, PARA>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, slim, ndt_bits, err);
 }
 
 // MC or PARA with the FORCE / MUSTA / HLLLAXF1 fluxes
 template <int DIR, bool PARA>
 __global__ void __launch_bounds__(256) k_flux_pt_ext(const double *__restrict__ p, const double *__restrict__ geo,
                                                      double *__restrict__ F, Dims d, int fv, double gam,
-                                                     double cour, double dxdir, int fluxmethod,
+                                                     double cour, double dxdir, int fluxmethod, int slim,
                                                      unsigned long long *__restrict__ ndt_bits,
                                                      int *__restrict__ err) {
   FaceRange fr = fv ? face_range_fv(d, DIR) : face_range(d, DIR);
-  flux_point_body<DIR, PARA, true>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  flux_point_body<DIR, PARA, true>(p, geo, F, d, fr, gam, cour, dxdir, fluxmethod, slim, ndt_bits, err);
 }
 
 template <int DIR, bool EXT>
@@ -950,45 +975,46 @@ void launch_bound(double *p, const double *geo, const Dims &d, double gammamax, 
   }
 }
 
-void launch_flux_point(int dir, bool para, bool fv, const double *p, const double *geo, double *F, const Dims &d,
+void launch_flux_point(int dir, int lim, bool fv, const double *p, const double *geo, double *F, const Dims &d,
                        double gam, double cour, double dxdir, int fluxmethod, unsigned long long *ndt_bits,
                        int *err, const Tune &tu, hipStream_t s) {
   FaceRange fr = fv ? face_range_fv(d, dir) : face_range(d, dir);
   int64_t n = (int64_t)fr.ni * fr.nj * fr.nk;
   dim3 g(nblocks(n, tu.bs_flux)), b(tu.bs_flux);
+  const bool para = lim == HG_LIM_PARA;
   if (fluxmethod >= 2) {
     if (para) {
-      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_ext<1, true>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_ext<2, true>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_ext<3, true>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_ext<1, true>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
+      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_ext<2, true>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
+      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_ext<3, true>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
     } else {
-      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_ext<1, false>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_ext<2, false>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_ext<3, false>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_ext<1, false>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
+      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_ext<2, false>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
+      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_ext<3, false>), g, b, 0, s, p, geo, F, d, (int)fv, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
     }
     return;
   }
   if (fv) {
     if (para) {
-      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_fv<1, true>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_fv<2, true>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_fv<3, true>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_fv<1, true>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
+      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_fv<2, true>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
+      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_fv<3, true>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
     } else {
-      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_fv<1, false>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_fv<2, false>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_fv<3, false>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+      if (dir == 1) hipLaunchKernelGGL((k_flux_pt_fv<1, false>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
+      if (dir == 2) hipLaunchKernelGGL((k_flux_pt_fv<2, false>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
+      if (dir == 3) hipLaunchKernelGGL((k_flux_pt_fv<3, false>), g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
     }
     return;
   }
   if (para) {
-    if (dir == 1) hipLaunchKernelGGL(k_flux_para<1>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-    if (dir == 2) hipLaunchKernelGGL(k_flux_para<2>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-    if (dir == 3) hipLaunchKernelGGL(k_flux_para<3>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+    if (dir == 1) hipLaunchKernelGGL(k_flux_para<1>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
+    if (dir == 2) hipLaunchKernelGGL(k_flux_para<2>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
+    if (dir == 3) hipLaunchKernelGGL(k_flux_para<3>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
     return;
   }
-  if (dir == 1) hipLaunchKernelGGL(k_flux_mc<1>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-  if (dir == 2) hipLaunchKernelGGL(k_flux_mc<2>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
-  if (dir == 3) hipLaunchKernelGGL(k_flux_mc<3>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  if (dir == 1) hipLaunchKernelGGL(k_flux_mc<1>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
+  if (dir == 2) hipLaunchKernelGGL(k_flux_mc<2>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
+  if (dir == 3) hipLaunchKernelGGL(k_flux_mc<3>, g, b, 0, s, p, geo, F, d, gam, cour, dxdir, fluxmethod, lim, ndt_bits, err);
 }
 
 // initbase.c:18 init_conservatives (primtoU of every active zone) into the

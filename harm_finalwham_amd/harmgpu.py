@@ -11,7 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libharmgpu.so")
 
-HG_LIM_MC, HG_LIM_PARA, HG_LIM_WENO5BND = 3, 4, 17
+HG_LIM_DONOR, HG_LIM_VANL, HG_LIM_MINM, HG_LIM_MC, HG_LIM_PARA, HG_LIM_WENO5BND = 0, 1, 2, 3, 4, 17
 HG_NOENOFLUX, HG_ENOFINITEVOLUME = 0, 3
 HG_FLUX_FORCE, HG_FLUX_MUSTA, HG_FLUX_HLLLAXF1 = 2, 3, 4
 HG_FLUX_LAXF, HG_FLUX_HLL = 0, 1

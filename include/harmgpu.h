@@ -48,6 +48,9 @@ extern "C" {
 #define HG_NPR 8
 
 /* reconstruction and flux choices (reference values of lim/fluxmethod, global.h:247-314) */
+#define HG_LIM_DONOR 0 /* no slope (interppoint.c:433) */
+#define HG_LIM_VANL 1  /* van Leer (interppoint.c:399) */
+#define HG_LIM_MINM 2  /* minmod of the one-sided and central slopes (interppoint.c:416) */
 #define HG_LIM_MC 3
 #define HG_LIM_PARA 4 /* para4 (interppoint.c:738) with PARA2LIM MC, WHICHPARA PARA4 */
 #define HG_LIM_WENO5BND 17
@@ -65,7 +68,7 @@ typedef struct hg_ctx hg_ctx;
 typedef struct {
   int n1, n2, n3;      /* active zones */
   int nb;              /* ghost layers per side (>= 6 for WENO5BND, initbase.c:792) */
-  int lim;             /* HG_LIM_MC, HG_LIM_PARA or HG_LIM_WENO5BND */
+  int lim;             /* HG_LIM_DONOR, _VANL, _MINM, _MC, _PARA or _WENO5BND */
   int fluxmethod;      /* HG_FLUX_LAXF, HG_FLUX_HLL, HG_FLUX_FORCE, HG_FLUX_MUSTA or HG_FLUX_HLLLAXF1 */
   double gam;          /* adiabatic index */
   double cour;         /* Courant factor */

@@ -100,7 +100,7 @@ void or_floors(int64_t n, const double *r, double rhomin, double uumin, double *
 /* ---- grid-level step (advance.c, flux.c, fluxct.c, step_ch.c, bounds.fishmon.c) ---- */
 typedef struct {
   int n1, n2, n3, nb;       /* active zones per direction and ghost width */
-  int lim;                  /* 0 = MC, 1 = WENO5BND, 2 = PARA (para4) */
+  int lim;                  /* 0 = three-point limiter (slim), 1 = WENO5BND, 2 = PARA (para4) */
   int flux;                 /* 0 = LAXF, 1 = HLL, 2 = FORCE, 3 = MUSTA, 4 = HLLLAXF1 */
   double gam, cour, dx[4];
   double rhomin, uumin, gammamax;
@@ -111,6 +111,7 @@ typedef struct {
   const double *gcov, *gcon, *gcovpert, *gdet, *conn, *floors;
   int timeorder;            /* 1 Euler, 2 (or 0) midpoint RK2, 3 TVD RK3, 4 RK4 (step_ch.c:442) */
   int fv;                   /* 1 = ENOFINITEVOLUME with avgscheme DONOR (advance.c:245) */
+  int slim;                 /* lim 0: the three-point limiter, DONOR 0, VANL 1, MINM 2 or MC (else) */
 } or_grid;
 
 /* one step on p (full array with ghosts, layout [i][j][k][pl] with

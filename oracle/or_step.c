@@ -103,9 +103,35 @@ int or_bound_prim(const or_grid *G, double *p)
 /* reconstruction                                                      */
 /* ------------------------------------------------------------------ */
 
-/* interppoint.c:24 slope_lim_pointtype + interppoint.c:387 slope_lim_3points (MC);
+/* interppoint.c:387 slope_lim_3points: MC (3), VANL (1), MINM (2), DONOR (0) */
+static double slope3(int lim, double yl, double yc, double yr)
+{
+  double Dqm, Dqp, Dqc, s;
+  if (lim == 3) {
+    Dqm = 2.0 * (yc - yl);
+    Dqp = 2.0 * (yr - yc);
+    Dqc = 0.5 * (yr - yl);
+    return MINMOD(Dqc, MINMOD(Dqm, Dqp));
+  }
+  if (lim == 1) {
+    Dqm = (yc - yl);
+    Dqp = (yr - yc);
+    s = Dqm * Dqp;
+    if (s <= 0.) return 0.;
+    return (2.0 * s / (Dqm + Dqp));
+  }
+  if (lim == 2) {
+    Dqm = 1.0 * (yc - yl);
+    Dqp = 1.0 * (yr - yc);
+    Dqc = 0.5 * (yr - yl);
+    return MINMOD(MINMOD(Dqm, Dqc), Dqp);
+  }
+  return 0.0;
+}
+
+/* interppoint.c:24 slope_lim_pointtype + interppoint.c:387 slope_lim_3points;
    dq over the range of set_interppoint_loop (interppoint.c:159) */
-static void mc_slopes(const or_dims *d, const double *p, int dir, double *dq)
+static void mc_slopes(const or_dims *d, const double *p, int dir, int slim, double *dq)
 {
   int i, j, k, pl;
   int is = -d->nb1, ie = d->n1 + d->nb1 - 1, js = -d->nb2, je = d->n2 + d->nb2 - 1;
@@ -121,11 +147,7 @@ static void mc_slopes(const or_dims *d, const double *p, int dir, double *dq)
           double yl = p[or_zidx(d, i - di, j - dj, k - dk) * 8 + pl];
           double yc = p[or_zidx(d, i, j, k) * 8 + pl];
           double yr = p[or_zidx(d, i + di, j + dj, k + dk) * 8 + pl];
-          double Dqm = 2.0 * (yc - yl);
-          double Dqp = 2.0 * (yr - yc);
-          double Dqc = 0.5 * (yr - yl);
-          double inner = MINMOD(Dqm, Dqp);
-          dq[or_zidx(d, i, j, k) * 8 + pl] = MINMOD(Dqc, inner);
+          dq[or_zidx(d, i, j, k) * 8 + pl] = slope3(slim, yl, yc, yr);
         }
 }
 
@@ -190,7 +212,7 @@ int or_reconstruct(const or_grid *G, const double *p, int dir, double *pleft, do
 {
   or_dims d = or_get_dims(G);
   if (G->lim == 0) {
-    mc_slopes(&d, p, dir, pleft);
+    mc_slopes(&d, p, dir, (G->slim >= 0 && G->slim <= 2) ? G->slim : 3, pleft);
     return 0;
   }
   if (G->lim == 2) {

@@ -53,9 +53,12 @@ def _p(a):
 # argument) -> (oracle lim, flux, timeorder, finite volume): MC + LaxF + RK2,
 # WENO5BND + HLL + RK2, PARA + HLL + RK4, the last with the finite-volume
 # step (the torus file's own choice), MC + LaxF + TVD RK3, MC + LaxF + Euler,
-# MC + FORCE, PARA + MUSTA, WENO5BND + HLLLAXF1 (midpoint RK2)
+# MC + FORCE, PARA + MUSTA, WENO5BND + HLLLAXF1 (midpoint RK2), WENO5BND
+# finite volume, VANL + LaxF, MINM + HLL, DONOR + LaxF (5th entry: the
+# three-point limiter, MC when absent)
 SCHEMES = {0: (0, 0, 2, 0), 1: (1, 1, 2, 0), 2: (2, 1, 4, 0), 3: (2, 1, 4, 1), 4: (0, 0, 3, 0), 5: (0, 0, 1, 0),
-           6: (0, 2, 2, 0), 7: (2, 3, 2, 0), 8: (1, 4, 2, 0), 9: (1, 1, 2, 1)}
+           6: (0, 2, 2, 0), 7: (2, 3, 2, 0), 8: (1, 4, 2, 0), 9: (1, 1, 2, 1),
+           10: (0, 0, 2, 0, 1), 11: (0, 1, 2, 0, 2), 12: (0, 0, 2, 0, 0)}
 
 
 class OrGrid(C.Structure):
@@ -67,7 +70,7 @@ class OrGrid(C.Structure):
                 ("gcov", C.POINTER(C.c_double)), ("gcon", C.POINTER(C.c_double)),
                 ("gcovpert", C.POINTER(C.c_double)), ("gdet", C.POINTER(C.c_double)),
                 ("conn", C.POINTER(C.c_double)), ("floors", C.POINTER(C.c_double)),
-                ("timeorder", C.c_int), ("fv", C.c_int)]
+                ("timeorder", C.c_int), ("fv", C.c_int), ("slim", C.c_int)]
 
 
 def c2p_batch(gcov, gcon, gdet, gcovpert, U, gam, guess):
@@ -104,7 +107,7 @@ class Grid:
     (FACE1, FACE2, FACE3, CORN1, CORN2, CORN3, CENT)."""
 
     def __init__(self, n1, n2, n3, nb, lim, flux, gam, cour, dx, gcov, gcon, gcovpert, gdet, conn,
-                 r_cent, defcon=1.0, safe=1.3, timeorder=2, fv=0):
+                 r_cent, defcon=1.0, safe=1.3, timeorder=2, fv=0, slim=3):
         self.keep = [np.ascontiguousarray(x, dtype=np.float64)
                      for x in (gcov, gcon, gcovpert, gdet, conn, floors_from_r(r_cent))]
         g = OrGrid()
@@ -114,7 +117,7 @@ class Grid:
             g.dx[i] = dx[i]
         g.rhomin, g.uumin, g.gammamax, g.defcon, g.safe = RHOMIN, UUMIN, GAMMAMAX, defcon, safe
         g.gcov, g.gcon, g.gcovpert, g.gdet, g.conn, g.floors = [_p(a) for a in self.keep]
-        g.timeorder, g.fv = timeorder, fv
+        g.timeorder, g.fv, g.slim = timeorder, fv, slim
         self.g = g
         self.shape = (n1 + 2 * nb * (n1 > 1), n2 + 2 * nb * (n2 > 1), n3 + 2 * nb * (n3 > 1), 8)
 
@@ -122,11 +125,11 @@ class Grid:
     def from_fixture(cls, d):
         par = d["params"]
         n1, n2, n3, nb = (int(x) for x in par[:4])
-        lim, flux, timeorder, fv = SCHEMES[int(d["weno"])]
+        lim, flux, timeorder, fv, slim = (SCHEMES[int(d["weno"])] + (3,))[:5]
         dx = [0.0, par[12], par[13], par[14]]
         r = d["xv_cent"][:, :, 1, 1]
         return cls(n1, n2, n3, nb, lim, flux, par[5], par[10], dx, d["gcov"], d["gcon"],
-                   d["gcovpert"], d["gdet"], d["conn"], r, timeorder=timeorder, fv=fv)
+                   d["gcovpert"], d["gdet"], d["conn"], r, timeorder=timeorder, fv=fv, slim=slim)
 
     def step(self, p, dt):
         p = np.ascontiguousarray(p, dtype=np.float64).copy()

@@ -21,7 +21,9 @@
  *       TIMEORDERs of step_ch.c:442); 6 MC + FORCE, 7 PARA + MUSTA,
  *       8 WENO5BND + HLLLAXF1, each with RK2 (the other fluxmethods of
  *       fluxcompute.c:100); 9 WENO5BND + HLL + RK2 with the finite-volume
- *       step.  Every mode's <weno> argument below is this scheme number.
+ *       step; 10 VANL + LaxF, 11 MINM + HLL, 12 DONOR + LaxF (RK2; the
+ *       other three-point limiters).  Every mode's <weno> argument below is
+ *       this scheme number.
  *   c2p <infile> <outfile> <gamma>
  *       Batched inversion: each input record holds a metric (gcov, gcon,
  *       gdet, gcovpert), the true primitives and a guess.  U is built with
@@ -133,9 +135,10 @@ static void harness_init(int scheme)
   init_defconsts();
   init_global();
   /* OVERRIDE: the hot-path configurations (init.fishmon.c:96-118 picks PARA/RK4/FV) */
-  lim = (scheme == 2 || scheme == 3 || scheme == 7) ? PARA : (use_weno || scheme == 8 || scheme == 9) ? WENO5BND : MC;
+  lim = (scheme == 2 || scheme == 3 || scheme == 7) ? PARA : (use_weno || scheme == 8 || scheme == 9) ? WENO5BND
+      : (scheme == 10) ? VANL : (scheme == 11) ? MINM : (scheme == 12) ? DONOR : MC;
   fluxmethod = (scheme == 6) ? FORCEFLUX : (scheme == 7) ? MUSTAFLUX : (scheme == 8) ? HLLLAXF1FLUX
-             : (scheme == 0 || scheme == 4 || scheme == 5) ? LAXFFLUX : HLLFLUX;
+             : (scheme == 0 || scheme == 4 || scheme == 5 || scheme == 10 || scheme == 12) ? LAXFFLUX : HLLFLUX;
   FLUXB = FLUXCTTOTH;
   TIMEORDER = (scheme == 2 || scheme == 3) ? 4 : (scheme == 4) ? 3 : (scheme == 5) ? 1 : 2;
   DOENOFLUX = (scheme == 3 || scheme == 9) ? ENOFINITEVOLUME : NOENOFLUX;

@@ -29,7 +29,9 @@ Fixtures
   t2d_fv_30.npz  2-D torus 32x32, the torus file's own scheme, 30 steps
   t3d_weno_10.npz 3-D torus 16x16x8, WENO5BND + HLL + RK2, 10 steps
   t3d_fv_10.npz  3-D torus 16x16x8, the torus file's own scheme, 10 steps
-        (these fourteen written alone by --para)
+  t2d_vanl.npz / t2d_minm.npz / t2d_donor.npz  2-D torus 32x32, VANL + LaxF,
+        MINM + HLL, DONOR + LaxF, 5 RK2 steps
+        (these seventeen written alone by --para)
 Each torus fixture carries the reference geometry tables, the initial
 primitives, the reconstruction states (or MC slopes) per direction, the
 fluxes after one fluxcalc()+flux_ct() and the primitives after steps.
@@ -403,6 +405,8 @@ def main_para():
     np.savez_compressed(os.path.join(HERE, "t2d_fv_30.npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), 3, 30))
     np.savez_compressed(os.path.join(HERE, "t3d_weno_10.npz"), **run_torus(os.path.join(REFDIR, "harness_t3d"), 1, 10))
     np.savez_compressed(os.path.join(HERE, "t3d_fv_10.npz"), **run_torus(os.path.join(REFDIR, "harness_t3d"), 3, 10))
+    for scheme, name in ((10, "t2d_vanl"), (11, "t2d_minm"), (12, "t2d_donor")):
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **run_torus(os.path.join(REFDIR, "harness_t2d"), scheme, 5))
 
 
 def main():

@@ -13,7 +13,8 @@ from golden_util import load
 pytestmark = pytest.mark.gpu
 
 TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para", "t2d_fv", "t3d_fv", "t2d_rk3", "t2d_euler",
-        "t2d_force", "t2d_musta", "t2d_hlllaxf1", "t2d_fvweno", "t3d_fvweno", "t2d_fv_30", "t3d_weno_10", "t3d_fv_10"]
+        "t2d_force", "t2d_musta", "t2d_hlllaxf1", "t2d_fvweno", "t3d_fvweno", "t2d_fv_30", "t3d_weno_10", "t3d_fv_10",
+        "t2d_vanl", "t2d_minm", "t2d_donor"]
 
 
 def scheme(d):
@@ -28,7 +29,10 @@ def scheme(d):
             6: (h.HG_LIM_MC, h.HG_FLUX_FORCE, 2, h.HG_NOENOFLUX),
             7: (h.HG_LIM_PARA, h.HG_FLUX_MUSTA, 2, h.HG_NOENOFLUX),
             8: (h.HG_LIM_WENO5BND, h.HG_FLUX_HLLLAXF1, 2, h.HG_NOENOFLUX),
-            9: (h.HG_LIM_WENO5BND, h.HG_FLUX_HLL, 2, h.HG_ENOFINITEVOLUME)}[int(d["weno"])]
+            9: (h.HG_LIM_WENO5BND, h.HG_FLUX_HLL, 2, h.HG_ENOFINITEVOLUME),
+            10: (h.HG_LIM_VANL, h.HG_FLUX_LAXF, 2, h.HG_NOENOFLUX),
+            11: (h.HG_LIM_MINM, h.HG_FLUX_HLL, 2, h.HG_NOENOFLUX),
+            12: (h.HG_LIM_DONOR, h.HG_FLUX_LAXF, 2, h.HG_NOENOFLUX)}[int(d["weno"])]
 
 
 def make_grid(d):

@@ -28,7 +28,8 @@ def test_c2p_failure_codes_present():
 
 
 TORI = ["t2d_mc", "t2d_weno", "t3d_weno", "t2d_para", "t3d_para", "t2d_fv", "t3d_fv", "t2d_rk3", "t2d_euler",
-        "t2d_force", "t2d_musta", "t2d_hlllaxf1", "t2d_fvweno", "t3d_fvweno", "t2d_fv_30", "t3d_weno_10", "t3d_fv_10"]
+        "t2d_force", "t2d_musta", "t2d_hlllaxf1", "t2d_fvweno", "t3d_fvweno", "t2d_fv_30", "t3d_weno_10", "t3d_fv_10",
+        "t2d_vanl", "t2d_minm", "t2d_donor"]
 
 
 @pytest.mark.parametrize("name", TORI)
