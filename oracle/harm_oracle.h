@@ -120,6 +120,9 @@ typedef struct {
 int or_step(const or_grid *G, double *p, double *dt_io, int64_t *nfail, int64_t *iters);
 /* one finite-volume step: ucons (layout of p, active zones used) is the
    conserved state carried between steps, replaced by the step's ucum */
+/* or_step that also returns the step's unew (the last substep's ucum, zero
+   outside the active zones), as restart_write (restart.c:240) dumps it */
+int or_step_unew(const or_grid *G, double *p, double *unew, double *dt_io, int64_t *nfail, int64_t *iters);
 int or_step_fv(const or_grid *G, double *p, double *ucons, double *dt_io, int64_t *nfail, int64_t *iters);
 /* initbase.c:18 init_conservatives: U(p) of the active zones into ucons */
 void or_cons_from_prims(const or_grid *G, const double *p, double *ucons);

@@ -621,7 +621,8 @@ done:
   return rc;
 }
 
-static int step_impl(const or_grid *G, double *p, double *ucons, double *dt_io, int64_t *nfail, int64_t *iters)
+static int step_impl(const or_grid *G, double *p, double *ucons, double *unew, double *dt_io, int64_t *nfail,
+                     int64_t *iters)
 {
   or_dims d = or_get_dims(G);
   size_t nz = (size_t)d.m1 * d.m2 * d.m3;
@@ -686,6 +687,8 @@ static int step_impl(const or_grid *G, double *p, double *ucons, double *dt_io, 
   /* step_ch.c:243-247 */
   if (ndt > G->safe * dt) ndt = G->safe * dt;
   *dt_io = ndt;
+  /* the unew restart_write dumps (restart.c:240): ucum of the last substep */
+  if (unew) memcpy(unew, ucum, nz * 8 * sizeof(double));
 done:
   free(pk);
   free(pk2);
@@ -698,13 +701,19 @@ done:
 int or_step(const or_grid *G, double *p, double *dt_io, int64_t *nfail, int64_t *iters)
 {
   if (G->fv) return 1;
-  return step_impl(G, p, NULL, dt_io, nfail, iters);
+  return step_impl(G, p, NULL, NULL, dt_io, nfail, iters);
+}
+
+int or_step_unew(const or_grid *G, double *p, double *unew, double *dt_io, int64_t *nfail, int64_t *iters)
+{
+  if (G->fv || !unew) return 1;
+  return step_impl(G, p, NULL, unew, dt_io, nfail, iters);
 }
 
 int or_step_fv(const or_grid *G, double *p, double *ucons, double *dt_io, int64_t *nfail, int64_t *iters)
 {
   if (!G->fv || !ucons) return 1;
-  return step_impl(G, p, ucons, dt_io, nfail, iters);
+  return step_impl(G, p, ucons, NULL, dt_io, nfail, iters);
 }
 
 void or_cons_from_prims(const or_grid *G, const double *p, double *ucons)

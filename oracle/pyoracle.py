@@ -40,6 +40,7 @@ def lib():
         _lib.or_reconstruct.argtypes = [C.POINTER(OrGrid), d, C.c_int, d, d]
         _lib.or_floors.argtypes = [C.c_int64, d, C.c_double, C.c_double, d]
         _lib.or_post_fixup.argtypes = [C.POINTER(OrGrid), d, i32, C.c_int]
+        _lib.or_step_unew.argtypes = [C.POINTER(OrGrid), d, d, d, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         _lib.or_step_fv.argtypes = [C.POINTER(OrGrid), d, d, d, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         _lib.or_cons_from_prims.argtypes = [C.POINTER(OrGrid), d, d]
     return _lib
@@ -141,6 +142,20 @@ class Grid:
         if rc:
             raise RuntimeError("oracle step failed")
         return p, float(dtv[0]), int(nf[0]), int(it[0])
+
+    def step_unew(self, p, dt):
+        """or_step that also returns unew (the last substep's ucum), the
+        array restart_write dumps beside p."""
+        p = np.ascontiguousarray(p, dtype=np.float64).copy()
+        U = np.zeros(self.shape)
+        dtv = np.array([dt])
+        nf = np.zeros(1, np.int64)
+        it = np.zeros(1, np.int64)
+        rc = lib().or_step_unew(C.byref(self.g), _p(p), _p(U), _p(dtv), nf.ctypes.data_as(C.POINTER(C.c_int64)),
+                                it.ctypes.data_as(C.POINTER(C.c_int64)))
+        if rc:
+            raise RuntimeError("oracle step failed")
+        return p, U, float(dtv[0])
 
     def cons_from_prims(self, p):
         """init_conservatives: U(p) of the active zones (zero elsewhere)."""

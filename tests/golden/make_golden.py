@@ -380,11 +380,15 @@ def main_rdump():
     them: the file's bytes, the parsed p and unew of the active zones and the
     header values (t, tf, nstep, a, gam, cour, dt, lim, fluxmethod, R0, Rin,
     Rout, hslope, RHOMIN, UUMIN, GAMMAMAX).  rdump_t2d: MC + LaxF + RK2;
-    rdump_t2d_fv: the torus file's finite-volume PARA + HLL + RK4 scheme."""
+    rdump_t2d_fv: the torus file's finite-volume PARA + HLL + RK4 scheme;
+    rdump_t2d_rk4, _rk3, _euler: PARA + HLL + RK4 without the finite-volume
+    scheme, MC + LaxF + TVD RK3 and MC + LaxF + Euler."""
     subprocess.run([BUILD, "t2d", "32", "32", "1"], check=True, stdout=subprocess.DEVNULL)
     h = os.path.join(REFDIR, "harness_t2d")
     np.savez_compressed(os.path.join(HERE, "rdump_t2d.npz"), **rdump_fixture(h, 0))
     np.savez_compressed(os.path.join(HERE, "rdump_t2d_fv.npz"), **rdump_fixture(h, 3))
+    for name, scheme in (("rk4", 2), ("rk3", 4), ("euler", 5)):
+        np.savez_compressed(os.path.join(HERE, "rdump_t2d_%s.npz" % name), **rdump_fixture(h, scheme))
 
 
 def main_para():

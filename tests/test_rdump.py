@@ -1,8 +1,9 @@
 """The reference's restart dumps (restart.c:206, text output): the files
-restart_write produced after two steps of the 32x32 torus (MC + RK2, and the
-torus file's finite-volume PARA + RK4 scheme) are parsed to exactly the
-values the reference's own restart_read gets from them, and the library
-writes the same bytes after the same steps."""
+restart_write produced after two steps of the 32x32 torus (MC + RK2, the
+torus file's finite-volume PARA + RK4 scheme, PARA + RK4 without it, MC +
+TVD RK3 and MC + Euler) are parsed to exactly the values the reference's own
+restart_read gets from them, and the oracle and the library write the same
+bytes after the same steps."""
 import os
 
 import numpy as np
@@ -18,7 +19,60 @@ def _file(tmp_path, name="rdump_t2d"):
     return d, str(path)
 
 
-@pytest.mark.parametrize("name", ["rdump_t2d", "rdump_t2d_fv"])
+# the reference's dumps and the step fixtures of the same scheme
+DUMPS = {"rdump_t2d": "t2d_mc", "rdump_t2d_fv": "t2d_fv", "rdump_t2d_rk4": "t2d_para", "rdump_t2d_rk3": "t2d_rk3",
+         "rdump_t2d_euler": "t2d_euler"}
+TIMEORDER = {"rdump_t2d": 2, "rdump_t2d_fv": 4, "rdump_t2d_rk4": 4, "rdump_t2d_rk3": 3, "rdump_t2d_euler": 1}
+
+
+def _dump_text(header, p, u, nb, n=32):
+    """restart_write's text (restart.c:240 with "%21.15g " per value, one
+    line per active zone, i fastest) of p and unew of a 2D grid."""
+    lines = [header]
+    for j in range(n):
+        for i in range(n):
+            z = (nb + i, nb + j, 0)
+            lines.append(b"".join(b"%21.15g " % v for v in p[z]) + b"".join(b"%21.15g " % v for v in u[z]))
+    return b"\n".join(lines) + b"\n"
+
+
+@pytest.mark.parametrize("name", [n for n in DUMPS if n != "rdump_t2d_fv"])
+def test_oracle_unew_writes_reference_bytes(name):
+    """The oracle's p and unew (the last substep's ucum) after the two steps
+    the reference took, formatted as restart_write formats them, are the
+    reference's dump byte for byte."""
+    import pyoracle as po
+    from golden_util import load
+    d = load(name)
+    text = bytes(d["text"])
+    base = load(DUMPS[name])
+    g = po.Grid.from_fixture(base)
+    p, dt = base["p_init"], float(base["dt_seq"][0])
+    for _ in range(2):
+        p, u, dt = g.step_unew(p, dt)
+    nb = int(base["params"][3])
+    assert _dump_text(text.split(b"\n", 1)[0], p, u, nb) == text
+
+
+def test_oracle_unew_planted_mistake_fails():
+    """The same check on a unew with one entry moved by one unit in the
+    last place fails (the formatting keeps 15 digits, so the move is chosen
+    where the text changes)."""
+    import pyoracle as po
+    from golden_util import load
+    d = load("rdump_t2d_rk3")
+    text = bytes(d["text"])
+    base = load("t2d_rk3")
+    g = po.Grid.from_fixture(base)
+    p, dt = base["p_init"], float(base["dt_seq"][0])
+    for _ in range(2):
+        p, u, dt = g.step_unew(p, dt)
+    nb = int(base["params"][3])
+    u[nb + 3, nb + 5, 0, 1] *= 1.0 + 1e-13
+    assert _dump_text(text.split(b"\n", 1)[0], p, u, nb) != text
+
+
+@pytest.mark.parametrize("name", list(DUMPS))
 def test_parse_matches_reference_reader(tmp_path, name):
     from harm_finalwham_amd import harmgpu
     d, path = _file(tmp_path, name)
@@ -32,7 +86,7 @@ def test_parse_matches_reference_reader(tmp_path, name):
     assert np.array_equal(np.array(got, dtype=np.float64), h)
     assert info["n"] == (32, 32, 1)
     # the header as the torus set-up writes it: outflow in r, polar axis, periodic phi
-    assert info["timeorder"] == (4 if name.endswith("_fv") else 2) and info["gammafail"] == 2500.0
+    assert info["timeorder"] == TIMEORDER[name] and info["gammafail"] == 2500.0
 
 
 def test_parse_rejects_other_grids_and_truncation(tmp_path):
@@ -72,11 +126,12 @@ def test_read_into_context(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name,base_name", [("rdump_t2d", "t2d_mc"), ("rdump_t2d_fv", "t2d_fv")])
+@pytest.mark.parametrize("name,base_name", list(DUMPS.items()))
 def test_write_matches_reference_bytes(tmp_path, name, base_name):
     """hg_rdump_write after the same two steps the reference took before its
     restart_write: the same bytes (the header line passed through).  The
-    finite-volume dump's unew is the carried conserved state."""
+    finite-volume dump's unew is the carried conserved state; the others'
+    is the last substep's ucum."""
     from golden_util import load
     from test_gpu_step import make_grid
     print("case rdump write", name, flush=True)
