@@ -11,6 +11,13 @@
  * FP64 and written in the reference's operation order; the library is built
  * with -O2 -ffp-contract=off so that no FMA contraction changes rounding.
  *
+ * Pinning: every function here is checked bit for bit against output of the
+ * reference's own code, compiled from its sources into oracle/_ref and run
+ * on the same inputs (tests/golden/make_golden.py writes those fixtures;
+ * the CPU tests under tests/ check them, and planted mistakes in
+ * test_oracle_golden.py, test_fixup_golden.py and test_rdump.py show that
+ * the checks can fail).  Parity unpinned: none.
+ *
  * Conventions (same as the reference, global.h:916-923):
  *   primitives  p = [rho, u, U1, U2, U3, B1, B2, B3]  (U^i relative 4-velocity)
  *   conserved   U = gdet * [rho u^t, T^t_t + rho u^t, T^t_i, B^i]
