@@ -115,20 +115,63 @@ def fp64_rate(logical, mean_ms):
             "gflop_per_launch": gf, "source": os.path.relpath(VALU_PROFILE, HERE)}
 
 
-def cpu_baseline(timeout_s=240):
-    """Reference C build (oracle/_ref/harness_b3d, gcc -O2, one core) on the
-    same torus with phi reduced to 8 zones: 256x128x8 = 262144 zones, 1 step."""
+def host_cores():
+    """Host cores this process may use, capped at 16 (a one-GPU box's share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def reference_timing(scheme, label, timeout_s=300):
+    """The reference C build (oracle/_ref/harness_b3d, gcc -O2, single
+    threaded) timing one step of the torus at 256x128x8 zones: once alone
+    (per core) and then as one process per host core at the same time (the
+    aggregate of the box's cores; the reference has no threading of its own)."""
     exe = os.path.join(HERE, "oracle", "_ref", "harness_b3d")
     if not os.path.exists(exe):
         return None
+
+    def launch(tmp):
+        return subprocess.Popen([exe, "time", str(scheme), "1"], cwd=tmp, stdout=subprocess.PIPE,
+                                stderr=subprocess.DEVNULL, text=True)
+
+    def collect(procs):
+        out = []
+        for p in procs:
+            so, _ = p.communicate(timeout=timeout_s)
+            if p.returncode != 0:
+                return None
+            out.append(json.loads(so.strip().splitlines()[-1]))
+        return out
+
+    ncores = host_cores()
     with tempfile.TemporaryDirectory() as tmp:
-        r = subprocess.run([exe, "time", "1", "1"], cwd=tmp, capture_output=True, text=True, timeout=timeout_s)
-    if r.returncode != 0:
-        return None
-    res = json.loads(r.stdout.strip().splitlines()[-1])
-    return {"value": res["zone_updates_per_s"], "unit": "zone-updates/s", "cores": 1, "kind": "reference",
-            "sample": "reference harness (gcc -O2 FP64, single thread), same torus and scheme at 256x128x8 "
-                      f"zones, {res['steps']} RK2 step, {res['seconds']:.1f} s timed"}
+        dirs = [os.path.join(tmp, str(k)) for k in range(ncores)]
+        for d in dirs:
+            os.makedirs(d)
+        one = collect([launch(dirs[0])])
+        if not one:
+            return None
+        t0 = time.perf_counter()
+        many = collect([launch(d) for d in dirs])
+        wall = time.perf_counter() - t0
+    per_core = one[0]["zone_updates_per_s"]
+    res = {"value": per_core, "unit": "zone-updates/s", "cores": 1, "kind": "reference",
+           "sample": f"reference harness (gcc -O2 FP64, single thread), {label} at 256x128x8 zones, "
+                     f"{one[0]['steps']} step, {one[0]['seconds']:.1f} s timed"}
+    if many:
+        zu = sum(r["zones"] * r["steps"] for r in many)
+        res["aggregate"] = {"value": zu / max(r["seconds"] for r in many), "unit": "zone-updates/s",
+                            "cores": ncores, "processes": ncores, "wall_s": wall,
+                            "per_process_s": [round(r["seconds"], 2) for r in many]}
+    return res
+
+
+def cpu_baseline():
+    """The reference build on the benchmark's scheme (WENO5BND + HLL + RK2)."""
+    return reference_timing(1, "WENO5BND + HLL + RK2, the same torus")
 
 
 def torus_scheme_bench(n1, n2, n3, steps, warmup=2):
@@ -160,79 +203,96 @@ def torus_scheme_bench(n1, n2, n3, steps, warmup=2):
             "kernels_ms_per_step": {k: v["total_ms"] / steps for k, v in prof.items()}}
 
 
-def torus_scheme_cpu_baseline(timeout_s=240):
-    """The reference build on the same scheme, 256x128x8 zones, one step."""
-    exe = os.path.join(HERE, "oracle", "_ref", "harness_b3d")
-    if not os.path.exists(exe):
-        return None
-    with tempfile.TemporaryDirectory() as tmp:
-        r = subprocess.run([exe, "time", "3", "1"], cwd=tmp, capture_output=True, text=True, timeout=timeout_s)
-    if r.returncode != 0:
-        return None
-    res = json.loads(r.stdout.strip().splitlines()[-1])
-    return {"value": res["zone_updates_per_s"], "unit": "zone-updates/s", "cores": 1, "kind": "reference",
-            "sample": "reference harness (gcc -O2 FP64, single thread), PARA + HLL + RK4 finite volume at "
-                      f"256x128x8 zones, {res['steps']} step, {res['seconds']:.1f} s timed"}
+def torus_scheme_cpu_baseline():
+    """The reference build on the torus file's own scheme."""
+    return reference_timing(3, "PARA + HLL + RK4 finite volume")
+
+
+def _c2p_solve(b, gam, mode, reps, mlp=None):
+    """Best-of-reps kernel time (HIP events on the launch stream) of one
+    batched inversion; returns (pr, pflag, iters, ms, counts)."""
+    import torch
+    from harm_finalwham_amd import harmgpu
+    best, cnt = None, None
+    n = b["U"].shape[1]
+    pr = b["p_guess"].clone()
+    pflag = torch.zeros(n, dtype=torch.int32, device="cuda")
+    iters = torch.zeros_like(pflag)
+    for _ in range(reps):
+        pr.copy_(b["p_guess"])
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        if mlp is None:
+            harmgpu.utoprim_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], b["gcovpert"], pr, pflag, iters, gam=gam,
+                                  mode=mode)
+        else:
+            cnt = harmgpu.utoprim_batch_learned(mlp, b["U"], b["gcov"], b["gcon"], b["gdet"], b["gcovpert"], pr,
+                                                pflag, iters, gam=gam)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    return pr, pflag, iters, best, cnt
+
+
+def c2p_errors(p, pref, fref, gam):
+    """Max errors over the zones pref solved: rho relative, u relative to
+    rho + gam u, velocity relative to max(1, the largest component)."""
+    ok = fref == 0
+    d = (p - pref)[:, ok].abs()
+    ref = pref[:, ok].abs()
+    return [float((d[0] / ref[0]).max()), float((d[1] / (ref[0] + gam * ref[1])).max()),
+            float((d[2:5].amax(dim=0) / ref[2:5].amax(dim=0).clamp(min=1.0)).max())]
+
+
+# algorithmic bytes per inversion: U 8, the upper triangles of gcov and gcon
+# 10 + 10, gdet, the guess's u and u~^i 4 doubles in; the solution 8 doubles,
+# the code and the step count out
+C2P_BYTES = (8 + 10 + 10 + 1 + 4 + 8) * 8 + 4 + 4
 
 
 def c2p_bench(n, reps=3):
     """Batched inversions (the second half of the headline metric): the
     benchmark distribution for gam = 4/3 and 5/3 generated on the GPU,
-    inverted by the plain FP64 kernel and by the mixed-precision persistent
-    kernel; best-of-reps kernel time with HIP events on the launch stream."""
-    import torch
+    inverted by the plain FP64 kernel (the reference's arithmetic), by the
+    converged-root kernel (HG_C2P_REFINED: the reference's codes, root
+    converged with double-double residuals) and by the mixed-precision
+    persistent kernel.  Errors are against the converged root."""
     from harm_finalwham_amd import harmgpu
-    res = {"n": n, "gam": [], "fp64_ms": [], "mixed_ms": [], "flags_identical": True, "fallbacks": 0,
-           "max_err": [0.0, 0.0, 0.0], "p999_u_rel": 0.0, "fp64_iters_per_zone": [], "mixed_iters_per_zone": []}
+    import torch
+    res = {"n": n, "gam": [], "fp64_ms": [], "refined_ms": [], "mixed_ms": [], "flags_identical": True,
+           "fallbacks": 0, "max_err_vs_converged": {"fp64": [0.0, 0.0, 0.0], "mixed": [0.0, 0.0, 0.0]},
+           "fp64_iters_per_zone": [], "mixed_iters_per_zone": [], "mixed_wavefront_max_iters": []}
     for gam in (4.0 / 3.0, 5.0 / 3.0):
         b = harmgpu.make_c2p_batch(n, gam=gam, seed=2024)
-        out = {}
-        for mode, key in ((harmgpu.HG_C2P_FP64, "fp64"), (harmgpu.HG_C2P_MIXED, "mixed")):
-            best = None
-            for _ in range(reps):
-                pr = b["p_guess"].clone()
-                pflag = torch.zeros(n, dtype=torch.int32, device="cuda")
-                iters = torch.zeros_like(pflag)
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                torch.cuda.synchronize()
-                e0.record()
-                harmgpu.utoprim_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], b["gcovpert"], pr, pflag, iters,
-                                      gam=gam, mode=mode)
-                e1.record()
-                torch.cuda.synchronize()
-                ms = e0.elapsed_time(e1)
-                best = ms if best is None else min(best, ms)
-            out[key] = (pr, pflag, iters, best)
-            if key == "mixed":
-                res["fallbacks"] += harmgpu.c2p_fallbacks()
-        p64, f64, i64, t64 = out["fp64"]
-        pmx, fmx, imx, tmx = out["mixed"]
+        p64, f64, i64, t64, _ = _c2p_solve(b, gam, harmgpu.HG_C2P_FP64, reps)
+        pre, fre, _, tre, _ = _c2p_solve(b, gam, harmgpu.HG_C2P_REFINED, reps)
+        pmx, fmx, imx, tmx, _ = _c2p_solve(b, gam, harmgpu.HG_C2P_MIXED, reps)
+        res["fallbacks"] += harmgpu.c2p_fallbacks()
         res["gam"].append(gam)
         res["fp64_ms"].append(t64)
+        res["refined_ms"].append(tre)
         res["mixed_ms"].append(tmx)
         res["fp64_iters_per_zone"].append(float(i64.double().mean()))
         res["mixed_iters_per_zone"].append(float(imx.double().mean()))
-        res["flags_identical"] &= bool(torch.equal(f64, fmx))
-        ok = f64 == 0
-        d = (pmx - p64)[:, ok].abs()
-        ref = p64[:, ok].abs()
-        err = [float((d[0] / ref[0]).max()), float((d[1] / (ref[0] + gam * ref[1])).max()),
-               float((d[2:5].amax(dim=0) / ref[2:5].amax(dim=0)).max())]
-        res["max_err"] = [max(a, c) for a, c in zip(res["max_err"], err)]
-        u_rel = d[1] / ref[1]
-        res["p999_u_rel"] = max(res["p999_u_rel"], float(torch.quantile(u_rel[: 1 << 20], 0.999)))
-        del b, out
+        res["mixed_wavefront_max_iters"].append(float(imx.view(-1, 64).amax(1).double().mean()))
+        res["flags_identical"] &= bool(torch.equal(f64, fmx)) and bool(torch.equal(f64, fre))
+        for key, p in (("fp64", p64), ("mixed", pmx)):
+            e = c2p_errors(p, pre, fre, gam)
+            res["max_err_vs_converged"][key] = [max(a, c) for a, c in zip(res["max_err_vs_converged"][key], e)]
+        del b, p64, pre, pmx
     tot = 2 * n
     res["fp64_solves_per_s"] = tot / (sum(res["fp64_ms"]) * 1e-3)
+    res["refined_solves_per_s"] = tot / (sum(res["refined_ms"]) * 1e-3)
     res["mixed_solves_per_s"] = tot / (sum(res["mixed_ms"]) * 1e-3)
     res["speedup"] = res["mixed_solves_per_s"] / res["fp64_solves_per_s"]
-    # algorithmic bytes per inversion of this layout: U 8, gcov 16, gcon 16,
-    # gdet, gcov'_tt, guess 8 doubles in; p 8 doubles + flag + iterations out
-    bpi = (8 + 16 + 16 + 1 + 1 + 8 + 8) * 8 + 4 + 4
-    res["bytes_per_inversion"] = bpi
-    res["mixed_hbm_frac"] = res["mixed_solves_per_s"] * bpi / (HBM_PEAK_GBS * 1e9)
-    res["err_note"] = ("max errors vs FP64 over successful zones: rho relative, u relative to rho + gam u, "
-                       "velocity relative to the largest component; p999_u_rel = 99.9th percentile of u relative")
+    res["bytes_per_inversion"] = C2P_BYTES
+    for key in ("fp64", "mixed", "refined"):
+        res[key + "_hbm_frac"] = res[key + "_solves_per_s"] * C2P_BYTES / (HBM_PEAK_GBS * 1e9)
+    res["err_note"] = ("max errors over solved zones against the converged root of the same FP64 system "
+                       "(HG_C2P_REFINED): rho relative, u relative to rho + gam u, velocity relative to "
+                       "max(1, largest component)")
     return res
 
 
@@ -255,84 +315,116 @@ def _timed(fn, reps):
     return r, best
 
 
+def step_inversions(n1=128, n2=128, nsteps=100, seed=7):
+    """The final-substep inversions of nsteps RK2 steps of the 2-D torus of
+    the plain-method configuration (128x128, MC + LaxF + flux-CT), one batch
+    per step (hg_dump_inversions): the step's own conserved states, metric
+    and guesses (the midpoint primitives, advance.c:193)."""
+    from harm_finalwham_amd import harmgpu
+    g = harmgpu.Grid(n1, n2, 1, nb=3, lim=harmgpu.HG_LIM_MC, fluxmethod=harmgpu.HG_FLUX_LAXF, gam=4.0 / 3.0,
+                     cour=0.8)
+    g.build_mks_geometry(a=0.9375, Rin=1.3, Rout=40.0, R0=0.0, hslope=0.3)
+    dt = g.init_torus(rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0, randfact=4e-2, seed=seed)
+    out = []
+    for _ in range(nsteps):
+        dt, _, _ = g.step_ch(dt)
+        out.append(g.dump_inversions())
+    g.close()
+    return out
+
+
+def _cat_batches(bs):
+    import torch
+    return {k: torch.cat([b[k] for b in bs], dim=-1).contiguous() for k in bs[0]}
+
+
+def _features_solved(b, gam):
+    """Features and targets of a batch the plain FP64 path solves."""
+    from harm_finalwham_amd import harmgpu
+    p64, f64, _, _, _ = _c2p_solve(b, gam, harmgpu.HG_C2P_FP64, 1)
+    X, y, ok = harmgpu.mlp_features(b["U"], b["gcov"], b["gcon"], b["gdet"], b["p_guess"], p_solved=p64, gam=gam)
+    sel = (ok == 1) & (f64 == 0)
+    return X[sel].contiguous(), y[sel].contiguous()
+
+
+def _eval_modes(b, gam, m, reps):
+    """Plain FP64, converged (refined), mixed from the plain start and mixed
+    from the learned start on one batch: times, codes, errors against the
+    converged root, Newton steps."""
+    import torch
+    from harm_finalwham_amd import harmgpu
+    n = b["U"].shape[1]
+    p64, f64, i64, t64, _ = _c2p_solve(b, gam, harmgpu.HG_C2P_FP64, reps)
+    pre, fre, _, tre, _ = _c2p_solve(b, gam, harmgpu.HG_C2P_REFINED, reps)
+    pmx, fmx, imx, tmx, _ = _c2p_solve(b, gam, harmgpu.HG_C2P_MIXED, reps)
+    nfb_mixed = harmgpu.c2p_fallbacks()
+    pl, fl, il, tl, cnt = _c2p_solve(b, gam, None, reps, mlp=m)
+    wmax = lambda it: float(it[: (n // 64) * 64].view(-1, 64).amax(1).double().mean())
+    return {
+        "n": n, "fp64_ms": t64, "refined_ms": tre, "mixed_ms": tmx, "learned_ms": tl,
+        "fp64_solves_per_s": n / (t64 * 1e-3), "mixed_solves_per_s": n / (tmx * 1e-3),
+        "learned_solves_per_s": n / (tl * 1e-3),
+        "learned_speedup_vs_fp64": t64 / tl, "learned_speedup_vs_mixed": tmx / tl,
+        "flags_identical": bool(torch.equal(f64, fl)) and bool(torch.equal(f64, fmx)),
+        "learned_starts": cnt[0], "fallbacks": {"learned": cnt[1], "mixed": nfb_mixed},
+        "max_err_vs_converged": {"learned": c2p_errors(pl, pre, fre, gam), "mixed": c2p_errors(pmx, pre, fre, gam),
+                                 "fp64": c2p_errors(p64, pre, fre, gam)},
+        "steps_per_zone": {"fp64": float(i64.double().mean()), "mixed": float(imx.double().mean()),
+                           "learned": float(il.double().mean())},
+        "wavefront_max_steps": {"fp64": wmax(i64), "mixed": wmax(imx), "learned": wmax(il)},
+    }, p64, f64
+
+
 def mlp_bench(n_train, n_eval, epochs=8, reps=3):
     """Learned Newton start: a network trained on n_train inversions of the
-    benchmark distribution (gam 4/3) that the plain FP64 path solved, then
-    n_eval held-out inversions solved plain (FP64), from the learned start,
-    and mixed precision; plus the matrix-core forward pass alone."""
+    benchmark distribution (gam 4/3) plus the plain-method torus's own step
+    inversions (steps 1-60 of 100), which the plain FP64 path solved; then
+    held-out inversions -- n_eval of the distribution and the torus's steps
+    61-100 -- solved plain (FP64), converged (refined), mixed precision from
+    the plain start and mixed precision from the learned start; plus the
+    matrix-core forward pass alone."""
     import torch
     from harm_finalwham_amd import harmgpu
     gam = 4.0 / 3.0
     res = {"n_train": n_train, "n_eval": n_eval, "epochs": epochs, "gam": gam}
-
-    def solve(b, mode=None, mlp=None):
-        pr = b["p_guess"].clone()
-        n = pr.shape[1]
-        pflag = torch.zeros(n, dtype=torch.int32, device="cuda")
-        iters = torch.zeros_like(pflag)
-
-        def run():
-            pr.copy_(b["p_guess"])
-            if mlp is None:
-                harmgpu.utoprim_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], b["gcovpert"], pr, pflag, iters,
-                                      gam=gam, mode=mode)
-                return None
-            return harmgpu.utoprim_batch_learned(mlp, b["U"], b["gcov"], b["gcon"], b["gdet"], b["gcovpert"], pr,
-                                                 pflag, iters, gam=gam)
-        cnt, ms = _timed(run, reps)
-        return pr, pflag, iters, cnt, ms
-
     tr = harmgpu.make_c2p_batch(n_train, gam=gam, seed=4096)
-    p64, f64, _, _, _ = solve(tr, harmgpu.HG_C2P_FP64)
-    X, y, ok = harmgpu.mlp_features(tr["U"], tr["gcov"], tr["gcon"], tr["gdet"], tr["p_guess"], p_solved=p64,
-                                    gam=gam)
-    sel = (ok == 1) & (f64 == 0)
-    X, y = X[sel].contiguous(), y[sel].contiguous()
-    del tr, p64, f64
-    mu, sd = X.mean(0), X.std(0)
-    sd[sd < 1e-6] = 1.0
+    X, y = _features_solved(tr, gam)
+    del tr
+    dumps = step_inversions()
+    steps_train = _cat_batches(dumps[:60])
+    steps_eval = _cat_batches(dumps[60:])
+    del dumps
+    X1, y1 = _features_solved(steps_train, gam)
+    del steps_train
+    res["train_samples"] = {"distribution": int(X.shape[0]), "torus_steps": int(X1.shape[0])}
+    X = torch.cat([X, X1]).contiguous()
+    y = torch.cat([y, y1]).contiguous()
+    del X1, y1
     m = harmgpu.Mlp(seed=7)
-    m.set_normalisation(mu.cpu().numpy(), sd.cpu().numpy())
-    perm = torch.randperm(X.shape[0], device="cuda", generator=torch.Generator("cuda").manual_seed(1))
-    X, y = X[perm].contiguous(), y[perm].contiguous()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    hist = []
-    for ep in range(epochs):
-        hist.append(m.train(X, y, epochs=1, batch=2048, lr=2e-3 if ep < epochs // 2 else 5e-4))
-    torch.cuda.synchronize()
-    res["train_s"] = time.perf_counter() - t0
-    res["train_samples"] = int(X.shape[0])
+    hist, t_train = train_mlp(m, X, y, epochs)
+    res["train_s"] = t_train
     res["loss_by_epoch"] = hist
     del X, y
 
     ev = harmgpu.make_c2p_batch(n_eval, gam=gam, seed=8192)
-    p64, f64, i64, _, t64 = solve(ev, harmgpu.HG_C2P_FP64)
-    pl, fl, il, cnt, tl = solve(ev, mlp=m)
-    _, fmx, imx, _, tmx = solve(ev, harmgpu.HG_C2P_MIXED)
+    dist, p64, f64 = _eval_modes(ev, gam, m, reps)
     Xe, ye, oke = harmgpu.mlp_features(ev["U"], ev["gcov"], ev["gcon"], ev["gdet"], ev["p_guess"], p_solved=p64,
                                        gam=gam)
     yp, tf = _timed(lambda: m.forward(Xe), reps)
     s = (oke == 1) & (f64 == 0)
-    okz = f64 == 0
-    d = (pl - p64)[:, okz].abs()
-    ref = p64[:, okz].abs()
+    dist["rms_logW_error_plain_guess"] = float(ye[s].pow(2).mean().sqrt())
+    dist["rms_logW_error_learned"] = float((yp - ye)[s].pow(2).mean().sqrt())
+    res["distribution"] = dist
+    del ev, Xe, ye, yp
+    st, p64, f64 = _eval_modes(steps_eval, gam, m, reps)
+    Xe, ye, oke = harmgpu.mlp_features(steps_eval["U"], steps_eval["gcov"], steps_eval["gcon"], steps_eval["gdet"],
+                                       steps_eval["p_guess"], p_solved=p64, gam=gam)
+    s = (oke == 1) & (f64 == 0)
+    st["rms_logW_error_plain_guess"] = float(ye[s].pow(2).mean().sqrt())
+    st["rms_logW_error_learned"] = float((m.forward(Xe) - ye)[s].pow(2).mean().sqrt())
+    st["source"] = "final-substep inversions of steps 61-100 of the 128x128 MC + LaxF torus"
+    res["torus_steps"] = st
     n = n_eval
-    res.update({
-        "fp64_ms": t64, "learned_ms": tl, "mixed_ms": tmx,
-        "fp64_solves_per_s": n / (t64 * 1e-3), "learned_solves_per_s": n / (tl * 1e-3),
-        "learned_speedup_vs_fp64": t64 / tl,
-        "flags_identical": bool(torch.equal(f64, fl)) and bool(torch.equal(f64, fmx)),
-        "learned_starts": cnt[0], "fallbacks": cnt[1],
-        "max_err_vs_fp64": [float((d[0] / ref[0]).max()), float((d[1] / (ref[0] + gam * ref[1])).max()),
-                            float((d[2:5].amax(dim=0) / ref[2:5].amax(dim=0)).max())],
-        "rms_logW_error_plain_guess": float(ye[s].pow(2).mean().sqrt()),
-        "rms_logW_error_learned": float((yp - ye)[s].pow(2).mean().sqrt()),
-        "evaluations_per_zone": {"fp64": float(i64.double().mean()), "learned": float(il.double().mean()),
-                                 "mixed": float(imx.double().mean())},
-        "wavefront_max_evaluations": {"fp64": float(i64.view(-1, 64).amax(1).double().mean()),
-                                      "learned": float(il.view(-1, 64).amax(1).double().mean())},
-    })
     tflops = MLP_FLOP * n / (tf * 1e-3) / 1e12
     res["forward"] = {"ms": tf, "inferences_per_s": n / (tf * 1e-3), "flop_per_inference": MLP_FLOP,
                       "roofline": {"bound": "mfma", "achieved": tflops, "peak": BF16_DENSE_TFLOPS,
@@ -340,9 +432,29 @@ def mlp_bench(n_train, n_eval, epochs=8, reps=3):
     return res
 
 
+def train_mlp(m, X, y, epochs, seed=1):
+    """Normalise the features, shuffle, and train with Adam (minibatch 2048,
+    learning rate 2e-3 then 5e-4); returns (loss by epoch, seconds)."""
+    import torch
+    mu, sd = X.mean(0), X.std(0)
+    sd[sd < 1e-6] = 1.0
+    m.set_normalisation(mu.cpu().numpy(), sd.cpu().numpy())
+    perm = torch.randperm(X.shape[0], device="cuda", generator=torch.Generator("cuda").manual_seed(seed))
+    X, y = X[perm].contiguous(), y[perm].contiguous()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = []
+    for ep in range(epochs):
+        hist.append(m.train(X, y, epochs=1, batch=2048, lr=2e-3 if ep < epochs // 2 else 5e-4))
+    torch.cuda.synchronize()
+    return hist, time.perf_counter() - t0
+
+
 def c2p_cpu_baseline(n=1 << 22):
     """The oracle's Noble 2D inversion (C, gcc -O2, one thread; bit-identical
-    to the reference's Utoprimgen) on n inversions of the same distribution."""
+    to the reference's Utoprimgen) on n inversions of the same distribution,
+    and the same split over one thread per host core (the C loop releases
+    the interpreter, so the threads run in parallel)."""
     import numpy as np
     import torch
     from harm_finalwham_amd import harmgpu
@@ -358,8 +470,19 @@ def c2p_cpu_baseline(n=1 << 22):
     t0 = time.perf_counter()
     pyoracle.c2p_batch(*args)
     dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "inversions/s", "cores": 1, "kind": "port",
-            "sample": f"oracle Noble 2D (C, gcc -O2, bit-identical to the reference) on {n} inversions, {dt:.2f} s"}
+    res = {"value": n / dt, "unit": "inversions/s", "cores": 1, "kind": "port",
+           "sample": f"oracle Noble 2D (C, gcc -O2, bit-identical to the reference) on {n} inversions, {dt:.2f} s"}
+    ncores = host_cores()
+    if ncores > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        parts = np.array_split(np.arange(n), ncores)
+        sub = [tuple(a[idx] if isinstance(a, np.ndarray) else a for a in args) for idx in parts]
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(ncores) as ex:
+            list(ex.map(lambda a: pyoracle.c2p_batch(*a), sub))
+        dta = time.perf_counter() - t0
+        res["aggregate"] = {"value": n / dta, "unit": "inversions/s", "cores": ncores, "threads": ncores}
+    return res
 
 
 def batched_bench(ntori=48, nsteps=10, seed=5):

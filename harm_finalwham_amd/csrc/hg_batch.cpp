@@ -205,13 +205,66 @@ extern "C" int hg_set_tune(hg_ctx *c, const hg_tune *t) {
   return 0;
 }
 
-// time nsteps steps from the saved state; restores the state afterwards
-static int time_steps(hg_ctx *c, const double *p0, double dt0, int nsteps, hipStream_t s, hipEvent_t e0,
-                      hipEvent_t e1, float &ms) {
+// The state a step reads or writes, saved before tuning and restored after:
+// the primitives and intermediate states, the conserved arrays (the
+// finite-volume scheme's carried state, ucum, and uf, which holds unew for
+// hg_rdump_write after a step), the failure codes, dt and the counters, with
+// the host flags that describe them.  Trials start from the saved p, dt and
+// carried state, so that every trial times the same work.
+struct Snapshot {
+  std::vector<std::pair<void *, size_t>> arrays;  // device array, bytes
+  std::vector<size_t> offs;
+  char *buf = nullptr;
+  bool ucons_stale = true, unew_in_uf = false, dg_faces_valid = false;
+  const double *flux_state = nullptr;
+  double dt = 0.;
+  unsigned long long cnt[2] = {0, 0};
+  size_t carried = 0;                              // arrays[0..carried): restored before every trial
+};
+
+static int snapshot_take(hg_ctx *c, Snapshot &sn, hipStream_t s) {
   int rc = 0;
-  size_t bytes = (size_t)c->d.spl * NPR * sizeof(double);
-  HIPTRY(hipMemcpyAsync(c->p, p0, bytes, hipMemcpyDeviceToDevice, s));
-  HIPTRY(hipMemcpyAsync(c->dt, &dt0, sizeof(double), hipMemcpyHostToDevice, s));
+  const size_t st = (size_t)c->d.spl * NPR * sizeof(double);
+  sn.arrays.push_back({c->p, st});
+  if (c->ucons) sn.arrays.push_back({c->ucons, st});
+  sn.carried = sn.arrays.size();
+  for (double *a : {c->pk, c->pk2, c->ucum, c->uf})
+    if (a) sn.arrays.push_back({a, st});
+  sn.arrays.push_back({c->pflag, (size_t)c->d.spl});
+  size_t tot = 0;
+  for (auto &a : sn.arrays) {
+    sn.offs.push_back(tot);
+    tot += (a.second + 255) & ~(size_t)255;
+  }
+  HIPTRY(hipMalloc((void **)&sn.buf, tot));
+  if (rc) { sn.buf = nullptr; return rc; }
+  HIPTRY(hipStreamSynchronize(s));
+  for (size_t i = 0; i < sn.arrays.size(); i++)
+    HIPTRY(hipMemcpy(sn.buf + sn.offs[i], sn.arrays[i].first, sn.arrays[i].second, hipMemcpyDeviceToDevice));
+  HIPTRY(hipMemcpy(&sn.dt, c->dt, sizeof(double), hipMemcpyDeviceToHost));
+  HIPTRY(hipMemcpy(sn.cnt, c->counters, sizeof sn.cnt, hipMemcpyDeviceToHost));
+  sn.ucons_stale = c->ucons_stale;
+  sn.unew_in_uf = c->unew_in_uf;
+  sn.dg_faces_valid = c->dg_faces_valid;
+  sn.flux_state = c->flux_state;
+  return rc;
+}
+
+// restore the first n saved arrays (asynchronously on s), dt and the carried
+// state's flag
+static int snapshot_restore(hg_ctx *c, const Snapshot &sn, size_t n, hipStream_t s) {
+  int rc = 0;
+  for (size_t i = 0; i < n; i++)
+    HIPTRY(hipMemcpyAsync(sn.arrays[i].first, sn.buf + sn.offs[i], sn.arrays[i].second, hipMemcpyDeviceToDevice, s));
+  HIPTRY(hipMemcpyAsync(c->dt, &sn.dt, sizeof(double), hipMemcpyHostToDevice, s));
+  c->ucons_stale = sn.ucons_stale;
+  return rc;
+}
+
+// time nsteps steps from the saved state
+static int time_steps(hg_ctx *c, const Snapshot &sn, int nsteps, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                      float &ms) {
+  int rc = snapshot_restore(c, sn, sn.carried, s);
   HIPTRY(hipEventRecord(e0, s));
   for (int k = 0; k < nsteps; k++) enqueue_step(c, s);
   HIPTRY(hipGetLastError());
@@ -229,25 +282,22 @@ extern "C" int hg_autotune(hg_ctx *c, int nsteps, hg_tune *best, void *stream) {
   if (nsteps < 1) nsteps = 1;
   hipStream_t s = (hipStream_t)stream;
   int rc = 0;
-  size_t bytes = (size_t)c->d.spl * NPR * sizeof(double);
-  double *p0 = nullptr;
-  double dt0 = 0.;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  HIPTRY(hipMalloc((void **)&p0, bytes));
+  Snapshot sn;
+  // no diagnostics recording during the trials: the recorded boundary terms
+  // stay those of the caller's last step
+  const bool rec = c->dg_record;
+  c->dg_record = false;
   HIPTRY(hipEventCreate(&e0));
   HIPTRY(hipEventCreate(&e1));
-  HIPTRY(hipStreamSynchronize(s));
-  HIPTRY(hipMemcpy(p0, c->p, bytes, hipMemcpyDeviceToDevice));
-  HIPTRY(hipMemcpy(&dt0, c->dt, sizeof(double), hipMemcpyDeviceToHost));
-  unsigned long long cnt[2] = {0, 0};
-  HIPTRY(hipMemcpy(cnt, c->counters, sizeof cnt, hipMemcpyDeviceToHost));
+  if (!rc) rc = snapshot_take(c, sn, s);
   Tune cur = c->tune;
   if (!rc) {
     // one warm-up, then coordinate descent over the shapes, one at a time
     float ms = 0.f;
-    rc = time_steps(c, p0, dt0, nsteps, s, e0, e1, ms);
+    rc = time_steps(c, sn, nsteps, s, e0, e1, ms);
     float best_ms = 0.f;
-    if (!rc) rc = time_steps(c, p0, dt0, nsteps, s, e0, e1, best_ms);
+    if (!rc) rc = time_steps(c, sn, nsteps, s, e0, e1, best_ms);
     struct Axis {
       int Tune::*field;
       std::vector<int> values;
@@ -265,7 +315,7 @@ extern "C" int hg_autotune(hg_ctx *c, int nsteps, hg_tune *best, void *stream) {
         Tune trial = cur;
         trial.*(ax.field) = v;
         c->tune = trial;
-        rc = time_steps(c, p0, dt0, nsteps, s, e0, e1, ms);
+        rc = time_steps(c, sn, nsteps, s, e0, e1, ms);
         if (rc) break;
         // a shape must win by 1% to replace the current one: timing noise
         // of a few steps is about that size
@@ -277,13 +327,18 @@ extern "C" int hg_autotune(hg_ctx *c, int nsteps, hg_tune *best, void *stream) {
     }
     c->tune = cur;
   }
-  // the state, dt and counters as they were
-  if (p0) {
-    HIPTRY(hipMemcpy(c->p, p0, bytes, hipMemcpyDeviceToDevice));
-    HIPTRY(hipMemcpy(c->dt, &dt0, sizeof(double), hipMemcpyHostToDevice));
-    HIPTRY(hipMemcpy(c->counters, cnt, sizeof cnt, hipMemcpyHostToDevice));
+  // the state, dt, counters and flags as they were
+  if (sn.buf) {
+    int r2 = snapshot_restore(c, sn, sn.arrays.size(), s);
+    if (!rc) rc = r2;
+    HIPTRY(hipStreamSynchronize(s));
+    HIPTRY(hipMemcpy(c->counters, sn.cnt, sizeof sn.cnt, hipMemcpyHostToDevice));
+    c->unew_in_uf = sn.unew_in_uf;
+    c->dg_faces_valid = sn.dg_faces_valid;
+    c->flux_state = sn.flux_state;
+    (void)hipFree(sn.buf);
   }
-  if (p0) (void)hipFree(p0);
+  c->dg_record = rec;
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (!rc && best) to_c(c->tune, best);

@@ -7,6 +7,7 @@
 // operations in the same order as the reference, hence the same iterates,
 // iteration counts and failure codes.
 #pragma once
+#include "hg_dd.h"
 #include "hg_phys.h"
 
 namespace hg {
@@ -182,8 +183,11 @@ __device__ __forceinline__ int c2p_finish(const C2PInv &c, double W, int retval,
 
 // Full FP64 inversion of one zone (utoprimgen.c:23 with UTOPRIM2DFINAL).
 // U: evolved conserved variables (with gdet); pr: guess in, solution out.
-__device__ __forceinline__ int utoprimgen_fp64(const double *U, const Geom &G, double gam, double *pr,
-                                               int &iters) {
+// The core also returns the invariants and the final Newton iterate (x0 = W,
+// x1 = vsq); newton is false when the set-up failed before the iteration.
+__device__ __forceinline__ int utoprimgen_fp64_core(const double *U, const Geom &G, double gam, double *pr,
+                                                    int &iters, C2PInv &c, double &x0, double &x1,
+                                                    bool &newton) {
   double Ug[NPR], Ut[NPR];
 #pragma unroll
   for (int i = 0; i < NPR; i++) Ug[i] = U[i] / G.g;
@@ -196,15 +200,16 @@ __device__ __forceinline__ int utoprimgen_fp64(const double *U, const Geom &G, d
   const double igr = (gam - 1.0) / gam;
   const double gsq_big = GAMMAFAIL * GAMMAFAIL;
   const double vsq_big = (gsq_big - 1.0) / (gsq_big);
-  C2PInv c;
   double W_last;
   iters = 0;
+  newton = false;
   int code = c2p_setup(Ut, G, pr, gam, c, W_last);
   if (code) return code;
+  newton = true;
   // find_root_2D_gen + general_newton_raphson without line search
-  double x0 = fabs(W_last);
+  x0 = fabs(W_last);
   double v0 = vsq_of_W(c, W_last);
-  double x1 = (v0 > 1.) ? vsq_big : v0;
+  x1 = (v0 > 1.) ? vsq_big : v0;
   double f = 1., df = 1., errx = 1.;
   int n_iter = 0;
   bool keep = true;
@@ -235,22 +240,43 @@ __device__ __forceinline__ int utoprimgen_fp64(const double *U, const Geom &G, d
   return code;
 }
 
+__device__ __forceinline__ int utoprimgen_fp64(const double *U, const Geom &G, double gam, double *pr,
+                                               int &iters) {
+  C2PInv c;
+  double x0 = 0., x1 = 0.;
+  bool newton;
+  return utoprimgen_fp64_core(U, G, gam, pr, iters, c, x0, x1, newton);
+}
+
 }  // namespace hg
 
 namespace hg {
 
 // ---------------------------------------------------------------------------
-// Mixed precision: the Newton iteration of utoprimgen_fp64 run in FP32 from
-// the same FP64 set-up (metric contractions, invariants, guess), then FP64
-// Newton polishes from the FP32 root until the reference's tolerance holds,
-// then the FP64 post-solve.  Whenever the FP32 phase does not converge, meets
-// a non-finite value, or the polish needs more than MIXED_MAX_POLISH steps,
-// the zone is inverted again by utoprimgen_fp64 from the original guess, so
-// every hard case keeps the reference's iterates and failure codes.
+// The converged root and the mixed-precision path.
+//
+// The reference iteration stops at |dW/W| <= NEWT_TOL with W where the FP64
+// residual is still within rounding noise of zero; at Lorentz factors near 50
+// and magnetisations near 1e3 that noise moves the primitives by up to ~1e-9
+// (vsq is recomputed from W and 1 / (1 - vsq) amplifies it), and two FP64
+// iterations from different starts land at different points of the noise.
+// The converged root of the same FP64 system (the invariants exactly as the
+// reference sets them up) is found by Newton steps whose residuals are
+// evaluated in double-double arithmetic (hg_dd.h) with the FP64 Jacobian of
+// newton_dx, and the primitives are formed from it in double-double:
+//   c2p_polish     FP64 steps to |dW/W| <= 1e-9, then double-double steps to
+//                  |dW/W| <= 1e-15 (false when either is not reached)
+//   c2p_finish_dd  c2p_finish with vsq, 1 - vsq, rho0, w, p, u and the
+//                  velocities in double-double, rounded once
+// The mixed-precision path runs the Newton iteration in FP32 from the
+// reference's FP64 set-up and start, to |dW/W| <= MIXED_TOL_F32, then polishes
+// and finishes as above; a zone whose FP32 phase or polish does not converge
+// is inverted again by the reference path (and polished), by the caller.
 // ---------------------------------------------------------------------------
 constexpr int MIXED_MAX_F32 = 40;       // FP32 Newton steps before falling back
 constexpr float MIXED_TOL_F32 = 2e-6f;  // |dW/W| that ends the FP32 phase
-constexpr int MIXED_MAX_POLISH = 4;     // FP64 steps allowed after it (2 expected)
+constexpr int POLISH_MAX_F64 = 3;       // FP64 steps of the polish (to |dW/W| <= 1e-9)
+constexpr int POLISH_MAX_DD = 2;        // double-double steps (to |dW/W| <= 1e-15)
 
 struct C2PInvF {
   float Bsq, QdotBsq, Qtsq, Qdotn, D;
@@ -258,7 +284,7 @@ struct C2PInvF {
 
 // newton_dx in FP32 (same algebra) with the hardware reciprocal and
 // reciprocal square root (about 1 ulp): the FP32 phase only has to bring W
-// within MIXED_TOL_F32 of the root, the FP64 polish does the rest
+// within MIXED_TOL_F32 of the root, the polish does the rest
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ void newton_dx_f32(const C2PInvF &c, float igr, float W, float vsq, float &dx0,
                                               float &dx1) {
@@ -289,95 +315,183 @@ __device__ __forceinline__ void newton_dx_f32(const C2PInvF &c, float igr, float
   dx1 = -(-t25 * t11 - 2.0f * t40 * t18) * t21 * t36;
 }
 
-// Returns the failure code; iters counts FP32 + FP64 Newton steps (or the
-// FP64 path's steps after a fallback); *fell_back tells which path answered.
-__device__ __forceinline__ int utoprimgen_mixed(const double *U, const Geom &G, double gam, double *pr,
-                                                int &iters, bool &fell_back) {
-  double guess[NPR];
-#pragma unroll
-  for (int i = 0; i < NPR; i++) guess[i] = pr[i];
-  fell_back = false;
-  double Ug[NPR], Ut[NPR];
-#pragma unroll
-  for (int i = 0; i < NPR; i++) Ug[i] = U[i] / G.g;
-  Ug[UU] -= Ug[RHO];
-  double alpha = 1.0 / sqrt(-G.gcon[0][0]);
-#pragma unroll
-  for (int i = 0; i < NPR; i++) Ut[i] = alpha * Ug[i];
-  const double igr = (gam - 1.0) / gam;
+// The start of utoprimgen_fp64's Newton loop from the set-up's W_last
+__device__ __forceinline__ void newton_start(const C2PInv &c, double W_last, double &x0, double &x1) {
   const double gsq_big = GAMMAFAIL * GAMMAFAIL;
   const double vsq_big = (gsq_big - 1.0) / (gsq_big);
+  x0 = fabs(W_last);
+  double v0 = vsq_of_W(c, W_last);
+  x1 = (v0 > 1.) ? vsq_big : v0;
+}
+
+// FP32 Newton from (x0, x1) with the guards of the reference loop; true when
+// |dW/W| <= MIXED_TOL_F32 was reached with finite iterates; iters counts steps
+__device__ __forceinline__ bool newton_f32(const C2PInv &c, double igr, double &x0, double &x1, int &iters) {
+  const double gsq_big = GAMMAFAIL * GAMMAFAIL;
+  const double vsq_big = (gsq_big - 1.0) / (gsq_big);
+  C2PInvF cf = {(float)c.Bsq, (float)c.QdotBsq, (float)c.Qtsq, (float)c.Qdotn, (float)c.D};
+  const float igrf = (float)igr, wf = (float)c.w;
+  const float gsq_bigf = (float)gsq_big, vsq_bigf = (float)vsq_big;
+  float w0 = (float)x0, w1 = (float)x1;
+  bool conv = false;
+  int n32 = 0;
+  while (n32 < MIXED_MAX_F32) {
+    float d0, d1;
+    newton_dx_f32(cf, igrf, w0, w1, d0, d1);
+    float wo = w0;
+    w0 += d0;
+    w1 += d1;
+    float err = (w0 == 0.f) ? fabsf(d0) : fabsf(d0 * frcp(w0));
+    w0 = fabsf(w0);
+    w0 = (w0 > gsq_bigf * wf) ? wo : w0;
+    w1 = (w1 > 1.f) ? vsq_bigf : w1;
+    n32++;
+    if (!isfinite(w0) || !isfinite(w1)) break;
+    if (err <= MIXED_TOL_F32) {
+      conv = true;
+      break;
+    }
+  }
+  iters += n32;
+  x0 = (double)w0;
+  x1 = (double)w1;
+  return conv;
+}
+
+// One Newton step with the residuals t11, t18 of func_vsq2 in double-double
+// and the FP64 Jacobian (newton_dx's terms)
+__device__ __forceinline__ void newton_dx_dd(const C2PInv &c, double igr, double W, double vsq, double &dx0,
+                                             double &dx1) {
+  const double Wsq = W * W, D = c.D, Bsq = c.Bsq, QdotBsq = c.QdotBsq;
+  const double gamma = 1.0 / sqrt(1.0 - vsq);
+  const double dPdW = (1.0 - vsq) * igr;
+  const double dPdvsq = (D * (gamma * 0.5 - 1.0) - (W - D)) * igr;
+  const double t2 = -0.5 * Bsq + dPdvsq, t3 = Bsq + W, t4 = t3 * t3, t9 = 1 / Wsq;
+  const double t16 = QdotBsq * t9, t21 = 1 / t3, t23 = 1 / W, t24 = t16 * t23, t25 = -1.0 + dPdW - t24;
+  const double t35 = t25 * t3 + (Bsq - 2.0 * dPdvsq) * (QdotBsq + vsq * Wsq * W) * t9 * t23;
+  const double t36 = 1 / t35;
+  // residuals: t11 = Qtsq - vsq (Bsq + W)^2 + QdotBsq (Bsq + 2 W) / W^2,
+  // t18 = -Qdotn - Bsq (1 + vsq) / 2 + QdotBsq / (2 W^2) - W + p(W, vsq)
+  const dd Wd = dd_of(W), vd = dd_of(vsq);
+  const dd T3 = two_sum(Bsq, W);
+  const dd q = dd_div(dd_of(QdotBsq), two_prod(W, W));
+  const dd r11 = dd_add(dd_sub(dd_of(c.Qtsq), dd_mul(vd, dd_mul(T3, T3))), dd_mul(q, two_sum(Bsq, 2.0 * W)));
+  const dd omv = two_sum(1.0, -vsq);
+  const dd gd = dd_div(dd_of(1.0), dd_sqrt(omv));
+  const dd wm = dd_sub(dd_mul(two_sum(W, -D), omv), dd_div(two_prod(vsq, D), dd_add(dd_of(1.0), gd)));
+  const dd r18 = dd_add(dd_add(dd_add(dd_of(-c.Qdotn), dd_mul_d(two_sum(1.0, vsq), -0.5 * Bsq)), dd_mul_d(q, 0.5)),
+                        dd_add(dd_neg(Wd), dd_mul_d(wm, igr)));
+  const double t11 = dd_val(r11), t18 = dd_val(r18);
+  dx0 = -(t2 * t11 + t4 * t18) * t21 * t36;
+  const double t40 = (vsq + t24) * t3;
+  dx1 = -(-t25 * t11 - 2.0 * t40 * t18) * t21 * t36;
+}
+
+// The polish: FP64 steps (at most n64; none when n64 = 0) until |dW/W| <=
+// 1e-9, then double-double steps until |dW/W| <= 1e-15.  True when both
+// tolerances were reached with finite iterates.
+__device__ __forceinline__ bool c2p_polish(const C2PInv &c, double igr, double &x0, double &x1, int n64,
+                                           int &iters) {
+  const double gsq_big = GAMMAFAIL * GAMMAFAIL;
+  const double vsq_big = (gsq_big - 1.0) / (gsq_big);
+  bool ok = (n64 == 0);
+  for (int k = 0; k < n64 && !ok; k++) {
+    double dx0, dx1, f, df;
+    newton_dx(c, igr, x0, x1, dx0, dx1, f, df);
+    double xo0 = x0;
+    x0 = fabs(x0 + dx0);
+    x0 = (x0 / c.w > gsq_big) ? xo0 : x0;
+    x1 += dx1;
+    x1 = (x1 > 1.) ? vsq_big : x1;
+    iters++;
+    ok = fabs(x0 - xo0) <= 1e-9 * x0;
+  }
+  if (!ok) return false;
+  ok = false;
+  for (int k = 0; k < POLISH_MAX_DD && !ok; k++) {
+    double dx0, dx1;
+    newton_dx_dd(c, igr, x0, x1, dx0, dx1);
+    double xo0 = x0;
+    x0 = fabs(x0 + dx0);
+    x0 = (x0 / c.w > gsq_big) ? xo0 : x0;
+    x1 += dx1;
+    x1 = (x1 > 1.) ? vsq_big : x1;
+    iters++;
+    ok = fabs(x0 - xo0) <= 1e-15 * x0;
+  }
+  return ok && isfinite(x0) && isfinite(x1);
+}
+
+// c2p_finish from a converged W with the primitives formed in double-double
+// (the checks and failure codes of c2p_finish, on the rounded values)
+__device__ __forceinline__ int c2p_finish_dd(const C2PInv &c, double W, double igr, double *prim_out,
+                                             bool &assign) {
+  const double gsq_big = GAMMAFAIL * GAMMAFAIL;
+  const double vsq_big = (gsq_big - 1.0) / (gsq_big);
+  const double maxnegvsq = (1.0 - 1.0 / (MAXNEGUTSQ + 1.0));
+  assign = false;
+  if (W == FAIL_VAL) return 101;
+  double Wtest = W / c.w;
+  assign = true;
+  if (Wtest <= 0. || Wtest > gsq_big) { assign = false; return FAIL_CONVW; }
+  const dd Wsq = two_prod(W, W), X = two_sum(c.Bsq, W);
+  dd vd = dd_div(dd_add(dd_mul_d(Wsq, c.Qtsq), dd_mul_d(two_sum(c.Bsq, 2.0 * W), c.QdotBsq)),
+                 dd_mul(Wsq, dd_mul(X, X)));
+  double vsq = dd_val(vd);
+  if ((vsq < 0.) && (fabs(vsq) < maxnegvsq)) {
+    vsq = 0.0;
+    vd = dd_of(0.0);
+  }
+  if ((vsq > vsq_big) || (vsq < 0.0)) { assign = false; return FAIL_CONVUTSQ; }
+  const dd omv = dd_sub(dd_of(1.0), vd);
+  const dd gt = dd_sqrt(omv);
+  const dd rho0 = dd_mul_d(gt, c.D);
+  const dd w = dd_mul_d(omv, W);
+  const dd p = dd_mul_d(dd_sub(w, rho0), igr);
+  const dd u = dd_sub(w, dd_add(rho0, p));
+  const double r = dd_val(rho0), uu = dd_val(u);
+  int flag = 0;
+  if ((r <= 0.) || (uu < 0.)) {
+    if ((r <= 0.) && (uu >= 0.)) flag = FAIL_RHONEG;
+    if ((r > 0.) && (uu < 0.)) flag = FAIL_UNEG;
+    if ((r <= 0.) && (uu < 0.)) flag = FAIL_RHOUNEG;
+  }
+  prim_out[RHO] = r;
+  prim_out[UU] = uu;
+  const dd f = dd_div(dd_div(dd_of(1.0), gt), X);  // gamma / (W + Bsq)
+  const dd iW = dd_div(dd_of(1.0), dd_of(W));
+#pragma unroll
+  for (int i = 1; i < 4; i++) {
+    dd Qt = dd_add(dd_of(c.Qcon[i]), two_prod(c.ncon[i], c.Qdotn));
+    dd v = dd_add(Qt, dd_mul(two_prod(c.QdotB, c.Bcon[i]), iW));
+    prim_out[U1 + i - 1] = dd_val(dd_mul(f, v));
+  }
+  return flag;
+}
+
+// The reference inversion of one zone with a converged root: utoprimgen_fp64
+// (its failure code, its iteration count and, for a failed zone, its
+// values), then, for a zone it solves, the polish from its final iterate and
+// the primitives in double-double; these replace the reference's values
+// when the polish converges.  *polished tells whether they did.
+__device__ __forceinline__ int utoprimgen_refined(const double *U, const Geom &G, double gam, double *pr,
+                                                  int &iters, bool &polished) {
   C2PInv c;
-  double W_last;
-  iters = 0;
-  int code = c2p_setup(Ut, G, guess, gam, c, W_last);
-  bool ok = (code == 0);
   double x0 = 0., x1 = 0.;
-  if (ok) {
-    // FP32 bulk
-    C2PInvF cf = {(float)c.Bsq, (float)c.QdotBsq, (float)c.Qtsq, (float)c.Qdotn, (float)c.D};
-    const float igrf = (float)igr, wf = (float)c.w;
-    const float gsq_bigf = (float)gsq_big, vsq_bigf = (float)vsq_big;
-    float w0 = (float)fabs(W_last);
-    double v0 = vsq_of_W(c, W_last);
-    float w1 = (float)((v0 > 1.) ? vsq_big : v0);
-    int n32 = 0;
-    bool conv = false;
-    while (n32 < MIXED_MAX_F32) {
-      float d0, d1;
-      newton_dx_f32(cf, igrf, w0, w1, d0, d1);
-      float wo = w0;
-      w0 += d0;
-      w1 += d1;
-      float err = (w0 == 0.f) ? fabsf(d0) : fabsf(d0 * frcp(w0));
-      w0 = fabsf(w0);
-      w0 = (w0 > gsq_bigf * wf) ? wo : w0;
-      w1 = (w1 > 1.f) ? vsq_bigf : w1;
-      n32++;
-      if (!isfinite(w0) || !isfinite(w1)) break;
-      if (err <= MIXED_TOL_F32) { conv = true; break; }
-    }
-    ok = conv;
-    iters = n32;
-    x0 = (double)w0;
-    x1 = (double)w1;
-  }
-  if (ok) {
-    // FP64 polish with the reference's stopping rule
-    double f = 1., df = 1., errx = 1.;
-    int np = 0;
-    bool done = false;
-    while (np < MIXED_MAX_POLISH) {
-      double dx0, dx1;
-      newton_dx(c, igr, x0, x1, dx0, dx1, f, df);
-      double xo0 = x0;
-      x0 += dx0;
-      x1 += dx1;
-      errx = (x0 == 0.) ? fabs(dx0) : fabs(dx0 / x0);
-      x0 = fabs(x0);
-      x0 = (x0 / c.w > gsq_big) ? xo0 : x0;
-      x1 = (x1 > 1.) ? vsq_big : x1;
-      np++;
-      if (fabs(errx) <= NEWT_TOL) { done = true; break; }
-    }
-    iters += np;
-    ok = done && isfinite(f) && isfinite(df) && isfinite(x0) && isfinite(x1);
-  }
-  if (!ok) {
-    // the reference path from the original guess
-    fell_back = true;
-#pragma unroll
-    for (int i = 0; i < NPR; i++) pr[i] = guess[i];
-    return utoprimgen_fp64(U, G, gam, pr, iters);
-  }
-#pragma unroll
-  for (int i = B1; i <= B3; i++) pr[i] = Ug[i];
+  bool newton;
+  polished = false;
+  int code = utoprimgen_fp64_core(U, G, gam, pr, iters, c, x0, x1, newton);
+  if (code != 0 || !newton) return code;
+  const double igr = (gam - 1.0) / gam;
+  int it = 0;
+  if (!c2p_polish(c, igr, x0, x1, POLISH_MAX_F64, it)) return code;
   double out[NPR];
   bool assign;
-  code = c2p_finish(c, x0, 0, igr, out, assign);
-  if (assign) {
+  if (c2p_finish_dd(c, x0, igr, out, assign) == 0 && assign) {
 #pragma unroll
     for (int i = 0; i < B1; i++) pr[i] = out[i];
+    iters += it;
+    polished = true;
   }
   return code;
 }

@@ -1,19 +1,36 @@
-// Batched con2prim on device memory: the generator of the benchmark
-// distribution and the mixed-precision persistent inversion kernel.
+// Batched con2prim on device memory (utoprimgen.c:23 over independent
+// zones): the generator of the benchmark distribution and the inversion
+// kernels of hg_utoprim_batch.
 //
 //   k_make_c2p_batch  one thread per inversion: Kerr-Schild metric at a random
 //                     (r, theta), random primitives, their conserved variables
 //                     (primtoU, FP64), a perturbed guess
-//   k_c2p_mixed       persistent grid; each wavefront takes 64 inversions at
-//                     a time from a device counter until the batch is done,
-//                     so wavefronts whose zones need more Newton steps do not
-//                     hold up the rest (utoprimgen_mixed per lane)
+//   k_c2p_batch       one thread per zone: the reference inversion
+//                     (HG_C2P_FP64), or the reference inversion with its root
+//                     converged and the primitives formed in double-double
+//                     (HG_C2P_REFINED)
+//   k_c2p_mixed       persistent work queue: each wavefront takes 4 x 64
+//                     zones at a time from a device counter; per lane the
+//                     FP64 set-up, the FP32 Newton iteration, the polish and
+//                     the double-double primitives; a zone that does not
+//                     converge joins a compacted list instead
+//   k_c2p_listed      persistent over that list: the HG_C2P_REFINED
+//                     inversion from the original guess
+//
+// Memory per zone: the inversion reads U (8), the upper triangles of the
+// symmetric gcov and gcon (10 + 10), gdet and the guess's u and u~^i (4),
+// and writes the field (3), the solved rho, u, u~^i (5, only where the
+// reference assigns them), the failure code and the iteration count:
+// 336 bytes.  gcovpert and the guess's rho and field are not read (the
+// inversion does not use them).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 
 #include "harmgpu.h"
 #include "hg_c2p.h"
+#include "hg_c2p_io.h"
 #include "hg_host.h"
 
 using namespace hg;
@@ -120,57 +137,183 @@ __global__ void __launch_bounds__(256) k_make_c2p_batch(int64_t n, double gam, d
   gcovpert[t] = G.gpert0;
 }
 
-__global__ void k_reset2(unsigned long long *c) {
-  if (threadIdx.x < 2) c[threadIdx.x] = 0ull;
+template <bool REFINED>
+__global__ void __launch_bounds__(256) k_c2p_batch(int64_t n, double gam, const double *__restrict__ U,
+                                                   const double *__restrict__ gcov, const double *__restrict__ gcon,
+                                                   const double *__restrict__ gdet, double *__restrict__ pr,
+                                                   int *__restrict__ pflag, int *__restrict__ iters) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  Geom G;
+  double u[NPR], p[NPR];
+  load_batch_zone(n, t, U, gcov, gcon, gdet, pr, G, u, p);
+  int it = 0, flag;
+  if (REFINED) {
+    bool pol;
+    flag = utoprimgen_refined(u, G, gam, p, it, pol);
+  } else {
+    flag = utoprimgen_fp64(u, G, gam, p, it);
+  }
+  store_batch_zone(n, t, pr, p, code_assigns(flag));
+  pflag[t] = flag;
+  if (iters) iters[t] = it;
 }
+
+__global__ void k_c2p_reset(unsigned long long *ctr) {
+  if (threadIdx.x < C2P_NCTR) ctr[threadIdx.x] = 0ull;
+}
+
+constexpr int MIXED_ROUNDS = 4;  // 64-zone rounds per work-queue grab
 
 __global__ void __launch_bounds__(256) k_c2p_mixed(int64_t n, double gam, const double *__restrict__ U,
                                                    const double *__restrict__ gcov,
                                                    const double *__restrict__ gcon,
-                                                   const double *__restrict__ gdet,
-                                                   const double *__restrict__ gcovpert, double *__restrict__ pr,
+                                                   const double *__restrict__ gdet, double *__restrict__ pr,
                                                    int *__restrict__ pflag, int *__restrict__ iters,
-                                                   unsigned long long *__restrict__ queue) {
+                                                   unsigned long long *__restrict__ ctr,
+                                                   uint32_t *__restrict__ list) {
   const int lane = threadIdx.x & 63;
+  const double igr = (gam - 1.0) / gam;
   for (;;) {
     unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(&queue[0], 64ull);
+    if (lane == 0) base = atomicAdd(&ctr[C2P_WORK], (unsigned long long)(64 * MIXED_ROUNDS));
     base = __shfl(base, 0, 64);
     if (base >= (unsigned long long)n) break;
-    int64_t t = (int64_t)base + lane;
-    bool fb = false;
-    if (t < n) {
-      Geom G;
+    for (int r = 0; r < MIXED_ROUNDS; r++) {
+      const int64_t t = (int64_t)base + 64 * r + lane;
+      bool fb = false;
+      if (t < n) {
+        Geom G;
+        double u[NPR], p[NPR], Ug[NPR], Ut[NPR];
+        load_batch_zone(n, t, U, gcov, gcon, gdet, pr, G, u, p);
 #pragma unroll
-      for (int a = 0; a < 4; a++)
+        for (int i = 0; i < NPR; i++) Ug[i] = u[i] / G.g;
+        Ug[UU] -= Ug[RHO];
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-          G.gcov[a][c] = gcov[(a * 4 + c) * n + t];
-          G.gcon[a][c] = gcon[(a * 4 + c) * n + t];
+        for (int i = B1; i <= B3; i++) p[i] = Ug[i];
+        const double alpha = 1.0 / sqrt(-G.gcon[0][0]);
+#pragma unroll
+        for (int i = 0; i < NPR; i++) Ut[i] = alpha * Ug[i];
+        C2PInv c;
+        double W_last;
+        int it = 0, flag = c2p_setup(Ut, G, p, gam, c, W_last);
+        if (flag == 0) {
+          double x0, x1;
+          newton_start(c, W_last, x0, x1);
+          fb = !mixed_zone(c, igr, x0, x1, p, it, flag);
         }
-      G.gpert0 = gcovpert[t];
-      G.g = gdet[t];
-      double u[NPR], p[NPR];
-#pragma unroll
-      for (int pl = 0; pl < NPR; pl++) {
-        u[pl] = U[pl * n + t];
-        p[pl] = pr[pl * n + t];
+        if (!fb) {
+          store_batch_zone(n, t, pr, p, code_assigns(flag));
+          pflag[t] = flag;
+          if (iters) iters[t] = it;
+        }
       }
-      int it = 0;
-      int flag = utoprimgen_mixed(u, G, gam, p, it, fb);
-#pragma unroll
-      for (int pl = 0; pl < NPR; pl++) pr[pl * n + t] = p[pl];
-      pflag[t] = flag;
-      if (iters) iters[t] = it;
+      list_append(fb, t, ctr, list);
     }
-    unsigned long long nfb = __popcll(__ballot(fb));
-    if (lane == 0 && nfb) atomicAdd(&queue[1], nfb);
   }
 }
 
-unsigned long long *g_queue[64];  // per device: {work counter, fallbacks of the last mixed batch}
+// the listed zones: HG_C2P_REFINED from the guess still in pr
+__global__ void __launch_bounds__(256) k_c2p_listed(int64_t n, double gam, const double *__restrict__ U,
+                                                    const double *__restrict__ gcov,
+                                                    const double *__restrict__ gcon,
+                                                    const double *__restrict__ gdet, double *__restrict__ pr,
+                                                    int *__restrict__ pflag, int *__restrict__ iters,
+                                                    unsigned long long *__restrict__ ctr,
+                                                    const uint32_t *__restrict__ list) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long len = ctr[C2P_LISTLEN];
+  for (;;) {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&ctr[C2P_LISTWORK], 64ull);
+    base = __shfl(base, 0, 64);
+    if (base >= len) break;
+    if (base + lane < len) {
+      const int64_t t = list[base + lane];
+      Geom G;
+      double u[NPR], p[NPR];
+      load_batch_zone(n, t, U, gcov, gcon, gdet, pr, G, u, p);
+      int it = 0;
+      bool pol;
+      int flag = utoprimgen_refined(u, G, gam, p, it, pol);
+      store_batch_zone(n, t, pr, p, code_assigns(flag));
+      pflag[t] = flag;
+      if (iters) iters[t] = it;
+    }
+  }
+}
+
+std::atomic<long long> g_last_fallbacks[64];  // per device: listed zones of the last mixed batch
+
+// blocks of 256 threads that stay resident on the whole device
+template <typename K>
+static int persistent_blocks(K kernel, int64_t want, int64_t &blocks) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0);
+  if (e != hipSuccess) {
+    set_error("hg_utoprim_batch: %s", hipGetErrorString(e));
+    return 1;
+  }
+  if (per_cu < 1) per_cu = 1;
+  blocks = (int64_t)ncu * per_cu;
+  if (blocks > want) blocks = want;
+  if (blocks < 1) blocks = 1;
+  return 0;
+}
 
 }  // namespace
+
+namespace hg {
+
+size_t c2p_scratch_bytes(int64_t n) { return C2P_NCTR * sizeof(unsigned long long) + (size_t)n * sizeof(uint32_t); }
+
+int launch_c2p_batch(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
+                     const double *gdet, double *pr, int *pflag, int *iters, bool refined, hipStream_t s) {
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (refined)
+    hipLaunchKernelGGL(k_c2p_batch<true>, dim3(nb), dim3(256), 0, s, n, gam, U, gcov, gcon, gdet, pr, pflag, iters);
+  else
+    hipLaunchKernelGGL(k_c2p_batch<false>, dim3(nb), dim3(256), 0, s, n, gam, U, gcov, gcon, gdet, pr, pflag, iters);
+  return 0;
+}
+
+int launch_c2p_batch_mixed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
+                           const double *gdet, double *pr, int *pflag, int *iters, void *scratch, hipStream_t s) {
+  unsigned long long *ctr = (unsigned long long *)scratch;
+  uint32_t *list = (uint32_t *)(ctr + C2P_NCTR);
+  int64_t b1 = 0, b2 = 0;
+  if (persistent_blocks(k_c2p_mixed, (n + 255) / 256, b1)) return 1;
+  if (persistent_blocks(k_c2p_listed, (n + 255) / 256, b2)) return 1;
+  hipLaunchKernelGGL(k_c2p_reset, dim3(1), dim3(64), 0, s, ctr);
+  hipLaunchKernelGGL(k_c2p_mixed, dim3((unsigned)b1), dim3(256), 0, s, n, gam, U, gcov, gcon, gdet, pr, pflag, iters,
+                     ctr, list);
+  hipLaunchKernelGGL(k_c2p_listed, dim3((unsigned)b2), dim3(256), 0, s, n, gam, U, gcov, gcon, gdet, pr, pflag,
+                     iters, ctr, (const uint32_t *)list);
+  return 0;
+}
+
+void launch_c2p_reset(void *scratch, hipStream_t s) {
+  hipLaunchKernelGGL(k_c2p_reset, dim3(1), dim3(64), 0, s, (unsigned long long *)scratch);
+}
+
+int launch_c2p_listed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
+                      const double *gdet, double *pr, int *pflag, int *iters, void *scratch, hipStream_t s) {
+  unsigned long long *ctr = (unsigned long long *)scratch;
+  int64_t b = 0;
+  if (persistent_blocks(k_c2p_listed, (n + 255) / 256, b)) return 1;
+  hipLaunchKernelGGL(k_c2p_listed, dim3((unsigned)b), dim3(256), 0, s, n, gam, U, gcov, gcon, gdet, pr, pflag, iters,
+                     ctr, (const uint32_t *)(ctr + C2P_NCTR));
+  return 0;
+}
+
+void c2p_note_fallbacks(long long nfb) {
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64) g_last_fallbacks[dev] = nfb;
+}
+
+}  // namespace hg
 
 extern "C" int hg_make_c2p_batch(int64_t n, double gam, double a, uint64_t seed, double *U, double *gcov,
                                  double *gcon, double *gdet, double *gcovpert, double *p_true, double *p_guess,
@@ -191,38 +334,58 @@ extern "C" int hg_make_c2p_batch(int64_t n, double gam, double a, uint64_t seed,
   return 0;
 }
 
-namespace hg {
-int launch_c2p_batch_mixed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
-                           const double *gdet, const double *gcovpert, double *pr, int *pflag, int *iters,
-                           hipStream_t s) {
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess && (dev < 0 || dev >= 64)) { set_error("hg_utoprim_batch: device index %d", dev); return 1; }
-  if (e == hipSuccess && !g_queue[dev]) e = hipMalloc((void **)&g_queue[dev], 2 * sizeof(unsigned long long));
-  int ncu = 0, per_cu = 0;
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_c2p_mixed, 256, 0);
-  if (e != hipSuccess) { set_error("hg_utoprim_batch: %s", hipGetErrorString(e)); return 1; }
-  if (per_cu < 1) per_cu = 1;
-  int64_t want = (n + 255) / 256;
-  int64_t blocks = (int64_t)ncu * per_cu;
-  if (blocks > want) blocks = want;
-  hipLaunchKernelGGL(k_reset2, dim3(1), dim3(64), 0, s, g_queue[dev]);
-  hipLaunchKernelGGL(k_c2p_mixed, dim3((unsigned)blocks), dim3(256), 0, s, n, gam, U, gcov, gcon, gdet, gcovpert,
-                     pr, pflag, iters, g_queue[dev]);
+// the list holds 32-bit zone indices
+static int check_batch_args(const char *who, int64_t n, const double *U, const double *gcov, const double *gcon,
+                            const double *gdet, const double *pr, const int32_t *pflag) {
+  if (n < 0) { set_error("%s: n < 0", who); return 1; }
+  if (n > (int64_t)0xFFFFFFFFll) { set_error("%s: n > 2^32 - 1", who); return 1; }
+  if (n > 0 && (!U || !gcov || !gcon || !gdet || !pr || !pflag)) {
+    set_error("%s: null argument", who);
+    return 1;
+  }
   return 0;
 }
-}  // namespace hg
+
+extern "C" int hg_utoprim_batch(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
+                                const double *gdet, const double *gcovpert, double *pr, int32_t *pflag,
+                                int32_t *iters, int mode, void *stream) {
+  (void)gcovpert;
+  if (check_batch_args("hg_utoprim_batch", n, U, gcov, gcon, gdet, pr, pflag)) return 1;
+  if (mode != HG_C2P_FP64 && mode != HG_C2P_MIXED && mode != HG_C2P_REFINED) {
+    set_error("hg_utoprim_batch: unknown mode %d", mode);
+    return 1;
+  }
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSuccess;
+  if (mode != HG_C2P_MIXED) {
+    launch_c2p_batch(n, gam, U, gcov, gcon, gdet, pr, pflag, iters, mode == HG_C2P_REFINED, s);
+  } else {
+    // scratch of this call only, stream-ordered: concurrent calls on other
+    // streams have their own counters and lists
+    void *scr = nullptr;
+    e = hipMallocAsync(&scr, c2p_scratch_bytes(n), s);
+    if (e != hipSuccess) { set_error("hg_utoprim_batch: scratch: %s", hipGetErrorString(e)); return 1; }
+    int rc = launch_c2p_batch_mixed(n, gam, U, gcov, gcon, gdet, pr, pflag, iters, scr, s);
+    unsigned long long ctr[C2P_NCTR] = {0, 0, 0};
+    if (!rc) e = hipMemcpyAsync(ctr, scr, sizeof ctr, hipMemcpyDeviceToHost, s);
+    hipError_t e2 = hipFreeAsync(scr, s);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (rc) return 1;
+    if (e == hipSuccess) c2p_note_fallbacks((long long)ctr[C2P_LISTLEN]);
+  }
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) { set_error("hg_utoprim_batch: %s", hipGetErrorString(e)); return 1; }
+  return 0;
+}
 
 extern "C" int hg_c2p_fallbacks(int64_t *nfallback) {
   if (!nfallback) { set_error("hg_c2p_fallbacks: null argument"); return 1; }
   int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
   *nfallback = 0;
-  if (e != hipSuccess || dev < 0 || dev >= 64 || !g_queue[dev]) return 0;
-  unsigned long long h[2];
-  e = hipMemcpy(h, g_queue[dev], sizeof h, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) { set_error("hg_c2p_fallbacks: %s", hipGetErrorString(e)); return 1; }
-  *nfallback = (int64_t)h[1];
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  *nfallback = (int64_t)g_last_fallbacks[dev].load();
   return 0;
 }

@@ -421,9 +421,15 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
   // the directions are independent (own interface states, fluxes and ndt
   // slot): j and k on the side streams, i on s, joined before the EMFs.
   // A profile runs them one after another so that every interval holds one
-  // kernel class; so does a graph capture (the capture of this fork onto the
-  // side streams crashed the HIP runtime, so it is not used there).
-  const bool conc = c->tune.concurrent && !c->prof_on && !c->capturing;
+  // kernel class; so does any stream capture, hg_graph_create's or a
+  // caller's: the fork records the context's own events onto its own side
+  // streams, which a capture would pull into the graph, and that path is
+  // kept out of captures (a graph runs its grids side by side instead).
+  bool conc = c->tune.concurrent && !c->prof_on && !c->capturing;
+  if (conc) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) conc = false;
+  }
   hipStream_t ds[4] = {nullptr, s, s, s};
   if (conc) {
     (void)hipEventRecord(c->ev_fork, s);
@@ -678,6 +684,9 @@ extern "C" int hg_step(hg_ctx *c, double *dt, int64_t *nfail, int64_t *iters, vo
   if (need_geometry(c, "hg_step")) return 1;
   if (!dt) { set_error("hg_step: null dt"); return 1; }
   hipStream_t s = (hipStream_t)stream;
+  // work the caller queued on s (hg_step_async) still reads dt and adds to
+  // the counters: let it finish before they are set
+  HIPCHECK(hipStreamSynchronize(s));
   if (hg_set_dt(c, *dt)) return 1;
   HIPCHECK(hipMemset(c->counters, 0, 2 * sizeof(unsigned long long)));
   enqueue_step(c, s);
@@ -688,24 +697,22 @@ extern "C" int hg_step(hg_ctx *c, double *dt, int64_t *nfail, int64_t *iters, vo
   return hg_get_counters(c, nfail, iters);
 }
 
-extern "C" int hg_utoprim_batch(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
-                                const double *gdet, const double *gcovpert, double *pr, int32_t *pflag,
-                                int32_t *iters, int mode, void *stream) {
-  if (n < 0) { set_error("hg_utoprim_batch: n < 0"); return 1; }
-  if (n == 0) return 0;
-  if (!U || !gcov || !gcon || !gdet || !gcovpert || !pr || !pflag) {
-    set_error("hg_utoprim_batch: null argument");
+// the final substep's inversions of the last step (advance.c:193-201): the
+// midpoint scheme inverts Uf (kept in uf) with the guess pb = pk
+extern "C" int hg_dump_inversions(hg_ctx *c, double *U, double *gcov, double *gcon, double *gdet, double *gcovpert,
+                                  double *guess, void *stream) {
+  if (need_geometry(c, "hg_dump_inversions")) return 1;
+  if (!U || !gcov || !gcon || !gdet || !gcovpert || !guess) {
+    set_error("hg_dump_inversions: null argument");
+    return 1;
+  }
+  if (c->cfg.timeorder != 2 || c->ucons || !c->unew_in_uf) {
+    set_error("hg_dump_inversions: needs a step of the midpoint (TIMEORDER 2) standard scheme since the "
+              "primitives were set");
     return 1;
   }
   hipStream_t s = (hipStream_t)stream;
-  if (mode == HG_C2P_FP64) {
-    launch_c2p_batch(n, gam, U, gcov, gcon, gdet, gcovpert, pr, pflag, iters, s);
-  } else if (mode == HG_C2P_MIXED) {
-    if (launch_c2p_batch_mixed(n, gam, U, gcov, gcon, gdet, gcovpert, pr, pflag, iters, s)) return 1;
-  } else {
-    set_error("hg_utoprim_batch: unknown mode %d", mode);
-    return 1;
-  }
+  launch_gather_inversions(c->uf, c->pk, c->geo, c->d, U, gcov, gcon, gdet, gcovpert, guess, s);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(s));
   return 0;

@@ -89,7 +89,18 @@ void launch_diag_faces(hg_ctx *c, const double *prim, const double *pglob, hipSt
 void launch_diag_stage(hg_ctx *c, int stage, hipStream_t s);
 // fixup_checksolution + fixup_utoprim on bounded primitives p (kernels only)
 void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s);
+// batched inversion (hg_c2p_batch.hip).  The mixed path's scratch holds
+// C2P_NCTR counters (work queue, list length, list work queue) and a list of
+// n 32-bit zone indices: c2p_scratch_bytes(n) bytes.
+constexpr int C2P_WORK = 0, C2P_LISTLEN = 1, C2P_LISTWORK = 2, C2P_NCTR = 3;
+size_t c2p_scratch_bytes(int64_t n);
+int launch_c2p_batch(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
+                     const double *gdet, double *pr, int *pflag, int *iters, bool refined, hipStream_t s);
 int launch_c2p_batch_mixed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
-                           const double *gdet, const double *gcovpert, double *pr, int *pflag, int *iters,
-                           hipStream_t s);
+                           const double *gdet, double *pr, int *pflag, int *iters, void *scratch, hipStream_t s);
+void launch_c2p_reset(void *scratch, hipStream_t s);
+// the zones a mixed or learned pass listed, inverted by HG_C2P_REFINED
+int launch_c2p_listed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
+                      const double *gdet, double *pr, int *pflag, int *iters, void *scratch, hipStream_t s);
+void c2p_note_fallbacks(long long nfb);
 }  // namespace hg

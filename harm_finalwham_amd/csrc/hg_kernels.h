@@ -56,9 +56,7 @@ void launch_update(const double *pi, const double *pb, double *pf, double *ubuf,
                    unsigned long long *counters, int8_t *pflag, int *err, const Tune &tu, hipStream_t s);
 void launch_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe, hipStream_t s);
 void launch_reset_ndt(unsigned long long *ndt_bits, hipStream_t s);
-void launch_c2p_batch(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
-                      const double *gdet, const double *gcovpert, double *pr, int *pflag, int *iters,
-                      hipStream_t s);
+
 
 // WENO5BND (hg_weno.hip): the per-zone dP/P inputs (2 x spl doubles in aux),
 // then the interface states along dir for every zone; pleft/pright have the
@@ -72,4 +70,9 @@ void launch_weno(int dir, const double *p, const double *aux, double *pleft, dou
 // initial time step over every zone (advance.c:957 set_dt), folded into *out
 void launch_set_dt(const double *p, const double *geo, const Dims &d, double gam, double cour, const double *dx,
                    unsigned long long *out, int *err, hipStream_t s);
+// the conserved states, zone-centre metric and guesses of a substep's
+// inversions as a batch in hg_utoprim_batch's layout
+void launch_gather_inversions(const double *ubuf, const double *pb, const double *geo, const Dims &d, double *U,
+                              double *gcov, double *gcon, double *gdet, double *gcovpert, double *guess,
+                              hipStream_t s);
 }  // namespace hg

@@ -7,7 +7,6 @@
 //   k_emf              flux-CT corner EMFs (fluxct.c:10)
 //   k_update           conserved update + source + Noble 2D + fixup1zone (advance.c:58)
 //   k_dt               time-step update (advance.c:231, step_ch.c:243)
-//   k_c2p_batch        batched inversion (utoprimgen.c:23)
 //   k_set_dt           initial time step of the set-up (advance.c:957)
 #include <hip/hip_runtime.h>
 
@@ -922,37 +921,46 @@ __global__ void k_reset_ndt(unsigned long long *ndt_bits) {
 }
 
 // ---------------------------------------------------------------------------
-// batched inversion, structure-of-arrays over n zones
+// the inversions of a substep as a batch (hg_dump_inversions): for every
+// active zone t = i + n1 (j + n2 k), the conserved state it inverted, the
+// zone-centre metric and the guess, in hg_utoprim_batch's layout
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_c2p_batch(int64_t n, double gam, const double *__restrict__ U,
-                                                   const double *__restrict__ gcov, const double *__restrict__ gcon,
-                                                   const double *__restrict__ gdet,
-                                                   const double *__restrict__ gcovpert, double *__restrict__ pr,
-                                                   int *__restrict__ pflag, int *__restrict__ iters) {
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_gather_inversions(const double *__restrict__ ubuf,
+                                                           const double *__restrict__ pb,
+                                                           const double *__restrict__ geo, Dims d,
+                                                           double *__restrict__ U, double *__restrict__ gcov,
+                                                           double *__restrict__ gcon, double *__restrict__ gdet,
+                                                           double *__restrict__ gcovpert,
+                                                           double *__restrict__ guess) {
+  const int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= n) return;
-  Geom G;
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      G.gcov[a][c] = gcov[(a * 4 + c) * n + t];
-      G.gcon[a][c] = gcon[(a * 4 + c) * n + t];
-    }
-  G.gpert0 = gcovpert[t];
-  G.g = gdet[t];
-  double u[NPR], p[NPR];
+  const int i = (int)(t % d.n1);
+  const int64_t r = t / d.n1;
+  const int j = (int)(r % d.n2), k = (int)(r / d.n2);
+  const int64_t z = d.idx(i, j, k);
+  const int64_t P = d.spl;
 #pragma unroll
   for (int pl = 0; pl < NPR; pl++) {
-    u[pl] = U[pl * n + t];
-    p[pl] = pr[pl * n + t];
+    U[pl * n + t] = ubuf[z + pl * P];
+    guess[pl * n + t] = pb[z + pl * P];
   }
-  int it = 0;
-  int flag = utoprimgen_fp64(u, G, gam, p, it);
+  const double *b = geo + (int64_t)GL_CENT * GCOMP * d.gplane + d.gidx(i, j);
 #pragma unroll
-  for (int pl = 0; pl < NPR; pl++) pr[pl * n + t] = p[pl];
-  pflag[t] = flag;
-  if (iters) iters[t] = it;
+  for (int c = 0; c < 16; c++) {
+    gcov[c * n + t] = b[c * d.gplane];
+    gcon[c * n + t] = b[(16 + c) * d.gplane];
+  }
+  gcovpert[t] = b[32 * d.gplane];
+  gdet[t] = b[33 * d.gplane];
+}
+
+void launch_gather_inversions(const double *ubuf, const double *pb, const double *geo, const Dims &d, double *U,
+                              double *gcov, double *gcon, double *gdet, double *gcovpert, double *guess,
+                              hipStream_t s) {
+  const int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
+  hipLaunchKernelGGL(k_gather_inversions, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ubuf, pb, geo, d, U,
+                     gcov, gcon, gdet, gcovpert, guess);
 }
 
 // ---------------------------------------------------------------------------
@@ -1110,13 +1118,6 @@ void launch_reset_ndt(unsigned long long *ndt_bits, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_ndt, dim3(1), dim3(64), 0, s, ndt_bits);
 }
 
-void launch_c2p_batch(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
-                      const double *gdet, const double *gcovpert, double *pr, int *pflag, int *iters,
-                      hipStream_t s) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_c2p_batch, dim3(nblocks(n, 256)), dim3(256), 0, s, n, gam, U, gcov, gcon, gdet, gcovpert,
-                     pr, pflag, iters);
-}
 
 }  // namespace hg
 

@@ -15,9 +15,11 @@
 //                   units (FP32), per-block weight gradients added to grad[]
 //   k_mlp_adam      Adam update of the FP32 weights
 //   k_mlp_refresh   BF16 copies of the hidden-layer weights
-//   k_c2p_learned   the FP64 inversion from the better of the learned and
-//                   the plain start; a learned start that does not converge
-//                   falls back to the reference path from the plain guess
+//   k_c2p_learned   the mixed-precision inversion (FP32 Newton, polish,
+//                   double-double primitives) from the better of the learned
+//                   and the plain start; a zone that does not converge joins
+//                   the list that the reference path (HG_C2P_REFINED)
+//                   inverts from the plain guess
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -25,6 +27,7 @@
 
 #include "harmgpu.h"
 #include "hg_c2p.h"
+#include "hg_c2p_io.h"
 #include "hg_host.h"
 
 using namespace hg;
@@ -74,25 +77,6 @@ __device__ __forceinline__ int setup_zone(const double *U, const Geom &G, double
   return c2p_setup(Ut, G, guess, gam, c, W_last);
 }
 
-__device__ __forceinline__ void load_zone(int64_t n, int64_t t, const double *U, const double *gcov,
-                                          const double *gcon, const double *gdet, const double *pr, Geom &G,
-                                          double *u, double *p) {
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-      G.gcov[a][b] = gcov[(a * 4 + b) * n + t];
-      G.gcon[a][b] = gcon[(a * 4 + b) * n + t];
-    }
-  G.gpert0 = 0.;
-  G.g = gdet[t];
-#pragma unroll
-  for (int pl = 0; pl < NPR; pl++) {
-    u[pl] = U[pl * n + t];
-    p[pl] = pr[pl * n + t];
-  }
-}
-
 // W = (rho + gam u) (1 + u~^2) of primitives p
 __device__ __forceinline__ double W_of_prims(const Geom &G, const double *p, double gam) {
   double utsq = 0.;
@@ -116,7 +100,7 @@ __global__ void __launch_bounds__(256) k_mlp_features(int64_t n, double gam, con
   if (t >= n) return;
   Geom G;
   double u[NPR], p[NPR], Ug[NPR];
-  load_zone(n, t, U, gcov, gcon, gdet, guess, G, u, p);
+  load_batch_zone(n, t, U, gcov, gcon, gdet, guess, G, u, p);
   C2PInv c;
   double W_last, alpha;
   int code = setup_zone(u, G, gam, p, c, W_last, alpha, Ug);
@@ -388,46 +372,21 @@ __global__ void k_mlp_refresh(const float *__restrict__ w, __bf16 *__restrict__ 
 
 // ---------------------------------------------------------------------------
 // inversion with the learned start.  The residual of the Newton system is
-// evaluated at the plain start (the reference's W from the previous
+// evaluated in FP64 at the plain start (the reference's W from the previous
 // primitives) and at the learned one, W_guess exp(y); the iteration starts
-// from the smaller.  From the plain start it is the reference iteration of
-// utoprimgen_fp64 (same arithmetic, same result bits).  From the learned
-// start, a zone that does not reach the reference tolerance within
-// LEARNED_MAX_ITER steps, or meets a non-finite value, is inverted again by
-// utoprimgen_fp64 from the plain guess.
+// from the smaller.  From there it is the mixed-precision inversion of
+// HG_C2P_MIXED (FP32 Newton, FP64 and double-double polish, double-double
+// primitives); a zone that does not converge joins the list that
+// HG_C2P_REFINED inverts from the plain guess.  counts[0] counts the zones
+// solved from the learned start.
 // ---------------------------------------------------------------------------
-constexpr int LEARNED_MAX_ITER = 30;
-
-// Newton loop of utoprimgen_fp64 from (x0, x1) whose first step (dx0, dx1,
-// f, df) is already evaluated; `iters` counts evaluations
-__device__ __forceinline__ void newton_continue(const C2PInv &c, double igr, double &x0, double &x1, double dx0,
-                                                double dx1, double &f, double &df, int max_iter, int &iters,
-                                                double &errx) {
-  const double gsq_big = GAMMAFAIL * GAMMAFAIL;
-  const double vsq_big = (gsq_big - 1.0) / (gsq_big);
-  int n_iter = 0;
-  while (true) {
-    double xo0 = x0;
-    x0 += dx0;
-    x1 += dx1;
-    errx = (x0 == 0.) ? fabs(dx0) : fabs(dx0 / x0);
-    x0 = fabs(x0);
-    x0 = (x0 / c.w > gsq_big) ? xo0 : x0;
-    x1 = (x1 > 1.) ? vsq_big : x1;
-    bool stop = (fabs(errx) <= NEWT_TOL) || (n_iter >= max_iter - 1);
-    n_iter++;
-    if (stop) break;
-    iters++;
-    newton_dx(c, igr, x0, x1, dx0, dx1, f, df);
-  }
-}
-
 __global__ void __launch_bounds__(64) k_c2p_learned(int64_t n, double gam, const double *__restrict__ U,
                                                     const double *__restrict__ gcov,
                                                     const double *__restrict__ gcon,
-                                                    const double *__restrict__ gdet,
-                                                    const double *__restrict__ gcovpert, double *__restrict__ pr,
+                                                    const double *__restrict__ gdet, double *__restrict__ pr,
                                                     int *__restrict__ pflag, int *__restrict__ iters, MlpDev m,
+                                                    unsigned long long *__restrict__ ctr,
+                                                    uint32_t *__restrict__ list,
                                                     unsigned long long *__restrict__ counts) {
   __shared__ float xin[64][NIN];
   __shared__ float h[16][NH];
@@ -441,8 +400,7 @@ __global__ void __launch_bounds__(64) k_c2p_learned(int64_t n, double gam, const
   double W_last = 1., alpha = 1.;
   int code = 1;
   if (live) {
-    load_zone(n, t, U, gcov, gcon, gdet, pr, G, u, p);
-    G.gpert0 = gcovpert[t];
+    load_batch_zone(n, t, U, gcov, gcon, gdet, pr, G, u, p);
     code = setup_zone(u, G, gam, p, c, W_last, alpha, Ug);
     float x[NIN];
     features(c, W_last, alpha, gam, x);
@@ -457,71 +415,46 @@ __global__ void __launch_bounds__(64) k_c2p_learned(int64_t n, double gam, const
     forward16(m, (const float(*)[NIN]) & xin[16 * g16][0], h, &yo[16 * g16]);
     __syncthreads();
   }
-  if (!live) return;
   const double igr = (gam - 1.0) / gam;
   const double gsq_big = GAMMAFAIL * GAMMAFAIL;
   const double vsq_big = (gsq_big - 1.0) / (gsq_big);
-  int it = 0, flag;
-  bool learned = false, ref_done = false;
-  double x0 = 0., x1 = 0., errx = 1., f = 1., df = 1.;
-  if (code == 0) {
-    // the plain start and its first step
-    double xp0 = fabs(W_last), vp = vsq_of_W(c, W_last);
-    double xp1 = (vp > 1.) ? vsq_big : vp;
-    double dp0, dp1, fp, dfp;
-    newton_dx(c, igr, xp0, xp1, dp0, dp1, fp, dfp);
-    // the learned start and its first step
-    double Wm = W_last * (double)__expf(yo[l]);
-    double xm0 = Wm, vm = vsq_of_W(c, Wm);
-    double xm1 = (vm > 1.) ? vsq_big : vm;
-    double dm0 = 0., dm1 = 0., fm = 0., dfm = 0.;
-    bool cand = isfinite(Wm) && Wm > 0.;
-    if (cand) newton_dx(c, igr, xm0, xm1, dm0, dm1, fm, dfm);
-    cand = cand && isfinite(fm) && isfinite(dm0) && isfinite(dm1) && (fm < fp || !isfinite(fp));
-    it = cand ? 2 : 1;
-    if (cand) {
-      x0 = xm0;
-      x1 = xm1;
-      f = fm;
-      df = dfm;
-      newton_continue(c, igr, x0, x1, dm0, dm1, f, df, LEARNED_MAX_ITER, it, errx);
-      learned = (fabs(errx) <= NEWT_TOL) && isfinite(f) && isfinite(df) && isfinite(x0) && isfinite(x1);
-    } else {
-      // the reference iteration, continued from its own first step
-      x0 = xp0;
-      x1 = xp1;
-      f = fp;
-      df = dfp;
-      newton_continue(c, igr, x0, x1, dp0, dp1, f, df, MAX_NEWT_ITER, it, errx);
-      ref_done = true;
-    }
-  }
-  if (learned || ref_done) {
-    int retval = 0;
-    if (!isfinite(f) || !isfinite(df) || !isfinite(x0) || !isfinite(x1)) retval = 1;
-    else if (fabs(errx) > MIN_NEWT_TOL) retval = 1;
+  bool fb = false, learned = false;
+  if (live) {
 #pragma unroll
     for (int i = B1; i <= B3; i++) p[i] = Ug[i];
-    double out[NPR];
-    bool assign;
-    flag = c2p_finish(c, x0, retval, igr, out, assign);
-    if (assign)
-#pragma unroll
-      for (int i = 0; i < B1; i++) p[i] = out[i];
-  } else {
-    // set-up failure, or the learned start did not converge: the plain
-    // guess and the reference arithmetic
-    int it2 = 0;
-    for (int pl = 0; pl < NPR; pl++) p[pl] = pr[pl * n + t];
-    flag = utoprimgen_fp64(u, G, gam, p, it2);
-    if (code == 0) atomicAdd(&counts[1], 1ull);
-    it += it2;
+    int it = 0, flag = code;
+    if (code == 0) {
+      // the plain start and the learned start, compared by their residuals
+      double x0, x1, dx0, dx1, fp, dfp;
+      newton_start(c, W_last, x0, x1);
+      newton_dx(c, igr, x0, x1, dx0, dx1, fp, dfp);
+      double Wm = W_last * (double)__expf(yo[l]);
+      bool cand = isfinite(Wm) && Wm > 0.;
+      double xm1 = 0., fm = 0., dfm = 0.;
+      if (cand) {
+        double vm = vsq_of_W(c, Wm);
+        xm1 = (vm > 1.) ? vsq_big : vm;
+        double d0, d1;
+        newton_dx(c, igr, Wm, xm1, d0, d1, fm, dfm);
+        cand = isfinite(fm) && isfinite(d0) && isfinite(d1) && (fm < fp || !isfinite(fp));
+      }
+      it = cand ? 2 : 1;
+      if (cand) {
+        x0 = Wm;
+        x1 = xm1;
+      }
+      fb = !mixed_zone(c, igr, x0, x1, p, it, flag);
+      learned = cand && !fb;
+    }
+    if (!fb) {
+      store_batch_zone(n, t, pr, p, code_assigns(flag));
+      pflag[t] = flag;
+      if (iters) iters[t] = it;
+    }
   }
-  if (learned) atomicAdd(&counts[0], 1ull);
-#pragma unroll
-  for (int pl = 0; pl < NPR; pl++) pr[pl * n + t] = p[pl];
-  pflag[t] = flag;
-  if (iters) iters[t] = it;
+  list_append(fb, t, ctr, list);
+  const unsigned long long ml = __ballot(learned);
+  if (l == 0 && ml) atomicAdd(&counts[0], (unsigned long long)__popcll(ml));
 }
 
 }  // namespace
@@ -696,26 +629,42 @@ extern "C" int hg_utoprim_batch_learned(const hg_mlp *m, int64_t n, double gam, 
                                         const double *gcov, const double *gcon, const double *gdet,
                                         const double *gcovpert, double *pr, int32_t *pflag, int32_t *iters,
                                         int64_t *counts, void *stream) {
-  if (!m || n < 0) { set_error("hg_utoprim_batch_learned: bad arguments"); return 1; }
+  (void)gcovpert;
+  if (!m || n < 0 || n > (int64_t)0xFFFFFFFFll) { set_error("hg_utoprim_batch_learned: bad arguments"); return 1; }
   if (n == 0) {
     if (counts) counts[0] = counts[1] = 0;
     return 0;
   }
-  if (!U || !gcov || !gcon || !gdet || !gcovpert || !pr || !pflag) {
+  if (!U || !gcov || !gcon || !gdet || !pr || !pflag) {
     set_error("hg_utoprim_batch_learned: null argument");
     return 1;
   }
   hipStream_t s = (hipStream_t)stream;
-  MLPCHECK(hipMemsetAsync(m->counts, 0, 2 * sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(k_c2p_learned, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, gam, U, gcov, gcon, gdet,
-                     gcovpert, pr, pflag, iters, dev_view(m), m->counts);
-  MLPCHECK(hipGetLastError());
-  MLPCHECK(hipStreamSynchronize(s));
+  // scratch of this call only (see hg_utoprim_batch)
+  void *scr = nullptr;
+  MLPCHECK(hipMallocAsync(&scr, c2p_scratch_bytes(n), s));
+  unsigned long long *ctr = (unsigned long long *)scr;
+  launch_c2p_reset(scr, s);
+  hipError_t e = hipMemsetAsync(m->counts, 0, 2 * sizeof(unsigned long long), s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_c2p_learned, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, gam, U, gcov, gcon, gdet,
+                       pr, pflag, iters, dev_view(m), ctr, (uint32_t *)(ctr + C2P_NCTR), m->counts);
+    if (launch_c2p_listed(n, gam, U, gcov, gcon, gdet, pr, pflag, iters, scr, s)) e = hipErrorUnknown;
+  }
+  unsigned long long h[2] = {0, 0}, len = 0;
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h, m->counts, sizeof h, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&len, ctr + C2P_LISTLEN, sizeof len, hipMemcpyDeviceToHost, s);
+  hipError_t e2 = hipFreeAsync(scr, s);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    set_error("hg_utoprim_batch_learned: %s", hipGetErrorString(e));
+    return 1;
+  }
   if (counts) {
-    unsigned long long h[2] = {0, 0};
-    MLPCHECK(hipMemcpy(h, m->counts, sizeof h, hipMemcpyDeviceToHost));
     counts[0] = (int64_t)h[0];
-    counts[1] = (int64_t)h[1];
+    counts[1] = (int64_t)len;
   }
   return 0;
 }

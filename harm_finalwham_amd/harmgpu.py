@@ -15,7 +15,7 @@ HG_LIM_DONOR, HG_LIM_VANL, HG_LIM_MINM, HG_LIM_MC, HG_LIM_PARA, HG_LIM_WENO5BND 
 HG_NOENOFLUX, HG_ENOFINITEVOLUME = 0, 3
 HG_FLUX_FORCE, HG_FLUX_MUSTA, HG_FLUX_HLLLAXF1 = 2, 3, 4
 HG_FLUX_LAXF, HG_FLUX_HLL = 0, 1
-HG_C2P_FP64, HG_C2P_MIXED = 0, 1
+HG_C2P_FP64, HG_C2P_MIXED, HG_C2P_REFINED = 0, 1, 2
 
 # every symbol include/harmgpu.h declares
 EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geometry_tables",
@@ -27,7 +27,8 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_mlp_create", "hg_mlp_destroy", "hg_mlp_nparams", "hg_mlp_get_params", "hg_mlp_set_params",
            "hg_mlp_features", "hg_mlp_forward", "hg_mlp_gradient", "hg_mlp_train",
            "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag", "hg_diagnostics", "hg_diag_record", "hg_rdump_parse",
-           "hg_rdump_read", "hg_set_conserved", "hg_get_conserved", "hg_rdump_write", "hg_advance_substep"]
+           "hg_rdump_read", "hg_set_conserved", "hg_get_conserved", "hg_rdump_write", "hg_advance_substep",
+           "hg_dump_inversions"]
 HG_PROF_NCLASS = 10
 
 
@@ -131,6 +132,7 @@ def lib():
     L.hg_get_dt.argtypes = [vp, d]
     L.hg_get_counters.argtypes = [vp, i64p, i64p]
     L.hg_utoprim_batch.argtypes = [C.c_int64, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, vp]
+    L.hg_dump_inversions.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
     L.hg_make_c2p_batch.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_uint64, vp, vp, vp, vp, vp,
                                     vp, vp, vp]
     L.hg_last_error.restype = C.c_char_p
@@ -348,6 +350,20 @@ class Grid:
         _check(lib().hg_autotune(self.h, nsteps, C.byref(t), stream))
         return t
 
+    def dump_inversions(self):
+        """The final substep's inversions of the last step as a batch
+        (hg_dump_inversions): a dict of torch float64 tensors in the layout
+        of utoprim_batch (U, gcov, gcon, gdet, gcovpert, p_guess)."""
+        import torch
+        n = self.n1 * self.n2 * self.n3
+        f = dict(dtype=torch.float64, device="cuda")
+        out = dict(U=torch.empty(8, n, **f), gcov=torch.empty(16, n, **f), gcon=torch.empty(16, n, **f),
+                   gdet=torch.empty(n, **f), gcovpert=torch.empty(n, **f), p_guess=torch.empty(8, n, **f))
+        ptr = lambda t: C.c_void_p(t.data_ptr())
+        _check(lib().hg_dump_inversions(self.h, *[ptr(out[k]) for k in ("U", "gcov", "gcon", "gdet", "gcovpert",
+                                                                         "p_guess")], None))
+        return out
+
     def get_dt(self):
         v = np.zeros(1)
         _check(lib().hg_get_dt(self.h, _d(v)))
@@ -411,7 +427,7 @@ def make_c2p_batch(n, gam=4.0 / 3.0, a=0.9375, seed=1, device="cuda"):
 
 
 def c2p_fallbacks():
-    """Zones of the last HG_C2P_MIXED batch that took the FP64 path."""
+    """Zones of the last HG_C2P_MIXED batch inverted by the HG_C2P_REFINED fallback."""
     v = C.c_int64()
     _check(lib().hg_c2p_fallbacks(C.byref(v)))
     return v.value

@@ -256,20 +256,49 @@ int hg_rdump_parse(const char *path, int n1, int n2, int n3, double *p_active, d
 int hg_rdump_read(hg_ctx *ctx, const char *path, hg_rdump_info *info, double *unew, void *stream);
 int hg_rdump_write(hg_ctx *ctx, const char *path, const char *header, void *stream);
 
-/* Batched Noble-2D inversion on device memory (utoprimgen.c:23).  All
-   arrays are structure-of-arrays over n zones:
-     U[8][n] evolved conserved variables (with gdet), gcov[16][n], gcon[16][n],
-     gdet[n], gcovpert[n] (the tt component), pr[8][n] guess in / solution out,
-     pflag[n] failure code out, iters[n] Newton iterations out (may be NULL).
-   mode: HG_C2P_FP64 (reference arithmetic) or HG_C2P_MIXED (FP64 set-up and
-   metric contractions, FP32 Newton to |dW/W| <= 2e-6, FP64 Newton polish to
-   the reference tolerance and FP64 post-solve; any zone the FP32 phase does
-   not settle is inverted again by the FP64 path, so failure codes match). */
+/* Batched Noble-2D inversion on device memory (utoprimgen.c:23 for each of
+   n independent zones).  All arrays are structure-of-arrays over n zones:
+     U[8][n] evolved conserved variables (with gdet), gcov[16][n], gcon[16][n]
+     (symmetric: only the upper triangles, rows 4a+b with a <= b, are read),
+     gdet[n], gcovpert[n] (the tt component; not read), pr[8][n] guess in /
+     solution out (the inversion reads the guess's u and u~^i; it writes the
+     field, and rho, u, u~^i where the reference assigns them: codes 0, 9, 10,
+     11), pflag[n] failure code out, iters[n] Newton steps out (may be NULL);
+     n < 2^32.
+   mode:
+     HG_C2P_FP64     the reference arithmetic: same iterates, codes, counts.
+     HG_C2P_REFINED  HG_C2P_FP64, then for each solved zone the root of the
+                     same FP64 system converged (Newton steps with the
+                     residuals in double-double arithmetic) and the primitives
+                     formed from it in double-double: the reference's codes
+                     and counts (plus the polish steps), values within ~1e-13
+                     of the exact root.  No counterpart in the reference (its
+                     EXTRA_NEWT_ITER, u2p_defs.h:77, continues the FP64
+                     iteration, which cannot go below its rounding noise).
+     HG_C2P_MIXED    the reference's FP64 set-up and start, the Newton
+                     iteration in FP32 to |dW/W| <= 2e-6, then the polish and
+                     primitives of HG_C2P_REFINED; a persistent work-queue
+                     kernel.  Zones whose FP32 phase or polish does not
+                     converge are inverted by HG_C2P_REFINED from the guess.
+                     Allocates its scratch (4 n + 24 bytes) stream-ordered on
+                     the given stream. */
 #define HG_C2P_FP64 0
 #define HG_C2P_MIXED 1
+#define HG_C2P_REFINED 2
 int hg_utoprim_batch(int64_t n, double gam, const double *U, const double *gcov,
                      const double *gcon, const double *gdet, const double *gcovpert,
                      double *pr, int32_t *pflag, int32_t *iters, int mode, void *stream);
+
+/* The inversions of the last step's final substep (advance.c:193-201,
+   Utoprimgen of Uf with the guess pb) as a batch for hg_utoprim_batch: for
+   each active zone t = i + n1 (j + n2 k), U[8][n] the conserved state it
+   inverted, gcov/gcon[16][n], gdet[n] and gcovpert[n] at the zone centre,
+   guess[8][n] the substep's pb; n = n1 n2 n3, device memory.  The midpoint
+   scheme (timeorder 2) without the finite-volume step, after hg_step or
+   hg_step_async.  No counterpart in the reference (the learned start's
+   training and evaluation data from the step itself). */
+int hg_dump_inversions(hg_ctx *ctx, double *U, double *gcov, double *gcon, double *gdet, double *gcovpert,
+                       double *guess, void *stream);
 
 /* Device-side generator of the batched-inversion benchmark distribution
    (see hg_utoprim_batch for the layouts): KS metric of spin a at r, theta,
@@ -279,8 +308,8 @@ int hg_make_c2p_batch(int64_t n, double gam, double a, uint64_t seed, double *U,
                       double *p_guess, void *stream);
 
 /* Number of zones of the last HG_C2P_MIXED batch on the current device that
-   were inverted by the FP64 path (the FP32 phase did not converge or the FP64
-   polish needed more than 4 steps).  No counterpart in the reference. */
+   were inverted by the HG_C2P_REFINED path (the FP32 phase or the polish did
+   not converge).  No counterpart in the reference. */
 int hg_c2p_fallbacks(int64_t *nfallback);
 
 /* Learned initial guess of the Noble-2D Newton iteration, no counterpart in
@@ -320,16 +349,17 @@ int hg_mlp_gradient(hg_mlp *m, int64_t n, const float *X, const float *Y, float 
 /* `epochs` passes of Adam over minibatches of `batch` (>= 64) samples in order */
 int hg_mlp_train(hg_mlp *m, int64_t n, const float *X, const float *Y, int epochs, int batch, float lr,
                  float *loss, void *stream);
-/* hg_utoprim_batch (HG_C2P_FP64 arithmetic) with a learned start.  The
-   residual of the Newton system is evaluated at the plain start and at
-   W_guess exp(y); the iteration starts from the smaller.  From the plain
-   start the zone follows the reference iteration bit for bit.  From the
-   learned start, a zone whose iteration does not reach the reference
-   tolerance within 30 steps, or meets a non-finite value, is inverted again
-   from its plain guess by the reference path.  Learned zones agree with the
-   plain path to the Newton tolerance, not bit for bit (another start).
-   iters counts residual evaluations, both starts included.  counts (may be
-   NULL) receives [zones solved from the learned start, fallbacks]. */
+/* hg_utoprim_batch's HG_C2P_MIXED inversion with a learned start.  The
+   residual of the Newton system is evaluated in FP64 at the plain start (the
+   reference's, from the guess) and at W_guess exp(y); the FP32 iteration
+   starts from the smaller, then the polish and double-double primitives of
+   HG_C2P_REFINED follow.  A zone whose FP32 phase or polish does not converge
+   is inverted by HG_C2P_REFINED from its plain guess.  The values therefore
+   agree with HG_C2P_REFINED to ~1e-13 (the same converged root), not bit for
+   bit.  iters counts Newton steps and residual evaluations, both starts
+   included.  counts (may be NULL) receives [zones solved from the learned
+   start, zones inverted by the fallback].  Allocates its scratch (4 n + 24
+   bytes) stream-ordered on the given stream. */
 int hg_utoprim_batch_learned(const hg_mlp *m, int64_t n, double gam, const double *U, const double *gcov,
                              const double *gcon, const double *gdet, const double *gcovpert, double *pr,
                              int32_t *pflag, int32_t *iters, int64_t counts[2], void *stream);
@@ -351,9 +381,11 @@ int hg_graph_destroy(hg_graph *g);
    and of the k reconstruction (0 = by line length), and the scheme along i
    (weno_xmarch 0: one lane per point with wavefront shuffles; n > 0: lanes
    across j and k marching along i in n segments).  hg_autotune times nsteps steps per
-   candidate (coordinate descent) from the current state, restores the state,
-   dt and counters, keeps the fastest shapes and returns them in *best (may be
-   NULL).  No counterpart in the reference. */
+   candidate (coordinate descent) from the current state, restores the state
+   (primitives, intermediate states, the finite-volume scheme's carried
+   conserved state, unew, failure codes), dt and counters, keeps the fastest
+   shapes and returns them in *best (may be NULL).  Diagnostics recording is
+   paused during the trials.  No counterpart in the reference. */
 typedef struct {
   int bs_flux, bs_emf, bs_update, bs_invert;
   int weno_xwaves, weno_nseg, weno_nseg3;

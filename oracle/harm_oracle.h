@@ -99,6 +99,11 @@ int or_limit_gamma(double gammamax, double *pr, const or_geom *g);
 void or_c2p_batch(int64_t n, const double *gcov, const double *gcon, const double *gdet,
                   const double *gcovpert, const double *U, double gam, double *pr_inout,
                   int32_t *pflag, int32_t *iters);
+/* the converged root of the reference's FP64 Noble 2D system in long double
+   (or_c2p_root.c) */
+void or_c2p_converged_batch(int64_t n, const double *gcov, const double *gcon, const double *gdet,
+                            const double *gcovpert, const double *U, double gam, double *pr_inout,
+                            int32_t *pflag, int start, const double *start_prims, double *Wv, double *resid);
 void or_primtoU_batch(int64_t n, const double *gcov, const double *gcon, const double *gdet,
                       const double *gcovpert, const double *pr, double gam, double *U);
 int or_inflow_check_r(double *pr, const or_geom *g, int side, double gammamax);

@@ -34,6 +34,7 @@ def lib():
         i64 = C.POINTER(C.c_int64)
         _lib.or_c2p_batch.argtypes = [C.c_int64, d, d, d, d, d, C.c_double, d, i32, i32]
         _lib.or_primtoU_batch.argtypes = [C.c_int64, d, d, d, d, d, C.c_double, d]
+        _lib.or_c2p_converged_batch.argtypes = [C.c_int64, d, d, d, d, d, C.c_double, d, i32, C.c_int, d, d, d]
         _lib.or_step.argtypes = [C.POINTER(OrGrid), d, d, i64, i64]
         _lib.or_fluxcalc.argtypes = [C.POINTER(OrGrid), d, d, d, d, d]
         _lib.or_bound_prim.argtypes = [C.POINTER(OrGrid), d]
@@ -84,6 +85,23 @@ def c2p_batch(gcov, gcon, gdet, gcovpert, U, gam, guess):
                        pflag.ctypes.data_as(C.POINTER(C.c_int32)),
                        iters.ctypes.data_as(C.POINTER(C.c_int32)))
     return pr, pflag, iters
+
+
+def c2p_converged(gcov, gcon, gdet, gcovpert, U, gam, guess, start_prims=None):
+    """The converged root of the reference's FP64 Noble 2D system in long
+    double (or_c2p_root.c): (primitives, reference codes, (W, vsq) of the
+    root, scaled residuals at the root).  start_prims: start the long double
+    Newton from these primitives instead of the reference's solution."""
+    n = U.shape[0]
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (gcov, gcon, gdet, gcovpert, U)]
+    pr = np.ascontiguousarray(guess, dtype=np.float64).copy()
+    pflag = np.zeros(n, np.int32)
+    Wv = np.zeros((n, 2))
+    res = np.zeros((n, 2))
+    sp = np.ascontiguousarray(start_prims if start_prims is not None else np.zeros((1, 8)), dtype=np.float64)
+    lib().or_c2p_converged_batch(n, *[_p(a) for a in arrs], gam, _p(pr), pflag.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 1 if start_prims is not None else 0, _p(sp), _p(Wv), _p(res))
+    return pr, pflag, Wv, res
 
 
 def primtoU_batch(gcov, gcon, gdet, gcovpert, pr, gam):

@@ -37,6 +37,13 @@
  *       inversion codes (see mode_fixup).
  *   diag <outdir> <weno 0|1> <nsteps>
  *       constotal, divbmaxavg and diag_flux after <nsteps> steps (mode_diag).
+ *   long <outdir> <scheme> <nsteps>
+ *       The dt and failure sequences over <nsteps> steps, then constotal()
+ *       and divbmaxavg() of the final state and the final state itself
+ *       (mode_long): the acceptance run of the full-size grids.
+ *
+ * Environment overrides of the set-up (absent: the values below):
+ *   HR_A    black-hole spin a;   HR_RIN  inner radius of the grid, Rin.
  *
  * The initialisation below follows init() in initbase.c:5-176 call for call;
  * the only additions are the parameter overrides marked OVERRIDE.
@@ -149,9 +156,9 @@ static void harness_init(int scheme)
   init_defgrid();
   init_grid();
   /* OVERRIDE: log-r / sin-theta MKS grid, r in [1.3,40], hslope 0.3 (init.fishmon.c:54-76 uses defcoord 9) */
-  a = 0.9375;
+  a = getenv("HR_A") ? atof(getenv("HR_A")) : 0.9375;
   R0 = 0.0;
-  Rin = 1.3;
+  Rin = getenv("HR_RIN") ? atof(getenv("HR_RIN")) : 1.3;
   Rout = 40.0;
   hslope = 0.3;
   defcoord = LOGRSINTH;
@@ -499,6 +506,47 @@ static int mode_diag(const char *outdir, int use_weno, int nsteps)
 }
 
 /*
+ * long mode: <nsteps> steps of the torus in the given scheme, recording dt
+ * and the number of failed inversions after every step (as mode_torus),
+ * then constotal() and divbmaxavg() of the final state (as mode_diag) and
+ * the final state.
+ */
+static int mode_long(const char *outdir, int scheme, int nsteps)
+{
+  SFTYPE vars[NPR], dmax, davg, dv[2];
+  long s1[1] = {NPR}, s2[1] = {2}, shp1[1];
+  FTYPE *dts = malloc(sizeof(FTYPE) * (nsteps + 1));
+  long *fails = malloc(sizeof(long) * (nsteps + 1));
+  int n;
+  mkdir(outdir, 0755);
+  harness_init(scheme);
+  dump_prims(outdir, "p_init", p);
+  dts[0] = dt;
+  fails[0] = 0;
+  for (n = 1; n <= nsteps; n++) {
+    nstroke = 0;
+    step_ch();
+    nstep++;
+    dts[n] = dt;
+    fails[n] = count_failed();
+    fprintf(stderr, "step %d t=%.17g dt=%.17g failed=%ld\n", n, t, dt, fails[n]);
+  }
+  if (constotal(0, vars) >= 1) { fprintf(stderr, "constotal failed\n"); return 3; }
+  divbmaxavg(p, &dmax, &davg);
+  dv[0] = dmax;
+  dv[1] = davg;
+  write_npy(outdir, "cons", "<f8", 1, s1, vars, sizeof(SFTYPE));
+  write_npy(outdir, "divb", "<f8", 1, s2, dv, sizeof(SFTYPE));
+  shp1[0] = nsteps + 1;
+  write_npy(outdir, "dt_seq", "<f8", 1, shp1, dts, sizeof(FTYPE));
+  write_npy(outdir, "fail_seq", "<i8", 1, shp1, fails, sizeof(long));
+  dump_prims(outdir, "p_final", p);
+  free(dts);
+  free(fails);
+  return 0;
+}
+
+/*
  * rdump mode: the torus over <nsteps> RK2 steps, then restart_write(0)
  * (restart.c:206), which writes dumps/rdump-0 in the working directory, and
  * the state as .npy (p with ghosts, unew of the active zones).
@@ -566,6 +614,7 @@ int main(int argc, char **argv)
   if (argc >= 5 && !strcmp(argv[1], "diag")) return mode_diag(argv[2], atoi(argv[3]), atoi(argv[4]));
   if (argc >= 5 && !strcmp(argv[1], "rdump")) return mode_rdump(argv[2], atoi(argv[3]), atoi(argv[4]));
   if (argc >= 4 && !strcmp(argv[1], "rdread")) return mode_rdread(argv[2], atoi(argv[3]));
+  if (argc >= 5 && !strcmp(argv[1], "long")) return mode_long(argv[2], atoi(argv[3]), atoi(argv[4]));
   fprintf(stderr, "usage: %s torus <outdir> <weno> <nsteps> | c2p <in> <out> <gamma> | time <weno> <nsteps> | fixup <in> <outdir> <weno> <finalstep>\n", argv[0]);
   return 1;
 }

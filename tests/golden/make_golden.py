@@ -43,6 +43,19 @@ With --long (and --full):
   f3d_weno_1.npz   3-D torus 256x128x64, WENO5BND + HLL, 1 RK2 step: dt,
         digests (states, geometry) and per-primitive sums only (the states
         are 182 MB)
+
+With --long50:
+  f3d_weno_50.npz  3-D torus 256x128x64, WENO5BND + HLL, 50 RK2 steps (about
+        an hour of one core here): dt and failure sequences, digests of the
+        initial and final states, per-primitive sums of the final state, and
+        the reference's constotal (dump_ener.c:596) and divbmaxavg
+        (dump_ener.c:482) of the final state
+
+With --c5:
+  c5_t2d_128.npz   2-D torus 128x128, WENO5BND + HLL, 10 RK2 steps, at the spin
+        and inner radius of one member of the 48-torus batch (a = 0.53,
+        Rin = 0.9 of the horizon radius; r_pmax 12 is the torus file's own):
+        dt and failure sequences, the final state
 """
 import hashlib
 import os
@@ -52,6 +65,9 @@ import tempfile
 
 import numpy as np
 
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+from c2p_sample import ks_metric, sample_c2p  # noqa: E402
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.abspath(os.path.join(HERE, "..", ".."))
 BUILD = os.path.join(REPO, "oracle", "refbuild", "build_ref.sh")
@@ -59,83 +75,6 @@ REFDIR = os.path.join(REPO, "oracle", "_ref")
 
 C2P_SEED = 20240611
 LOCS = [0, 1, 2, 3, 4, 5, 7]  # FACE1..3, CORN1..3, CENT (global.h:946-953); CORNT unused
-
-
-def ks_metric(r, th, a):
-    """Kerr-Schild metric in (t, r, theta, phi) at BL radius r, theta
-    (metric.c:697 set_gcov_ksmetric) and its closed-form inverse."""
-    n = r.shape[0]
-    s, c = np.sin(th), np.cos(th)
-    s2 = s * s
-    rho2 = r * r + a * a * c * c
-    gcov = np.zeros((n, 4, 4))
-    gcov[:, 0, 0] = -1.0 + 2.0 * r / rho2
-    gcov[:, 0, 1] = gcov[:, 1, 0] = 2.0 * r / rho2
-    gcov[:, 0, 3] = gcov[:, 3, 0] = -2.0 * a * r * s2 / rho2
-    gcov[:, 1, 1] = 1.0 + 2.0 * r / rho2
-    gcov[:, 1, 3] = gcov[:, 3, 1] = -a * s2 * (1.0 + 2.0 * r / rho2)
-    gcov[:, 2, 2] = rho2
-    gcov[:, 3, 3] = s2 * (rho2 + a * a * s2 * (1.0 + 2.0 * r / rho2))
-    delta = r * r - 2.0 * r + a * a
-    gcon = np.zeros((n, 4, 4))
-    gcon[:, 0, 0] = -(1.0 + 2.0 * r / rho2)
-    gcon[:, 0, 1] = gcon[:, 1, 0] = 2.0 * r / rho2
-    gcon[:, 1, 1] = delta / rho2
-    gcon[:, 1, 3] = gcon[:, 3, 1] = a / rho2
-    gcon[:, 2, 2] = 1.0 / rho2
-    gcon[:, 3, 3] = 1.0 / (rho2 * s2)
-    gdet = rho2 * s
-    gpert = np.zeros((n, 4))
-    gpert[:, 0] = 2.0 * r / rho2
-    gpert[:, 1] = 2.0 * r / rho2
-    gpert[:, 2] = gcov[:, 2, 2] - 1.0
-    gpert[:, 3] = gcov[:, 3, 3] - 1.0
-    return gcov, gcon, gdet, gpert
-
-
-def sample_c2p(n, rng, a=0.9375):
-    """The batched-inversion distribution: rho log-U[1e-6,1], u log-U[1e-8,1e2],
-    Lorentz factor log-U[1,50] with isotropic direction, b^2/rho log-U[1e-4,1e3]
-    with isotropic direction, KS metric at r U[1.3,40], theta U(0.01,pi-0.01)."""
-    r = rng.uniform(1.3, 40.0, n)
-    th = rng.uniform(0.01, np.pi - 0.01, n)
-    gcov, gcon, gdet, gpert = ks_metric(r, th, a)
-    rho = 10.0 ** rng.uniform(-6, 0, n)
-    u = 10.0 ** rng.uniform(-8, 2, n)
-    lor = 10.0 ** rng.uniform(0, np.log10(50.0), n)
-    sig = 10.0 ** rng.uniform(-4, 3, n)
-    gs = gcov[:, 1:, 1:]
-
-    def unit(v):
-        nrm = np.sqrt(np.einsum("ni,nij,nj->n", v, gs, v))
-        return v / nrm[:, None]
-
-    ev = unit(rng.normal(size=(n, 3)))
-    ut = np.sqrt(lor * lor - 1.0)[:, None] * ev
-    eb = unit(rng.normal(size=(n, 3)))
-    # b^2 for unit-amplitude B (phys.c:849 get_state), then rescale to b^2/rho = sig
-    alpha = 1.0 / np.sqrt(-gcon[:, 0, 0])
-    beta = gcon[:, 0, 1:] * (alpha * alpha)[:, None]
-    gam_n = np.sqrt(1.0 + np.einsum("ni,nij,nj->n", ut, gs, ut))
-    ucon = np.zeros((n, 4))
-    ucon[:, 0] = gam_n / alpha
-    ucon[:, 1:] = ut - ucon[:, :1] * beta
-    ucov = np.einsum("nij,nj->ni", gcov, ucon)
-    bt = np.einsum("ni,ni->n", eb, ucov[:, 1:])
-    bcon = np.zeros((n, 4))
-    bcon[:, 0] = bt
-    bcon[:, 1:] = (eb + bt[:, None] * ucon[:, 1:]) / ucon[:, :1]
-    bsq1 = np.einsum("ni,nij,nj->n", bcon, gcov, bcon)
-    amp = np.sqrt(sig * rho / bsq1)
-    p = np.zeros((n, 8))
-    p[:, 0] = rho
-    p[:, 1] = u
-    p[:, 2:5] = ut
-    p[:, 5:8] = amp[:, None] * eb
-    guess = p.copy()
-    guess[:, 1] *= 1.0 + 1e-2 * rng.normal(size=n)
-    guess[:, 2:5] *= 1.0 + 1e-2 * rng.normal(size=(n, 3))
-    return gcov, gcon, gdet, gpert, p, guess
 
 
 def edge_cases(rng):
@@ -225,6 +164,50 @@ def main_long(full):
         d = run_torus_long(os.path.join(REFDIR, "harness_" + tag), weno, nsteps, keep)
         np.savez_compressed(os.path.join(HERE, name), **d)
         print(name, os.path.getsize(os.path.join(HERE, name)))
+
+
+def run_long(harness, scheme, nsteps, env=None):
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "out")
+        subprocess.run([harness, "long", out, str(scheme), str(nsteps)], cwd=tmp, check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                       env=dict(os.environ, **(env or {})))
+        d = {f[:-4]: np.load(os.path.join(out, f)) for f in os.listdir(out) if f.endswith(".npy")}
+    d["scheme"] = np.int32(scheme)
+    d["nsteps"] = np.int32(nsteps)
+    return d
+
+
+def main_long50():
+    """The benchmark grid at the configuration's stated length: 50 RK2 steps
+    of the 256x128x64 WENO5BND/HLL torus."""
+    subprocess.run([BUILD, "f3d", "256", "128", "64"], check=True, stdout=subprocess.DEVNULL)
+    d = run_long(os.path.join(REFDIR, "harness_f3d"), 1, 50)
+    out = {k: d[k] for k in ("dt_seq", "fail_seq", "cons", "divb", "scheme", "nsteps")}
+    out["digest_init"] = np.array(state_digest(d["p_init"]))
+    out["digest_final"] = np.array(state_digest(d["p_final"]))
+    out["sum_final"] = d["p_final"].reshape(-1, 8).sum(axis=0)
+    np.savez_compressed(os.path.join(HERE, "f3d_weno_50.npz"), **out)
+
+
+C5_A = 0.53
+
+
+def c5_rin(a):
+    """0.9 of the outer horizon radius r_+ = 1 + sqrt(1 - a^2)."""
+    return 0.9 * (1.0 + np.sqrt(1.0 - a * a))
+
+
+def main_c5():
+    """One 128x128 member of the uneven 48-torus batch, WENO5BND + HLL."""
+    subprocess.run([BUILD, "c5", "128", "128", "1"], check=True, stdout=subprocess.DEVNULL)
+    env = {"HR_A": repr(C5_A), "HR_RIN": repr(float(c5_rin(C5_A)))}
+    d = run_long(os.path.join(REFDIR, "harness_c5"), 1, 10, env)
+    out = {k: d[k] for k in ("dt_seq", "fail_seq", "cons", "divb", "scheme", "nsteps", "p_final")}
+    out["a"] = np.float64(C5_A)
+    out["Rin"] = np.float64(c5_rin(C5_A))
+    out["digest_init"] = np.array(state_digest(d["p_init"]))
+    np.savez_compressed(os.path.join(HERE, "c5_t2d_128.npz"), **out)
 
 
 FIXUP_CODES = np.array([101, 101, 101, 2, 3, 4, 9, 10, 11, 14, 15], np.int32)
@@ -420,6 +403,10 @@ def main():
         return main_rdump()
     if "--diag" in sys.argv:
         return main_diag()
+    if "--long50" in sys.argv:
+        return main_long50()
+    if "--c5" in sys.argv:
+        return main_c5()
     if "--long" in sys.argv:
         return main_long("--full" in sys.argv)
     if "--fixup" in sys.argv:

@@ -59,19 +59,29 @@ def test_graph_rejects_duplicate_contexts():
         harmgpu.Graph([g, g], nsteps=1)
 
 
-def test_autotune_keeps_results():
+@pytest.mark.parametrize("scheme", ["weno", "fv"])
+def test_autotune_keeps_results(scheme):
+    """The tuner's trials leave no trace: primitives, dt, failure codes and,
+    in the finite-volume scheme, the carried conserved state match an
+    untuned twin before and after three more steps."""
     from harm_finalwham_amd import harmgpu
-    print("case autotune", flush=True)
-    a = torus(*SPECS[2])
-    b = torus(*SPECS[2])
+    print("case autotune", scheme, flush=True)
+    a = torus(*SPECS[2], scheme=scheme)
+    b = torus(*SPECS[2], scheme=scheme)
+    for g in (a, b):
+        g.step_async(2)  # a state that is not the initial one
     t = b.autotune(nsteps=1)
     assert t.bs_flux in (64, 128, 256) and t.weno_xwaves in (1, 2, 4)
     assert b.get_dt() == a.get_dt()
     assert np.array_equal(b.get_prims(), a.get_prims())
     for g in (a, b):
         g.step_async(3)
+        assert g.get_dt() > 0
     assert b.get_dt() == a.get_dt()
     assert np.array_equal(b.get_prims(), a.get_prims())
+    assert np.array_equal(b.get_pflag(), a.get_pflag())
+    if scheme == "fv":
+        assert np.array_equal(b.get_conserved(), a.get_conserved())
     bad = harmgpu.HgTune(**{**t.as_dict(), "bs_flux": 100})
     with pytest.raises(harmgpu.HarmError):
         b.set_tune(bad)
