@@ -273,9 +273,10 @@ namespace hg {
 // and finishes as above; a zone whose FP32 phase or polish does not converge
 // is inverted again by the reference path (and polished), by the caller.
 // ---------------------------------------------------------------------------
-constexpr int MIXED_MAX_F32 = 40;       // FP32 Newton steps before falling back
-constexpr float MIXED_TOL_F32 = 2e-6f;  // |dW/W| that ends the FP32 phase
-constexpr int POLISH_MAX_F64 = 3;       // FP64 steps of the polish (to |dW/W| <= 1e-9)
+constexpr int MIXED_MAX_F32 = 40;         // FP32 Newton steps before falling back
+constexpr float MIXED_TOL_F32 = 2e-6f;    // |dW/W| that ends the FP32 phase
+constexpr float MIXED_STALL_F32 = 1e-3f;  // below it, a step that shrank less than 4x ends it too
+constexpr int POLISH_MAX_F64 = 4;         // FP64 steps of the polish (to |dW/W| <= 1e-9)
 constexpr int POLISH_MAX_DD = 2;        // double-double steps (to |dW/W| <= 1e-15)
 
 struct C2PInvF {
@@ -335,6 +336,7 @@ __device__ __forceinline__ bool newton_f32(const C2PInv &c, double igr, double &
   float w0 = (float)x0, w1 = (float)x1;
   bool conv = false;
   int n32 = 0;
+  float errp = 1e30f;
   while (n32 < MIXED_MAX_F32) {
     float d0, d1;
     newton_dx_f32(cf, igrf, w0, w1, d0, d1);
@@ -347,10 +349,15 @@ __device__ __forceinline__ bool newton_f32(const C2PInv &c, double igr, double &
     w1 = (w1 > 1.f) ? vsq_bigf : w1;
     n32++;
     if (!isfinite(w0) || !isfinite(w1)) break;
-    if (err <= MIXED_TOL_F32) {
+    // converged; or near the root (|dW/W| <= MIXED_STALL_F32) with the
+    // steps no longer shrinking quadratically: FP32 rounding of the
+    // residual (amplified by 1 / (1 - vsq)) sets the floor, the FP64 polish
+    // takes it from here
+    if (err <= MIXED_TOL_F32 || (err <= MIXED_STALL_F32 && err > 0.25f * errp)) {
       conv = true;
       break;
     }
+    errp = err;
   }
   iters += n32;
   x0 = (double)w0;

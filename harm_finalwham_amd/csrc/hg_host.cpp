@@ -471,36 +471,6 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
   prof_mark(c, -1, s);
 }
 
-// the field fluxes rewritten from the EMFs (fluxct.c:720-781), on the host,
-// for hg_fluxcalc's diagnostic output only
-static void rewrite_field_fluxes(const Dims &d, double *F1, double *F2, double *F3, const double *e1,
-                                 const double *e2, const double *e3) {
-  int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
-  auto Z = [&](int i, int j, int k) { return (size_t)d.idx(i, j, k); };
-  for (int i = 0; i <= d.n1 - 1 + s1; i++)
-    for (int j = 0; j <= d.n2 - 1 + s2; j++)
-      for (int k = 0; k <= d.n3 - 1 + s3; k++) {
-        size_t z = Z(i, j, k);
-        int ip = s1 ? i + 1 : i, jp = s2 ? j + 1 : j, kp = s3 ? k + 1 : k;
-        size_t P = (size_t)d.spl;
-        if (s1 && j < d.n2 && k < d.n3) {
-          F1[z + B1 * P] = 0.;
-          F1[z + B2 * P] = 0.5 * (e3[z] + e3[Z(i, jp, k)]);
-          F1[z + B3 * P] = -0.5 * (e2[z] + e2[Z(i, j, kp)]);
-        }
-        if (s2 && i < d.n1 && k < d.n3) {
-          F2[z + B1 * P] = -0.5 * (e3[z] + e3[Z(ip, j, k)]);
-          F2[z + B2 * P] = 0.;
-          F2[z + B3 * P] = 0.5 * (e1[z] + e1[Z(i, j, kp)]);
-        }
-        if (s3 && i < d.n1 && j < d.n2) {
-          F3[z + B1 * P] = 0.5 * (e2[z] + e2[Z(ip, j, k)]);
-          F3[z + B2 * P] = -0.5 * (e1[z] + e1[Z(i, jp, k)]);
-          F3[z + B3 * P] = 0.;
-        }
-      }
-}
-
 extern "C" int hg_fluxcalc(hg_ctx *c, double *F1, double *F2, double *F3, double ndt[3], void *stream) {
   if (need_geometry(c, "hg_fluxcalc")) return 1;
   hipStream_t s = (hipStream_t)stream;
@@ -514,14 +484,16 @@ extern "C" int hg_fluxcalc(hg_ctx *c, double *F1, double *F2, double *F3, double
     for (int k = 0; k < 3; k++) memcpy(&ndt[k], &b[k], sizeof(double));
   }
   const Dims &d = c->d;
-  std::vector<double> h[3], e[3];
+  // flux_ct's rewrite of the field fluxes (fluxct.c:720-781), on the device;
+  // the step itself rebuilds them from the EMFs inside k_update
+  launch_fluxct_fields(c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2], d, s);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s));
+  std::vector<double> h[3];
   for (int k = 0; k < 3; k++) {
     h[k].resize((size_t)d.spl * NPR);
-    e[k].resize((size_t)d.spl);
     HIPCHECK(hipMemcpy(h[k].data(), c->F[k], h[k].size() * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(e[k].data(), c->emf[k], e[k].size() * sizeof(double), hipMemcpyDeviceToHost));
   }
-  rewrite_field_fluxes(d, h[0].data(), h[1].data(), h[2].data(), e[0].data(), e[1].data(), e[2].data());
   double *outs[3] = {F1, F2, F3};
   for (int k = 0; k < 3; k++) {
     if (!outs[k]) continue;

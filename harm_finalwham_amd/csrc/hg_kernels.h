@@ -31,6 +31,8 @@ void launch_cons_from_prims(const double *p, const double *geo, double *U, const
 void launch_flux_lr(int dir, const double *pleft, const double *pright, const double *geo, double *F,
                     const Dims &d, double gam, double cour, double dxdir, int fluxmethod,
                     unsigned long long *ndt_bits, int *err, const Tune &tu, bool fv, hipStream_t s);
+void launch_fluxct_fields(double *F1, double *F2, double *F3, const double *e1, const double *e2, const double *e3,
+                          const Dims &d, hipStream_t s);
 void launch_emf(const double *F1, const double *F2, const double *F3, const double *geo, const double *corn,
                 double *e1, double *e2, double *e3, const Dims &d, const Tune &tu, hipStream_t s);
 // per-zone update (-> ubuf) then inversion + fixup (-> pf); ubuf has the

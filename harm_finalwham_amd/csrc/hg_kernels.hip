@@ -1084,6 +1084,45 @@ void launch_emf(const double *F1, const double *F2, const double *F3, const doub
   hipLaunchKernelGGL(k_emf, dim3(nblocks(n, tu.bs_emf)), dim3(tu.bs_emf), 0, s, F1, F2, F3, geo, corn, e1, e2, e3, d);
 }
 
+// fluxct.c:720-781: the field fluxes rebuilt from the corner EMFs, in place
+// (F_i[B_i] = 0, the transverse components the averages of two EMFs), over
+// the faces i = 0..n1, j = 0..n2, k = 0..n3 of the dimensions present
+__global__ void k_fluxct_fields(double *__restrict__ F1, double *__restrict__ F2, double *__restrict__ F3,
+                                const double *__restrict__ e1, const double *__restrict__ e2,
+                                const double *__restrict__ e3, Dims d) {
+  const int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  const int ni = d.n1 + s1, nj = d.n2 + s2, nk = d.n3 + s3;
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ni * nj * nk) return;
+  const int i = (int)(t % ni);
+  const int64_t r = t / ni;
+  const int k = (int)(r % nk), j = (int)(r / nk);
+  const int64_t z = d.idx(i, j, k), P = d.spl;
+  const int64_t zip = z + s1, zjp = z + (s2 ? d.sj : 0), zkp = z + (s3 ? d.sk : 0);
+  if (s1 && j < d.n2 && k < d.n3) {
+    F1[z + B1 * P] = 0.;
+    F1[z + B2 * P] = 0.5 * (e3[z] + e3[zjp]);
+    F1[z + B3 * P] = -0.5 * (e2[z] + e2[zkp]);
+  }
+  if (s2 && i < d.n1 && k < d.n3) {
+    F2[z + B1 * P] = -0.5 * (e3[z] + e3[zip]);
+    F2[z + B2 * P] = 0.;
+    F2[z + B3 * P] = 0.5 * (e1[z] + e1[zkp]);
+  }
+  if (s3 && i < d.n1 && j < d.n2) {
+    F3[z + B1 * P] = 0.5 * (e2[z] + e2[zip]);
+    F3[z + B2 * P] = -0.5 * (e1[z] + e1[zjp]);
+    F3[z + B3 * P] = 0.;
+  }
+}
+
+void launch_fluxct_fields(double *F1, double *F2, double *F3, const double *e1, const double *e2, const double *e3,
+                          const Dims &d, hipStream_t s) {
+  int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  int64_t n = (int64_t)(d.n1 + s1) * (d.n2 + s2) * (d.n3 + s3);
+  hipLaunchKernelGGL(k_fluxct_fields, dim3(nblocks(n, 256)), dim3(256), 0, s, F1, F2, F3, e1, e2, e3, d);
+}
+
 void launch_update(const double *pi, const double *pb, double *pf, double *ubuf, const double *F1,
                    const double *F2, const double *F3, const double *e1, const double *e2, const double *e3,
                    const double *geo, const double *conn, const double *floors, const double *dtp,

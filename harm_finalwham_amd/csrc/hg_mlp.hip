@@ -16,8 +16,10 @@
 //   k_mlp_adam      Adam update of the FP32 weights
 //   k_mlp_refresh   BF16 copies of the hidden-layer weights
 //   k_c2p_learned   the mixed-precision inversion (FP32 Newton, polish,
-//                   double-double primitives) from the better of the learned
-//                   and the plain start; a zone that does not converge joins
+//                   double-double primitives) from the learned start where
+//                   the network predicts a large correction and the start's
+//                   first FP64 Newton step passes a size check, else from
+//                   the plain start; a zone that does not converge joins
 //                   the list that the reference path (HG_C2P_REFINED)
 //                   inverts from the plain guess
 #include <hip/hip_runtime.h>
@@ -35,6 +37,10 @@ using namespace hg;
 namespace {
 
 constexpr int NIN = 8, NH = 64;
+// the learned start: used where |log(W_pred / W_guess)| exceeds LEARNED_MIN_CORR,
+// kept where its first FP64 Newton step is at most LEARNED_MAX_STEP W
+constexpr float LEARNED_MIN_CORR = 0.25f;
+constexpr double LEARNED_MAX_STEP = 0.5;
 // parameter block (FP32): W1[8][64] b1[64] W2[64][64] b2[64] W3[64] b3 mu[8] sd[8]
 constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NIN * NH, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
               OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NH, OFF_MU = OFF_B3 + 1, OFF_SD = OFF_MU + NIN,
@@ -424,25 +430,32 @@ __global__ void __launch_bounds__(64) k_c2p_learned(int64_t n, double gam, const
     for (int i = B1; i <= B3; i++) p[i] = Ug[i];
     int it = 0, flag = code;
     if (code == 0) {
-      // the plain start and the learned start, compared by their residuals
-      double x0, x1, dx0, dx1, fp, dfp;
-      newton_start(c, W_last, x0, x1);
-      newton_dx(c, igr, x0, x1, dx0, dx1, fp, dfp);
+      // the plain start from the previous primitives unless the network
+      // predicts a correction of W by more than a factor exp(LEARNED_MIN_CORR)
+      // (its error on zones the plain start suits is well below that); the
+      // learned start is then checked by the FP64 Newton step its residual
+      // gives, and taken, with that step as the first Newton step, when the
+      // step is finite and moves W by at most LEARNED_MAX_STEP of itself
+      double x0, x1;
       double Wm = W_last * (double)__expf(yo[l]);
-      bool cand = isfinite(Wm) && Wm > 0.;
-      double xm1 = 0., fm = 0., dfm = 0.;
+      bool cand = fabsf(yo[l]) > LEARNED_MIN_CORR && isfinite(Wm) && Wm > 0.;
+      double d0 = 0., d1 = 0., fm = 0., dfm = 0., xm1 = 0.;
       if (cand) {
         double vm = vsq_of_W(c, Wm);
         xm1 = (vm > 1.) ? vsq_big : vm;
-        double d0, d1;
         newton_dx(c, igr, Wm, xm1, d0, d1, fm, dfm);
-        cand = isfinite(fm) && isfinite(d0) && isfinite(d1) && (fm < fp || !isfinite(fp));
+        cand = isfinite(d0) && isfinite(d1) && fabs(d0) <= LEARNED_MAX_STEP * Wm;
       }
-      it = cand ? 2 : 1;
       if (cand) {
-        x0 = Wm;
-        x1 = xm1;
+        // the step, with the reference loop's guards on W and vsq
+        x0 = fabs(Wm + d0);
+        x0 = (x0 / c.w > gsq_big) ? Wm : x0;
+        x1 = xm1 + d1;
+        x1 = (x1 > 1.) ? vsq_big : x1;
+      } else {
+        newton_start(c, W_last, x0, x1);
       }
+      it = (fabsf(yo[l]) > LEARNED_MIN_CORR) ? 1 : 0;  // the learned start's evaluation
       fb = !mixed_zone(c, igr, x0, x1, p, it, flag);
       learned = cand && !fb;
     }

@@ -448,11 +448,31 @@ __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, co
   // before step t = e0-3 the y window holds y(t-6..t+1); points below e0-5 are never read
 #pragma unroll
   for (int a = 4; a < 8; a++) yw[a] = yl[(int64_t)(e0 - 9 + a) * sd];
+  // the loads of step t are issued one step ahead (step t-1), so that their
+  // latency hides behind a step's arithmetic; indices are clamped to the
+  // last ones read (the final step's prefetch is never used)
+  const int tlast = e1 + 3;
+  double ny = yl[(int64_t)(e0 - 1) * sd];
+  double npa = 1., nua = 1., nrho = 0.;
+  if (e0 - 5 >= -2) {
+    npa = a0[(int64_t)(e0 - 5) * sd];
+    nua = a1[(int64_t)(e0 - 5) * sd];
+  }
   // par/upar window before step t: (t-5..t-3); loaded from t-2 on
-  for (int t = e0 - 3; t <= e1 + 3; t++) {
+  for (int t = e0 - 3; t <= tlast; t++) {
+    const double cy = ny, cpa = npa, cua = nua, crho = nrho;
+    {
+      const int tn = (t + 1 < tlast) ? t + 1 : tlast;
+      ny = yl[(int64_t)(tn + 2) * sd];
+      if (tn - 2 >= -2) {
+        npa = a0[(int64_t)(tn - 2) * sd];
+        nua = a1[(int64_t)(tn - 2) * sd];
+      }
+      if (tn - 3 >= e0) nrho = rl[(int64_t)(tn - 3) * sd];
+    }
 #pragma unroll
     for (int a = 0; a < 7; a++) yw[a] = yw[a + 1];
-    yw[7] = yl[(int64_t)(t + 2) * sd];
+    yw[7] = cy;
     double w[3], m0, lof0;
     stage_c(yw + 5, w, m0, lof0);
 #pragma unroll
@@ -475,12 +495,12 @@ __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, co
     UA[0] = UA[1];
     UA[1] = UA[2];
     if (t - 2 >= -2) {
-      PA[2] = a0[(int64_t)(t - 2) * sd];
-      UA[2] = a1[(int64_t)(t - 2) * sd];
+      PA[2] = cpa;
+      UA[2] = cua;
     }
     // E(t-3)
     if (t - 3 >= e0) {
-      double dPi = dp_jump(rl[(int64_t)(t - 3) * sd], PA[0], PA[1], PA[2], UA[0], UA[1], UA[2]);
+      double dPi = dp_jump(crho, PA[0], PA[1], PA[2], UA[0], UA[1], UA[2]);
       double outl, outr;
       stage_e(yw + 2, M0[0], W0[1], W1[0], W2[0], LD[0], LD[1], LD[2], dPi, outl, outr);
       ol[(int64_t)(t - 3) * sd] = outl;

@@ -289,6 +289,7 @@ int hg_utoprim_batch(int64_t n, double gam, const double *U, const double *gcov,
                      const double *gcon, const double *gdet, const double *gcovpert,
                      double *pr, int32_t *pflag, int32_t *iters, int mode, void *stream);
 
+
 /* The inversions of the last step's final substep (advance.c:193-201,
    Utoprimgen of Uf with the guess pb) as a batch for hg_utoprim_batch: for
    each active zone t = i + n1 (j + n2 k), U[8][n] the conserved state it
@@ -349,15 +350,17 @@ int hg_mlp_gradient(hg_mlp *m, int64_t n, const float *X, const float *Y, float 
 /* `epochs` passes of Adam over minibatches of `batch` (>= 64) samples in order */
 int hg_mlp_train(hg_mlp *m, int64_t n, const float *X, const float *Y, int epochs, int batch, float lr,
                  float *loss, void *stream);
-/* hg_utoprim_batch's HG_C2P_MIXED inversion with a learned start.  The
-   residual of the Newton system is evaluated in FP64 at the plain start (the
-   reference's, from the guess) and at W_guess exp(y); the FP32 iteration
-   starts from the smaller, then the polish and double-double primitives of
-   HG_C2P_REFINED follow.  A zone whose FP32 phase or polish does not converge
-   is inverted by HG_C2P_REFINED from its plain guess.  The values therefore
-   agree with HG_C2P_REFINED to ~1e-13 (the same converged root), not bit for
-   bit.  iters counts Newton steps and residual evaluations, both starts
-   included.  counts (may be NULL) receives [zones solved from the learned
+/* hg_utoprim_batch's HG_C2P_MIXED inversion with a learned start.  Where
+   the network predicts |y| > 0.25 (the plain guess off by more than ~28% in
+   W), the Newton step of the FP64 system is evaluated at W_guess exp(y); when
+   it is finite and at most half of W it is taken as the first step and the
+   FP32 iteration continues from there, else the FP32 iteration starts from
+   the plain start (the reference's, from the guess).  The polish and
+   double-double primitives of HG_C2P_REFINED follow.  A zone whose FP32
+   phase or polish does not converge is inverted by HG_C2P_REFINED from its
+   plain guess.  The values therefore agree with HG_C2P_REFINED to ~1e-13
+   (the same converged root), not bit for bit.  iters counts Newton steps and
+   the learned start's evaluation.  counts (may be NULL) receives [zones solved from the learned
    start, zones inverted by the fallback].  Allocates its scratch (4 n + 24
    bytes) stream-ordered on the given stream. */
 int hg_utoprim_batch_learned(const hg_mlp *m, int64_t n, double gam, const double *U, const double *gcov,

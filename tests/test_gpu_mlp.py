@@ -201,8 +201,12 @@ def _torus_dumps(n1, n2, nsteps, seed=7):
 
 def test_dumped_step_inversions_reproduce_the_step():
     """The batch hg_dump_inversions returns is what the step inverted: the
-    FP64 batch inversion of it gives the step's primitives bit for bit in
-    every zone that no floor, gamma limit or fixup touched afterwards."""
+    FP64 batch inversion of it gives the step's primitives in every zone that
+    no floor (fixup.c:1542, rescaletype 0: rho >= 1e-4 r^-1.5, u >= 1e-6
+    r^-2.5) or pole row (POLEDEATH) touched afterwards -- bit for bit where
+    the zone's metric table is exactly symmetric, to 1e-13 where gcov's
+    mirrored entries differ in the last bit (the batch reads the upper
+    triangles, the step the full tables)."""
     import torch
     from harm_finalwham_amd import harmgpu
     print("case dump inversions 32x32", flush=True)
@@ -212,14 +216,29 @@ def test_dumped_step_inversions_reproduce_the_step():
     pr = b["p_guess"].clone()
     pflag = torch.zeros(n, dtype=torch.int32, device="cuda")
     harmgpu.utoprim_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], b["gcovpert"], pr, pflag, None)
+    assert (pflag.cpu().numpy() == 0).all()
     p = g.get_prims()[3:-3, 3:-3, 0, :]                      # active zones [i][j][pl]
     pa = pr.cpu().numpy().T.reshape(32, 32, 8).transpose(1, 0, 2)  # t = i + 32 j -> [i][j][pl]
-    same = np.all(pa == p, axis=2)
-    print("zones reproduced bit for bit: %d of %d" % (same.sum(), same.size), flush=True)
-    assert (pflag.cpu().numpy() == 0).all()
-    assert same.mean() > 0.8
     # the field is copied through both paths
     assert np.array_equal(pa[..., 5:8], p[..., 5:8])
+    geo = harmgpu.mks_geometry_tables(32, 32, 1, 3)
+    r = geo["r_cent"][3:-3, 3:-3]
+    rhoflr = np.maximum(1e-4 * r ** -1.5, 1e-20)
+    uuflr = np.maximum(1e-6 * r ** -2.5, 1e-20)
+    free = (pa[..., 0] > rhoflr * (1 + 1e-9)) & (pa[..., 1] > uuflr * (1 + 1e-9))
+    free[:, 0] = free[:, -1] = False
+    gc = geo["gcov"][3:-4, 3:-4, 6]
+    sym = np.all(gc == np.swapaxes(gc, -1, -2), axis=(-1, -2))
+    same = np.all(pa == p, axis=2)
+    scale = np.maximum(np.abs(p[..., :5]), (p[..., 0] + 4.0 / 3.0 * p[..., 1])[..., None])
+    rel = (np.abs(pa[..., :5] - p[..., :5]) / scale).max(axis=2)
+    print("zones above the floors: %d of %d; symmetric metric among them %d, reproduced bit for bit %d; "
+          "largest relative difference elsewhere %.3g" % (free.sum(), free.size, (free & sym).sum(),
+                                                         (free & sym & same).sum(), rel[free & ~sym].max()),
+          flush=True)
+    assert (free & sym).sum() >= 50
+    assert same[free & sym].all()
+    assert rel[free].max() <= 1e-13
 
 
 def test_learned_start_on_step_inversions():
