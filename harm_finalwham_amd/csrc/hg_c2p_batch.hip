@@ -27,6 +27,7 @@
 
 #include <atomic>
 #include <cmath>
+#include <mutex>
 
 #include "harmgpu.h"
 #include "hg_c2p.h"
@@ -378,6 +379,78 @@ extern "C" int hg_utoprim_batch(int64_t n, double gam, const double *U, const do
   if (e == hipSuccess) e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) { set_error("hg_utoprim_batch: %s", hipGetErrorString(e)); return 1; }
+  return 0;
+}
+
+// hg_utoprim_batch_noalloc: the scratch of HG_C2P_MIXED (counters, then the
+// 32-bit list) in a static device array; host code uses only the per-device
+// address returned by hipGetSymbolAddress, never the array's name
+static __device__ __align__(256) unsigned char scratch[C2P_NCTR * sizeof(unsigned long long) +
+                                                       (size_t)HG_C2P_NOALLOC_MAX_N * sizeof(uint32_t)];
+static void *g_scratch_ptr[64];
+static std::mutex g_scratch_mu;
+
+static int noalloc_scratch(void **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) { set_error("hg_utoprim_batch_noalloc: %s", hipGetErrorString(e)); return 1; }
+  if (dev < 0 || dev >= 64) { set_error("hg_utoprim_batch_noalloc: device %d out of range", dev); return 1; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error("hg_utoprim_batch_noalloc: scratch address: %s", hipGetErrorString(e));
+      return 1;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+extern "C" int hg_utoprim_batch_noalloc(int64_t n, double gam, const double *U, const double *gcov,
+                                        const double *gcon, const double *gdet, const double *gcovpert,
+                                        double *pr, int32_t *pflag, int32_t *iters, int mode, void *stream) {
+  (void)gcovpert;
+  if (check_batch_args("hg_utoprim_batch_noalloc", n, U, gcov, gcon, gdet, pr, pflag)) return 1;
+  if (mode != HG_C2P_FP64 && mode != HG_C2P_MIXED && mode != HG_C2P_REFINED) {
+    set_error("hg_utoprim_batch_noalloc: unknown mode %d", mode);
+    return 1;
+  }
+  if (n > HG_C2P_NOALLOC_MAX_N) {
+    set_error("hg_utoprim_batch_noalloc: n = %lld exceeds the static scratch (%lld zones)", (long long)n,
+              (long long)HG_C2P_NOALLOC_MAX_N);
+    return 1;
+  }
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipSuccess;
+  if (mode != HG_C2P_MIXED) {
+    launch_c2p_batch(n, gam, U, gcov, gcon, gdet, pr, pflag, iters, mode == HG_C2P_REFINED, s);
+  } else {
+    void *scr = nullptr;
+    if (noalloc_scratch(&scr)) return 1;
+    // the counters are cleared by k_c2p_reset inside the launch sequence
+    if (launch_c2p_batch_mixed(n, gam, U, gcov, gcon, gdet, pr, pflag, iters, scr, s)) return 1;
+  }
+  // under stream capture the launches are only recorded: no wait, and the
+  // fallback count (hg_c2p_fallbacks) is not updated
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  e = hipStreamIsCapturing(s, &cs);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess && cs == hipStreamCaptureStatusNone) {
+    if (mode == HG_C2P_MIXED) {
+      unsigned long long ctr[C2P_NCTR] = {0, 0, 0};
+      void *scr = nullptr;
+      if (noalloc_scratch(&scr)) return 1;
+      e = hipMemcpyAsync(ctr, scr, sizeof ctr, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e == hipSuccess) c2p_note_fallbacks((long long)ctr[C2P_LISTLEN]);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  if (e != hipSuccess) { set_error("hg_utoprim_batch_noalloc: %s", hipGetErrorString(e)); return 1; }
   return 0;
 }
 

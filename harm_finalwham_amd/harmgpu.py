@@ -28,7 +28,7 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_mlp_features", "hg_mlp_forward", "hg_mlp_gradient", "hg_mlp_train",
            "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag", "hg_diagnostics", "hg_diag_record", "hg_rdump_parse",
            "hg_rdump_read", "hg_set_conserved", "hg_get_conserved", "hg_rdump_write", "hg_advance_substep",
-           "hg_dump_inversions"]
+           "hg_dump_inversions", "hg_utoprim_batch_noalloc"]
 HG_PROF_NCLASS = 10
 
 
@@ -132,6 +132,7 @@ def lib():
     L.hg_get_dt.argtypes = [vp, d]
     L.hg_get_counters.argtypes = [vp, i64p, i64p]
     L.hg_utoprim_batch.argtypes = [C.c_int64, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, vp]
+    L.hg_utoprim_batch_noalloc.argtypes = [C.c_int64, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, vp]
     L.hg_dump_inversions.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
     L.hg_make_c2p_batch.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_uint64, vp, vp, vp, vp, vp,
                                     vp, vp, vp]
@@ -407,6 +408,17 @@ def utoprim_batch(U, gcov, gcon, gdet, gcovpert, pr, pflag, iters=None, gam=4.0 
     ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
     _check(lib().hg_utoprim_batch(n, gam, ptr(U), ptr(gcov), ptr(gcon), ptr(gdet), ptr(gcovpert), ptr(pr),
                                   ptr(pflag), ptr(iters), mode, stream))
+
+
+def utoprim_batch_noalloc(U, gcov, gcon, gdet, gcovpert, pr, pflag, iters=None, gam=4.0 / 3.0,
+                          mode=HG_C2P_FP64, stream=None):
+    """utoprim_batch through hg_utoprim_batch_noalloc: the same inversion
+    with its scratch in the library's static device array (n <= 2^24; calls
+    that overlap on one GPU race)."""
+    n = U.shape[1]
+    ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+    _check(lib().hg_utoprim_batch_noalloc(n, gam, ptr(U), ptr(gcov), ptr(gcon), ptr(gdet), ptr(gcovpert),
+                                          ptr(pr), ptr(pflag), ptr(iters), mode, stream))
 
 
 def make_c2p_batch(n, gam=4.0 / 3.0, a=0.9375, seed=1, device="cuda"):

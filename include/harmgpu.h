@@ -289,6 +289,20 @@ int hg_utoprim_batch(int64_t n, double gam, const double *U, const double *gcov,
                      const double *gcon, const double *gdet, const double *gcovpert,
                      double *pr, int32_t *pflag, int32_t *iters, int mode, void *stream);
 
+/* hg_utoprim_batch without any allocation.  No counterpart in the reference:
+   for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+   arguments, kernels and results as hg_utoprim_batch.  HG_C2P_MIXED keeps its
+   work-queue counters and compacted fallback list in a static device array
+   of the library (HG_C2P_NOALLOC_MAX_N zones: 4 * 2^24 + 24 bytes, held on
+   every GPU the library runs on until the process ends); larger n is
+   rejected.  Every call on one GPU shares that GPU's array: calls that
+   overlap in time on one GPU (other streams, other threads) race and must
+   not be issued. */
+#define HG_C2P_NOALLOC_MAX_N ((int64_t)1 << 24)
+int hg_utoprim_batch_noalloc(int64_t n, double gam, const double *U, const double *gcov,
+                             const double *gcon, const double *gdet, const double *gcovpert,
+                             double *pr, int32_t *pflag, int32_t *iters, int mode, void *stream);
 
 /* The inversions of the last step's final substep (advance.c:193-201,
    Utoprimgen of Uf with the guess pb) as a batch for hg_utoprim_batch: for
