@@ -429,7 +429,7 @@ def mlp_bench(n_train, n_eval, epochs=8, reps=3):
     res["forward"] = {"ms": tf, "inferences_per_s": n / (tf * 1e-3), "flop_per_inference": MLP_FLOP,
                       "roofline": {"bound": "mfma", "achieved": tflops, "peak": BF16_DENSE_TFLOPS,
                                    "unit": "TFLOP/s", "frac": tflops / BF16_DENSE_TFLOPS}}
-    return res
+    return res, m
 
 
 def train_mlp(m, X, y, epochs, seed=1):
@@ -485,12 +485,15 @@ def c2p_cpu_baseline(n=1 << 22):
     return res
 
 
-def batched_bench(ntori=48, nsteps=10, seed=5):
+def batched_bench(ntori=48, nsteps=10, seed=5, mlp=None):
     """48 independent 2-D tori of mixed sizes, spins a ~ U[0, 0.98] and
     pressure-maximum radii r_pmax ~ U[10, 14] (WENO5BND + HLL + flux-CT +
     Noble 2D): nsteps RK2 steps of all of them in one HIP graph launch, with
     launch shapes autotuned once per grid size, against the same grids stepped
-    one after the other; every grid is compared bit for bit."""
+    one after the other; every grid is compared bit for bit.  First with the
+    reference's FP64 inversion, then, with a trained network mlp, with the
+    mixed-precision inversion from the learned start (hg_set_inversion) in a
+    new graph of the same grids."""
     import math
     import numpy as np
     import torch
@@ -538,13 +541,39 @@ def batched_bench(ntori=48, nsteps=10, seed=5):
     identical = all(gb.get_dt() == ga.get_dt() and np.array_equal(gb.get_prims(), ga.get_prims())
                     for gb, ga in zip(batch, alone))
     graph.close()
+    learned = None
+    if mlp is not None:
+        for g in batch + alone:
+            g.set_inversion(harmgpu.HG_C2P_LEARNED, mlp)
+        graph = harmgpu.Graph(batch, nsteps)
+        graph.launch()
+        for g in alone:
+            g.step_async(nsteps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        graph.launch()
+        torch.cuda.synchronize()
+        t_lg = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for g in alone:
+            g.step_async(nsteps)
+        torch.cuda.synchronize()
+        t_ls = time.perf_counter() - t0
+        same = all(gb.get_dt() == ga.get_dt() and np.array_equal(gb.get_prims(), ga.get_prims())
+                   for gb, ga in zip(batch, alone))
+        graph.close()
+        learned = {"inversion": "mixed precision from the learned start (HG_C2P_LEARNED)",
+                   "graph_zone_updates_per_s": zones * nsteps / t_lg,
+                   "sequential_zone_updates_per_s": zones * nsteps / t_ls,
+                   "graph_speedup": t_ls / t_lg, "vs_fp64_inversion_graph": t_graph / t_lg,
+                   "bitwise_identical_to_single_runs": same}
     res = {"tori": ntori, "steps": nsteps, "zones": zones,
            "sizes": {f"{a}x{b}": sum(1 for sp in specs if sp[0] == (a, b)) for a, b in sizes},
            "graph_zone_updates_per_s": zones * nsteps / t_graph,
            "sequential_zone_updates_per_s": zones * nsteps / t_seq,
            "graph_speedup": t_seq / t_graph, "bitwise_identical_to_single_runs": identical,
            "tuned_shapes": {f"{k[0]}x{k[1]}": v.as_dict() for k, v in tuned.items()},
-           "setup_s": t_setup}
+           "setup_s": t_setup, "learned_inversion": learned}
     for g in batch + alone:
         g.close()
     return res
@@ -649,11 +678,11 @@ def main():
         if not args.no_cpu_baseline:
             c2p["cpu_baseline"] = c2p_cpu_baseline()
         print(f"# con2prim: {json.dumps(c2p)}", file=sys.stderr, flush=True)
-    learned = None
+    learned, net = None, None
     if args.mlp_eval_n > 0:
         if args.c2p_n <= 0:
             g.close()
-        learned = mlp_bench(args.mlp_train_n, args.mlp_eval_n)
+        learned, net = mlp_bench(args.mlp_train_n, args.mlp_eval_n)
         print(f"# learned start: {json.dumps(learned)}", file=sys.stderr, flush=True)
     torus_scheme = None
     if args.torus_scheme_steps > 0:
@@ -664,7 +693,7 @@ def main():
         print(f"# torus scheme: {json.dumps(torus_scheme)}", file=sys.stderr, flush=True)
     batched = None
     if args.batch_tori > 0:
-        batched = batched_bench(args.batch_tori)
+        batched = batched_bench(args.batch_tori, mlp=net)
         print(f"# batched: {json.dumps(batched)}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     out = {

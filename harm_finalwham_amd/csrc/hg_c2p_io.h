@@ -79,4 +79,73 @@ __device__ __forceinline__ bool mixed_zone(const C2PInv &c, double igr, double x
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// features of one inversion from its FP64 set-up (scale free: ratios to D)
+// ---------------------------------------------------------------------------
+// (ratios formed in FP64, logarithms in FP32: the network sees ~16 bits)
+__device__ __forceinline__ void c2p_features(const C2PInv &c, double W_last, double alpha, double gam, float *x) {
+  const double tiny = 1e-300;
+  double D = c.D;
+  double E = fmax(-c.Qdotn, tiny);
+  double iD = 1.0 / D;
+  x[0] = logf((float)(E * iD));                               // energy per rest mass
+  x[1] = log1pf((float)(c.Qtsq / (E * E)));                   // momentum^2 / energy^2
+  x[2] = log1pf((float)(c.Bsq * iD));                         // magnetisation
+  x[3] = (float)(c.QdotBsq / (c.Bsq * c.Qtsq + tiny));        // cos^2 (S, B)
+  x[4] = log10f((float)D);                                    // scale
+  x[5] = (float)alpha;                                        // lapse
+  x[6] = logf((float)(fmax(W_last, tiny) * iD));              // the plain guess
+  x[7] = (float)gam;                                          // adiabatic index
+}
+
+// the FP64 set-up shared by features and the learned inversion
+__device__ __forceinline__ int c2p_setup_zone(const double *U, const Geom &G, double gam, const double *guess,
+                                          C2PInv &c, double &W_last, double &alpha, double *Ug) {
+  double Ut[NPR];
+#pragma unroll
+  for (int i = 0; i < NPR; i++) Ug[i] = U[i] / G.g;
+  Ug[UU] -= Ug[RHO];
+  alpha = 1.0 / sqrt(-G.gcon[0][0]);
+#pragma unroll
+  for (int i = 0; i < NPR; i++) Ut[i] = alpha * Ug[i];
+  return c2p_setup(Ut, G, guess, gam, c, W_last);
+}
+
+// The start of the mixed iteration from the network's prediction y =
+// log(W / W_guess): the plain start from the previous primitives unless
+// |y| > LEARNED_MIN_CORR (the network's error on zones the plain start suits
+// is well below that); the learned start W_guess e^y is then checked by the
+// FP64 Newton step its residual gives, and taken, with that step as the
+// first Newton step (the reference loop's guards on W and vsq), when the
+// step is finite and moves W by at most LEARNED_MAX_STEP of itself.
+// Returns the evaluations spent (0 or 1); *used tells whether the learned
+// start was taken.
+constexpr float LEARNED_MIN_CORR = 0.25f;
+constexpr double LEARNED_MAX_STEP = 0.5;
+__device__ __forceinline__ int learned_start(const C2PInv &c, double igr, double W_last, float y, double &x0,
+                                             double &x1, bool &used) {
+  const double gsq_big = GAMMAFAIL * GAMMAFAIL;
+  const double vsq_big = (gsq_big - 1.0) / (gsq_big);
+  const bool tried = fabsf(y) > LEARNED_MIN_CORR;
+  const double Wm = W_last * (double)__expf(y);
+  bool cand = tried && isfinite(Wm) && Wm > 0.;
+  double d0 = 0., d1 = 0., fm = 0., dfm = 0., xm1 = 0.;
+  if (cand) {
+    const double vm = vsq_of_W(c, Wm);
+    xm1 = (vm > 1.) ? vsq_big : vm;
+    newton_dx(c, igr, Wm, xm1, d0, d1, fm, dfm);
+    cand = isfinite(d0) && isfinite(d1) && fabs(d0) <= LEARNED_MAX_STEP * Wm;
+  }
+  if (cand) {
+    x0 = fabs(Wm + d0);
+    x0 = (x0 / c.w > gsq_big) ? Wm : x0;
+    x1 = xm1 + d1;
+    x1 = (x1 > 1.) ? vsq_big : x1;
+  } else {
+    newton_start(c, W_last, x0, x1);
+  }
+  used = cand;
+  return tried ? 1 : 0;
+}
+
 }  // namespace hg

@@ -130,6 +130,8 @@ extern "C" int hg_ctx_destroy(hg_ctx *c) {
     if (c->ev_join[k]) (void)hipEventDestroy(c->ev_join[k]);
   }
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->lx) (void)hipFree(c->lx);
+  if (c->ly) (void)hipFree(c->ly);
   fixup_free(c);
   diag_free(c);
   for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
@@ -522,9 +524,15 @@ static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, const
     c->dg_faces_valid = finalstep != 0;
   }
   prof_mark(c, HG_PROF_UPDATE, s);
-  launch_update(pi, pb, pf, c->uf, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2], c->geo, c->conn,
-                c->floors, c->dt, rk, finalstep, c->d, f.gam, f.dx, f.gammamax, c->counters, c->pflag, c->err,
-                c->tune, s);
+  launch_update(pi, pb, c->uf, c->F[0], c->F[1], c->F[2], c->emf[0], c->emf[1], c->emf[2], c->geo, c->conn, c->dt,
+                rk, c->d, f.gam, f.dx, c->err, c->tune, s);
+  if (c->c2p_mode == HG_C2P_LEARNED) {
+    // the network's start for every inversion of this substep
+    launch_step_features(c->uf, pb, c->geo, c->d, f.gam, c->lx, s);
+    launch_mlp_forward(c->mlp, (int64_t)c->d.n1 * c->d.n2 * c->d.n3, c->lx, c->ly, s);
+  }
+  launch_invert(c->uf, pb, pf, c->geo, c->floors, rk, finalstep, c->d, f.gam, f.gammamax, c->counters, c->pflag,
+                c->c2p_mode == HG_C2P_MIXED ? 1 : c->c2p_mode == HG_C2P_LEARNED ? 2 : 0, c->ly, c->err, c->tune, s);
   if (finalstep) launch_dt(c->ndt_bits, c->dt, f.defcon, f.safe, s);
   // step_ch.c:127 and :166 call bound_prim twice per substep; the second call
   // is not a repeat of the first: POLEDEATH has zeroed U2 in the rows next to
@@ -693,5 +701,24 @@ extern "C" int hg_dump_inversions(hg_ctx *c, double *U, double *gcov, double *gc
 extern "C" int hg_get_config(const hg_ctx *c, hg_config *out) {
   if (!c || !out) { set_error("hg_get_config: null argument"); return 1; }
   *out = c->cfg;
+  return 0;
+}
+
+// the inversion the step uses (advance.c:193-201); the learned mode's
+// feature and prediction buffers are allocated here, never in the step
+extern "C" int hg_set_inversion(hg_ctx *c, int mode, const hg_mlp *m) {
+  if (!c) { set_error("hg_set_inversion: null context"); return 1; }
+  if (mode != HG_C2P_FP64 && mode != HG_C2P_MIXED && mode != HG_C2P_LEARNED) {
+    set_error("hg_set_inversion: unknown mode %d", mode);
+    return 1;
+  }
+  if (mode == HG_C2P_LEARNED && !m) { set_error("hg_set_inversion: HG_C2P_LEARNED needs a network"); return 1; }
+  if (mode == HG_C2P_LEARNED && !c->lx) {
+    const size_t n = (size_t)c->d.n1 * c->d.n2 * c->d.n3;
+    HIPCHECK(hipMalloc(&c->lx, n * C2P_NFEAT * sizeof(float)));
+    HIPCHECK(hipMalloc(&c->ly, n * sizeof(float)));
+  }
+  c->c2p_mode = mode;
+  c->mlp = mode == HG_C2P_LEARNED ? m : nullptr;
   return 0;
 }

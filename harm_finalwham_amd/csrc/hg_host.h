@@ -31,6 +31,11 @@ struct hg_ctx {
   hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
   bool capturing = false;                    // inside hg_graph_create: no side streams
   double *aux = nullptr;                     // WENO5BND per-zone dP/P inputs
+  // the step's inversion (hg_set_inversion): mode, and for the learned start
+  // the network with its per-zone features and predictions
+  int c2p_mode = HG_C2P_FP64;
+  const hg_mlp *mlp = nullptr;
+  float *lx = nullptr, *ly = nullptr;
   double *uf = nullptr;                      // updated conserved variables before the inversion
   double *geo = nullptr, *corn = nullptr, *conn = nullptr, *floors = nullptr;
   double *dt = nullptr;
@@ -103,4 +108,7 @@ void launch_c2p_reset(void *scratch, hipStream_t s);
 int launch_c2p_listed(int64_t n, double gam, const double *U, const double *gcov, const double *gcon,
                       const double *gdet, double *pr, int *pflag, int *iters, void *scratch, hipStream_t s);
 void c2p_note_fallbacks(long long nfb);
+// the network's predictions y[n] from features X[n][8] (hg_mlp.hip; kernels only)
+void launch_mlp_forward(const hg_mlp *m, int64_t n, const float *X, float *y, hipStream_t s);
+constexpr int C2P_NFEAT = 8;  // features per inversion of the learned start
 }  // namespace hg

@@ -51,11 +51,22 @@ struct RkStage {
                   // replaced by ucum on the last substep; null otherwise
   double cuf0 = 1.0, cuf1 = 0.0;
 };
-void launch_update(const double *pi, const double *pb, double *pf, double *ubuf, const double *F1,
-                   const double *F2, const double *F3, const double *e1, const double *e2, const double *e3,
-                   const double *geo, const double *conn, const double *floors, const double *dtp,
-                   const RkStage &rk, int finalstep, const Dims &d, double gam, const double *dx, double gammamax,
-                   unsigned long long *counters, int8_t *pflag, int *err, const Tune &tu, hipStream_t s);
+void launch_update(const double *pi, const double *pb, double *ubuf, const double *F1, const double *F2,
+                   const double *F3, const double *e1, const double *e2, const double *e3, const double *geo,
+                   const double *conn, const double *dtp, const RkStage &rk, const Dims &d, double gam,
+                   const double *dx, int *err, const Tune &tu, hipStream_t s);
+// the inversion of ubuf with the guess pb, fixup1zone on the final substep
+// -> pf; mode 0: the reference's FP64 iteration, 1: mixed precision to the
+// converged root, 2: the same from the learned start (ylearn[zone], the
+// network's log(W / W_guess) in k_invert's zone order)
+void launch_invert(const double *ubuf, const double *pb, double *pf, const double *geo, const double *floors,
+                   const RkStage &rk, int finalstep, const Dims &d, double gam, double gammamax,
+                   unsigned long long *counters, int8_t *pflag, int mode, const float *ylearn, int *err,
+                   const Tune &tu, hipStream_t s);
+// the learned start's features of the inversions launch_invert will do
+// (8 floats per zone, k_invert's zone order)
+void launch_step_features(const double *ubuf, const double *pb, const double *geo, const Dims &d, double gam,
+                          float *X, hipStream_t s);
 void launch_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe, hipStream_t s);
 void launch_reset_ndt(unsigned long long *ndt_bits, hipStream_t s);
 

@@ -12,6 +12,7 @@
 
 #include "harmgpu.h"
 #include "hg_c2p.h"
+#include "hg_c2p_io.h"
 #include "hg_grid.h"
 #include "hg_kernels.h"
 
@@ -833,14 +834,53 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
   }
 }
 
-template <bool FV>
+// The inversion of one zone in the step's mode: the reference's FP64
+// iteration (INV_FP64), or the mixed-precision path of the batched inversion
+// (FP32 Newton from the plain start, or from the learned start for
+// INV_LEARNED, then the polish to the converged root and double-double
+// primitives) with the converged-root reference inversion (HG_C2P_REFINED)
+// from the plain guess for any zone that path does not settle
+constexpr int INV_FP64 = 0, INV_MIXED = 1, INV_LEARNED = 2;
+template <int MODE>
+__device__ __forceinline__ int invert_zone(const double *Uf, const Geom &G, double gam, double *pr, float ylearn,
+                                           int &iters) {
+  if (MODE == INV_FP64) return utoprimgen_fp64(Uf, G, gam, pr, iters);
+  {
+    double Ug[NPR], p2[NPR], W_last, alpha;
+#pragma unroll
+    for (int i = 0; i < NPR; i++) p2[i] = pr[i];
+    for (int i = B1; i <= B3; i++) p2[i] = Uf[i] / G.g;  // as the set-up's Ug
+    C2PInv c;
+    int code = c2p_setup_zone(Uf, G, gam, p2, c, W_last, alpha, Ug);
+    if (code == 0) {
+      const double igr = (gam - 1.0) / gam;
+      double x0, x1;
+      int it = 0, flag;
+      bool used;
+      if (MODE == INV_LEARNED)
+        it = learned_start(c, igr, W_last, ylearn, x0, x1, used);
+      else
+        newton_start(c, W_last, x0, x1);
+      if (mixed_zone(c, igr, x0, x1, p2, it, flag)) {
+#pragma unroll
+        for (int i = 0; i < NPR; i++) pr[i] = p2[i];
+        iters = it;
+        return flag;
+      }
+    }
+  }
+  bool pol;
+  return utoprimgen_refined(Uf, G, gam, pr, iters, pol);
+}
+
+template <bool FV, int MODE>
 __global__ void __launch_bounds__(256) k_invert(const double *__restrict__ ubuf, const double *__restrict__ pb,
                                                 double *__restrict__ pf, const double *__restrict__ geo,
                                                 const double *__restrict__ floors, int finalstep, Dims d,
                                                 double gam, double gammamax,
                                                 unsigned long long *__restrict__ counters,
                                                 int8_t *__restrict__ pflag, double *__restrict__ ucons,
-                                                int *__restrict__ err) {
+                                                const float *__restrict__ ylearn, int *__restrict__ err) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
   unsigned long long nfail = 0, nit = 0;
@@ -861,7 +901,7 @@ __global__ void __launch_bounds__(256) k_invert(const double *__restrict__ ubuf,
     }
     bool bad = false;
     int iters = 0;
-    int flag = utoprimgen_fp64(Uf, G, gam, pr, iters);
+    int flag = invert_zone<MODE>(Uf, G, gam, pr, MODE == INV_LEARNED ? ylearn[t] : 0.f, iters);
     pflag[z] = (int8_t)flag;
     nit = (unsigned long long)iters;
     nfail = flag ? 1ull : 0ull;
@@ -1123,11 +1163,10 @@ void launch_fluxct_fields(double *F1, double *F2, double *F3, const double *e1, 
   hipLaunchKernelGGL(k_fluxct_fields, dim3(nblocks(n, 256)), dim3(256), 0, s, F1, F2, F3, e1, e2, e3, d);
 }
 
-void launch_update(const double *pi, const double *pb, double *pf, double *ubuf, const double *F1,
-                   const double *F2, const double *F3, const double *e1, const double *e2, const double *e3,
-                   const double *geo, const double *conn, const double *floors, const double *dtp,
-                   const RkStage &rk, int finalstep, const Dims &d, double gam, const double *dx, double gammamax,
-                   unsigned long long *counters, int8_t *pflag, int *err, const Tune &tu, hipStream_t s) {
+void launch_update(const double *pi, const double *pb, double *ubuf, const double *F1, const double *F2,
+                   const double *F3, const double *e1, const double *e2, const double *e3, const double *geo,
+                   const double *conn, const double *dtp, const RkStage &rk, const Dims &d, double gam,
+                   const double *dx, int *err, const Tune &tu, hipStream_t s) {
   int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
   dim3 g(nblocks(n, tu.bs_update)), b(tu.bs_update);
   const double i1 = 1.0 / dx[1], i2 = 1.0 / dx[2], i3 = 1.0 / dx[3];
@@ -1140,13 +1179,77 @@ void launch_update(const double *pi, const double *pb, double *pf, double *ubuf,
   else
     hipLaunchKernelGGL((k_update<false, false>), g, b, 0, s, pi, pb, ubuf, F1, F2, F3, e1, e2, e3, geo, conn, dtp, rk,
                        d, gam, i1, i2, i3, err);
+}
+
+template <int MODE>
+static void launch_invert_mode(const double *ubuf, const double *pb, double *pf, const double *geo,
+                               const double *floors, const RkStage &rk, int finalstep, const Dims &d, double gam,
+                               double gammamax, unsigned long long *counters, int8_t *pflag, const float *ylearn,
+                               int *err, const Tune &tu, hipStream_t s) {
+  int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
   dim3 gi(nblocks(n, tu.bs_invert)), bi(tu.bs_invert);
   if (rk.ucons && finalstep)
-    hipLaunchKernelGGL(k_invert<true>, gi, bi, 0, s, ubuf, pb, pf, geo, floors, finalstep, d, gam, gammamax, counters,
-                       pflag, rk.ucons, err);
+    hipLaunchKernelGGL((k_invert<true, MODE>), gi, bi, 0, s, ubuf, pb, pf, geo, floors, finalstep, d, gam, gammamax,
+                       counters, pflag, rk.ucons, ylearn, err);
   else
-    hipLaunchKernelGGL(k_invert<false>, gi, bi, 0, s, ubuf, pb, pf, geo, floors, finalstep, d, gam, gammamax, counters,
-                       pflag, nullptr, err);
+    hipLaunchKernelGGL((k_invert<false, MODE>), gi, bi, 0, s, ubuf, pb, pf, geo, floors, finalstep, d, gam, gammamax,
+                       counters, pflag, nullptr, ylearn, err);
+}
+
+void launch_invert(const double *ubuf, const double *pb, double *pf, const double *geo, const double *floors,
+                   const RkStage &rk, int finalstep, const Dims &d, double gam, double gammamax,
+                   unsigned long long *counters, int8_t *pflag, int mode, const float *ylearn, int *err,
+                   const Tune &tu, hipStream_t s) {
+  if (mode == INV_MIXED)
+    launch_invert_mode<INV_MIXED>(ubuf, pb, pf, geo, floors, rk, finalstep, d, gam, gammamax, counters, pflag,
+                                  ylearn, err, tu, s);
+  else if (mode == INV_LEARNED)
+    launch_invert_mode<INV_LEARNED>(ubuf, pb, pf, geo, floors, rk, finalstep, d, gam, gammamax, counters, pflag,
+                                    ylearn, err, tu, s);
+  else
+    launch_invert_mode<INV_FP64>(ubuf, pb, pf, geo, floors, rk, finalstep, d, gam, gammamax, counters, pflag,
+                                 ylearn, err, tu, s);
+}
+
+// the learned start's features of the inversions k_invert will do (its zone
+// order; the guess pb with the field of ubuf, as the set-up sees it)
+__global__ void __launch_bounds__(256) k_step_features(const double *__restrict__ ubuf,
+                                                       const double *__restrict__ pb,
+                                                       const double *__restrict__ geo, Dims d, double gam,
+                                                       float *__restrict__ X) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)d.n1 * d.n2 * d.n3) return;
+  int i = (int)(t % d.n1);
+  int64_t r = t / d.n1;
+  int k = (int)(r % d.n3);
+  int j = (int)(r / d.n3);
+  int64_t z = d.idx(i, j, k);
+  const int64_t P = d.spl;
+  Geom G;
+  load_geom(geo, GL_CENT, d.gplane, d.gidx(i, j), G);
+  double Uf[NPR], pr[NPR], Ug[NPR], W_last = 1., alpha = 1.;
+#pragma unroll
+  for (int pl = 0; pl < NPR; pl++) {
+    Uf[pl] = ubuf[z + pl * P];
+    pr[pl] = pb[z + pl * P];
+  }
+  for (int pl = B1; pl <= B3; pl++) pr[pl] = Uf[pl] / G.g;
+  C2PInv c;
+  float x[C2P_NFEAT];
+  if (c2p_setup_zone(Uf, G, gam, pr, c, W_last, alpha, Ug) == 0) {
+    c2p_features(c, W_last, alpha, gam, x);
+  } else {
+#pragma unroll
+    for (int q = 0; q < C2P_NFEAT; q++) x[q] = 0.f;
+  }
+#pragma unroll
+  for (int q = 0; q < C2P_NFEAT; q++) X[t * C2P_NFEAT + q] = x[q];
+}
+
+void launch_step_features(const double *ubuf, const double *pb, const double *geo, const Dims &d, double gam,
+                          float *X, hipStream_t s) {
+  int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
+  hipLaunchKernelGGL(k_step_features, dim3(nblocks(n, 256)), dim3(256), 0, s, ubuf, pb, geo, d, gam, X);
 }
 
 void launch_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe, hipStream_t s) {

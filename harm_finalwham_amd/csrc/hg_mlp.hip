@@ -37,10 +37,6 @@ using namespace hg;
 namespace {
 
 constexpr int NIN = 8, NH = 64;
-// the learned start: used where |log(W_pred / W_guess)| exceeds LEARNED_MIN_CORR,
-// kept where its first FP64 Newton step is at most LEARNED_MAX_STEP W
-constexpr float LEARNED_MIN_CORR = 0.25f;
-constexpr double LEARNED_MAX_STEP = 0.5;
 // parameter block (FP32): W1[8][64] b1[64] W2[64][64] b2[64] W3[64] b3 mu[8] sd[8]
 constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NIN * NH, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
               OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NH, OFF_MU = OFF_B3 + 1, OFF_SD = OFF_MU + NIN,
@@ -50,38 +46,6 @@ constexpr int NTRAIN = OFF_MU;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-// ---------------------------------------------------------------------------
-// features of one inversion from its FP64 set-up (scale free: ratios to D)
-// ---------------------------------------------------------------------------
-// (ratios formed in FP64, logarithms in FP32: the network sees ~16 bits)
-__device__ __forceinline__ void features(const C2PInv &c, double W_last, double alpha, double gam, float *x) {
-  const double tiny = 1e-300;
-  double D = c.D;
-  double E = fmax(-c.Qdotn, tiny);
-  double iD = 1.0 / D;
-  x[0] = logf((float)(E * iD));                               // energy per rest mass
-  x[1] = log1pf((float)(c.Qtsq / (E * E)));                   // momentum^2 / energy^2
-  x[2] = log1pf((float)(c.Bsq * iD));                         // magnetisation
-  x[3] = (float)(c.QdotBsq / (c.Bsq * c.Qtsq + tiny));        // cos^2 (S, B)
-  x[4] = log10f((float)D);                                    // scale
-  x[5] = (float)alpha;                                        // lapse
-  x[6] = logf((float)(fmax(W_last, tiny) * iD));              // the plain guess
-  x[7] = (float)gam;                                          // adiabatic index
-}
-
-// the FP64 set-up shared by features and the learned inversion
-__device__ __forceinline__ int setup_zone(const double *U, const Geom &G, double gam, const double *guess,
-                                          C2PInv &c, double &W_last, double &alpha, double *Ug) {
-  double Ut[NPR];
-#pragma unroll
-  for (int i = 0; i < NPR; i++) Ug[i] = U[i] / G.g;
-  Ug[UU] -= Ug[RHO];
-  alpha = 1.0 / sqrt(-G.gcon[0][0]);
-#pragma unroll
-  for (int i = 0; i < NPR; i++) Ut[i] = alpha * Ug[i];
-  return c2p_setup(Ut, G, guess, gam, c, W_last);
-}
 
 // W = (rho + gam u) (1 + u~^2) of primitives p
 __device__ __forceinline__ double W_of_prims(const Geom &G, const double *p, double gam) {
@@ -109,9 +73,9 @@ __global__ void __launch_bounds__(256) k_mlp_features(int64_t n, double gam, con
   load_batch_zone(n, t, U, gcov, gcon, gdet, guess, G, u, p);
   C2PInv c;
   double W_last, alpha;
-  int code = setup_zone(u, G, gam, p, c, W_last, alpha, Ug);
+  int code = c2p_setup_zone(u, G, gam, p, c, W_last, alpha, Ug);
   float x[NIN];
-  features(c, W_last, alpha, gam, x);
+  c2p_features(c, W_last, alpha, gam, x);
   bool good = (code == 0);
 #pragma unroll
   for (int k = 0; k < NIN; k++) {
@@ -407,9 +371,9 @@ __global__ void __launch_bounds__(64) k_c2p_learned(int64_t n, double gam, const
   int code = 1;
   if (live) {
     load_batch_zone(n, t, U, gcov, gcon, gdet, pr, G, u, p);
-    code = setup_zone(u, G, gam, p, c, W_last, alpha, Ug);
+    code = c2p_setup_zone(u, G, gam, p, c, W_last, alpha, Ug);
     float x[NIN];
-    features(c, W_last, alpha, gam, x);
+    c2p_features(c, W_last, alpha, gam, x);
 #pragma unroll
     for (int k = 0; k < NIN; k++) xin[l][k] = (x[k] - m.w[OFF_MU + k]) / m.w[OFF_SD + k];
   } else {
@@ -422,40 +386,15 @@ __global__ void __launch_bounds__(64) k_c2p_learned(int64_t n, double gam, const
     __syncthreads();
   }
   const double igr = (gam - 1.0) / gam;
-  const double gsq_big = GAMMAFAIL * GAMMAFAIL;
-  const double vsq_big = (gsq_big - 1.0) / (gsq_big);
   bool fb = false, learned = false;
   if (live) {
 #pragma unroll
     for (int i = B1; i <= B3; i++) p[i] = Ug[i];
     int it = 0, flag = code;
     if (code == 0) {
-      // the plain start from the previous primitives unless the network
-      // predicts a correction of W by more than a factor exp(LEARNED_MIN_CORR)
-      // (its error on zones the plain start suits is well below that); the
-      // learned start is then checked by the FP64 Newton step its residual
-      // gives, and taken, with that step as the first Newton step, when the
-      // step is finite and moves W by at most LEARNED_MAX_STEP of itself
       double x0, x1;
-      double Wm = W_last * (double)__expf(yo[l]);
-      bool cand = fabsf(yo[l]) > LEARNED_MIN_CORR && isfinite(Wm) && Wm > 0.;
-      double d0 = 0., d1 = 0., fm = 0., dfm = 0., xm1 = 0.;
-      if (cand) {
-        double vm = vsq_of_W(c, Wm);
-        xm1 = (vm > 1.) ? vsq_big : vm;
-        newton_dx(c, igr, Wm, xm1, d0, d1, fm, dfm);
-        cand = isfinite(d0) && isfinite(d1) && fabs(d0) <= LEARNED_MAX_STEP * Wm;
-      }
-      if (cand) {
-        // the step, with the reference loop's guards on W and vsq
-        x0 = fabs(Wm + d0);
-        x0 = (x0 / c.w > gsq_big) ? Wm : x0;
-        x1 = xm1 + d1;
-        x1 = (x1 > 1.) ? vsq_big : x1;
-      } else {
-        newton_start(c, W_last, x0, x1);
-      }
-      it = (fabsf(yo[l]) > LEARNED_MIN_CORR) ? 1 : 0;  // the learned start's evaluation
+      bool cand;
+      it = learned_start(c, igr, W_last, yo[l], x0, x1, cand);
       fb = !mixed_zone(c, igr, x0, x1, p, it, flag);
       learned = cand && !fb;
     }
@@ -576,6 +515,11 @@ extern "C" int hg_mlp_features(int64_t n, double gam, const double *U, const dou
   MLPCHECK(hipGetLastError());
   MLPCHECK(hipStreamSynchronize(s));
   return 0;
+}
+
+void hg::launch_mlp_forward(const hg_mlp *m, int64_t n, const float *X, float *y, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, X, y, dev_view(m));
 }
 
 extern "C" int hg_mlp_forward(const hg_mlp *m, int64_t n, const float *X, float *y, void *stream) {

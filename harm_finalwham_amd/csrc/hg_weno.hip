@@ -407,10 +407,14 @@ __global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, co
 // DIR 1 marches along i with lane = (j, k): no halo lanes, but every load of
 // a wavefront touches 64 rows (the rows stay in cache over the next steps).
 // ---------------------------------------------------------------------------
+// Block order: the 8 primitive blocks of one (lane block, segment) unit read
+// the same dP/P inputs and densities, so they are dispatched back to back to
+// one XCD (work-groups go to the 8 XCDs round robin by linear index): block
+// b runs on XCD b % 8, takes primitive (b / 8) % 8 of unit (b / 64) * 8 + b % 8.
 template <int DIR>
 __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, const double *__restrict__ aux,
                                                 double *__restrict__ pleft, double *__restrict__ pright, Dims d,
-                                                int nseg, int seglen, int wide) {
+                                                int nseg, int seglen, int wide, int nlb) {
   const int n = (DIR == 1) ? d.n1 : (DIR == 2) ? d.n2 : d.n3;
   // the lane's two transverse indices: a (fastest) and o, over -1..n or,
   // for the finite-volume loops, every ghost row
@@ -420,12 +424,15 @@ __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, co
   const int so = (DIR == 2) ? h3 : (DIR == 3) ? h2 : h3;
   const int ca = ((DIR == 1) ? d.n2 : d.n1) + 2 * sa;
   const int co = ((DIR == 2) ? d.n3 : (DIR == 3) ? d.n2 : d.n3) + 2 * so;
-  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const unsigned slot = blockIdx.x >> 3;
+  const int pl = (int)(slot & 7u);
+  const unsigned unit = (slot >> 3) * 8u + (blockIdx.x & 7u);
+  if (unit >= (unsigned)(nlb * nseg)) return;
+  const int seg = (int)(unit / (unsigned)nlb);
+  const int64_t g = (int64_t)(unit % (unsigned)nlb) * blockDim.x + threadIdx.x;
   if (g >= (int64_t)ca * co) return;
   const int a_ = -sa + (int)(g % ca);
   const int o = -so + (int)(g / ca);
-  const int seg = blockIdx.y % nseg;
-  const int pl = blockIdx.y / nseg;
   const int e0 = -1 + seg * seglen;
   const int e1 = (e0 + seglen - 1 < n) ? e0 + seglen - 1 : n;
   const int64_t zl = (DIR == 1) ? d.idx(0, a_, o) : (DIR == 2) ? d.idx(a_, 0, o) : d.idx(a_, o, 0);
@@ -538,13 +545,17 @@ void launch_weno(int dir, const double *p, const double *aux, double *pleft, dou
     if (nseg > n + 2) nseg = n + 2;
     int seglen = (n + 2 + nseg - 1) / nseg;
     nseg = (n + 2 + seglen - 1) / seglen;  // every segment starts at or before point n
-    dim3 grid((unsigned)((lanes + 255) / 256), (unsigned)(NPR * nseg));
+    // units of (lane block, segment), padded to whole groups of 8, times
+    // the 8 primitives (NPR = 8)
+    const int nlb = (int)((lanes + 255) / 256);
+    const int64_t units = ((int64_t)nlb * nseg + 7) / 8 * 8;
+    dim3 grid((unsigned)(units * NPR));
     if (dir == 1)
-      hipLaunchKernelGGL(k_weno_m<1>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen, wide);
+      hipLaunchKernelGGL(k_weno_m<1>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen, wide, nlb);
     else if (dir == 2)
-      hipLaunchKernelGGL(k_weno_m<2>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen, wide);
+      hipLaunchKernelGGL(k_weno_m<2>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen, wide, nlb);
     else
-      hipLaunchKernelGGL(k_weno_m<3>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen, wide);
+      hipLaunchKernelGGL(k_weno_m<3>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen, wide, nlb);
   }
 }
 

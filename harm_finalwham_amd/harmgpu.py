@@ -15,7 +15,7 @@ HG_LIM_DONOR, HG_LIM_VANL, HG_LIM_MINM, HG_LIM_MC, HG_LIM_PARA, HG_LIM_WENO5BND 
 HG_NOENOFLUX, HG_ENOFINITEVOLUME = 0, 3
 HG_FLUX_FORCE, HG_FLUX_MUSTA, HG_FLUX_HLLLAXF1 = 2, 3, 4
 HG_FLUX_LAXF, HG_FLUX_HLL = 0, 1
-HG_C2P_FP64, HG_C2P_MIXED, HG_C2P_REFINED = 0, 1, 2
+HG_C2P_FP64, HG_C2P_MIXED, HG_C2P_REFINED, HG_C2P_LEARNED = 0, 1, 2, 3
 
 # every symbol include/harmgpu.h declares
 EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geometry_tables",
@@ -28,7 +28,7 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_mlp_features", "hg_mlp_forward", "hg_mlp_gradient", "hg_mlp_train",
            "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag", "hg_diagnostics", "hg_diag_record", "hg_rdump_parse",
            "hg_rdump_read", "hg_set_conserved", "hg_get_conserved", "hg_rdump_write", "hg_advance_substep",
-           "hg_dump_inversions", "hg_utoprim_batch_noalloc"]
+           "hg_dump_inversions", "hg_utoprim_batch_noalloc", "hg_set_inversion"]
 HG_PROF_NCLASS = 10
 
 
@@ -132,6 +132,7 @@ def lib():
     L.hg_get_dt.argtypes = [vp, d]
     L.hg_get_counters.argtypes = [vp, i64p, i64p]
     L.hg_utoprim_batch.argtypes = [C.c_int64, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, vp]
+    L.hg_set_inversion.argtypes = [vp, C.c_int, vp]
     L.hg_utoprim_batch_noalloc.argtypes = [C.c_int64, C.c_double, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, vp]
     L.hg_dump_inversions.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
     L.hg_make_c2p_batch.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_uint64, vp, vp, vp, vp, vp,
@@ -254,6 +255,13 @@ class Grid:
 
     def bound_prim(self):
         _check(lib().hg_bound_prim(self.h, None))
+
+    def set_inversion(self, mode, mlp=None):
+        """The step's inversion (hg_set_inversion): HG_C2P_FP64 (the
+        reference's, default), HG_C2P_MIXED, or HG_C2P_LEARNED with the
+        network mlp (which must stay alive while the grid uses it)."""
+        self._mlp = mlp
+        _check(lib().hg_set_inversion(self.h, int(mode), mlp.h if mlp is not None else None))
 
     def post_fixup(self, codes=None, finalstep=1):
         """bound_prim, post_fixup (fixup.c:37), bound_prim on the current

@@ -381,6 +381,25 @@ int hg_utoprim_batch_learned(const hg_mlp *m, int64_t n, double gam, const doubl
                              const double *gcon, const double *gdet, const double *gcovpert, double *pr,
                              int32_t *pflag, int32_t *iters, int64_t counts[2], void *stream);
 
+/* The inversion the step uses for every zone (advance.c:193-201, Utoprimgen
+   of Uf with the guess pb), no counterpart in the reference:
+     HG_C2P_FP64     the reference's FP64 iteration, bit for bit (default);
+     HG_C2P_MIXED    hg_utoprim_batch's HG_C2P_MIXED path: FP32 Newton from
+                     the reference's start, the polish to the converged root
+                     of the same FP64 system and double-double primitives;
+                     a zone it does not settle is inverted by HG_C2P_REFINED
+                     from its guess in the same kernel;
+     HG_C2P_LEARNED  HG_C2P_MIXED from the learned start of network m
+                     (hg_utoprim_batch_learned's rule), the network
+                     evaluated on the matrix cores before each substep's
+                     inversion; m must outlive its use by the context.
+   The failure codes are the reference's; the primitives of solved zones
+   are the converged root (within ~1e-13 of it), not the reference's last
+   iterate (up to ~1e-9 apart at Lorentz factors near 50).  Allocates the
+   learned mode's buffers (36 bytes per zone) here, not in the step. */
+#define HG_C2P_LEARNED 3
+int hg_set_inversion(hg_ctx *ctx, int mode, const hg_mlp *m);
+
 /* Several grids advanced by one HIP graph, no counterpart in the reference
    (which runs one grid per process): the graph holds nsteps RK2 steps of
    every context, one concurrent branch per context, and each launch advances

@@ -1,0 +1,124 @@
+"""The step's inversion modes (hg_set_inversion): the mixed-precision
+inversion and its learned start inside the step (advance.c:193-201 inverts
+every zone each substep), against the reference's FP64 inversion on the same
+torus, and a HIP graph of uneven tori in the learned mode against the same
+grids run one by one.
+
+The mixed path converges each zone's root of the reference's FP64 system
+(utoprim_2d_final.c:1189 func_vsq2) instead of stopping at the reference's
+tolerance, so the primitives differ from the FP64 step by the reference's own
+rounding noise of the root; failure counts and the dt sequence stay the
+reference's."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def torus(n1, n2, a=0.9375, rmax=12.0, seed=7):
+    from harm_finalwham_amd import harmgpu
+    g = harmgpu.Grid(n1, n2, 1, nb=6, lim=harmgpu.HG_LIM_WENO5BND, fluxmethod=harmgpu.HG_FLUX_HLL)
+    rh = 1.0 + math.sqrt(1.0 - a * a)
+    g.build_mks_geometry(a=a, Rin=0.9 * rh, Rout=40.0, R0=0.0, hslope=0.3)
+    dt = g.init_torus(rin=6.0, rmax=rmax, kappa=1e-3, beta=100.0, randfact=4e-2, seed=seed)
+    return g, dt
+
+
+def trained_network(seed=3):
+    """A network trained on the final-substep inversions of a 64x64 torus
+    (the step's own data, hg_dump_inversions), solved by the FP64 path."""
+    import torch
+    from harm_finalwham_amd import harmgpu
+    g, dt = torus(64, 64, seed=seed)
+    Xs, ys = [], []
+    for _ in range(4):
+        dt, _, _ = g.step_ch(dt)
+        b = g.dump_inversions()
+        pr = b["p_guess"].clone()
+        n = pr.shape[1]
+        pflag = torch.zeros(n, dtype=torch.int32, device="cuda")
+        harmgpu.utoprim_batch(b["U"], b["gcov"], b["gcon"], b["gdet"], b["gcovpert"], pr, pflag, None)
+        X, y, ok = harmgpu.mlp_features(b["U"], b["gcov"], b["gcon"], b["gdet"], b["p_guess"], p_solved=pr)
+        sel = (ok == 1) & (pflag == 0)
+        Xs.append(X[sel])
+        ys.append(y[sel])
+    g.close()
+    X, y = torch.cat(Xs).contiguous(), torch.cat(ys).contiguous()
+    m = harmgpu.Mlp(seed=11)
+    mu, sd = X.mean(0), X.std(0)
+    sd[sd < 1e-6] = 1.0
+    m.set_normalisation(mu.cpu().numpy(), sd.cpu().numpy())
+    for _ in range(3):
+        m.train(X, y, epochs=1, batch=2048, lr=2e-3)
+    return m
+
+
+@pytest.mark.parametrize("mode_name", ["HG_C2P_MIXED", "HG_C2P_LEARNED"])
+def test_step_inversion_mode_against_reference_inversion(mode_name):
+    from harm_finalwham_amd import harmgpu
+    mode = getattr(harmgpu, mode_name)
+    print("case step inversion", mode_name, "96x128 WENO5BND + HLL, 4 steps", flush=True)
+    m = trained_network() if mode == harmgpu.HG_C2P_LEARNED else None
+    ref, dt_r = torus(96, 128)
+    alt, dt_a = torus(96, 128)
+    alt.set_inversion(mode, m)
+    assert dt_r == dt_a
+    worst = 0.0
+    for step in range(4):
+        dt_r, nf_r, it_r = ref.step_ch(dt_r)
+        dt_a, nf_a, it_a = alt.step_ch(dt_a)
+        assert nf_r == nf_a, (step, nf_r, nf_a)
+        assert abs(dt_a - dt_r) <= 1e-10 * dt_r, (step, dt_r, dt_a)
+        pr, pa = ref.get_prims(), alt.get_prims()
+        # rho, u relative to rho + gam u, velocities relative to max(1, |u~|)
+        w = pr[..., 0] + 4.0 / 3.0 * np.abs(pr[..., 1])
+        e_rho = np.abs(pa[..., 0] - pr[..., 0]) / np.abs(pr[..., 0]).clip(1e-300)
+        e_u = np.abs(pa[..., 1] - pr[..., 1]) / w.clip(1e-300)
+        vm = np.abs(pr[..., 2:5]).max(axis=-1).clip(1.0)
+        e_v = (np.abs(pa[..., 2:5] - pr[..., 2:5]).max(axis=-1)) / vm
+        worst = max(worst, float(max(e_rho.max(), e_u.max(), e_v.max())))
+        print("step", step, "Newton steps", it_r, it_a, "max difference", worst, flush=True)
+        # one step: the two inversions' difference alone; later steps also
+        # carry it through the reconstruction's limiter switches
+        assert worst <= (1e-10 if step == 0 else 1e-6)
+    ref.close()
+    alt.close()
+
+
+def test_graph_with_learned_inversion_matches_single_runs():
+    """Uneven tori with the learned inversion in one graph, two replays
+    with eager work in between: every grid bit for bit as alone."""
+    import torch
+    from harm_finalwham_amd import harmgpu
+    print("case graph of 5 tori, learned inversion", flush=True)
+    m = trained_network()
+    specs = [((64, 64), 0.2, 10.5, 1), ((96, 128), 0.6, 12.5, 2), ((128, 128), 0.9375, 12.0, 3),
+             ((64, 64), 0.95, 13.5, 4), ((96, 128), 0.0, 11.0, 5)]
+    made = []
+    for (n1, n2), a, rmax, sd in specs:
+        pair = []
+        for _ in range(2):
+            g, _ = torus(n1, n2, a, rmax, sd)
+            g.set_inversion(harmgpu.HG_C2P_LEARNED, m)
+            pair.append(g)
+        made.append(pair)
+    batch = [p[0] for p in made]
+    alone = [p[1] for p in made]
+    graph = harmgpu.Graph(batch, nsteps=3)
+    x = torch.ones(1 << 20, device="cuda", dtype=torch.float64)
+    for replay in range(2):
+        graph.launch()
+        for g in alone:
+            g.step_async(3)
+        torch.cuda.synchronize()
+        for k, (gb, ga) in enumerate(zip(batch, alone)):
+            assert gb.get_dt() == ga.get_dt(), (replay, k)
+            assert np.array_equal(gb.get_prims(), ga.get_prims()), (replay, k)
+        x = x * 1.0000001 + 1.0
+        torch.cuda.synchronize()
+        print("replay", replay, "bit-identical", flush=True)
+    graph.close()
+    for g in batch + alone:
+        g.close()
