@@ -10,7 +10,8 @@
 //
 //   k_mlp_features  FP64 set-up of the inversion -> 8 features per zone, and
 //                   the training target log(W_solved / W_guess)
-//   k_mlp_forward   features -> prediction (MFMA)
+//   k_mlp_forward   features -> prediction (MFMA; each wavefront keeps the
+//                   network in registers and runs over 16-sample tiles)
 //   k_mlp_backward  forward + backward of a 64-sample chunk on the vector
 //                   units (FP32), per-block weight gradients added to grad[]
 //   k_mlp_adam      Adam update of the FP32 weights
@@ -107,9 +108,31 @@ __global__ void __launch_bounds__(256) k_mlp_features(int64_t n, double gam, con
 // ---------------------------------------------------------------------------
 struct MlpDev {
   const float *w;       // FP32 parameters (biases, W3, normalisation)
-  const __bf16 *w1b;    // BF16 W1 as [32][64]: rows 0..7 and 8..15 W1, rest 0
-  const __bf16 *w2b;    // BF16 W2 [64][64]
+  const __bf16 *w1b;    // BF16 W1 B-fragments: [nt 4][lane 64][j 8] of the K = 32 layer-1 matrix
+  const __bf16 *w2b;    // BF16 W2 B-fragments: [ks 2][nt 4][lane 64][j 8]
 };
+
+// the network as one wavefront holds it: the B-fragments of both hidden
+// layers (lane l = 16 q + r holds rows 8 q .. 8 q + 7 of column 16 nt + r),
+// and the biases and output weights of the lane's columns 16 nt + r
+struct MlpRegs {
+  bf16x8 w1[4], w2[8];
+  float b1[4], b2[4], w3[4], b3;
+};
+
+__device__ __forceinline__ void mlp_load(const MlpDev &m, MlpRegs &R) {
+  const int l = threadIdx.x & 63, r = l & 15;
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+    R.w1[nt] = *(const bf16x8 *)(m.w1b + (nt * 64 + l) * 8);
+    R.b1[nt] = m.w[OFF_B1 + 16 * nt + r];
+    R.b2[nt] = m.w[OFF_B2 + 16 * nt + r];
+    R.w3[nt] = m.w[OFF_W3 + 16 * nt + r];
+  }
+#pragma unroll
+  for (int f = 0; f < 8; f++) R.w2[f] = *(const bf16x8 *)(m.w2b + (f * 64 + l) * 8);
+  R.b3 = m.w[OFF_B3];
+}
 
 __device__ __forceinline__ __bf16 bf_hi(float v) { return (__bf16)v; }
 
@@ -122,7 +145,11 @@ __device__ __forceinline__ float act(float x) {
 __device__ __forceinline__ __bf16 bf_lo(float v) { return (__bf16)(v - (float)(__bf16)v); }
 
 // 16 samples: xin[16][8] (normalised) -> y[16]; h[16][64] float LDS scratch
-__device__ __forceinline__ void forward16(const MlpDev &m, const float (*xin)[NIN], float (*h)[NH], float *y) {
+// for the layer-1 activations (the C layout of layer 1 -- sample 4 q + i in
+// lane 16 q + r -- is not the A layout of layer 2, sample r in lane 16 q + r);
+// layer 3 is summed per lane over its columns, then across the 16 lanes of
+// each sample group
+__device__ __forceinline__ void forward16(const MlpRegs &R, const float (*xin)[NIN], float (*h)[NH], float *y) {
   const int l = threadIdx.x & 63, q = l >> 4, r = l & 15;
   // layer 1
   bf16x8 a;
@@ -133,13 +160,10 @@ __device__ __forceinline__ void forward16(const MlpDev &m, const float (*xin)[NI
   }
 #pragma unroll
   for (int nt = 0; nt < 4; nt++) {
-    bf16x8 b;
-#pragma unroll
-    for (int j = 0; j < 8; j++) b[j] = m.w1b[(8 * q + j) * NH + 16 * nt + r];
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, R.w1[nt], acc, 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < 4; i++) h[4 * q + i][16 * nt + r] = act(acc[i] + m.w[OFF_B1 + 16 * nt + r]);
+    for (int i = 0; i < 4; i++) h[4 * q + i][16 * nt + r] = act(acc[i] + R.b1[nt]);
   }
   __syncthreads();
   // layer 2: K = 64 in two steps, BF16 inputs
@@ -153,45 +177,54 @@ __device__ __forceinline__ void forward16(const MlpDev &m, const float (*xin)[NI
 #pragma unroll
     for (int j = 0; j < 8; j++) a2[j] = bf_hi(h[r][kb + j]);
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
-      bf16x8 b2;
-#pragma unroll
-      for (int j = 0; j < 8; j++) b2[j] = m.w2b[(kb + j) * NH + 16 * nt + r];
-      acc2[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2, acc2[nt], 0, 0, 0);
-    }
+    for (int nt = 0; nt < 4; nt++)
+      acc2[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, R.w2[4 * ks + nt], acc2[nt], 0, 0, 0);
   }
-  __syncthreads();
+  // layer 3: lane (q, r) sums its four columns for samples 4 q + i
+  float part[4];
 #pragma unroll
-  for (int nt = 0; nt < 4; nt++)
+  for (int i = 0; i < 4; i++) {
+    float sacc = 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; i++) h[4 * q + i][16 * nt + r] = act(acc2[nt][i] + m.w[OFF_B2 + 16 * nt + r]);
-  __syncthreads();
-  // layer 3 on the vector units: lanes 0..15 one sample each
-  if (l < 16) {
-    float s = m.w[OFF_B3];
-    for (int k = 0; k < NH; k++) s += h[l][k] * m.w[OFF_W3 + k];
-    y[l] = s;
+    for (int nt = 0; nt < 4; nt++) sacc += act(acc2[nt][i] + R.b2[nt]) * R.w3[nt];
+    part[i] = sacc;
   }
+#pragma unroll
+  for (int off = 8; off >= 1; off >>= 1)
+#pragma unroll
+    for (int i = 0; i < 4; i++) part[i] += __shfl_xor(part[i], off, 64);
+  if (r == 0)
+#pragma unroll
+    for (int i = 0; i < 4; i++) y[4 * q + i] = R.b3 + part[i];
   __syncthreads();
 }
 
+// one wavefront per block holds the network in registers and runs over
+// tiles of 16 samples (grid-stride)
 __global__ void __launch_bounds__(64) k_mlp_forward(int64_t n, const float *__restrict__ X, float *__restrict__ y,
                                                     MlpDev m) {
   __shared__ float xin[16][NIN];
   __shared__ float h[16][NH];
   __shared__ float yo[16];
   const int l = threadIdx.x;
-  for (int64_t base = (int64_t)blockIdx.x * 64; base < (int64_t)blockIdx.x * 64 + 64; base += 16) {
-    if (base >= n) break;
-    if (l < 16) {
-      int64_t t = base + l;
+  MlpRegs R;
+  mlp_load(m, R);
+  float mu = 0.f, isd = 1.f;
+  if (l < NIN) {
+    mu = m.w[OFF_MU + l];
+    isd = m.w[OFF_SD + l];
+  }
+  for (int64_t base = (int64_t)blockIdx.x * 16; base < n; base += (int64_t)gridDim.x * 16) {
+    // lane 8 s + k (s < 8) normalises feature k of sample s, twice
 #pragma unroll
-      for (int k = 0; k < NIN; k++)
-        xin[l][k] = (t < n) ? (X[t * NIN + k] - m.w[OFF_MU + k]) / m.w[OFF_SD + k] : 0.f;
+    for (int half = 0; half < 2; half++) {
+      const int s = half * 8 + (l >> 3), k = l & 7;
+      const int64_t t = base + s;
+      const float mk = __shfl(mu, k, 64), sk = __shfl(isd, k, 64);
+      xin[s][k] = (t < n) ? (X[t * NIN + k] - mk) / sk : 0.f;
     }
     __syncthreads();
-    forward16(m, xin, h, yo);
-    __syncthreads();
+    forward16(R, xin, h, yo);
     if (l < 16 && base + l < n) y[base + l] = yo[l];
     __syncthreads();
   }
@@ -313,13 +346,20 @@ __global__ void __launch_bounds__(64) k_mlp_backward(int64_t n, const float *__r
   }
 }
 
-// BF16 copies of W1 (as [32][64], rows 0..7 and 8..15 both W1) and W2
+// BF16 B-fragments of the hidden layers for lane l = 16 q + r, element j:
+// layer 1 [nt][l][j] = W1x[8 q + j][16 nt + r] with W1x the K = 32 matrix
+// whose rows 0..7 and 8..15 are both W1 (the input's hi and lo halves) and
+// rows 16..31 zero; layer 2 [ks][nt][l][j] = W2[32 ks + 8 q + j][16 nt + r]
 __device__ __forceinline__ void refresh_bf16(int e, const float *w, __bf16 *w1b, __bf16 *w2b) {
-  if (e < 32 * NH) {
-    int row = e / NH, col = e % NH;
-    w1b[e] = (row < 2 * NIN) ? (__bf16)w[OFF_W1 + (row % NIN) * NH + col] : (__bf16)0.0f;
+  const int j = e & 7, l = (e >> 3) & 63, q = l >> 4, r = l & 15;
+  if (e < 4 * 64 * 8) {
+    const int nt = e >> 9, row = 8 * q + j;
+    w1b[e] = (row < 2 * NIN) ? (__bf16)w[OFF_W1 + (row % NIN) * NH + 16 * nt + r] : (__bf16)0.0f;
   }
-  if (e < NH * NH) w2b[e] = (__bf16)w[OFF_W2 + e];
+  if (e < 8 * 64 * 8) {
+    const int f = e >> 9, ks = f >> 2, nt = f & 3;
+    w2b[e] = (__bf16)w[OFF_W2 + (32 * ks + 8 * q + j) * NH + 16 * nt + r];
+  }
 }
 
 __global__ void k_mlp_adam(float *__restrict__ w, float *__restrict__ g, float *__restrict__ m1,
@@ -381,8 +421,10 @@ __global__ void __launch_bounds__(64) k_c2p_learned(int64_t n, double gam, const
     for (int k = 0; k < NIN; k++) xin[l][k] = 0.f;
   }
   __syncthreads();
+  MlpRegs R;
+  mlp_load(m, R);
   for (int g16 = 0; g16 < 4; g16++) {
-    forward16(m, (const float(*)[NIN]) & xin[16 * g16][0], h, &yo[16 * g16]);
+    forward16(R, (const float(*)[NIN]) & xin[16 * g16][0], h, &yo[16 * g16]);
     __syncthreads();
   }
   const double igr = (gam - 1.0) / gam;
@@ -419,6 +461,13 @@ struct hg_mlp {
 };
 
 static MlpDev dev_view(const hg_mlp *m) { return MlpDev{m->w, m->w1b, m->w2b}; }
+
+// the forward kernel's one-wavefront blocks: one per 16-sample tile up to
+// 8192 (32 per CU), each then loops over its share of the tiles
+static unsigned forward_blocks(int64_t n) {
+  const int64_t tiles = (n + 15) / 16;
+  return (unsigned)(tiles < 8192 ? tiles : 8192);
+}
 
 #define MLPCHECK(call)                                                              \
   do {                                                                              \
@@ -519,14 +568,14 @@ extern "C" int hg_mlp_features(int64_t n, double gam, const double *U, const dou
 
 void hg::launch_mlp_forward(const hg_mlp *m, int64_t n, const float *X, float *y, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, X, y, dev_view(m));
+  hipLaunchKernelGGL(k_mlp_forward, dim3(forward_blocks(n)), dim3(64), 0, s, n, X, y, dev_view(m));
 }
 
 extern "C" int hg_mlp_forward(const hg_mlp *m, int64_t n, const float *X, float *y, void *stream) {
   if (!m || n < 0 || (n > 0 && (!X || !y))) { set_error("hg_mlp_forward: bad arguments"); return 1; }
   if (n == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, X, y, dev_view(m));
+  hipLaunchKernelGGL(k_mlp_forward, dim3(forward_blocks(n)), dim3(64), 0, s, n, X, y, dev_view(m));
   MLPCHECK(hipGetLastError());
   MLPCHECK(hipStreamSynchronize(s));
   return 0;
