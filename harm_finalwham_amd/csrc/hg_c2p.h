@@ -276,7 +276,9 @@ namespace hg {
 constexpr int MIXED_MAX_F32 = 40;         // FP32 Newton steps before falling back
 constexpr float MIXED_TOL_F32 = 2e-6f;    // |dW/W| that ends the FP32 phase
 constexpr float MIXED_STALL_F32 = 1e-3f;  // below it, a step that shrank less than 4x ends it too
-constexpr int POLISH_MAX_F64 = 4;         // FP64 steps of the polish (to |dW/W| <= 1e-9)
+constexpr int POLISH_MAX_F64 = 8;         // FP64 steps of the polish (to |dW/W| <= 1e-9); from the
+                                          // FP32 floor at Lorentz factors near 50 (|dW/W| ~ 1e-3) it
+                                          // can take six
 constexpr int POLISH_MAX_DD = 2;        // double-double steps (to |dW/W| <= 1e-15)
 
 struct C2PInvF {
