@@ -61,7 +61,7 @@ def profile_summary(prof, zones):
 DEVICE_KERNELS = {"recon+flux1": ("hg::k_weno_x", "hg::k_flux_lr<1>"),
                   "recon+flux2": ("hg::k_weno_m<2>", "hg::k_flux_lr<2>"),
                   "recon+flux3": ("hg::k_weno_m<3>", "hg::k_flux_lr<3>"),
-                  "emf": ("hg::k_emf",), "update": ("hg::k_update<false, false>", "hg::k_invert<false>")}
+                  "emf": ("hg::k_emf",), "update": ("hg::k_update<false, false>", "hg::k_invert<false, 0>")}
 TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "hbm_traffic_torus3d_256x128x64_weno_hll.md")
 VALU_PROFILE = os.path.join(HERE, "profiles", "valu_issue_torus3d_256x128x64_weno_hll.md")
 FP64_PEAK_TFLOPS = 78.6
