@@ -264,8 +264,12 @@ namespace hg {
 // reference sets them up) is found by Newton steps whose residuals are
 // evaluated in double-double arithmetic (hg_dd.h) with the FP64 Jacobian of
 // newton_dx, and the primitives are formed from it in double-double:
-//   c2p_polish     FP64 steps to |dW/W| <= 1e-9, then double-double steps to
-//                  |dW/W| <= 1e-15 (false when either is not reached)
+//   c2p_polish     FP64 steps to |dW/W| <= 1e-9, then double-double steps
+//                  until one moves W by at most 1e-12 of itself (false when
+//                  either is not reached).  Newton converges quadratically
+//                  there: a step of 1e-12 W leaves an error near 1e-24 W, so
+//                  that step's W is the root to the last bit and needs no
+//                  further step to confirm it
 //   c2p_finish_dd  c2p_finish with vsq, 1 - vsq, rho0, w, p, u and the
 //                  velocities in double-double, rounded once
 // The mixed-precision path runs the Newton iteration in FP32 from the
@@ -279,7 +283,7 @@ constexpr float MIXED_STALL_F32 = 1e-3f;  // below it, a step that shrank less t
 constexpr int POLISH_MAX_F64 = 8;         // FP64 steps of the polish (to |dW/W| <= 1e-9); from the
                                           // FP32 floor at Lorentz factors near 50 (|dW/W| ~ 1e-3) it
                                           // can take six
-constexpr int POLISH_MAX_DD = 2;        // double-double steps (to |dW/W| <= 1e-15)
+constexpr int POLISH_MAX_DD = 3;          // double-double steps (until one is <= 1e-12 W)
 
 struct C2PInvF {
   float Bsq, QdotBsq, Qtsq, Qdotn, D;
@@ -397,7 +401,7 @@ __device__ __forceinline__ void newton_dx_dd(const C2PInv &c, double igr, double
 }
 
 // The polish: FP64 steps (at most n64; none when n64 = 0) until |dW/W| <=
-// 1e-9, then double-double steps until |dW/W| <= 1e-15.  True when both
+// 1e-9, then double-double steps until one moves W by <= 1e-12 W.  True when both
 // tolerances were reached with finite iterates.
 __device__ __forceinline__ bool c2p_polish(const C2PInv &c, double igr, double &x0, double &x1, int n64,
                                            int &iters) {
@@ -426,7 +430,7 @@ __device__ __forceinline__ bool c2p_polish(const C2PInv &c, double igr, double &
     x1 += dx1;
     x1 = (x1 > 1.) ? vsq_big : x1;
     iters++;
-    ok = fabs(x0 - xo0) <= 1e-15 * x0;
+    ok = fabs(x0 - xo0) <= 1e-12 * x0;
   }
   return ok && isfinite(x0) && isfinite(x1);
 }
