@@ -64,15 +64,18 @@ __device__ __forceinline__ void list_append(bool fb, int64_t t, unsigned long lo
   if (fb) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)t;
 }
 
-// the FP32 bulk, polish and double-double primitives of one zone whose
-// set-up succeeded; false: the zone goes to the list
+// the FP32 bulk, the polish and the primitives of one zone whose set-up
+// succeeded; false: the zone goes to the list
 __device__ __forceinline__ bool mixed_zone(const C2PInv &c, double igr, double x0, double x1, double *p, int &it,
                                            int &flag) {
   if (!newton_f32(c, igr, x0, x1, it)) return false;
   if (!c2p_polish(c, igr, x0, x1, POLISH_MAX_F64, it)) return false;
+  // the primitives from the converged W with the reference's FP64 formulas
+  // (utoprim_2d_final.c:335-441): rounding of 1 - vsq moves them by at most
+  // ~1e-12 of the double-double values HG_C2P_REFINED forms
   double out[NPR];
   bool assign;
-  flag = c2p_finish_dd(c, x0, igr, out, assign);
+  flag = c2p_finish(c, x0, 0, igr, out, assign);
   if (assign)
 #pragma unroll
     for (int i = 0; i < B1; i++) p[i] = out[i];

@@ -276,10 +276,13 @@ int hg_rdump_write(hg_ctx *ctx, const char *path, const char *header, void *stre
                      EXTRA_NEWT_ITER, u2p_defs.h:77, continues the FP64
                      iteration, which cannot go below its rounding noise).
      HG_C2P_MIXED    the reference's FP64 set-up and start, the Newton
-                     iteration in FP32 to |dW/W| <= 2e-6, then the polish and
-                     primitives of HG_C2P_REFINED; a persistent work-queue
-                     kernel.  Zones whose FP32 phase or polish does not
-                     converge are inverted by HG_C2P_REFINED from the guess.
+                     iteration in FP32 to |dW/W| <= 2e-6 (or to its rounding
+                     floor), then the polish of HG_C2P_REFINED and the
+                     primitives from the converged W with the reference's
+                     FP64 formulas (within ~1e-12 of HG_C2P_REFINED's); a
+                     persistent work-queue kernel.  Zones whose FP32 phase or
+                     polish does not converge are inverted by HG_C2P_REFINED
+                     from the guess.
                      Allocates its scratch (4 n + 24 bytes) stream-ordered on
                      the given stream. */
 #define HG_C2P_FP64 0
@@ -372,7 +375,7 @@ int hg_mlp_train(hg_mlp *m, int64_t n, const float *X, const float *Y, int epoch
    the plain start (the reference's, from the guess).  The polish and
    double-double primitives of HG_C2P_REFINED follow.  A zone whose FP32
    phase or polish does not converge is inverted by HG_C2P_REFINED from its
-   plain guess.  The values therefore agree with HG_C2P_REFINED to ~1e-13
+   plain guess.  The values therefore agree with HG_C2P_REFINED to ~1e-12
    (the same converged root), not bit for bit.  iters counts Newton steps and
    the learned start's evaluation.  counts (may be NULL) receives [zones solved from the learned
    start, zones inverted by the fallback].  Allocates its scratch (4 n + 24
