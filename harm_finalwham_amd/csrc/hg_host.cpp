@@ -713,10 +713,22 @@ extern "C" int hg_set_inversion(hg_ctx *c, int mode, const hg_mlp *m) {
     return 1;
   }
   if (mode == HG_C2P_LEARNED && !m) { set_error("hg_set_inversion: HG_C2P_LEARNED needs a network"); return 1; }
-  if (mode == HG_C2P_LEARNED && !c->lx) {
+  if (mode == HG_C2P_LEARNED && !(c->lx && c->ly)) {
+    // both buffers or neither: a failed allocation leaves the context as it was
     const size_t n = (size_t)c->d.n1 * c->d.n2 * c->d.n3;
-    HIPCHECK(hipMalloc(&c->lx, n * C2P_NFEAT * sizeof(float)));
-    HIPCHECK(hipMalloc(&c->ly, n * sizeof(float)));
+    float *lx = nullptr, *ly = nullptr;
+    hipError_t e = hipMalloc(&lx, n * C2P_NFEAT * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&ly, n * sizeof(float));
+    if (e != hipSuccess) {
+      if (lx) (void)hipFree(lx);
+      if (ly) (void)hipFree(ly);
+      set_error("hg_set_inversion: %s", hipGetErrorString(e));
+      return 1;
+    }
+    if (c->lx) (void)hipFree(c->lx);
+    if (c->ly) (void)hipFree(c->ly);
+    c->lx = lx;
+    c->ly = ly;
   }
   c->c2p_mode = mode;
   c->mlp = mode == HG_C2P_LEARNED ? m : nullptr;

@@ -209,18 +209,14 @@ __global__ void __launch_bounds__(64) k_mlp_forward(int64_t n, const float *__re
   const int l = threadIdx.x;
   MlpRegs R;
   mlp_load(m, R);
-  float mu = 0.f, isd = 1.f;
-  if (l < NIN) {
-    mu = m.w[OFF_MU + l];
-    isd = m.w[OFF_SD + l];
-  }
+  // lane 8 s + k (s < 8) normalises feature k of sample s and of s + 8
+  const int k = l & 7;
+  const float mk = m.w[OFF_MU + k], sk = m.w[OFF_SD + k];
   for (int64_t base = (int64_t)blockIdx.x * 16; base < n; base += (int64_t)gridDim.x * 16) {
-    // lane 8 s + k (s < 8) normalises feature k of sample s, twice
 #pragma unroll
     for (int half = 0; half < 2; half++) {
-      const int s = half * 8 + (l >> 3), k = l & 7;
+      const int s = half * 8 + (l >> 3);
       const int64_t t = base + s;
-      const float mk = __shfl(mu, k, 64), sk = __shfl(isd, k, 64);
       xin[s][k] = (t < n) ? (X[t * NIN + k] - mk) / sk : 0.f;
     }
     __syncthreads();

@@ -122,3 +122,17 @@ def test_graph_with_learned_inversion_matches_single_runs():
     graph.close()
     for g in batch + alone:
         g.close()
+
+
+def test_set_inversion_rejects_bad_arguments():
+    from harm_finalwham_amd import harmgpu
+    g, _ = torus(64, 64)
+    with pytest.raises(harmgpu.HarmError):
+        g.set_inversion(7)
+    with pytest.raises(harmgpu.HarmError):
+        g.set_inversion(harmgpu.HG_C2P_LEARNED, None)  # the learned mode needs a network
+    with pytest.raises(harmgpu.HarmError):
+        g.set_inversion(harmgpu.HG_C2P_REFINED)  # a batch mode, not a step mode
+    g.set_inversion(harmgpu.HG_C2P_MIXED)
+    g.set_inversion(harmgpu.HG_C2P_FP64)
+    g.close()
