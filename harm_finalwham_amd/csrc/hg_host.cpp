@@ -68,6 +68,15 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
   if (cfg->nb < need) { set_error("hg_ctx_create: nb=%d < %d ghost layers needed", cfg->nb, need); return 1; }
   if (cfg->n3 > 1 && cfg->n3 < cfg->nb) { set_error("hg_ctx_create: periodic n3=%d < nb=%d", cfg->n3, cfg->nb); return 1; }
   if (cfg->n2 > 1 && cfg->n2 < cfg->nb) { set_error("hg_ctx_create: n2=%d < nb=%d", cfg->n2, cfg->nb); return 1; }
+  {
+    // the reconstruction kernels address a primitive plane with 32-bit byte offsets
+    const Dims dd = make_dims(cfg->n1, cfg->n2, cfg->n3, cfg->nb);
+    if ((uint64_t)dd.spl * sizeof(double) > (uint64_t)UINT32_MAX) {
+      set_error("hg_ctx_create: %lld zones with ghosts exceed the 2^29 per primitive plane the kernels address",
+                (long long)dd.spl);
+      return 1;
+    }
+  }
   hg_ctx *c = new hg_ctx();
   c->cfg = *cfg;
   c->d = make_dims(cfg->n1, cfg->n2, cfg->n3, cfg->nb);
@@ -89,7 +98,7 @@ extern "C" int hg_ctx_create(const hg_config *cfg, hg_ctx **out) {
     rc |= dalloc(&c->pr, ns);
     if (d.n2 > 1) { rc |= dalloc(&c->pl2, ns); rc |= dalloc(&c->pr2, ns); }
     if (d.n3 > 1) { rc |= dalloc(&c->pl3, ns); rc |= dalloc(&c->pr3, ns); }
-    rc |= dalloc(&c->aux, 2 * nz);
+    rc |= dalloc(&c->aux, 5 * nz);
   }
   rc |= dalloc(&c->geo, (size_t)4 * GCOMP * d.gplane);
   rc |= dalloc(&c->corn, (size_t)3 * d.gplane);

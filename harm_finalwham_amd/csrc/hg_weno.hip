@@ -6,7 +6,8 @@
 //   A  dP/P jump indicator (interpline.c:2148) from the zone's p u^t (1+u_t)
 //      and the enthalpy-like term at the neighbours; the two per-zone terms do
 //      not depend on the direction and are computed once per substep
-//      (k_weno_prep)
+//      (k_weno_prep), the indicator itself once per zone and direction
+//      (k_weno_dpi), not once per primitive
 //   C  smooth-monotonicity indicator + order-3 weights (smonointerp.c:15,
 //      reconstructeno.c:1406, :702), from y[s-2..s+2]
 //   D  weight ratios -> lower-order fraction (reconstructeno.c:1962), from
@@ -353,12 +354,41 @@ __global__ void __launch_bounds__(256) k_weno_prep(const double *__restrict__ p,
 }
 
 // ---------------------------------------------------------------------------
+// stage A per direction: dP/P at every zone of the box above whose two
+// neighbours along that direction are in the box too, into aux planes 2, 3, 4
+// (directions 1, 2, 3).  It does not depend on the primitive reconstructed,
+// so the reconstruction kernels of all 8 primitives read it from here.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_weno_dpi(const double *__restrict__ p, double *__restrict__ aux, Dims d,
+                                                  int wide) {
+  const int e1 = d.n1 > 1 ? (wide ? d.nb1 : 2) : 0, e2 = d.n2 > 1 ? (wide ? d.nb2 : 2) : 0;
+  const int e3 = d.n3 > 1 ? (wide ? d.nb3 : 2) : 0;
+  const int b1 = d.n1 + 2 * e1, b2 = d.n2 + 2 * e2, b3 = d.n3 + 2 * e3;
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)b1 * b2 * b3) return;
+  int i = -e1 + (int)(t % b1);
+  int64_t r = t / b1;
+  int k = -e3 + (int)(r % b3), j = -e2 + (int)(r / b3);
+  const int64_t z = d.idx(i, j, k), P = d.spl;
+  const double *par = aux, *upar = aux + P;
+  const double rho = p[RHO * P + z];
+  if (d.n1 > 1 && i > -e1 && i < d.n1 + e1 - 1)
+    aux[2 * P + z] = dp_jump(rho, par[z - 1], par[z], par[z + 1], upar[z - 1], upar[z], upar[z + 1]);
+  if (d.n2 > 1 && j > -e2 && j < d.n2 + e2 - 1)
+    aux[3 * P + z] =
+        dp_jump(rho, par[z - d.sj], par[z], par[z + d.sj], upar[z - d.sj], upar[z], upar[z + d.sj]);
+  if (d.n3 > 1 && k > -e3 && k < d.n3 + e3 - 1)
+    aux[4 * P + z] =
+        dp_jump(rho, par[z - d.sk], par[z], par[z + d.sk], upar[z - d.sk], upar[z], upar[z + d.sk]);
+}
+
+// ---------------------------------------------------------------------------
 // along i: one lane per point; wavefront w covers points s0-3 .. s0+60 and
 // writes the interface pair of s0 .. s0+57
 // ---------------------------------------------------------------------------
 constexpr int XOUT = 58;
 
-__global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, const double *__restrict__ aux,
+__global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, const double *__restrict__ dpi,
                                                 double *__restrict__ pleft, double *__restrict__ pright, Dims d,
                                                 int nseg, int wide) {
   const int lane = threadIdx.x & 63;
@@ -390,8 +420,7 @@ __global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, co
   double lofm = __shfl_up(lofd, 1, 64);
   double lofp = __shfl_down(lofd, 1, 64);
   if (lane >= 3 && lane < 3 + XOUT && s <= n) {
-    const double *a0 = aux + zl, *a1 = aux + d.spl + zl;
-    double dPi = dp_jump(p[RHO * d.spl + zl + s], a0[s - 1], a0[s], a0[s + 1], a1[s - 1], a1[s], a1[s + 1]);
+    const double dPi = dpi[zl + s];
     double ol, orr;
     stage_e(y + 2, m0, w[0], w[1], w[2], lofm, lofd, lofp, dPi, ol, orr);
     int64_t z = (int64_t)pl * d.spl + zl + s;
@@ -407,12 +436,27 @@ __global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, co
 // DIR 1 marches along i with lane = (j, k): no halo lanes, but every load of
 // a wavefront touches 64 rows (the rows stay in cache over the next steps).
 // ---------------------------------------------------------------------------
+// A row address that is the same in every lane, pinned to scalar registers
+// as a global-memory pointer: loads and stores take it as their scalar base
+// and add the lane's 32-bit offset, where the compiler would otherwise keep a
+// 64-bit per-lane pointer in two vector registers for every array it walks.
+typedef __attribute__((address_space(1))) char gchar;
+__device__ __forceinline__ gchar *uniform_row(const void *base, int64_t off) {
+  const uint64_t v = (uint64_t)base + (uint64_t)off;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (gchar *)(((uint64_t)hi << 32) | lo);
+}
+
 // Block order: the 8 primitive blocks of one (lane block, segment) unit read
-// the same dP/P inputs and densities, so they are dispatched back to back to
-// one XCD (work-groups go to the 8 XCDs round robin by linear index): block
-// b runs on XCD b % 8, takes primitive (b / 8) % 8 of unit (b / 64) * 8 + b % 8.
+// the same dP/P indicators, so they are dispatched back to back to one XCD
+// (work-groups go to the 8 XCDs round robin by linear index): block b runs
+// on XCD b % 8, takes primitive (b / 8) % 8 of unit (b / 64) * 8 + b % 8.
+// Addressing: the primitive plane and the line position t are uniform over
+// the block, so every load and store is a uniform row address plus one
+// 32-bit per-lane byte offset (the context keeps a plane under 4 GiB).
 template <int DIR>
-__global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, const double *__restrict__ aux,
+__global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, const double *__restrict__ dpi,
                                                 double *__restrict__ pleft, double *__restrict__ pright, Dims d,
                                                 int nseg, int seglen, int wide, int nlb) {
   const int n = (DIR == 1) ? d.n1 : (DIR == 2) ? d.n2 : d.n3;
@@ -437,13 +481,22 @@ __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, co
   const int e1 = (e0 + seglen - 1 < n) ? e0 + seglen - 1 : n;
   const int64_t zl = (DIR == 1) ? d.idx(0, a_, o) : (DIR == 2) ? d.idx(a_, 0, o) : d.idx(a_, o, 0);
   const int64_t sd = (DIR == 1) ? 1 : (DIR == 2) ? d.sj : d.sk;
-  const double *yl = p + (int64_t)pl * d.spl + zl;
-  const double *rl = p + (int64_t)RHO * d.spl + zl;
-  const double *a0 = aux + zl, *a1 = aux + d.spl + zl;
-  double *ol = pleft + (int64_t)pl * d.spl + zl, *orr = pright + (int64_t)pl * d.spl + zl;
-  // windows: yw = y(t-5..t+2), W0 = w0(t-4..t), W1/W2/M0 = (t-3..t), L0 = lof0(t-2..t),
-  // LD = stage-D fraction (t-4..t-2), PA/UA = par/upar (t-4..t-2)
-  double yw[8], W0[5], W1[4], W2[4], M0[4], L0[3], LD[3], PA[3], UA[3];
+  // lowest point read along the line (y at e0-5); row offsets count from it
+  // so that the uniform part of every address is non-negative
+  const int lo = e0 - 5;
+  const uint32_t zb = (uint32_t)((zl + (int64_t)lo * sd) * (int64_t)sizeof(double));
+  const int64_t rowb = sd * (int64_t)sizeof(double);
+  const char *yb = (const char *)(p + (int64_t)pl * d.spl);
+  const char *db = (const char *)dpi;
+  char *lb = (char *)(pleft + (int64_t)pl * d.spl), *rb = (char *)(pright + (int64_t)pl * d.spl);
+#define WM_AT(base, t) (uniform_row((base), (int64_t)((t) - lo) * rowb) + zb)
+#define WM_LD(base, t) (*(const __attribute__((address_space(1))) double *)WM_AT(base, t))
+#define WM_ST(base, t) (*(__attribute__((address_space(1))) double *)WM_AT(base, t))
+  // windows: yw = y(t-5..t+2), W0 = w0(t-4..t), W1/W2/M0 = (t-3..t),
+  // LD = stage-D fraction (t-4..t-2); the cusp fractions of t-2..t, each
+  // 0 or 1, are bits 2..0 of lofb
+  double yw[8], W0[5], W1[4], W2[4], M0[4], LD[3];
+  unsigned lofb = 0u;
 #pragma unroll
   for (int a = 0; a < 8; a++) yw[a] = 0.;
 #pragma unroll
@@ -451,31 +504,30 @@ __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, co
 #pragma unroll
   for (int a = 0; a < 4; a++) { W1[a] = 1.; W2[a] = 1.; M0[a] = 0.; }
 #pragma unroll
-  for (int a = 0; a < 3; a++) { L0[a] = 0.; LD[a] = 0.; PA[a] = 1.; UA[a] = 1.; }
+  for (int a = 0; a < 3; a++) LD[a] = 0.;
   // before step t = e0-3 the y window holds y(t-6..t+1); points below e0-5 are never read
 #pragma unroll
-  for (int a = 4; a < 8; a++) yw[a] = yl[(int64_t)(e0 - 9 + a) * sd];
-  // the loads of step t are issued one step ahead (step t-1), so that their
-  // latency hides behind a step's arithmetic; indices are clamped to the
-  // last ones read (the final step's prefetch is never used)
+  for (int a = 4; a < 8; a++) yw[a] = WM_LD(yb, e0 - 9 + a);
+  // The loads of step t are issued one step ahead (step t-1), so that their
+  // latency hides behind a step's arithmetic; indices are clamped to points
+  // read anyway (the final step's prefetch and the indicator before e0 are
+  // never used).  Every step issues the same memory operations, two loads
+  // then two stores: the steps before the first output store their
+  // (unused) values at e0, which the step of e0 overwrites, and two such
+  // stores follow the first loads.  The wait for a step's loads then leaves
+  // the previous step's stores in flight; with stores on some paths only it
+  // would have to wait for them too.
   const int tlast = e1 + 3;
-  double ny = yl[(int64_t)(e0 - 1) * sd];
-  double npa = 1., nua = 1., nrho = 0.;
-  if (e0 - 5 >= -2) {
-    npa = a0[(int64_t)(e0 - 5) * sd];
-    nua = a1[(int64_t)(e0 - 5) * sd];
-  }
-  // par/upar window before step t: (t-5..t-3); loaded from t-2 on
+  double ny = WM_LD(yb, e0 - 1);
+  double ndp = WM_LD(db, e0);
+  WM_ST(lb, e0) = 0.;
+  WM_ST(rb, e0) = 0.;
   for (int t = e0 - 3; t <= tlast; t++) {
-    const double cy = ny, cpa = npa, cua = nua, crho = nrho;
+    const double cy = ny, cdp = ndp;
     {
       const int tn = (t + 1 < tlast) ? t + 1 : tlast;
-      ny = yl[(int64_t)(tn + 2) * sd];
-      if (tn - 2 >= -2) {
-        npa = a0[(int64_t)(tn - 2) * sd];
-        nua = a1[(int64_t)(tn - 2) * sd];
-      }
-      if (tn - 3 >= e0) nrho = rl[(int64_t)(tn - 3) * sd];
+      ny = WM_LD(yb, tn + 2);
+      ndp = WM_LD(db, (tn - 3 > e0) ? tn - 3 : e0);
     }
 #pragma unroll
     for (int a = 0; a < 7; a++) yw[a] = yw[a + 1];
@@ -490,30 +542,21 @@ __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, co
     W1[3] = w[1];
     W2[3] = w[2];
     M0[3] = m0;
-    L0[0] = L0[1];
-    L0[1] = L0[2];
-    L0[2] = lof0;
+    lofb = ((lofb << 1) | (lof0 != 0.0 ? 1u : 0u)) & 7u;
     // D(t-2)
     LD[0] = LD[1];
     LD[1] = LD[2];
-    LD[2] = stage_d(W2[1], W0[0], W0[2], W2[3], L0[0]);
-    PA[0] = PA[1];
-    PA[1] = PA[2];
-    UA[0] = UA[1];
-    UA[1] = UA[2];
-    if (t - 2 >= -2) {
-      PA[2] = cpa;
-      UA[2] = cua;
-    }
+    LD[2] = stage_d(W2[1], W0[0], W0[2], W2[3], (lofb & 4u) ? 1.0 : 0.0);
     // E(t-3)
-    if (t - 3 >= e0) {
-      double dPi = dp_jump(crho, PA[0], PA[1], PA[2], UA[0], UA[1], UA[2]);
-      double outl, outr;
-      stage_e(yw + 2, M0[0], W0[1], W1[0], W2[0], LD[0], LD[1], LD[2], dPi, outl, outr);
-      ol[(int64_t)(t - 3) * sd] = outl;
-      orr[(int64_t)(t - 3) * sd] = outr;
-    }
+    double outl = 0., outr = 0.;
+    if (t - 3 >= e0) stage_e(yw + 2, M0[0], W0[1], W1[0], W2[0], LD[0], LD[1], LD[2], cdp, outl, outr);
+    const int ts = (t - 3 > e0) ? t - 3 : e0;
+    WM_ST(lb, ts) = outl;
+    WM_ST(rb, ts) = outr;
   }
+#undef WM_AT
+#undef WM_LD
+#undef WM_ST
 }
 
 void launch_weno_prep(const double *p, const double *geo, double *aux, const Dims &d, double gam, int wide,
@@ -522,6 +565,7 @@ void launch_weno_prep(const double *p, const double *geo, double *aux, const Dim
   const int e3 = d.n3 > 1 ? (wide ? d.nb3 : 2) : 0;
   int64_t n = (int64_t)(d.n1 + 2 * e1) * (d.n2 + 2 * e2) * (d.n3 + 2 * e3);
   hipLaunchKernelGGL(k_weno_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, geo, aux, d, gam, wide);
+  hipLaunchKernelGGL(k_weno_dpi, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, aux, d, wide);
 }
 
 void launch_weno(int dir, const double *p, const double *aux, double *pleft, double *pright, const Dims &d,
@@ -529,11 +573,13 @@ void launch_weno(int dir, const double *p, const double *aux, double *pleft, dou
   // transverse halo rows (the kernels use the same widths)
   int s1 = d.n1 > 1 ? (wide ? d.nb1 : 1) : 0, s2 = d.n2 > 1 ? (wide ? d.nb2 : 1) : 0;
   int s3 = d.n3 > 1 ? (wide ? d.nb3 : 1) : 0;
+  // the dP/P indicators of this direction (k_weno_dpi)
+  const double *dpi = aux + (int64_t)(1 + dir) * d.spl;
   if (dir == 1 && tu.weno_xmarch == 0) {
     int nseg = (d.n1 + 2 + XOUT - 1) / XOUT;
     int64_t nw = (int64_t)nseg * NPR * (d.n2 + 2 * s2) * (d.n3 + 2 * s3);
     int wpb = tu.weno_xwaves;
-    hipLaunchKernelGGL(k_weno_x, dim3((unsigned)((nw + wpb - 1) / wpb)), dim3(64 * wpb), 0, s, p, aux, pleft,
+    hipLaunchKernelGGL(k_weno_x, dim3((unsigned)((nw + wpb - 1) / wpb)), dim3(64 * wpb), 0, s, p, dpi, pleft,
                        pright, d, nseg, wide);
   } else {
     int n = (dir == 1) ? d.n1 : (dir == 2) ? d.n2 : d.n3;
@@ -551,11 +597,11 @@ void launch_weno(int dir, const double *p, const double *aux, double *pleft, dou
     const int64_t units = ((int64_t)nlb * nseg + 7) / 8 * 8;
     dim3 grid((unsigned)(units * NPR));
     if (dir == 1)
-      hipLaunchKernelGGL(k_weno_m<1>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen, wide, nlb);
+      hipLaunchKernelGGL(k_weno_m<1>, grid, dim3(256), 0, s, p, dpi, pleft, pright, d, nseg, seglen, wide, nlb);
     else if (dir == 2)
-      hipLaunchKernelGGL(k_weno_m<2>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen, wide, nlb);
+      hipLaunchKernelGGL(k_weno_m<2>, grid, dim3(256), 0, s, p, dpi, pleft, pright, d, nseg, seglen, wide, nlb);
     else
-      hipLaunchKernelGGL(k_weno_m<3>, grid, dim3(256), 0, s, p, aux, pleft, pright, d, nseg, seglen, wide, nlb);
+      hipLaunchKernelGGL(k_weno_m<3>, grid, dim3(256), 0, s, p, dpi, pleft, pright, d, nseg, seglen, wide, nlb);
   }
 }
 
