@@ -28,3 +28,14 @@ def test_context_rejects_bad_config_without_gpu_work():
     from harm_finalwham_amd import harmgpu
     with pytest.raises(harmgpu.HarmError):
         harmgpu.Grid(0, 8, 1)
+
+
+def test_context_rejects_planes_past_32_bit_offsets_without_gpu_work():
+    """The reconstruction kernels address a primitive plane with 32-bit byte
+    offsets: a grid whose plane (ghosts included) holds 2^29 zones or more is
+    refused before any device allocation."""
+    from harm_finalwham_amd import harmgpu
+    with pytest.raises(harmgpu.HarmError, match="2\\^29"):
+        harmgpu.Grid(2048, 2048, 160, nb=6)  # 2060 x 2060 x 172 zones with ghosts
+    with pytest.raises(harmgpu.HarmError, match="2\\^29"):
+        harmgpu.Grid(2036, 2036, 116, nb=6)  # exactly 2^29 zones with ghosts
