@@ -71,13 +71,15 @@ void launch_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double 
 void launch_reset_ndt(unsigned long long *ndt_bits, hipStream_t s);
 
 
-// WENO5BND (hg_weno.hip): the per-zone dP/P inputs and the indicator of each
-// direction (5 x spl doubles in aux), then the interface states along dir for
-// every zone; pleft/pright have the layout of the primitives
+// WENO5BND (hg_weno.hip): the per-zone dP/P inputs (launch_weno_prep), then
+// per direction the dP/P indicator and the interface states along dir for
+// every zone (launch_weno; aux holds 5 x spl doubles: the two inputs and the
+// three directions' indicators); pleft/pright have the layout of the
+// primitives
 // wide: the finite-volume flux loops, which span every transverse ghost row
 void launch_weno_prep(const double *p, const double *geo, double *aux, const Dims &d, double gam, int wide,
                       hipStream_t s);
-void launch_weno(int dir, const double *p, const double *aux, double *pleft, double *pright, const Dims &d,
+void launch_weno(int dir, const double *p, double *aux, double *pleft, double *pright, const Dims &d,
                  const Tune &tu, int wide, hipStream_t s);
 
 // initial time step over every zone (advance.c:957 set_dt), folded into *out

@@ -354,32 +354,36 @@ __global__ void __launch_bounds__(256) k_weno_prep(const double *__restrict__ p,
 }
 
 // ---------------------------------------------------------------------------
-// stage A per direction: dP/P at every zone of the box above whose two
-// neighbours along that direction are in the box too, into aux planes 2, 3, 4
-// (directions 1, 2, 3).  It does not depend on the primitive reconstructed,
-// so the reconstruction kernels of all 8 primitives read it from here.
+// stage A along DIR: dP/P at every zone of the box above whose two
+// neighbours along DIR are in the box too, into aux plane 1 + DIR.  It does
+// not depend on the primitive reconstructed, so the reconstruction kernels
+// of all 8 primitives read it from here; each direction's runs on that
+// direction's stream, ahead of its reconstruction.
+// Work-groups go to the 8 XCDs round robin by linear index; each XCD takes
+// a contiguous eighth of the box instead, so that the neighbours a zone
+// reads were fetched into that XCD's L2 by the blocks just before it.
 // ---------------------------------------------------------------------------
+template <int DIR>
 __global__ void __launch_bounds__(256) k_weno_dpi(const double *__restrict__ p, double *__restrict__ aux, Dims d,
                                                   int wide) {
   const int e1 = d.n1 > 1 ? (wide ? d.nb1 : 2) : 0, e2 = d.n2 > 1 ? (wide ? d.nb2 : 2) : 0;
   const int e3 = d.n3 > 1 ? (wide ? d.nb3 : 2) : 0;
   const int b1 = d.n1 + 2 * e1, b2 = d.n2 + 2 * e2, b3 = d.n3 + 2 * e3;
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const unsigned xcd = blockIdx.x & 7u, q = blockIdx.x >> 3, base = gridDim.x >> 3, rem = gridDim.x & 7u;
+  const unsigned lb = xcd * base + (xcd < rem ? xcd : rem) + q;  // a bijection of [0, gridDim.x)
+  int64_t t = lb * (int64_t)blockDim.x + threadIdx.x;
   if (t >= (int64_t)b1 * b2 * b3) return;
   int i = -e1 + (int)(t % b1);
   int64_t r = t / b1;
   int k = -e3 + (int)(r % b3), j = -e2 + (int)(r / b3);
   const int64_t z = d.idx(i, j, k), P = d.spl;
   const double *par = aux, *upar = aux + P;
-  const double rho = p[RHO * P + z];
-  if (d.n1 > 1 && i > -e1 && i < d.n1 + e1 - 1)
-    aux[2 * P + z] = dp_jump(rho, par[z - 1], par[z], par[z + 1], upar[z - 1], upar[z], upar[z + 1]);
-  if (d.n2 > 1 && j > -e2 && j < d.n2 + e2 - 1)
-    aux[3 * P + z] =
-        dp_jump(rho, par[z - d.sj], par[z], par[z + d.sj], upar[z - d.sj], upar[z], upar[z + d.sj]);
-  if (d.n3 > 1 && k > -e3 && k < d.n3 + e3 - 1)
-    aux[4 * P + z] =
-        dp_jump(rho, par[z - d.sk], par[z], par[z + d.sk], upar[z - d.sk], upar[z], upar[z + d.sk]);
+  const int s = (DIR == 1) ? i : (DIR == 2) ? j : k, e = (DIR == 1) ? e1 : (DIR == 2) ? e2 : e3;
+  const int n = (DIR == 1) ? d.n1 : (DIR == 2) ? d.n2 : d.n3;
+  const int64_t sd = (DIR == 1) ? 1 : (DIR == 2) ? d.sj : d.sk;
+  if (s > -e && s < n + e - 1)
+    aux[(1 + DIR) * P + z] =
+        dp_jump(p[RHO * P + z], par[z - sd], par[z], par[z + sd], upar[z - sd], upar[z], upar[z + sd]);
 }
 
 // ---------------------------------------------------------------------------
@@ -565,15 +569,23 @@ void launch_weno_prep(const double *p, const double *geo, double *aux, const Dim
   const int e3 = d.n3 > 1 ? (wide ? d.nb3 : 2) : 0;
   int64_t n = (int64_t)(d.n1 + 2 * e1) * (d.n2 + 2 * e2) * (d.n3 + 2 * e3);
   hipLaunchKernelGGL(k_weno_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, geo, aux, d, gam, wide);
-  hipLaunchKernelGGL(k_weno_dpi, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, aux, d, wide);
 }
 
-void launch_weno(int dir, const double *p, const double *aux, double *pleft, double *pright, const Dims &d,
+void launch_weno(int dir, const double *p, double *aux, double *pleft, double *pright, const Dims &d,
                  const Tune &tu, int wide, hipStream_t s) {
   // transverse halo rows (the kernels use the same widths)
   int s1 = d.n1 > 1 ? (wide ? d.nb1 : 1) : 0, s2 = d.n2 > 1 ? (wide ? d.nb2 : 1) : 0;
   int s3 = d.n3 > 1 ? (wide ? d.nb3 : 1) : 0;
-  // the dP/P indicators of this direction (k_weno_dpi)
+  // the dP/P indicators of this direction, over the box of k_weno_prep
+  {
+    const int e1 = d.n1 > 1 ? (wide ? d.nb1 : 2) : 0, e2 = d.n2 > 1 ? (wide ? d.nb2 : 2) : 0;
+    const int e3 = d.n3 > 1 ? (wide ? d.nb3 : 2) : 0;
+    const int64_t nbox = (int64_t)(d.n1 + 2 * e1) * (d.n2 + 2 * e2) * (d.n3 + 2 * e3);
+    const dim3 gb((unsigned)((nbox + 255) / 256));
+    if (dir == 1) hipLaunchKernelGGL(k_weno_dpi<1>, gb, dim3(256), 0, s, p, aux, d, wide);
+    else if (dir == 2) hipLaunchKernelGGL(k_weno_dpi<2>, gb, dim3(256), 0, s, p, aux, d, wide);
+    else hipLaunchKernelGGL(k_weno_dpi<3>, gb, dim3(256), 0, s, p, aux, d, wide);
+  }
   const double *dpi = aux + (int64_t)(1 + dir) * d.spl;
   if (dir == 1 && tu.weno_xmarch == 0) {
     int nseg = (d.n1 + 2 + XOUT - 1) / XOUT;
