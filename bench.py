@@ -52,15 +52,16 @@ def profile_summary(prof, zones):
         v["mean_ms"] = v["total_ms"] / v["launches"]
         v["achieved_GBs"] = v["bytes_per_zone"] * zones / (v["mean_ms"] * 1e-3) / 1e9
     out["bound"] = dict(total_ms=prof["bound"][0], launches=prof["bound"][1])
-    if "fixup" in prof:
-        out["fixup"] = dict(total_ms=prof["fixup"][0], launches=prof["fixup"][1])
+    for k in ("fixup", "prep"):
+        if prof.get(k, (0.0, 0))[1]:
+            out[k] = dict(total_ms=prof[k][0], launches=prof[k][1])
     return out
 
 
 # device kernels of each logical kernel (WENO5BND build)
-DEVICE_KERNELS = {"recon+flux1": ("hg::k_weno_x", "hg::k_flux_lr<1>"),
-                  "recon+flux2": ("hg::k_weno_m<2>", "hg::k_flux_lr<2>"),
-                  "recon+flux3": ("hg::k_weno_m<3>", "hg::k_flux_lr<3>"),
+DEVICE_KERNELS = {"recon+flux1": ("hg::k_weno_dpi<1>", "hg::k_weno_x", "hg::k_flux_lr<1>"),
+                  "recon+flux2": ("hg::k_weno_dpi<2>", "hg::k_weno_m<2>", "hg::k_flux_lr<2>"),
+                  "recon+flux3": ("hg::k_weno_dpi<3>", "hg::k_weno_m<3>", "hg::k_flux_lr<3>"),
                   "emf": ("hg::k_emf",), "update": ("hg::k_update<false, false>", "hg::k_invert<false, 0>")}
 TRAFFIC_PROFILE = os.path.join(HERE, "profiles", "hbm_traffic_torus3d_256x128x64_weno_hll.md")
 VALU_PROFILE = os.path.join(HERE, "profiles", "valu_issue_torus3d_256x128x64_weno_hll.md")

@@ -410,8 +410,8 @@ extern "C" int hg_profile_read(hg_ctx *c, double ms[HG_PROF_NCLASS], int64_t lau
 }
 
 extern "C" const char *hg_profile_class_name(int k) {
-  static const char *names[HG_PROF_NCLASS] = {"bound", "recon1", "recon2", "recon3", "flux1",
-                                              "flux2", "flux3", "emf",    "update", "fixup"};
+  static const char *names[HG_PROF_NCLASS] = {"bound", "recon1", "recon2", "recon3", "flux1", "flux2",
+                                              "flux3", "emf",    "update", "fixup",  "prep"};
   return (k >= 0 && k < HG_PROF_NCLASS) ? names[k] : "";
 }
 
@@ -425,9 +425,10 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
   int nn[4] = {0, d.n1, d.n2, d.n3};
   launch_reset_ndt(c->ndt_bits, s);
   if (f.lim == HG_LIM_WENO5BND) {
-    // the dP/P inputs are shared by the three directions; timed with direction 1
-    prof_mark(c, HG_PROF_RECON1, s);
+    // the dP/P inputs are shared by the three directions
+    prof_mark(c, HG_PROF_PREP, s);
     launch_weno_prep(pb, c->geo, c->aux, d, f.gam, f.doenoflux == HG_ENOFINITEVOLUME, s);
+    prof_mark(c, -1, s);
   }
   // the directions are independent (own interface states, fluxes and ndt
   // slot): j and k on the side streams, i on s, joined before the EMFs.
@@ -463,7 +464,7 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
                         f.gam, f.cour, f.dx[dir], f.fluxmethod, c->ndt_bits + dir - 1, c->err, c->tune, sd);
       prof_mark(c, -1, sd);
     } else {
-      if (dir != 1) prof_mark(c, HG_PROF_RECON1 + dir - 1, sd);
+      prof_mark(c, HG_PROF_RECON1 + dir - 1, sd);
       launch_weno(dir, pb, c->aux, pls[dir], prs[dir], d, c->tune, f.doenoflux == HG_ENOFINITEVOLUME, sd);
       prof_mark(c, HG_PROF_FLUX1 + dir - 1, sd);
       launch_flux_lr(dir, pls[dir], prs[dir], c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,

@@ -29,7 +29,7 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag", "hg_diagnostics", "hg_diag_record", "hg_rdump_parse",
            "hg_rdump_read", "hg_set_conserved", "hg_get_conserved", "hg_rdump_write", "hg_advance_substep",
            "hg_dump_inversions", "hg_utoprim_batch_noalloc", "hg_set_inversion"]
-HG_PROF_NCLASS = 10
+HG_PROF_NCLASS = 11
 
 
 class HarmError(RuntimeError):

@@ -449,7 +449,8 @@ const char *hg_last_error(void);
 #define HG_PROF_EMF 7
 #define HG_PROF_UPDATE 8 /* update + source + inversion + fixup1zone (+ dt) */
 #define HG_PROF_FIXUP 9  /* post_fixup: neighbour check and average of failed zones */
-#define HG_PROF_NCLASS 10
+#define HG_PROF_PREP 10  /* WENO5BND: the per-zone dP/P inputs the three directions share */
+#define HG_PROF_NCLASS 11
 int hg_profile(hg_ctx *ctx, int enable);
 int hg_profile_read(hg_ctx *ctx, double ms[HG_PROF_NCLASS], int64_t launches[HG_PROF_NCLASS]);
 const char *hg_profile_class_name(int k);
