@@ -7,7 +7,8 @@
 // flag it sets is seen by the zones after it.  Here:
 //   k_fx_check    one thread per active zone: the zone's gamma; a zone with
 //                 gamma >= 2 (a candidate) stores, per neighbour, whether the
-//                 gamma ratio and the u ratio would vote, and joins a list
+//                 gamma ratio and the u ratio would vote, and joins a list;
+//                 any other zone whose inversion failed joins a second list
 //   k_fx_resolve  one workgroup: the candidates' flags as the sequential loop
 //                 leaves them.  A candidate's flag depends on the flags of its
 //                 three earlier neighbours (i,j-1), (i-1,j), (i-1,j-1) as the
@@ -15,8 +16,9 @@
 //                 Jacobi sweeps over the candidates reach that unique fixed
 //                 point (the dependencies follow the loop order, so they
 //                 cannot cycle) after at most (list length + 1) sweeps
-//   k_fx_average  one thread per active zone with a nonzero flag: the new
-//                 primitives and flag into a list.  A neighbour earlier in the
+//   k_fx_average  one thread per entry of the two lists whose flag is
+//                 nonzero (every zone the average can change is in one of
+//                 them): the new primitives and flag into a list.  A neighbour earlier in the
 //                 loop order is seen with the flag the loop gave it (FIXED
 //                 for every code the accounting resets), a later one with its
 //                 checked flag, a ghost with its bound_pflag copy; values are
@@ -51,7 +53,8 @@ struct FxBufs {
   int *fix_z;         // zones the average changed
   double *fix_v;      // their RHO..U3 (5 x cap, structure of arrays)
   int8_t *fix_flag;   // their final flag
-  unsigned *count;    // [candidates, fixes]
+  int *fail_z;        // failed inversions that are not candidates
+  unsigned *count;    // [candidates, fixes, failed non-candidates]
   int cap;
 };
 
@@ -130,13 +133,17 @@ __global__ void __launch_bounds__(256) k_fx_check(const double *__restrict__ p, 
   int k = (int)(r % d.n3);
   int j = (int)(r / d.n3);
   int64_t z = d.idx(i, j, k);
-  b.flag1[z] = b.flag0[z];
+  const int8_t f0 = b.flag0[z];
+  b.flag1[z] = f0;
   double g;
   if (zone_gamma(p, geo, d, i, j, k, g)) {
     atomicOr(err, 8);
     return;
   }
-  if (!(g >= 2.0)) return;
+  if (!(g >= 2.0)) {
+    if (f0 != 0) b.fail_z[atomicAdd(&b.count[2], 1u)] = (int)z;
+    return;
+  }
   const int s1 = d.n1 > 1, s2 = d.n2 > 1;
   const double uu = p[z + UU * d.spl];
   unsigned bits = 0;
@@ -230,19 +237,13 @@ __device__ __forceinline__ int avg_flag(const Dims &d, const int8_t *flag0, cons
   return earlier ? fx_final(f, finalstep) : f;
 }
 
-__global__ void __launch_bounds__(256) k_fx_average(const double *__restrict__ p, const double *__restrict__ geo,
-                                                    Dims d, FxBufs b, int finalstep, double gammamax, double gam,
-                                                    double *__restrict__ ucons, int *__restrict__ err) {
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
-  if (t >= total) return;
-  int i = (int)(t % d.n1);
-  int64_t r = t / d.n1;
-  int k = (int)(r % d.n3);
-  int j = (int)(r / d.n3);
-  int64_t z = d.idx(i, j, k);
-  const int f = b.flag1[z];
-  if (f == 0) return;
+// one zone of the average (z: padded index of an active zone with flag f != 0)
+__device__ __forceinline__ void fx_average_zone(const double *__restrict__ p, const double *__restrict__ geo,
+                                                const Dims &d, const FxBufs &b, int64_t z, int f, int finalstep,
+                                                double gammamax, double gam, double *__restrict__ ucons,
+                                                int *__restrict__ err) {
+  int i, j, k;
+  unidx(d, z, i, j, k);
   const int64_t P = d.spl;
   const int cnt = fx_counted(f, finalstep);
   if (cnt == 2) {
@@ -350,6 +351,20 @@ __global__ void __launch_bounds__(256) k_fx_average(const double *__restrict__ p
   b.fix_flag[slot] = (int8_t)fx_final(f, finalstep);
 }
 
+// over the candidates (their flags as k_fx_resolve left them) and the other
+// failed inversions; a few blocks, since the lists are short next to the
+// grid and the average takes the whole register file of a SIMD lane
+__global__ void __launch_bounds__(256) k_fx_average(const double *__restrict__ p, const double *__restrict__ geo,
+                                                    Dims d, FxBufs b, int finalstep, double gammamax, double gam,
+                                                    double *__restrict__ ucons, int *__restrict__ err) {
+  const unsigned nc = b.count[0], n = nc + b.count[2];
+  for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const int64_t z = (s < nc) ? b.cand_z[s] : b.fail_z[s - nc];
+    const int f = b.flag1[z];
+    if (f != 0) fx_average_zone(p, geo, d, b, z, f, finalstep, gammamax, gam, ucons, err);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_fx_apply(double *__restrict__ p, Dims d, FxBufs b) {
   const unsigned n = b.count[1];
   for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
@@ -360,7 +375,7 @@ __global__ void __launch_bounds__(256) k_fx_apply(double *__restrict__ p, Dims d
 }
 
 __global__ void k_fx_reset(unsigned *count) {
-  if (threadIdx.x < 2) count[threadIdx.x] = 0u;
+  if (threadIdx.x < 3) count[threadIdx.x] = 0u;
 }
 
 }  // namespace
@@ -386,7 +401,8 @@ int fixup_alloc(hg_ctx *c) {
   get((void **)&c->fx_fix_z, (size_t)act * sizeof(int));
   get((void **)&c->fx_fix_v, (size_t)act * 5 * sizeof(double));
   get((void **)&c->fx_fix_flag, (size_t)act);
-  get((void **)&c->fx_count, 2 * sizeof(unsigned));
+  get((void **)&c->fx_fail_z, (size_t)act * sizeof(int));
+  get((void **)&c->fx_count, 3 * sizeof(unsigned));
   if (e == hipSuccess) e = hipMemset(c->pflag, 0, (size_t)d.spl);
   if (e != hipSuccess) {
     set_error("hg_ctx_create: %s", hipGetErrorString(e));
@@ -397,7 +413,7 @@ int fixup_alloc(hg_ctx *c) {
 
 void fixup_free(hg_ctx *c) {
   void *b[] = {c->pflag, c->pchk, c->fx_cand_z, c->fx_cand_bits, c->fx_cand_new, c->fx_fix_z, c->fx_fix_v,
-               c->fx_fix_flag, c->fx_count};
+               c->fx_fix_flag, c->fx_fail_z, c->fx_count};
   for (void *p : b)
     if (p) (void)hipFree(p);
 }
@@ -412,6 +428,7 @@ static FxBufs bufs(hg_ctx *c) {
   b.fix_z = c->fx_fix_z;
   b.fix_v = c->fx_fix_v;
   b.fix_flag = c->fx_fix_flag;
+  b.fail_z = c->fx_fail_z;
   b.count = c->fx_count;
   b.cap = (int)((int64_t)c->d.n1 * c->d.n2 * c->d.n3);
   return b;
@@ -427,7 +444,7 @@ void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s) {
   hipLaunchKernelGGL(k_fx_reset, dim3(1), dim3(64), 0, s, b.count);
   hipLaunchKernelGGL(k_fx_check, dim3(nb), dim3(256), 0, s, p, c->geo, d, b, c->err);
   hipLaunchKernelGGL(k_fx_resolve, dim3(1), dim3(1024), 0, s, d, b, c->err);
-  hipLaunchKernelGGL(k_fx_average, dim3(nb), dim3(256), 0, s, p, c->geo, d, b, finalstep, c->cfg.gammamax,
+  hipLaunchKernelGGL(k_fx_average, dim3(256), dim3(256), 0, s, p, c->geo, d, b, finalstep, c->cfg.gammamax,
                      c->cfg.gam, c->ucons, c->err);
   hipLaunchKernelGGL(k_fx_apply, dim3(256), dim3(256), 0, s, p, d, b);
 }

@@ -48,6 +48,7 @@ struct hg_ctx {
   uint16_t *fx_cand_bits = nullptr;
   int8_t *fx_cand_new = nullptr, *fx_fix_flag = nullptr;
   double *fx_fix_v = nullptr;
+  int *fx_fail_z = nullptr;
   unsigned *fx_count = nullptr;
   // diagnostics (hg_diag.hip): scratch on first use, time-integrated
   // boundary fluxes (diag.c pcum) of the global region
