@@ -98,16 +98,17 @@ __device__ __forceinline__ void block_sum_to(unsigned long long a, unsigned long
 // ---------------------------------------------------------------------------
 __global__ void k_bound_x1(double *__restrict__ p, const double *__restrict__ geo, Dims d, double gammamax,
                            int *__restrict__ err) {
-  // threads: ((side*nb1 + g) * n3 + k) * n2 + j
+  // threads: ((k * n2 + j) * 2 + side) * nb1 + g: the ghost layers of a row
+  // on consecutive lanes (contiguous in memory, one source zone per side)
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = 2LL * d.nb1 * d.n2 * d.n3;
   if (t >= total) return;
-  int j = t % d.n2;
-  int64_t r = t / d.n2;
-  int k = r % d.n3;
-  r /= d.n3;
-  int g = r % d.nb1;
-  int side = (int)(r / d.nb1);
+  int g = (int)(t % d.nb1);
+  int64_t r = t / d.nb1;
+  int side = (int)(r % 2);
+  r /= 2;
+  int j = (int)(r % d.n2);
+  int k = (int)(r / d.n2);
   int i = side ? d.n1 + g : -1 - g;
   int isrc = side ? d.n1 - 1 : 0;
   double pr[NPR];
