@@ -213,13 +213,50 @@ __device__ __forceinline__ bool cusp(const double *y) {
   return false;
 }
 
+// Division by a constant divisor d through the steps of the hardware's
+// correctly rounded FP64 division (hipcc lowers x / d to v_div_scale of d
+// and of x, v_rcp_f64, two Newton steps on the reciprocal, the quotient,
+// its residual, v_div_fmas, v_div_fixup) with the reciprocal part done once:
+// for a finite non-zero d of normal magnitude and a numerator of magnitude
+// in [2^-660, 2^300] v_div_scale changes neither operand nor sets the flag
+// that makes v_div_fmas rescale, and v_div_fixup returns the corrected
+// quotient, so these steps give x / d bit for bit; a zero numerator gives
+// +0 for both signs.  Callers check div_num_ok and divide in full otherwise.
+__device__ __forceinline__ double rcp_refined(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-d, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ double div_core(double x, double d, double y) {
+  const double q = x * y;
+  const double r = fma(-d, q, x);
+  return fma(r, y, q);
+}
+__device__ __forceinline__ bool div_num_ok(double x) {
+  const double a = fabs(x);
+  return a == 0.0 || (a >= 0x1p-660 && a <= 0x1p300);
+}
+
 // smonointerp.c:296 compute_mono_indicator_point_eno5
 __device__ __forceinline__ double mono_eno5(const double *yin) {
-  double ld1 = (((-3 * yin[-2] + yin[2]) + (-10 * yin[-1] - 6 * yin[1])) + 18 * yin[0]) / 12.;
-  double ld2 = (((11 * yin[-2] - yin[2]) + (-20 * yin[-1] + 4 * yin[1])) + 6 * yin[0]) / 12.;
+  // the four divisions by 12 (their zero quotients' signs cannot reach the
+  // result: the minmods below return a literal 0 for a zero argument)
+  const double n1 = (((-3 * yin[-2] + yin[2]) + (-10 * yin[-1] - 6 * yin[1])) + 18 * yin[0]);
+  const double n2 = (((11 * yin[-2] - yin[2]) + (-20 * yin[-1] + 4 * yin[1])) + 6 * yin[0]);
+  const double n4 = (((-yin[-2] + 3 * yin[2]) + (6 * yin[-1] + 10 * yin[1])) - 18 * yin[0]);
+  const double n5 = (((-yin[-2] + 11 * yin[2]) + (4 * yin[-1] - 20 * yin[1])) + 6 * yin[0]);
+  const double y12 = rcp_refined(12.);
+  double ld1 = div_core(n1, 12., y12), ld2 = div_core(n2, 12., y12);
+  double rd1 = div_core(n4, 12., y12), rd2 = div_core(n5, 12., y12);
+  if (!(div_num_ok(n1) && div_num_ok(n2) && div_num_ok(n4) && div_num_ok(n5))) {
+    ld1 = n1 / 12.;
+    ld2 = n2 / 12.;
+    rd1 = n4 / 12.;
+    rd2 = n5 / 12.;
+  }
   double ld3 = (((-3 * yin[-2] - yin[2]) + (10 * yin[-1] + 6 * yin[1])) - 12 * yin[0]) / 2.;
-  double rd1 = (((-yin[-2] + 3 * yin[2]) + (6 * yin[-1] + 10 * yin[1])) - 18 * yin[0]) / 12.;
-  double rd2 = (((-yin[-2] + 11 * yin[2]) + (4 * yin[-1] - 20 * yin[1])) + 6 * yin[0]) / 12.;
   double rd3 = (((yin[-2] + 3 * yin[2]) - (6 * yin[-1] + 10 * yin[1])) + 12 * yin[0]) / 2.;
   ld1 = wminmod(ld1, yin[-1] - yin[-2]);
   rd1 = wminmod(rd1, yin[2] - yin[1]);
