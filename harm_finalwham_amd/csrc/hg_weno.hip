@@ -72,9 +72,66 @@ __device__ __forceinline__ double sum5(const double *a) {
   return s;
 }
 
+// Division by a constant divisor d through the steps of the hardware's
+// correctly rounded FP64 division (hipcc lowers x / d to v_div_scale of d
+// and of x, v_rcp_f64, two Newton steps on the reciprocal, the quotient,
+// its residual, v_div_fmas, v_div_fixup) with the reciprocal part done once:
+// for a finite non-zero d of normal magnitude and a numerator of magnitude
+// in [2^-660, 2^300] v_div_scale changes neither operand nor sets the flag
+// that makes v_div_fmas rescale, and v_div_fixup returns the corrected
+// quotient, so these steps give x / d bit for bit; a zero numerator gives
+// +0 for both signs.  Callers check div_num_ok and divide in full otherwise.
+__device__ __forceinline__ double rcp_refined(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-d, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ double div_core(double x, double d, double y) {
+  const double q = x * y;
+  const double r = fma(-d, q, x);
+  return fma(r, y, q);
+}
+__device__ __forceinline__ bool div_num_ok(double x) {
+  const double a = fabs(x);
+  return a == 0.0 || (a >= 0x1p-660 && a <= 0x1p300);
+}
+
+__device__ __forceinline__ bool div_den_ok(double d) { return d >= 0x1p-300 && d <= 0x1p300; }
+
+// x[k] / d for K numerators of one positive divisor, none of them -0.  SD
+// false: K full divisions.  The shuffle kernel along i runs faster with the
+// shared reciprocal, the marching kernels (whose loop the range checks and
+// their branches lengthen) without.
+template <int K, bool SD>
+__device__ __forceinline__ void div_shared(double *x, double d) {
+  if (!SD) {
+#pragma unroll
+    for (int k = 0; k < K; k++) x[k] /= d;
+    return;
+  }
+  const double y = rcp_refined(d);
+  double q[K];
+  bool ok = div_den_ok(d);
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    q[k] = div_core(x[k], d, y);
+    ok = ok && div_num_ok(x[k]);
+  }
+  if (ok) {
+#pragma unroll
+    for (int k = 0; k < K; k++) x[k] = q[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; k++) x[k] /= d;
+  }
+}
+
 // order-3 unoptimised weights (reconstructeno.c:1406) for a point far from
 // the line ends; u points at the cell.  Returns the cusp-driven lower-order
 // fraction (1 for a cusp, else 0).
+template <bool SD>
 __device__ __forceinline__ double weights3(double mono0, double mono1, const double *u, double *w) {
   if (mono0 == 1) {
     double sw = 1.0 / 3;
@@ -99,21 +156,22 @@ __device__ __forceinline__ double weights3(double mono0, double mono1, const dou
 #pragma unroll
   for (int s = 0; s < 3; s++) si[s] += e;
   double ss = sum3(si[0], si[1], si[2]);
-#pragma unroll
-  for (int s = 0; s < 3; s++) si[s] /= (ss + DBL_MIN);
+  // sums of squares and positive terms: no numerator is -0
+  div_shared<3, SD>(si, ss + DBL_MIN);
 #pragma unroll
   for (int s = 0; s < 3; s++) {
     double t = si[s] + WENO_EPS / 1.;
     t *= t;
-    w[s] = 1. / t;
+    w[s] = 1.;
+    div_shared<1, SD>(&w[s], t);
   }
   double sw = sum3(w[0], w[1], w[2]);
-#pragma unroll
-  for (int s = 0; s < 3; s++) w[s] /= (sw + 0);
+  div_shared<3, SD>(w, sw + 0);
   return lof;
 }
 
 // order-2 unoptimised + optimised weights (reconstructeno.c:1406 and :1617 with order 2)
+template <bool SD>
 __device__ __forceinline__ void weights2(double mono0, const double *u, double *wl, double *wr) {
   double w[2];
   if (mono0 == 1) {
@@ -130,27 +188,26 @@ __device__ __forceinline__ void weights2(double mono0, const double *u, double *
   si[0] += e;
   si[1] += e;
   double ss = sum2(si[0], si[1]);
-  si[0] /= (ss + DBL_MIN);
-  si[1] /= (ss + DBL_MIN);
+  // squares, positive sums and their products with positive constants: no
+  // numerator is -0
+  div_shared<2, SD>(si, ss + DBL_MIN);
 #pragma unroll
   for (int s = 0; s < 2; s++) {
     double t = si[s] + WENO_EPS / 1.;
     t *= t;
-    w[s] = 1. / t;
+    w[s] = 1.;
+    div_shared<1, SD>(&w[s], t);
   }
   double sw = sum2(w[0], w[1]);
-  w[0] /= (sw + 0);
-  w[1] /= (sw + 0);
+  div_shared<2, SD>(w, sw + 0);
   wl[0] = w[0] * (1. / 4.);
   wl[1] = w[1] * (3. / 4);
   wr[0] = w[0] * (3. / 4.);
   wr[1] = w[1] * (1. / 4.);
   double sl = sum2(wl[0], wl[1]);
-  wl[0] /= (sl + DBL_MIN);
-  wl[1] /= (sl + DBL_MIN);
+  div_shared<2, SD>(wl, sl + DBL_MIN);
   double sr = sum2(wr[0], wr[1]);
-  wr[0] /= (sr + DBL_MIN);
-  wr[1] /= (sr + DBL_MIN);
+  div_shared<2, SD>(wr, sr + DBL_MIN);
 }
 
 // reconstructeno.c:730 eno_cvt for order 3, left (C2L) or right (C2R) face
@@ -213,32 +270,6 @@ __device__ __forceinline__ bool cusp(const double *y) {
   return false;
 }
 
-// Division by a constant divisor d through the steps of the hardware's
-// correctly rounded FP64 division (hipcc lowers x / d to v_div_scale of d
-// and of x, v_rcp_f64, two Newton steps on the reciprocal, the quotient,
-// its residual, v_div_fmas, v_div_fixup) with the reciprocal part done once:
-// for a finite non-zero d of normal magnitude and a numerator of magnitude
-// in [2^-660, 2^300] v_div_scale changes neither operand nor sets the flag
-// that makes v_div_fmas rescale, and v_div_fixup returns the corrected
-// quotient, so these steps give x / d bit for bit; a zero numerator gives
-// +0 for both signs.  Callers check div_num_ok and divide in full otherwise.
-__device__ __forceinline__ double rcp_refined(double d) {
-  double y = __builtin_amdgcn_rcp(d);
-  double e = fma(-d, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-d, y, 1.0);
-  return fma(y, e, y);
-}
-__device__ __forceinline__ double div_core(double x, double d, double y) {
-  const double q = x * y;
-  const double r = fma(-d, q, x);
-  return fma(r, y, q);
-}
-__device__ __forceinline__ bool div_num_ok(double x) {
-  const double a = fabs(x);
-  return a == 0.0 || (a >= 0x1p-660 && a <= 0x1p300);
-}
-
 // smonointerp.c:296 compute_mono_indicator_point_eno5
 __device__ __forceinline__ double mono_eno5(const double *yin) {
   // the four divisions by 12 (their zero quotients' signs cannot reach the
@@ -284,11 +315,12 @@ __device__ __forceinline__ double eno5(bool right, const double *y) {
 // the stages at one point
 // ---------------------------------------------------------------------------
 // C: y points at the cell, y[-2..2] are read
+template <bool SD>
 __device__ __forceinline__ void stage_c(const double *y, double w[3], double &m0, double &lof0) {
   double m1;
   if (cusp(y)) { m0 = 0; m1 = 1; }
   else { m0 = mono_eno5(y); m1 = m0; }
-  lof0 = weights3(m0, m1, y, w);
+  lof0 = weights3<SD>(m0, m1, y, w);
   if (m0 > 0.0 && m0 < 1.0) {  // reconstructeno.c:702 mono_adjust_unoptimized_weights
     double simpleweight = 1.0 / 3.0;
 #pragma unroll
@@ -312,6 +344,7 @@ __device__ __forceinline__ double dp_jump(double rho, double parm, double par0, 
 }
 
 // E: y points at the cell; lofm/lof0/lofp are the stage-D fractions at s-1, s, s+1
+template <bool SD>
 __device__ __forceinline__ void stage_e(const double *y, double m0, double w0, double w1, double w2, double lofm,
                                         double lof0, double lofp, double dPi, double &outl, double &outr) {
   double shock = wmin((fabs(dPi) - 0.5 * REDUCE_DP) / (0.5 * REDUCE_DP), 1.);
@@ -327,14 +360,14 @@ __device__ __forceinline__ void stage_e(const double *y, double m0, double w0, d
   double wl[3] = {w0 * (1. / 16.), w1 * (5. / 8.), w2 * (5. / 16.)};
   double wr[3] = {w0 * (5. / 16.), w1 * (5. / 8.), w2 * (1. / 16.)};
   double sl = sum3(wl[0], wl[1], wl[2]), sr = sum3(wr[0], wr[1], wr[2]);
-#pragma unroll
-  for (int c = 0; c < 3; c++) { wl[c] /= (sl + DBL_MIN); wr[c] /= (sr + DBL_MIN); }
+  div_shared<3, SD>(wl, sl + DBL_MIN);
+  div_shared<3, SD>(wr, sr + DBL_MIN);
   double l2[2] = {0., 0.}, r2[2] = {0., 0.};
   bool two = (0.0 < lof);
   if (two) {
 #pragma unroll
     for (int c = 0; c < 3; c++) { wl[c] *= (1. - lof); wr[c] *= (1. - lof); }
-    weights2(m0, y, l2, r2);
+    weights2<SD>(m0, y, l2, r2);
     l2[0] *= lof; l2[1] *= lof; r2[0] *= lof; r2[1] *= lof;
   }
   double outv[2];
@@ -452,7 +485,7 @@ __global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, co
   if (s <= n + 3) {
 #pragma unroll
     for (int a = 0; a < 5; a++) y[a] = yl[s - 2 + a];
-    stage_c(y + 2, w, m0, lof0);
+    stage_c<true>(y + 2, w, m0, lof0);
   }
   // stage D needs the weights of s-2 and s+2 (lanes -2 and +2)
   double w0m2 = __shfl_up(w[0], 2, 64);
@@ -463,7 +496,7 @@ __global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, co
   if (lane >= 3 && lane < 3 + XOUT && s <= n) {
     const double dPi = dpi[zl + s];
     double ol, orr;
-    stage_e(y + 2, m0, w[0], w[1], w[2], lofm, lofd, lofp, dPi, ol, orr);
+    stage_e<true>(y + 2, m0, w[0], w[1], w[2], lofm, lofd, lofp, dPi, ol, orr);
     int64_t z = (int64_t)pl * d.spl + zl + s;
     pleft[z] = ol;
     pright[z] = orr;
@@ -574,7 +607,7 @@ __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, co
     for (int a = 0; a < 7; a++) yw[a] = yw[a + 1];
     yw[7] = cy;
     double w[3], m0, lof0;
-    stage_c(yw + 5, w, m0, lof0);
+    stage_c<false>(yw + 5, w, m0, lof0);
 #pragma unroll
     for (int a = 0; a < 4; a++) W0[a] = W0[a + 1];
     W0[4] = w[0];
@@ -590,7 +623,7 @@ __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, co
     LD[2] = stage_d(W2[1], W0[0], W0[2], W2[3], (lofb & 4u) ? 1.0 : 0.0);
     // E(t-3)
     double outl = 0., outr = 0.;
-    if (t - 3 >= e0) stage_e(yw + 2, M0[0], W0[1], W1[0], W2[0], LD[0], LD[1], LD[2], cdp, outl, outr);
+    if (t - 3 >= e0) stage_e<false>(yw + 2, M0[0], W0[1], W1[0], W2[0], LD[0], LD[1], LD[2], cdp, outl, outr);
     const int ts = (t - 3 > e0) ? t - 3 : e0;
     WM_ST(lb, ts) = outl;
     WM_ST(rb, ts) = outr;
