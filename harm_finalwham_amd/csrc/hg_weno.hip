@@ -32,6 +32,7 @@
 
 #include <cfloat>
 
+#include "hg_div.h"
 #include "hg_kernels.h"
 #include "hg_phys.h"
 
@@ -70,62 +71,6 @@ __device__ __forceinline__ double sum5(const double *a) {
   s += a[1] + a[3];
   s += a[2] * 1;
   return s;
-}
-
-// Division by a constant divisor d through the steps of the hardware's
-// correctly rounded FP64 division (hipcc lowers x / d to v_div_scale of d
-// and of x, v_rcp_f64, two Newton steps on the reciprocal, the quotient,
-// its residual, v_div_fmas, v_div_fixup) with the reciprocal part done once:
-// for a finite non-zero d of normal magnitude and a numerator of magnitude
-// in [2^-660, 2^300] v_div_scale changes neither operand nor sets the flag
-// that makes v_div_fmas rescale, and v_div_fixup returns the corrected
-// quotient, so these steps give x / d bit for bit; a zero numerator gives
-// +0 for both signs.  Callers check div_num_ok and divide in full otherwise.
-__device__ __forceinline__ double rcp_refined(double d) {
-  double y = __builtin_amdgcn_rcp(d);
-  double e = fma(-d, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-d, y, 1.0);
-  return fma(y, e, y);
-}
-__device__ __forceinline__ double div_core(double x, double d, double y) {
-  const double q = x * y;
-  const double r = fma(-d, q, x);
-  return fma(r, y, q);
-}
-__device__ __forceinline__ bool div_num_ok(double x) {
-  const double a = fabs(x);
-  return a == 0.0 || (a >= 0x1p-660 && a <= 0x1p300);
-}
-
-__device__ __forceinline__ bool div_den_ok(double d) { return d >= 0x1p-300 && d <= 0x1p300; }
-
-// x[k] / d for K numerators of one positive divisor, none of them -0.  SD
-// false: K full divisions.  The shuffle kernel along i runs faster with the
-// shared reciprocal, the marching kernels (whose loop the range checks and
-// their branches lengthen) without.
-template <int K, bool SD>
-__device__ __forceinline__ void div_shared(double *x, double d) {
-  if (!SD) {
-#pragma unroll
-    for (int k = 0; k < K; k++) x[k] /= d;
-    return;
-  }
-  const double y = rcp_refined(d);
-  double q[K];
-  bool ok = div_den_ok(d);
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    q[k] = div_core(x[k], d, y);
-    ok = ok && div_num_ok(x[k]);
-  }
-  if (ok) {
-#pragma unroll
-    for (int k = 0; k < K; k++) x[k] = q[k];
-  } else {
-#pragma unroll
-    for (int k = 0; k < K; k++) x[k] /= d;
-  }
 }
 
 // order-3 unoptimised weights (reconstructeno.c:1406) for a point far from
@@ -274,19 +219,13 @@ __device__ __forceinline__ bool cusp(const double *y) {
 __device__ __forceinline__ double mono_eno5(const double *yin) {
   // the four divisions by 12 (their zero quotients' signs cannot reach the
   // result: the minmods below return a literal 0 for a zero argument)
-  const double n1 = (((-3 * yin[-2] + yin[2]) + (-10 * yin[-1] - 6 * yin[1])) + 18 * yin[0]);
-  const double n2 = (((11 * yin[-2] - yin[2]) + (-20 * yin[-1] + 4 * yin[1])) + 6 * yin[0]);
-  const double n4 = (((-yin[-2] + 3 * yin[2]) + (6 * yin[-1] + 10 * yin[1])) - 18 * yin[0]);
-  const double n5 = (((-yin[-2] + 11 * yin[2]) + (4 * yin[-1] - 20 * yin[1])) + 6 * yin[0]);
-  const double y12 = rcp_refined(12.);
-  double ld1 = div_core(n1, 12., y12), ld2 = div_core(n2, 12., y12);
-  double rd1 = div_core(n4, 12., y12), rd2 = div_core(n5, 12., y12);
-  if (!(div_num_ok(n1) && div_num_ok(n2) && div_num_ok(n4) && div_num_ok(n5))) {
-    ld1 = n1 / 12.;
-    ld2 = n2 / 12.;
-    rd1 = n4 / 12.;
-    rd2 = n5 / 12.;
-  }
+  const double n[4] = {(((-3 * yin[-2] + yin[2]) + (-10 * yin[-1] - 6 * yin[1])) + 18 * yin[0]),
+                        (((11 * yin[-2] - yin[2]) + (-20 * yin[-1] + 4 * yin[1])) + 6 * yin[0]),
+                        (((-yin[-2] + 3 * yin[2]) + (6 * yin[-1] + 10 * yin[1])) - 18 * yin[0]),
+                        (((-yin[-2] + 11 * yin[2]) + (4 * yin[-1] - 20 * yin[1])) + 6 * yin[0])};
+  double q[4];
+  div_const4(n, 12., rcp_refined(12.), q);
+  double ld1 = q[0], ld2 = q[1], rd1 = q[2], rd2 = q[3];
   double ld3 = (((-3 * yin[-2] - yin[2]) + (10 * yin[-1] + 6 * yin[1])) - 12 * yin[0]) / 2.;
   double rd3 = (((yin[-2] + 3 * yin[2]) - (6 * yin[-1] + 10 * yin[1])) + 12 * yin[0]) / 2.;
   ld1 = wminmod(ld1, yin[-1] - yin[-2]);

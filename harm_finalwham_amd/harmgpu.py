@@ -28,7 +28,7 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_mlp_features", "hg_mlp_forward", "hg_mlp_gradient", "hg_mlp_train",
            "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag", "hg_diagnostics", "hg_diag_record", "hg_rdump_parse",
            "hg_rdump_read", "hg_set_conserved", "hg_get_conserved", "hg_rdump_write", "hg_advance_substep",
-           "hg_dump_inversions", "hg_utoprim_batch_noalloc", "hg_set_inversion"]
+           "hg_dump_inversions", "hg_utoprim_batch_noalloc", "hg_set_inversion", "hg_check_division"]
 HG_PROF_NCLASS = 11
 
 
@@ -138,6 +138,7 @@ def lib():
     L.hg_make_c2p_batch.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_uint64, vp, vp, vp, vp, vp,
                                     vp, vp, vp]
     L.hg_last_error.restype = C.c_char_p
+    L.hg_check_division.argtypes = [C.c_int64, C.c_uint64, i64p]
     L.hg_c2p_fallbacks.argtypes = [i64p]
     L.hg_graph_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.POINTER(vp)]
     L.hg_graph_launch.argtypes = [vp, vp]
@@ -444,6 +445,15 @@ def make_c2p_batch(n, gam=4.0 / 3.0, a=0.9375, seed=1, device="cuda"):
                                                                             "gcovpert", "p_true", "p_guess")],
                                    None))
     return out
+
+
+def check_division(n, seed=1):
+    """hg_check_division: mismatches of the shared-reciprocal division
+    against the hardware's x / d over n generated operand sets ([shared
+    divisor, reciprocal, divisor 12, sets on the shortcut])."""
+    r = np.zeros(4, dtype=np.int64)
+    _check(lib().hg_check_division(int(n), int(seed), r.ctypes.data_as(C.POINTER(C.c_int64))))
+    return [int(v) for v in r]
 
 
 def c2p_fallbacks():

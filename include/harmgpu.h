@@ -437,6 +437,15 @@ int hg_autotune(hg_ctx *ctx, int nsteps, hg_tune *best, void *stream);
 
 const char *hg_last_error(void);
 
+/* Device self-check of the shared-reciprocal FP64 division the WENO5BND
+   kernels use (a test hook, no counterpart in the reference): n generated
+   operand sets from seed; result[0], [1], [2] count the mismatches against
+   the hardware's own x / d of three numerators sharing one divisor, of 1 / d,
+   and of four signed numerators of the divisor 12 (a zero quotient compared
+   without its sign); result[3] counts the sets inside the shortcut's
+   ranges.  n at most 2^32. */
+int hg_check_division(int64_t n, uint64_t seed, int64_t result[4]);
+
 /* Kernel-class timing, no counterpart in the reference (its timing is the
    wall clock of the whole step, diag.c): while a profile is open, every
    launch of hg_step/hg_step_async/hg_fluxcalc is bracketed by HIP events on
