@@ -132,11 +132,22 @@ HG_HD void maxwell(const double *pr, const State &q, const Geom &G, Maxwell &M) 
     M.Mcon[k][0] = pr[B1 + k - 1];
     M.Mcon[0][k] = -M.Mcon[k][0];
   }
+  // The space-space entries (B^j u^k - B^k u^j) / u^t of phys.c:178, three
+  // divisions instead of nine, same bits for finite products B^j u^k (every
+  // state a step holds): a diagonal numerator is a product less itself, +0,
+  // and +0 / u^t = +0 (u^t > 0).  Entry [k][j] subtracts the same two
+  // products the other way round, the exact negative of [j][k]'s numerator
+  // unless the two products are equal: then both numerators are the zero of
+  // their own subtraction, whose quotient by u^t > 0 is that zero itself.
 #pragma unroll
   for (int j = 1; j < 4; j++)
 #pragma unroll
-    for (int k = 1; k < 4; k++)
-      M.Mcon[j][k] = (pr[B1 + j - 1] * q.ucon[k] - pr[B1 + k - 1] * q.ucon[j]) / q.ucon[0];
+    for (int k = j + 1; k < 4; k++) {
+      const double a = pr[B1 + j - 1] * q.ucon[k], b = pr[B1 + k - 1] * q.ucon[j];
+      const double njk = a - b;
+      M.Mcon[j][k] = njk / q.ucon[0];
+      M.Mcon[k][j] = (njk == 0.0) ? (b - a) : -M.Mcon[j][k];
+    }
   double a01 = 0., a02 = 0., a03 = 0., a12 = 0., a23 = 0., a31 = 0.;
 #pragma unroll
   for (int j = 0; j < 4; j++)
@@ -192,9 +203,11 @@ HG_HD void primtoflux(const double *pr, const State &q, const Maxwell &M,
   fl[RHO] = pr[RHO] * q.ucon[dir];
   mhd_row(pr, dir, q, M, plus1ud0, gam, &fl[UU]);
   if (dir > 0) {
-    fl[B1] = (pr[B1] * q.ucon[dir] - pr[B1 + dir - 1] * q.ucon[1]) / q.ucon[0];
-    fl[B2] = (pr[B2] * q.ucon[dir] - pr[B1 + dir - 1] * q.ucon[2]) / q.ucon[0];
-    fl[B3] = (pr[B3] * q.ucon[dir] - pr[B1 + dir - 1] * q.ucon[3]) / q.ucon[0];
+    // (B^k u^dir - B^dir u^k) / u^t is the Maxwell entry [k][dir] of the
+    // same state, the same operations (maxwell above)
+    fl[B1] = M.Mcon[1][dir];
+    fl[B2] = M.Mcon[2][dir];
+    fl[B3] = M.Mcon[3][dir];
   } else {
     fl[B1] = pr[B1];
     fl[B2] = pr[B2];
