@@ -436,12 +436,12 @@ static FxBufs bufs(hg_ctx *c) {
 
 // fixup_checksolution + fixup_utoprim on the bounded primitives p, with the
 // inversion codes in c->pflag; kernels only (capturable)
-void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s) {
+void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s, bool reset) {
   const Dims &d = c->d;
   FxBufs b = bufs(c);
   int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
   unsigned nb = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(k_fx_reset, dim3(1), dim3(64), 0, s, b.count);
+  if (reset) hipLaunchKernelGGL(k_fx_reset, dim3(1), dim3(64), 0, s, b.count);
   hipLaunchKernelGGL(k_fx_check, dim3(nb), dim3(256), 0, s, p, c->geo, d, b, c->err);
   hipLaunchKernelGGL(k_fx_resolve, dim3(1), dim3(1024), 0, s, d, b, c->err);
   hipLaunchKernelGGL(k_fx_average, dim3(256), dim3(256), 0, s, p, c->geo, d, b, finalstep, c->cfg.gammamax,

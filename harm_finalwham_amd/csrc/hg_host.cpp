@@ -423,13 +423,6 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
   c->flux_state = pb;
   c->dg_faces_valid = false;
   int nn[4] = {0, d.n1, d.n2, d.n3};
-  launch_reset_ndt(c->ndt_bits, s);
-  if (f.lim == HG_LIM_WENO5BND) {
-    // the dP/P inputs are shared by the three directions
-    prof_mark(c, HG_PROF_PREP, s);
-    launch_weno_prep(pb, c->geo, c->aux, d, f.gam, f.doenoflux == HG_ENOFINITEVOLUME, s);
-    prof_mark(c, -1, s);
-  }
   // the directions are independent (own interface states, fluxes and ndt
   // slot): j and k on the side streams, i on s, joined before the EMFs.
   // A profile runs them one after another so that every interval holds one
@@ -441,6 +434,20 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
   if (conc) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) conc = false;
+  }
+  // Run one after another, the three directions' dP/P indicators take one
+  // launch with the shared inputs' (fewer launches for the small grids of a
+  // graph); a profile keeps them with their directions, as the concurrent
+  // step runs them.  The prep kernel also resets the ndt slots.
+  const bool weno = f.lim == HG_LIM_WENO5BND;
+  const bool all_dpi = !conc && !c->prof_on;
+  if (weno) {
+    // the dP/P inputs are shared by the three directions
+    prof_mark(c, HG_PROF_PREP, s);
+    launch_weno_prep(pb, c->geo, c->aux, d, f.gam, f.doenoflux == HG_ENOFINITEVOLUME, all_dpi, c->ndt_bits, s);
+    prof_mark(c, -1, s);
+  } else {
+    launch_reset_ndt(c->ndt_bits, s);
   }
   hipStream_t ds[4] = {nullptr, s, s, s};
   if (conc) {
@@ -465,7 +472,7 @@ static void enqueue_fluxcalc(hg_ctx *c, const double *pb, hipStream_t s) {
       prof_mark(c, -1, sd);
     } else {
       prof_mark(c, HG_PROF_RECON1 + dir - 1, sd);
-      launch_weno(dir, pb, c->aux, pls[dir], prs[dir], d, c->tune, f.doenoflux == HG_ENOFINITEVOLUME, sd);
+      launch_weno(dir, pb, c->aux, pls[dir], prs[dir], d, c->tune, f.doenoflux == HG_ENOFINITEVOLUME, !all_dpi, sd);
       prof_mark(c, HG_PROF_FLUX1 + dir - 1, sd);
       launch_flux_lr(dir, pls[dir], prs[dir], c->geo, c->F[dir - 1], d, f.gam, f.cour, f.dx[dir], f.fluxmethod,
                      c->ndt_bits + dir - 1, c->err, c->tune, f.doenoflux == HG_ENOFINITEVOLUME, sd);
@@ -542,7 +549,8 @@ static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, const
     launch_mlp_forward(c->mlp, (int64_t)c->d.n1 * c->d.n2 * c->d.n3, c->lx, c->ly, s);
   }
   launch_invert(c->uf, pb, pf, c->geo, c->floors, rk, finalstep, c->d, f.gam, f.gammamax, c->counters, c->pflag,
-                c->c2p_mode == HG_C2P_MIXED ? 1 : c->c2p_mode == HG_C2P_LEARNED ? 2 : 0, c->ly, c->err, c->tune, s);
+                c->c2p_mode == HG_C2P_MIXED ? 1 : c->c2p_mode == HG_C2P_LEARNED ? 2 : 0, c->ly, c->fx_count,
+                c->err, c->tune, s);
   if (finalstep) launch_dt(c->ndt_bits, c->dt, f.defcon, f.safe, s);
   // step_ch.c:127 and :166 call bound_prim twice per substep; the second call
   // is not a repeat of the first: POLEDEATH has zeroed U2 in the rows next to
@@ -550,7 +558,7 @@ static void enqueue_substep(hg_ctx *c, double *pi, double *pb, double *pf, const
   prof_mark(c, HG_PROF_BOUND, s);
   launch_bound(pf, c->geo, c->d, f.gammamax, c->err, s);
   prof_mark(c, HG_PROF_FIXUP, s);
-  launch_post_fixup(c, pf, finalstep, s);
+  launch_post_fixup(c, pf, finalstep, s, false);
   prof_mark(c, HG_PROF_BOUND, s);
   launch_bound(pf, c->geo, c->d, f.gammamax, c->err, s);
   prof_mark(c, -1, s);

@@ -94,7 +94,9 @@ int diag_alloc(hg_ctx *c);
 void launch_diag_faces(hg_ctx *c, const double *prim, const double *pglob, hipStream_t s);
 void launch_diag_stage(hg_ctx *c, int stage, hipStream_t s);
 // fixup_checksolution + fixup_utoprim on bounded primitives p (kernels only)
-void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s);
+// reset false: the list counters were cleared already (by the substep's
+// inversion kernel)
+void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s, bool reset = true);
 // batched inversion (hg_c2p_batch.hip).  The mixed path's scratch holds
 // C2P_NCTR counters (work queue, list length, list work queue) and a list of
 // n 32-bit zone indices: c2p_scratch_bytes(n) bytes.

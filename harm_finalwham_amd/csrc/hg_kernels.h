@@ -58,11 +58,12 @@ void launch_update(const double *pi, const double *pb, double *ubuf, const doubl
 // the inversion of ubuf with the guess pb, fixup1zone on the final substep
 // -> pf; mode 0: the reference's FP64 iteration, 1: mixed precision to the
 // converged root, 2: the same from the learned start (ylearn[zone], the
-// network's log(W / W_guess) in k_invert's zone order)
+// network's log(W / W_guess) in k_invert's zone order); fx_reset (may be
+// null): post_fixup's three list counters, cleared by the kernel
 void launch_invert(const double *ubuf, const double *pb, double *pf, const double *geo, const double *floors,
                    const RkStage &rk, int finalstep, const Dims &d, double gam, double gammamax,
-                   unsigned long long *counters, int8_t *pflag, int mode, const float *ylearn, int *err,
-                   const Tune &tu, hipStream_t s);
+                   unsigned long long *counters, int8_t *pflag, int mode, const float *ylearn, unsigned *fx_reset,
+                   int *err, const Tune &tu, hipStream_t s);
 // the learned start's features of the inversions launch_invert will do
 // (8 floats per zone, k_invert's zone order)
 void launch_step_features(const double *ubuf, const double *pb, const double *geo, const Dims &d, double gam,
@@ -77,10 +78,13 @@ void launch_reset_ndt(unsigned long long *ndt_bits, hipStream_t s);
 // three directions' indicators); pleft/pright have the layout of the
 // primitives
 // wide: the finite-volume flux loops, which span every transverse ghost row
+// launch_weno_prep: also every direction's indicator when all_dpi, and the
+// substep's ndt slots reset when ndt_reset is given; launch_weno: the
+// direction's indicator first when with_dpi
 void launch_weno_prep(const double *p, const double *geo, double *aux, const Dims &d, double gam, int wide,
-                      hipStream_t s);
+                      bool all_dpi, unsigned long long *ndt_reset, hipStream_t s);
 void launch_weno(int dir, const double *p, double *aux, double *pleft, double *pright, const Dims &d,
-                 const Tune &tu, int wide, hipStream_t s);
+                 const Tune &tu, int wide, bool with_dpi, hipStream_t s);
 
 // initial time step over every zone (advance.c:957 set_dt), folded into *out
 void launch_set_dt(const double *p, const double *geo, const Dims &d, double gam, double cour, const double *dx,

@@ -881,7 +881,11 @@ __global__ void __launch_bounds__(256) k_invert(const double *__restrict__ ubuf,
                                                 double gam, double gammamax,
                                                 unsigned long long *__restrict__ counters,
                                                 int8_t *__restrict__ pflag, double *__restrict__ ucons,
-                                                const float *__restrict__ ylearn, int *__restrict__ err) {
+                                                const float *__restrict__ ylearn, unsigned *__restrict__ fx_reset,
+                                                int *__restrict__ err) {
+  // the post_fixup lists of this substep start empty (launch_post_fixup
+  // then skips its own reset)
+  if (fx_reset && blockIdx.x == 0 && threadIdx.x < 3) fx_reset[threadIdx.x] = 0u;
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
   unsigned long long nfail = 0, nit = 0;
@@ -1186,30 +1190,30 @@ template <int MODE>
 static void launch_invert_mode(const double *ubuf, const double *pb, double *pf, const double *geo,
                                const double *floors, const RkStage &rk, int finalstep, const Dims &d, double gam,
                                double gammamax, unsigned long long *counters, int8_t *pflag, const float *ylearn,
-                               int *err, const Tune &tu, hipStream_t s) {
+                               unsigned *fx_reset, int *err, const Tune &tu, hipStream_t s) {
   int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
   dim3 gi(nblocks(n, tu.bs_invert)), bi(tu.bs_invert);
   if (rk.ucons && finalstep)
     hipLaunchKernelGGL((k_invert<true, MODE>), gi, bi, 0, s, ubuf, pb, pf, geo, floors, finalstep, d, gam, gammamax,
-                       counters, pflag, rk.ucons, ylearn, err);
+                       counters, pflag, rk.ucons, ylearn, fx_reset, err);
   else
     hipLaunchKernelGGL((k_invert<false, MODE>), gi, bi, 0, s, ubuf, pb, pf, geo, floors, finalstep, d, gam, gammamax,
-                       counters, pflag, nullptr, ylearn, err);
+                       counters, pflag, nullptr, ylearn, fx_reset, err);
 }
 
 void launch_invert(const double *ubuf, const double *pb, double *pf, const double *geo, const double *floors,
                    const RkStage &rk, int finalstep, const Dims &d, double gam, double gammamax,
-                   unsigned long long *counters, int8_t *pflag, int mode, const float *ylearn, int *err,
-                   const Tune &tu, hipStream_t s) {
+                   unsigned long long *counters, int8_t *pflag, int mode, const float *ylearn, unsigned *fx_reset,
+                   int *err, const Tune &tu, hipStream_t s) {
   if (mode == INV_MIXED)
     launch_invert_mode<INV_MIXED>(ubuf, pb, pf, geo, floors, rk, finalstep, d, gam, gammamax, counters, pflag,
-                                  ylearn, err, tu, s);
+                                  ylearn, fx_reset, err, tu, s);
   else if (mode == INV_LEARNED)
     launch_invert_mode<INV_LEARNED>(ubuf, pb, pf, geo, floors, rk, finalstep, d, gam, gammamax, counters, pflag,
-                                    ylearn, err, tu, s);
+                                    ylearn, fx_reset, err, tu, s);
   else
     launch_invert_mode<INV_FP64>(ubuf, pb, pf, geo, floors, rk, finalstep, d, gam, gammamax, counters, pflag,
-                                 ylearn, err, tu, s);
+                                 ylearn, fx_reset, err, tu, s);
 }
 
 // the learned start's features of the inversions k_invert will do (its zone

@@ -328,7 +328,11 @@ __device__ __forceinline__ void stage_e(const double *y, double m0, double w0, d
 // over the box [-2, n+1] of every direction with more than one zone
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_weno_prep(const double *__restrict__ p, const double *__restrict__ geo,
-                                                   double *__restrict__ aux, Dims d, double gam, int wide) {
+                                                   double *__restrict__ aux, Dims d, double gam, int wide,
+                                                   unsigned long long *__restrict__ ndt_reset) {
+  // the substep's flux kernels take the minimum of their ndt into these
+  if (ndt_reset && blockIdx.x == 0 && threadIdx.x < 3)
+    ndt_reset[threadIdx.x] = (unsigned long long)__double_as_longlong(1.e+50);
   // wide (the finite-volume flux loops): every ghost layer
   const int e1 = d.n1 > 1 ? (wide ? d.nb1 : 2) : 0, e2 = d.n2 > 1 ? (wide ? d.nb2 : 2) : 0;
   const int e3 = d.n3 > 1 ? (wide ? d.nb3 : 2) : 0;
@@ -363,11 +367,13 @@ __global__ void __launch_bounds__(256) k_weno_prep(const double *__restrict__ p,
 }
 
 // ---------------------------------------------------------------------------
-// stage A along DIR: dP/P at every zone of the box above whose two
-// neighbours along DIR are in the box too, into aux plane 1 + DIR.  It does
-// not depend on the primitive reconstructed, so the reconstruction kernels
-// of all 8 primitives read it from here; each direction's runs on that
-// direction's stream, ahead of its reconstruction.
+// stage A along DIR (DIR 0: every direction of the grid): dP/P at every
+// zone of the box above whose two neighbours along that direction are in the
+// box too, into aux plane 1 + direction.  It does not depend on the
+// primitive reconstructed, so the reconstruction kernels of all 8 primitives
+// read it from here.  When the directions run concurrently each direction's
+// runs on that direction's stream, ahead of its reconstruction; otherwise
+// one launch serves all three.
 // Work-groups go to the 8 XCDs round robin by linear index; each XCD takes
 // a contiguous eighth of the box instead, so that the neighbours a zone
 // reads were fetched into that XCD's L2 by the blocks just before it.
@@ -387,12 +393,16 @@ __global__ void __launch_bounds__(256) k_weno_dpi(const double *__restrict__ p, 
   int k = -e3 + (int)(r % b3), j = -e2 + (int)(r / b3);
   const int64_t z = d.idx(i, j, k), P = d.spl;
   const double *par = aux, *upar = aux + P;
-  const int s = (DIR == 1) ? i : (DIR == 2) ? j : k, e = (DIR == 1) ? e1 : (DIR == 2) ? e2 : e3;
-  const int n = (DIR == 1) ? d.n1 : (DIR == 2) ? d.n2 : d.n3;
-  const int64_t sd = (DIR == 1) ? 1 : (DIR == 2) ? d.sj : d.sk;
-  if (s > -e && s < n + e - 1)
-    aux[(1 + DIR) * P + z] =
-        dp_jump(p[RHO * P + z], par[z - sd], par[z], par[z + sd], upar[z - sd], upar[z], upar[z + sd]);
+#pragma unroll
+  for (int dir = 1; dir <= 3; dir++) {
+    if (DIR != 0 && dir != DIR) continue;
+    const int s = (dir == 1) ? i : (dir == 2) ? j : k, e = (dir == 1) ? e1 : (dir == 2) ? e2 : e3;
+    const int n = (dir == 1) ? d.n1 : (dir == 2) ? d.n2 : d.n3;
+    const int64_t sd = (dir == 1) ? 1 : (dir == 2) ? d.sj : d.sk;
+    if (n > 1 && s > -e && s < n + e - 1)
+      aux[(1 + dir) * P + z] =
+          dp_jump(p[RHO * P + z], par[z - sd], par[z], par[z + sd], upar[z - sd], upar[z], upar[z + sd]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -573,20 +583,23 @@ __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, co
 }
 
 void launch_weno_prep(const double *p, const double *geo, double *aux, const Dims &d, double gam, int wide,
-                      hipStream_t s) {
+                      bool all_dpi, unsigned long long *ndt_reset, hipStream_t s) {
   const int e1 = d.n1 > 1 ? (wide ? d.nb1 : 2) : 0, e2 = d.n2 > 1 ? (wide ? d.nb2 : 2) : 0;
   const int e3 = d.n3 > 1 ? (wide ? d.nb3 : 2) : 0;
   int64_t n = (int64_t)(d.n1 + 2 * e1) * (d.n2 + 2 * e2) * (d.n3 + 2 * e3);
-  hipLaunchKernelGGL(k_weno_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, geo, aux, d, gam, wide);
+  hipLaunchKernelGGL(k_weno_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, geo, aux, d, gam, wide,
+                     ndt_reset);
+  if (all_dpi) hipLaunchKernelGGL(k_weno_dpi<0>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, aux, d, wide);
 }
 
 void launch_weno(int dir, const double *p, double *aux, double *pleft, double *pright, const Dims &d,
-                 const Tune &tu, int wide, hipStream_t s) {
+                 const Tune &tu, int wide, bool with_dpi, hipStream_t s) {
   // transverse halo rows (the kernels use the same widths)
   int s1 = d.n1 > 1 ? (wide ? d.nb1 : 1) : 0, s2 = d.n2 > 1 ? (wide ? d.nb2 : 1) : 0;
   int s3 = d.n3 > 1 ? (wide ? d.nb3 : 1) : 0;
   // the dP/P indicators of this direction, over the box of k_weno_prep
-  {
+  // (unless launch_weno_prep formed every direction's)
+  if (with_dpi) {
     const int e1 = d.n1 > 1 ? (wide ? d.nb1 : 2) : 0, e2 = d.n2 > 1 ? (wide ? d.nb2 : 2) : 0;
     const int e3 = d.n3 > 1 ? (wide ? d.nb3 : 2) : 0;
     const int64_t nbox = (int64_t)(d.n1 + 2 * e1) * (d.n2 + 2 * e2) * (d.n3 + 2 * e3);
