@@ -492,7 +492,9 @@ def batched_bench(ntori=48, nsteps=10, seed=5, mlp=None):
     Noble 2D): nsteps RK2 steps of all of them in one HIP graph launch, with
     launch shapes autotuned once per grid size, against the same grids stepped
     one after the other; every grid is compared bit for bit.  First with the
-    reference's FP64 inversion, then, with a trained network mlp, with the
+    reference's FP64 inversion, then the same grids in a grouped graph (one
+    launch per kernel over all of them) with the FP64 and with the
+    mixed-precision inversion, then, with a trained network mlp, with the
     mixed-precision inversion from the learned start (hg_set_inversion) in a
     new graph of the same grids."""
     import math
@@ -542,6 +544,31 @@ def batched_bench(ntori=48, nsteps=10, seed=5, mlp=None):
     identical = all(gb.get_dt() == ga.get_dt() and np.array_equal(gb.get_prims(), ga.get_prims())
                     for gb, ga in zip(batch, alone))
     graph.close()
+
+    def grouped_run(label):
+        # one launch per kernel over all the grids (hg_graph_create_grouped)
+        gg = harmgpu.Graph(batch, nsteps, grouped=True)
+        gg.launch()
+        for g in alone:
+            g.step_async(nsteps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gg.launch()
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        for g in alone:
+            g.step_async(nsteps)
+        torch.cuda.synchronize()
+        same = all(gb.get_dt() == ga.get_dt() and np.array_equal(gb.get_prims(), ga.get_prims())
+                   for gb, ga in zip(batch, alone))
+        gg.close()
+        return {"inversion": label, "graph_zone_updates_per_s": zones * nsteps / t,
+                "speedup_vs_branch_graph": t_graph / t, "bitwise_identical_to_single_runs": same}
+
+    grouped = grouped_run("the reference's FP64 iteration (HG_C2P_FP64)")
+    for g in batch + alone:
+        g.set_inversion(harmgpu.HG_C2P_MIXED)
+    grouped_mixed = grouped_run("mixed precision (HG_C2P_MIXED)")
     learned = None
     if mlp is not None:
         for g in batch + alone:
@@ -574,7 +601,8 @@ def batched_bench(ntori=48, nsteps=10, seed=5, mlp=None):
            "sequential_zone_updates_per_s": zones * nsteps / t_seq,
            "graph_speedup": t_seq / t_graph, "bitwise_identical_to_single_runs": identical,
            "tuned_shapes": {f"{k[0]}x{k[1]}": v.as_dict() for k, v in tuned.items()},
-           "setup_s": t_setup, "learned_inversion": learned}
+           "setup_s": t_setup, "grouped": grouped, "grouped_mixed_inversion": grouped_mixed,
+           "learned_inversion": learned}
     for g in batch + alone:
         g.close()
     return res

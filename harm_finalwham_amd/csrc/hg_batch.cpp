@@ -7,11 +7,17 @@
 // replays need no host work.  The step enqueues kernels only (no memset or
 // copy nodes).
 //
+// hg_graph_create_grouped captures the same steps as one launch per kernel of
+// a substep over the blocks of all the grids (GridView tables on the device,
+// hg_kernels.h): the graph then holds ~20 launches per substep instead of
+// ~20 per substep and grid, which the runtime would mostly run one at a time.
+//
 // hg_autotune times the step under different launch shapes (block sizes,
 // reconstruction wavefronts per block and line segments) and keeps the
 // fastest; the shapes do not change any result bit.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <vector>
 
 #include "harmgpu.h"
@@ -22,6 +28,7 @@ using namespace hg;
 struct hg_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  void *tables = nullptr;  // grouped: the views and block tables the launches read
 };
 
 #define HIPTRY(call)                                                               \
@@ -115,6 +122,174 @@ extern "C" int hg_graph_create(hg_ctx *const *ctxs, int n, int nsteps, hg_graph 
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// grouped graph
+// ---------------------------------------------------------------------------
+namespace {
+
+// the views of the RK2 midpoint substeps of one context (enqueue_step:
+// pb = p, pk; pf = pk, p; pi = p)
+void make_views(hg_ctx *c, GridView v[2]) {
+  const hg_config &f = c->cfg;
+  for (int sub = 0; sub < 2; sub++) {
+    GridView &w = v[sub];
+    memset(&w, 0, sizeof w);
+    w.d = c->d;
+    w.pi = c->p;
+    w.pb = sub ? c->pk : c->p;
+    w.pf = sub ? c->p : c->pk;
+    w.geo = c->geo;
+    w.corn = c->corn;
+    w.conn = c->conn;
+    w.floors = c->floors;
+    double *pls[3] = {c->pl, c->pl2, c->pl3}, *prs[3] = {c->pr, c->pr2, c->pr3};
+    for (int k = 0; k < 3; k++) {
+      w.F[k] = c->F[k];
+      w.emf[k] = c->emf[k];
+      w.pl[k] = pls[k];
+      w.pr[k] = prs[k];
+    }
+    w.aux = c->aux;
+    w.uf = c->uf;
+    w.dt = c->dt;
+    w.ndt_bits = c->ndt_bits;
+    w.counters = c->counters;
+    w.pflag = c->pflag;
+    w.fx = fixup_bufs(c);
+    w.err = c->err;
+    // enqueue_step's midpoint coefficients without ucum
+    w.rk = sub ? RkStage{1.0, 0.0, 0.0, 1.0, nullptr, 0, 1, nullptr} : RkStage{0.5, 0.0, 0.0, 0.0, nullptr, 1, 0, nullptr};
+    w.gam = f.gam;
+    w.gammamax = f.gammamax;
+    w.cour = f.cour;
+    w.defcon = f.defcon;
+    w.safe = f.safe;
+    for (int k = 1; k < 4; k++) {
+      w.dx[k] = f.dx[k];
+      w.idx[k] = 1.0 / f.dx[k];
+    }
+    w.finalstep = sub;
+    w.fluxmethod = f.fluxmethod;
+    weno_group_shapes(w, c->tune);
+  }
+}
+
+const char *grouped_unsupported(const hg_ctx *c, const hg_ctx *c0) {
+  const hg_config &f = c->cfg;
+  if (f.lim != HG_LIM_WENO5BND) return "reconstruction other than WENO5BND";
+  if (f.doenoflux == HG_ENOFINITEVOLUME) return "the finite-volume scheme";
+  if (f.timeorder != 2) return "a timeorder other than 2";
+  if (f.fluxmethod != HG_FLUX_LAXF && f.fluxmethod != HG_FLUX_HLL) return "a flux other than LAXF or HLL";
+  if (c->c2p_mode != HG_C2P_FP64 && c->c2p_mode != HG_C2P_MIXED) return "an inversion mode other than FP64 or MIXED";
+  if (c->c2p_mode != c0->c2p_mode) return "an inversion mode unlike the first context's";
+  if (c->dg_record) return "diagnostics recording on";
+  if (c->ucons) return "a carried conserved state";
+  return nullptr;
+}
+
+}  // namespace
+
+extern "C" int hg_graph_create_grouped(hg_ctx *const *ctxs, int n, int nsteps, hg_graph **out) {
+  if (!ctxs || !out || n < 1 || nsteps < 1) {
+    set_error("hg_graph_create_grouped: need n >= 1 contexts, nsteps >= 1 and an output");
+    return 1;
+  }
+  *out = nullptr;
+  for (int i = 0; i < n; i++) {
+    if (!ctxs[i] || !ctxs[i]->have_geometry) {
+      set_error("hg_graph_create_grouped: context %d missing or without geometry", i);
+      return 1;
+    }
+    if (ctxs[i]->prof_on) {
+      set_error("hg_graph_create_grouped: context %d has a profile open", i);
+      return 1;
+    }
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == ctxs[i]) {
+        set_error("hg_graph_create_grouped: context %d appears twice", i);
+        return 1;
+      }
+    if (const char *why = grouped_unsupported(ctxs[i], ctxs[0])) {
+      set_error("hg_graph_create_grouped: context %d has %s (hg_graph_create takes it)", i, why);
+      return 1;
+    }
+  }
+  // host tables: 2n views (substep-major), then per class n + 1 block offsets
+  std::vector<GridView> views(2 * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    GridView v[2];
+    make_views(ctxs[i], v);
+    views[i] = v[0];
+    views[n + i] = v[1];
+  }
+  std::vector<unsigned> first((size_t)GK_NCLASS * (n + 1));
+  unsigned total[GK_NCLASS];
+  for (int k = 0; k < GK_NCLASS; k++) {
+    uint64_t acc = 0;
+    for (int i = 0; i < n; i++) {
+      first[(size_t)k * (n + 1) + i] = (unsigned)acc;
+      acc += group_blocks(k, views[i]);
+    }
+    if (acc > 0x7fffffffull) {
+      set_error("hg_graph_create_grouped: %llu blocks in one launch exceed the grid limit",
+                (unsigned long long)acc);
+      return 1;
+    }
+    first[(size_t)k * (n + 1) + n] = (unsigned)acc;
+    total[k] = (unsigned)acc;
+  }
+  const size_t vbytes = views.size() * sizeof(GridView), vpad = (vbytes + 255) / 256 * 256;
+  const size_t bytes = vpad + first.size() * sizeof(unsigned);
+  int rc = 0;
+  hg_graph *g = new hg_graph();
+  HIPTRY(hipMalloc(&g->tables, bytes));
+  if (!rc) HIPTRY(hipMemcpy(g->tables, views.data(), vbytes, hipMemcpyHostToDevice));
+  if (!rc) HIPTRY(hipMemcpy((char *)g->tables + vpad, first.data(), first.size() * sizeof(unsigned),
+                            hipMemcpyHostToDevice));
+  const GridView *dv = (const GridView *)g->tables;
+  const unsigned *dfirst = (const unsigned *)((char *)g->tables + vpad);
+  const int mode = ctxs[0]->c2p_mode == HG_C2P_MIXED ? 1 : 0;
+  hipStream_t cap = nullptr;
+  if (!rc) HIPTRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  if (!rc) HIPTRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+  if (!rc) {
+    auto L = [&](int k, int sub) { launch_group(k, dv + (size_t)sub * n, dfirst + (size_t)k * (n + 1), n, total[k], mode, cap); };
+    // one substep of step_ch.c:94-166 on every grid: fluxcalc (directions in
+    // turn, every direction's dP/P indicator in one launch), update,
+    // inversion, dt on the final substep, bound_prim, post_fixup, bound_prim
+    const int order[] = {GK_PREP, GK_DPI, GK_WENO1, GK_FLUX1, GK_WENO2, GK_FLUX2, GK_WENO3, GK_FLUX3,
+                         GK_EMF, GK_UPDATE, GK_INVERT, GK_DT, GK_BOUND1, GK_BOUND2, GK_BOUND3,
+                         GK_FXCHECK, GK_FXRESOLVE, GK_FXAVG, GK_FXAPPLY, GK_BOUND1, GK_BOUND2, GK_BOUND3};
+    for (int step = 0; step < nsteps; step++)
+      for (int sub = 0; sub < 2; sub++)
+        for (int k : order)
+          if (k != GK_DT || sub == 1) L(k, sub);
+    HIPTRY(hipGetLastError());
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(cap, &graph);
+    if (e != hipSuccess && !rc) {
+      set_error("hg_graph_create_grouped: capture failed: %s", hipGetErrorString(e));
+      rc = 1;
+    }
+    g->graph = graph;
+    if (!rc) HIPTRY(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  }
+  if (cap) (void)hipStreamDestroy(cap);
+  if (rc) {
+    (void)hg_graph_destroy(g);
+    return 1;
+  }
+  // the host-side record of each context after the captured steps, as
+  // enqueue_step leaves it
+  for (int i = 0; i < n; i++) {
+    ctxs[i]->unew_in_uf = true;
+    ctxs[i]->flux_state = ctxs[i]->pk;
+    ctxs[i]->dg_faces_valid = false;
+  }
+  *out = g;
+  return 0;
+}
+
 extern "C" int hg_graph_launch(hg_graph *g, void *stream) {
   if (!g || !g->exec) {
     set_error("hg_graph_launch: no graph");
@@ -132,6 +307,7 @@ extern "C" int hg_graph_destroy(hg_graph *g) {
   if (!g) return 0;
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->tables) (void)hipFree(g->tables);
   delete g;
   return 0;
 }

@@ -32,6 +32,7 @@
 
 #include "harmgpu.h"
 #include "hg_host.h"
+#include "hg_kernels.h"
 #include "hg_phys.h"
 
 using namespace hg;
@@ -43,20 +44,6 @@ constexpr int FX_GAMMAPERC = 12, FX_UPERC = 13, FX_U2AVG1 = 14, FX_U2AVG2 = 15;
 constexpr int FX_RHONEG = 9, FX_UNEG = 10, FX_RHOUNEG = 11;
 constexpr double GAMMAPERCDIFFMAX = 2.0;  // init.fishmon.h:178
 constexpr double UPERCDIFFMAX = 10.0;     // init.fishmon.h:179
-
-struct FxBufs {
-  int8_t *flag0;      // inversion codes (active zones)
-  int8_t *flag1;      // codes after the neighbour check
-  int *cand_z;        // candidate zone indices
-  uint16_t *cand_bits;// per neighbour l: bit l gamma vote, bit 8+l u vote
-  int8_t *cand_new;   // Jacobi scratch
-  int *fix_z;         // zones the average changed
-  double *fix_v;      // their RHO..U3 (5 x cap, structure of arrays)
-  int8_t *fix_flag;   // their final flag
-  int *fail_z;        // failed inversions that are not candidates
-  unsigned *count;    // [candidates, fixes, failed non-candidates]
-  int cap;
-};
 
 __device__ __forceinline__ int64_t zidx(const Dims &d, int i, int j, int k) { return d.idx(i, j, k); }
 
@@ -123,9 +110,9 @@ __device__ __forceinline__ int vote_result(int f0, unsigned bits, const int *nfl
   return f;
 }
 
-__global__ void __launch_bounds__(256) k_fx_check(const double *__restrict__ p, const double *__restrict__ geo,
-                                                  Dims d, FxBufs b, int *__restrict__ err) {
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void fx_check_body(const double *__restrict__ p, const double *__restrict__ geo,
+                                              const Dims &d, const FxBufs &b, int *__restrict__ err, unsigned bid) {
+  int64_t t = bid * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
   if (t >= total) return;
   int i = (int)(t % d.n1);
@@ -166,6 +153,11 @@ __global__ void __launch_bounds__(256) k_fx_check(const double *__restrict__ p, 
   b.cand_bits[slot] = (uint16_t)bits;
 }
 
+__global__ void __launch_bounds__(256) k_fx_check(const double *__restrict__ p, const double *__restrict__ geo,
+                                                  Dims d, FxBufs b, int *__restrict__ err) {
+  fx_check_body(p, geo, d, b, err, blockIdx.x);
+}
+
 // zone (i,j,k) of a padded index
 __device__ __forceinline__ void unidx(const Dims &d, int64_t z, int &i, int &j, int &k) {
   i = (int)(z % d.m1) - d.nb1;
@@ -174,7 +166,7 @@ __device__ __forceinline__ void unidx(const Dims &d, int64_t z, int &i, int &j, 
   k = (int)(r / d.m2) - d.nb3;
 }
 
-__global__ void __launch_bounds__(1024) k_fx_resolve(Dims d, FxBufs b, int *__restrict__ err) {
+__device__ __forceinline__ void fx_resolve_body(const Dims &d, const FxBufs &b, int *__restrict__ err) {
   __shared__ int changed;
   const unsigned n = b.count[0];
   if (n == 0) return;
@@ -211,6 +203,10 @@ __global__ void __launch_bounds__(1024) k_fx_resolve(Dims d, FxBufs b, int *__re
       break;
     }
   }
+}
+
+__global__ void __launch_bounds__(1024) k_fx_resolve(Dims d, FxBufs b, int *__restrict__ err) {
+  fx_resolve_body(d, b, err);
 }
 
 // fixup_utoprim accounting (fixup.c:1345-1420): does the loop reset a zone
@@ -354,25 +350,71 @@ __device__ __forceinline__ void fx_average_zone(const double *__restrict__ p, co
 // over the candidates (their flags as k_fx_resolve left them) and the other
 // failed inversions; a few blocks, since the lists are short next to the
 // grid and the average takes the whole register file of a SIMD lane
-__global__ void __launch_bounds__(256) k_fx_average(const double *__restrict__ p, const double *__restrict__ geo,
-                                                    Dims d, FxBufs b, int finalstep, double gammamax, double gam,
-                                                    double *__restrict__ ucons, int *__restrict__ err) {
+__device__ __forceinline__ void fx_average_body(const double *__restrict__ p, const double *__restrict__ geo,
+                                                const Dims &d, const FxBufs &b, int finalstep, double gammamax,
+                                                double gam, double *__restrict__ ucons, int *__restrict__ err,
+                                                unsigned bid, unsigned nblk) {
   const unsigned nc = b.count[0], n = nc + b.count[2];
-  for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+  for (unsigned s = bid * blockDim.x + threadIdx.x; s < n; s += nblk * blockDim.x) {
     const int64_t z = (s < nc) ? b.cand_z[s] : b.fail_z[s - nc];
     const int f = b.flag1[z];
     if (f != 0) fx_average_zone(p, geo, d, b, z, f, finalstep, gammamax, gam, ucons, err);
   }
 }
 
-__global__ void __launch_bounds__(256) k_fx_apply(double *__restrict__ p, Dims d, FxBufs b) {
+__global__ void __launch_bounds__(256) k_fx_average(const double *__restrict__ p, const double *__restrict__ geo,
+                                                    Dims d, FxBufs b, int finalstep, double gammamax, double gam,
+                                                    double *__restrict__ ucons, int *__restrict__ err) {
+  fx_average_body(p, geo, d, b, finalstep, gammamax, gam, ucons, err, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void fx_apply_body(double *__restrict__ p, const Dims &d, const FxBufs &b, unsigned bid,
+                                              unsigned nblk) {
   const unsigned n = b.count[1];
-  for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+  for (unsigned s = bid * blockDim.x + threadIdx.x; s < n; s += nblk * blockDim.x) {
     int64_t z = b.fix_z[s];
     for (int pl = RHO; pl <= U3; pl++) p[z + pl * d.spl] = b.fix_v[(int64_t)pl * b.cap + s];
     b.flag0[z] = b.fix_flag[s];
   }
 }
+
+__global__ void __launch_bounds__(256) k_fx_apply(double *__restrict__ p, Dims d, FxBufs b) {
+  fx_apply_body(p, d, b, blockIdx.x, gridDim.x);
+}
+
+// grouped launches (one per class over several grids; the standard scheme,
+// so no carried conserved state)
+#define FX_GROUP_PRELUDE                              \
+  const int g_ = group_of(first, n, blockIdx.x);      \
+  const GridView &v = views[g_];                      \
+  const unsigned bid = blockIdx.x - first[g_], nblk = first[g_ + 1] - first[g_]; \
+  (void)bid;                                          \
+  (void)nblk;
+
+__global__ void __launch_bounds__(256) kg_fx_check(const GridView *__restrict__ views,
+                                                   const unsigned *__restrict__ first, int n) {
+  FX_GROUP_PRELUDE
+  fx_check_body(v.pf, v.geo, v.d, v.fx, v.err, bid);
+}
+
+__global__ void __launch_bounds__(1024) kg_fx_resolve(const GridView *__restrict__ views,
+                                                      const unsigned *__restrict__ first, int n) {
+  FX_GROUP_PRELUDE
+  fx_resolve_body(v.d, v.fx, v.err);
+}
+
+__global__ void __launch_bounds__(256) kg_fx_average(const GridView *__restrict__ views,
+                                                     const unsigned *__restrict__ first, int n) {
+  FX_GROUP_PRELUDE
+  fx_average_body(v.pf, v.geo, v.d, v.fx, v.finalstep, v.gammamax, v.gam, nullptr, v.err, bid, nblk);
+}
+
+__global__ void __launch_bounds__(256) kg_fx_apply(const GridView *__restrict__ views,
+                                                   const unsigned *__restrict__ first, int n) {
+  FX_GROUP_PRELUDE
+  fx_apply_body(v.pf, v.d, v.fx, bid, nblk);
+}
+#undef FX_GROUP_PRELUDE
 
 __global__ void k_fx_reset(unsigned *count) {
   if (threadIdx.x < 3) count[threadIdx.x] = 0u;
@@ -418,7 +460,7 @@ void fixup_free(hg_ctx *c) {
     if (p) (void)hipFree(p);
 }
 
-static FxBufs bufs(hg_ctx *c) {
+FxBufs fixup_bufs(hg_ctx *c) {
   FxBufs b;
   b.flag0 = c->pflag;
   b.flag1 = c->pchk;
@@ -438,7 +480,7 @@ static FxBufs bufs(hg_ctx *c) {
 // inversion codes in c->pflag; kernels only (capturable)
 void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s, bool reset) {
   const Dims &d = c->d;
-  FxBufs b = bufs(c);
+  FxBufs b = fixup_bufs(c);
   int64_t n = (int64_t)d.n1 * d.n2 * d.n3;
   unsigned nb = (unsigned)((n + 255) / 256);
   if (reset) hipLaunchKernelGGL(k_fx_reset, dim3(1), dim3(64), 0, s, b.count);
@@ -447,6 +489,24 @@ void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s, bool 
   hipLaunchKernelGGL(k_fx_average, dim3(256), dim3(256), 0, s, p, c->geo, d, b, finalstep, c->cfg.gammamax,
                      c->cfg.gam, c->ucons, c->err);
   hipLaunchKernelGGL(k_fx_apply, dim3(256), dim3(256), 0, s, p, d, b);
+}
+
+// grouped: the list kernels take 8 blocks per grid (their lists are short
+// next to the grid; the single-grid launch takes 256)
+unsigned group_blocks_fixup(int k, const GridView &v) {
+  const int64_t n = (int64_t)v.d.n1 * v.d.n2 * v.d.n3;
+  if (k == GK_FXCHECK) return (unsigned)((n + 255) / 256);
+  if (k == GK_FXRESOLVE) return 1u;
+  if (k == GK_FXAVG || k == GK_FXAPPLY) return 8u;
+  return 0u;
+}
+
+void launch_group_fixup(int k, const GridView *views, const unsigned *first, int n, unsigned total,
+                        hipStream_t s) {
+  if (k == GK_FXCHECK) hipLaunchKernelGGL(kg_fx_check, dim3(total), dim3(256), 0, s, views, first, n);
+  if (k == GK_FXRESOLVE) hipLaunchKernelGGL(kg_fx_resolve, dim3(total), dim3(1024), 0, s, views, first, n);
+  if (k == GK_FXAVG) hipLaunchKernelGGL(kg_fx_average, dim3(total), dim3(256), 0, s, views, first, n);
+  if (k == GK_FXAPPLY) hipLaunchKernelGGL(kg_fx_apply, dim3(total), dim3(256), 0, s, views, first, n);
 }
 
 }  // namespace hg

@@ -97,6 +97,7 @@ void launch_diag_stage(hg_ctx *c, int stage, hipStream_t s);
 // reset false: the list counters were cleared already (by the substep's
 // inversion kernel)
 void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s, bool reset = true);
+FxBufs fixup_bufs(hg_ctx *c);
 // batched inversion (hg_c2p_batch.hip).  The mixed path's scratch holds
 // C2P_NCTR counters (work queue, list length, list work queue) and a list of
 // n 32-bit zone indices: c2p_scratch_bytes(n) bytes.

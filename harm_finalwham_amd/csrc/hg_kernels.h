@@ -86,6 +86,75 @@ void launch_weno_prep(const double *p, const double *geo, double *aux, const Dim
 void launch_weno(int dir, const double *p, double *aux, double *pleft, double *pright, const Dims &d,
                  const Tune &tu, int wide, bool with_dpi, hipStream_t s);
 
+// post_fixup's buffers of one grid (hg_fixup.hip)
+struct FxBufs {
+  int8_t *flag0;       // inversion codes (active zones)
+  int8_t *flag1;       // codes after the neighbour check
+  int *cand_z;         // candidate zone indices
+  uint16_t *cand_bits; // per neighbour l: bit l gamma vote, bit 8+l u vote
+  int8_t *cand_new;    // Jacobi scratch
+  int *fix_z;          // zones the average changed
+  double *fix_v;       // their RHO..U3 (5 x cap, structure of arrays)
+  int8_t *fix_flag;    // their final flag
+  int *fail_z;         // failed inversions that are not candidates
+  unsigned *count;     // [candidates, fixes, failed non-candidates]
+  int cap;
+};
+
+// ---------------------------------------------------------------------------
+// Grouped launches: one launch of a kernel class covers the blocks of several
+// grids.  Grid g owns blocks [first[g], first[g+1]) of the launch (none when
+// the class has no work on it) and reads its arguments from views[g]; inside,
+// the kernel runs the same per-zone code as its single-grid launch, with the
+// grid's own block numbering.  The classes whose block order is laid out for
+// the XCDs (work-groups go to the 8 XCDs round robin) get a multiple of 8
+// blocks per grid, so that block b % 8 is the same in both numberings.
+// ---------------------------------------------------------------------------
+struct GridView {
+  Dims d;
+  const double *pi, *pb;  // this substep's primitives: Ui from pi, fluxes and guess from pb
+  double *pf;             // its result, bounded and fixed up in place
+  const double *geo, *corn, *conn, *floors;
+  double *F[3], *emf[3], *pl[3], *pr[3], *aux, *uf, *dt;
+  unsigned long long *ndt_bits, *counters;
+  int8_t *pflag;
+  FxBufs fx;
+  int *err;
+  RkStage rk;
+  double gam, gammamax, cour, defcon, safe;
+  double dx[4], idx[4];   // idx = 1 / dx, as launch_update forms it
+  int finalstep, fluxmethod;
+  int nseg[4], seglen[4], nlb[4];  // reconstruction line segments per direction
+};
+
+__device__ __forceinline__ int group_of(const unsigned *__restrict__ first, int n, unsigned b) {
+  int lo = 0, hi = n - 1;  // the last grid g with first[g] <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+enum GroupClass {
+  GK_PREP, GK_DPI, GK_WENO1, GK_WENO2, GK_WENO3, GK_FLUX1, GK_FLUX2, GK_FLUX3, GK_EMF, GK_UPDATE, GK_INVERT,
+  GK_DT, GK_BOUND1, GK_BOUND2, GK_BOUND3, GK_FXCHECK, GK_FXRESOLVE, GK_FXAVG, GK_FXAPPLY, GK_NCLASS
+};
+// blocks of class k on one grid (0: no work there), and one grouped launch of
+// class k over n grids (total = first[n] blocks; mode: the inversion's)
+unsigned group_blocks(int k, const GridView &v);
+void launch_group(int k, const GridView *views, const unsigned *first, int n, unsigned total, int mode,
+                  hipStream_t s);
+// the per-TU parts of the two above
+unsigned group_blocks_weno(int k, const GridView &v);
+void launch_group_weno(int k, const GridView *views, const unsigned *first, int n, unsigned total, hipStream_t s);
+unsigned group_blocks_fixup(int k, const GridView &v);
+void launch_group_fixup(int k, const GridView *views, const unsigned *first, int n, unsigned total, hipStream_t s);
+// the reconstruction's line segments of a view (v.d set), as launch_weno
+// forms them for the standard scheme with the tune's segment requests
+void weno_group_shapes(GridView &v, const Tune &tu);
+
 // initial time step over every zone (advance.c:957 set_dt), folded into *out
 void launch_set_dt(const double *p, const double *geo, const Dims &d, double gam, double cour, const double *dx,
                    unsigned long long *out, int *err, hipStream_t s);

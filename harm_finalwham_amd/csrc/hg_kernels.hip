@@ -96,11 +96,11 @@ __device__ __forceinline__ void block_sum_to(unsigned long long a, unsigned long
 // ---------------------------------------------------------------------------
 // ghost fill, bounds.fishmon.c:29 (EXTRAP 0, POLEDEATH 1, POSDEFMETRIC 0)
 // ---------------------------------------------------------------------------
-__global__ void k_bound_x1(double *__restrict__ p, const double *__restrict__ geo, Dims d, double gammamax,
-                           int *__restrict__ err) {
+__device__ __forceinline__ void bound_x1_body(double *__restrict__ p, const double *__restrict__ geo, const Dims &d,
+                                              double gammamax, int *__restrict__ err, unsigned bid) {
   // threads: ((k * n2 + j) * 2 + side) * nb1 + g: the ghost layers of a row
   // on consecutive lanes (contiguous in memory, one source zone per side)
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t t = bid * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = 2LL * d.nb1 * d.n2 * d.n3;
   if (t >= total) return;
   int g = (int)(t % d.nb1);
@@ -123,9 +123,14 @@ __global__ void k_bound_x1(double *__restrict__ p, const double *__restrict__ ge
   for (int pl = 0; pl < NPR; pl++) p[dst + pl * d.spl] = pr[pl];
 }
 
-__global__ void k_bound_x2(double *__restrict__ p, Dims d) {
+__global__ void k_bound_x1(double *__restrict__ p, const double *__restrict__ geo, Dims d, double gammamax,
+                           int *__restrict__ err) {
+  bound_x1_body(p, geo, d, gammamax, err, blockIdx.x);
+}
+
+__device__ __forceinline__ void bound_x2_body(double *__restrict__ p, const Dims &d, unsigned bid) {
   // threads: ((slot * n3 + k) * m1 + ii), slot in [0, 2*nb2)
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t t = bid * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = 2LL * d.nb2 * d.n3 * d.m1;
   if (t >= total) return;
   int ii = t % d.m1;
@@ -159,9 +164,11 @@ __global__ void k_bound_x2(double *__restrict__ p, Dims d) {
   }
 }
 
-__global__ void k_bound_x3(double *__restrict__ p, Dims d) {
+__global__ void k_bound_x2(double *__restrict__ p, Dims d) { bound_x2_body(p, d, blockIdx.x); }
+
+__device__ __forceinline__ void bound_x3_body(double *__restrict__ p, const Dims &d, unsigned bid) {
   // threads: ((slot * m2 + jj) * m1 + ii), slot in [0, 2*nb3)
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t t = bid * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = 2LL * d.nb3 * d.m2 * d.m1;
   if (t >= total) return;
   int ii = t % d.m1;
@@ -176,6 +183,8 @@ __global__ void k_bound_x3(double *__restrict__ p, Dims d) {
 #pragma unroll
   for (int pl = 0; pl < NPR; pl++) p[dst + pl * d.spl] = p[src + pl * d.spl];
 }
+
+__global__ void k_bound_x3(double *__restrict__ p, Dims d) { bound_x3_body(p, d, blockIdx.x); }
 
 // ---------------------------------------------------------------------------
 // Riemann flux at one face from left/right states, fluxcompute.c:12
@@ -563,8 +572,8 @@ __device__ __forceinline__ void flux_lr_body(const double *__restrict__ pleft,
                                                  const double *__restrict__ geo, double *__restrict__ F, Dims d,
                                                  FaceRange fr, double gam, double cour, double dxdir,
                                                  int fluxmethod, unsigned long long *__restrict__ ndt_bits,
-                                                 int *__restrict__ err) {
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+                                                 int *__restrict__ err, unsigned bid) {
+  int64_t t = bid * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)fr.ni * fr.nj * fr.nk;
   double dtij = 1.e+50;
   if (t < total) {
@@ -604,7 +613,7 @@ __global__ void __launch_bounds__(256) k_flux_lr(const double *__restrict__ plef
                                                  double gam, double cour, double dxdir, int fluxmethod,
                                                  unsigned long long *__restrict__ ndt_bits, int *__restrict__ err) {
   FaceRange fr = face_range(d, DIR);
-  flux_lr_body<DIR, false>(pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  flux_lr_body<DIR, false>(pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err, blockIdx.x);
 }
 
 // the finite-volume flux loops (every transverse ghost row)
@@ -616,7 +625,7 @@ __global__ void __launch_bounds__(256) k_flux_lr_fv(const double *__restrict__ p
                                                     unsigned long long *__restrict__ ndt_bits,
                                                     int *__restrict__ err) {
   FaceRange fr = face_range_fv(d, DIR);
-  flux_lr_body<DIR, false>(pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  flux_lr_body<DIR, false>(pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err, blockIdx.x);
 }
 
 // WENO5BND states with the FORCE / MUSTA / HLLLAXF1 fluxes
@@ -628,7 +637,7 @@ __global__ void __launch_bounds__(256) k_flux_lr_ext(const double *__restrict__ 
                                                      int fluxmethod, unsigned long long *__restrict__ ndt_bits,
                                                      int *__restrict__ err) {
   FaceRange fr = fv ? face_range_fv(d, DIR) : face_range(d, DIR);
-  flux_lr_body<DIR, true>(pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err);
+  flux_lr_body<DIR, true>(pleft, pright, geo, F, d, fr, gam, cour, dxdir, fluxmethod, ndt_bits, err, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -636,12 +645,14 @@ __global__ void __launch_bounds__(256) k_flux_lr_ext(const double *__restrict__ 
 // field fluxes divided by gdet at their faces, averaged to the corner,
 // multiplied by gdet at the corner.
 // ---------------------------------------------------------------------------
-__global__ void k_emf(const double *__restrict__ F1, const double *__restrict__ F2, const double *__restrict__ F3,
-                      const double *__restrict__ geo, const double *__restrict__ corn, double *__restrict__ emf1,
-                      double *__restrict__ emf2, double *__restrict__ emf3, Dims d) {
+__device__ __forceinline__ void emf_body(const double *__restrict__ F1, const double *__restrict__ F2,
+                                         const double *__restrict__ F3, const double *__restrict__ geo,
+                                         const double *__restrict__ corn, double *__restrict__ emf1,
+                                         double *__restrict__ emf2, double *__restrict__ emf3, const Dims &d,
+                                         unsigned bid) {
   int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
   int ni = d.n1 + s1, nj = d.n2 + s2, nk = d.n3 + s3;
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t t = bid * (int64_t)blockDim.x + threadIdx.x;
   if (t >= (int64_t)ni * nj * nk) return;
   int i = (int)(t % ni);
   int64_t r = t / ni;
@@ -684,6 +695,12 @@ __global__ void k_emf(const double *__restrict__ F1, const double *__restrict__ 
   emf3[z] = e3 * corn[2 * d.gplane + g0];
 }
 
+__global__ void k_emf(const double *__restrict__ F1, const double *__restrict__ F2, const double *__restrict__ F3,
+                      const double *__restrict__ geo, const double *__restrict__ corn, double *__restrict__ emf1,
+                      double *__restrict__ emf2, double *__restrict__ emf3, Dims d) {
+  emf_body(F1, F2, F3, geo, corn, emf1, emf2, emf3, d, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // per-zone update, advance.c:143-213, in two kernels:
 //   k_update: U(pi) + dt * (flux divergence + source) -> Uf
@@ -693,15 +710,15 @@ __global__ void k_emf(const double *__restrict__ F1, const double *__restrict__ 
 // round trip through HBM costs less than the occupancy it loses)
 // ---------------------------------------------------------------------------
 template <bool UCUM, bool FV>
-__global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, const double *__restrict__ pb,
-                                                double *__restrict__ ubuf, const double *__restrict__ F1,
-                                                const double *__restrict__ F2, const double *__restrict__ F3,
-                                                const double *__restrict__ emf1, const double *__restrict__ emf2,
-                                                const double *__restrict__ emf3, const double *__restrict__ geo,
-                                                const double *__restrict__ conn, const double *__restrict__ dtp,
-                                                RkStage rk, Dims d, double gam, double idx1, double idx2,
-                                                double idx3, int *__restrict__ err) {
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void update_body(const double *__restrict__ pi, const double *__restrict__ pb,
+                                            double *__restrict__ ubuf, const double *__restrict__ F1,
+                                            const double *__restrict__ F2, const double *__restrict__ F3,
+                                            const double *__restrict__ emf1, const double *__restrict__ emf2,
+                                            const double *__restrict__ emf3, const double *__restrict__ geo,
+                                            const double *__restrict__ conn, const double *__restrict__ dtp,
+                                            const RkStage &rk, const Dims &d, double gam, double idx1, double idx2,
+                                            double idx3, int *__restrict__ err, unsigned bid) {
+  int64_t t = bid * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
   if (t < total) {
     // i fastest, then k (geometry of one j row stays in L2)
@@ -835,6 +852,19 @@ __global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, c
   }
 }
 
+template <bool UCUM, bool FV>
+__global__ void __launch_bounds__(256) k_update(const double *__restrict__ pi, const double *__restrict__ pb,
+                                                double *__restrict__ ubuf, const double *__restrict__ F1,
+                                                const double *__restrict__ F2, const double *__restrict__ F3,
+                                                const double *__restrict__ emf1, const double *__restrict__ emf2,
+                                                const double *__restrict__ emf3, const double *__restrict__ geo,
+                                                const double *__restrict__ conn, const double *__restrict__ dtp,
+                                                RkStage rk, Dims d, double gam, double idx1, double idx2,
+                                                double idx3, int *__restrict__ err) {
+  update_body<UCUM, FV>(pi, pb, ubuf, F1, F2, F3, emf1, emf2, emf3, geo, conn, dtp, rk, d, gam, idx1, idx2, idx3, err,
+                        blockIdx.x);
+}
+
 // The inversion of one zone in the step's mode: the reference's FP64
 // iteration (INV_FP64), or the mixed-precision path of the batched inversion
 // (FP32 Newton from the plain start, or from the learned start for
@@ -875,18 +905,17 @@ __device__ __forceinline__ int invert_zone(const double *Uf, const Geom &G, doub
 }
 
 template <bool FV, int MODE>
-__global__ void __launch_bounds__(256) k_invert(const double *__restrict__ ubuf, const double *__restrict__ pb,
-                                                double *__restrict__ pf, const double *__restrict__ geo,
-                                                const double *__restrict__ floors, int finalstep, Dims d,
-                                                double gam, double gammamax,
-                                                unsigned long long *__restrict__ counters,
-                                                int8_t *__restrict__ pflag, double *__restrict__ ucons,
-                                                const float *__restrict__ ylearn, unsigned *__restrict__ fx_reset,
-                                                int *__restrict__ err) {
+__device__ __forceinline__ void invert_body(const double *__restrict__ ubuf, const double *__restrict__ pb,
+                                            double *__restrict__ pf, const double *__restrict__ geo,
+                                            const double *__restrict__ floors, int finalstep, const Dims &d,
+                                            double gam, double gammamax, unsigned long long *__restrict__ counters,
+                                            int8_t *__restrict__ pflag, double *__restrict__ ucons,
+                                            const float *__restrict__ ylearn, unsigned *__restrict__ fx_reset,
+                                            int *__restrict__ err, unsigned bid) {
   // the post_fixup lists of this substep start empty (launch_post_fixup
   // then skips its own reset)
-  if (fx_reset && blockIdx.x == 0 && threadIdx.x < 3) fx_reset[threadIdx.x] = 0u;
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (fx_reset && bid == 0 && threadIdx.x < 3) fx_reset[threadIdx.x] = 0u;
+  int64_t t = bid * (int64_t)blockDim.x + threadIdx.x;
   int64_t total = (int64_t)d.n1 * d.n2 * d.n3;
   unsigned long long nfail = 0, nit = 0;
   if (t < total) {
@@ -949,9 +978,22 @@ __global__ void __launch_bounds__(256) k_invert(const double *__restrict__ ubuf,
   block_sum_to(nfail, nit, &counters[0], &counters[1]);
 }
 
+template <bool FV, int MODE>
+__global__ void __launch_bounds__(256) k_invert(const double *__restrict__ ubuf, const double *__restrict__ pb,
+                                                double *__restrict__ pf, const double *__restrict__ geo,
+                                                const double *__restrict__ floors, int finalstep, Dims d,
+                                                double gam, double gammamax,
+                                                unsigned long long *__restrict__ counters,
+                                                int8_t *__restrict__ pflag, double *__restrict__ ucons,
+                                                const float *__restrict__ ylearn, unsigned *__restrict__ fx_reset,
+                                                int *__restrict__ err) {
+  invert_body<FV, MODE>(ubuf, pb, pf, geo, floors, finalstep, d, gam, gammamax, counters, pflag, ucons, ylearn,
+                        fx_reset, err, blockIdx.x);
+}
+
 // advance.c:231 ndt = defcon / (1/ndt1 + 1/ndt2 + 1/ndt3); on the final
 // substep step_ch.c:243-247 limits the growth and stores the next dt
-__global__ void k_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe) {
+__device__ __forceinline__ void dt_body(unsigned long long *ndt_bits, double *dtp, double defcon, double safe) {
   double n1 = __longlong_as_double((long long)ndt_bits[0]);
   double n2 = __longlong_as_double((long long)ndt_bits[1]);
   double n3 = __longlong_as_double((long long)ndt_bits[2]);
@@ -960,6 +1002,72 @@ __global__ void k_dt(unsigned long long *ndt_bits, double *dtp, double defcon, d
   if (ndt > safe * dt) ndt = safe * dt;
   dtp[0] = ndt;
 }
+
+__global__ void k_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double safe) {
+  dt_body(ndt_bits, dtp, defcon, safe);
+}
+
+// ---------------------------------------------------------------------------
+// grouped launches of the kernels above (GridView, group_of)
+// ---------------------------------------------------------------------------
+#define HG_GROUP_PRELUDE                               \
+  const int g_ = group_of(first, n, blockIdx.x);       \
+  const GridView &v = views[g_];                       \
+  const unsigned bid = blockIdx.x - first[g_];
+
+__global__ void __launch_bounds__(128) kg_bound_x1(const GridView *__restrict__ views,
+                                                   const unsigned *__restrict__ first, int n) {
+  HG_GROUP_PRELUDE
+  bound_x1_body(v.pf, v.geo, v.d, v.gammamax, v.err, bid);
+}
+
+__global__ void __launch_bounds__(256) kg_bound_x2(const GridView *__restrict__ views,
+                                                   const unsigned *__restrict__ first, int n) {
+  HG_GROUP_PRELUDE
+  bound_x2_body(v.pf, v.d, bid);
+}
+
+__global__ void __launch_bounds__(256) kg_bound_x3(const GridView *__restrict__ views,
+                                                   const unsigned *__restrict__ first, int n) {
+  HG_GROUP_PRELUDE
+  bound_x3_body(v.pf, v.d, bid);
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) kg_flux_lr(const GridView *__restrict__ views,
+                                                  const unsigned *__restrict__ first, int n) {
+  HG_GROUP_PRELUDE
+  flux_lr_body<DIR, false>(v.pl[DIR - 1], v.pr[DIR - 1], v.geo, v.F[DIR - 1], v.d, face_range(v.d, DIR), v.gam,
+                           v.cour, v.dx[DIR], v.fluxmethod, v.ndt_bits + DIR - 1, v.err, bid);
+}
+
+__global__ void __launch_bounds__(256) kg_emf(const GridView *__restrict__ views, const unsigned *__restrict__ first,
+                                              int n) {
+  HG_GROUP_PRELUDE
+  emf_body(v.F[0], v.F[1], v.F[2], v.geo, v.corn, v.emf[0], v.emf[1], v.emf[2], v.d, bid);
+}
+
+__global__ void __launch_bounds__(256) kg_update(const GridView *__restrict__ views,
+                                                 const unsigned *__restrict__ first, int n) {
+  HG_GROUP_PRELUDE
+  update_body<false, false>(v.pi, v.pb, v.uf, v.F[0], v.F[1], v.F[2], v.emf[0], v.emf[1], v.emf[2], v.geo, v.conn,
+                            v.dt, v.rk, v.d, v.gam, v.idx[1], v.idx[2], v.idx[3], v.err, bid);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) kg_invert(const GridView *__restrict__ views,
+                                                 const unsigned *__restrict__ first, int n) {
+  HG_GROUP_PRELUDE
+  invert_body<false, MODE>(v.uf, v.pb, v.pf, v.geo, v.floors, v.finalstep, v.d, v.gam, v.gammamax, v.counters,
+                           v.pflag, nullptr, nullptr, v.fx.count, v.err, bid);
+}
+
+__global__ void kg_dt(const GridView *__restrict__ views, const unsigned *__restrict__ first, int n) {
+  HG_GROUP_PRELUDE
+  (void)bid;
+  if (threadIdx.x == 0) dt_body(v.ndt_bits, v.dt, v.defcon, v.safe);
+}
+#undef HG_GROUP_PRELUDE
 
 __global__ void k_reset_ndt(unsigned long long *ndt_bits) {
   if (threadIdx.x < 3) ndt_bits[threadIdx.x] = (unsigned long long)__double_as_longlong(1.e+50);
@@ -1263,6 +1371,55 @@ void launch_dt(unsigned long long *ndt_bits, double *dtp, double defcon, double 
 
 void launch_reset_ndt(unsigned long long *ndt_bits, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_ndt, dim3(1), dim3(64), 0, s, ndt_bits);
+}
+
+// grouped launches: the block counts of the single-grid launchers above at
+// their default block sizes
+unsigned group_blocks(int k, const GridView &v) {
+  const Dims &d = v.d;
+  const int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  const int64_t act = (int64_t)d.n1 * d.n2 * d.n3;
+  switch (k) {
+    case GK_BOUND1: return s1 ? nblocks(2LL * d.nb1 * d.n2 * d.n3, 128) : 0u;
+    case GK_BOUND2: return s2 ? nblocks(2LL * d.nb2 * d.n3 * d.m1, 256) : 0u;
+    case GK_BOUND3: return s3 ? nblocks(2LL * d.nb3 * d.m2 * d.m1, 256) : 0u;
+    case GK_FLUX1: case GK_FLUX2: case GK_FLUX3: {
+      const int dir = 1 + k - GK_FLUX1;
+      if (!((dir == 1) ? s1 : (dir == 2) ? s2 : s3)) return 0u;
+      const FaceRange fr = face_range(d, dir);
+      return nblocks((int64_t)fr.ni * fr.nj * fr.nk, 256);
+    }
+    case GK_EMF: return nblocks((int64_t)(d.n1 + s1) * (d.n2 + s2) * (d.n3 + s3), 256);
+    case GK_UPDATE: case GK_INVERT: return nblocks(act, 256);
+    case GK_DT: return 1u;
+    case GK_FXCHECK: case GK_FXRESOLVE: case GK_FXAVG: case GK_FXAPPLY: return group_blocks_fixup(k, v);
+    default: return group_blocks_weno(k, v);
+  }
+}
+
+void launch_group(int k, const GridView *views, const unsigned *first, int n, unsigned total, int mode,
+                  hipStream_t s) {
+  if (total == 0) return;
+  const dim3 g(total);
+  switch (k) {
+    case GK_BOUND1: hipLaunchKernelGGL(kg_bound_x1, g, dim3(128), 0, s, views, first, n); return;
+    case GK_BOUND2: hipLaunchKernelGGL(kg_bound_x2, g, dim3(256), 0, s, views, first, n); return;
+    case GK_BOUND3: hipLaunchKernelGGL(kg_bound_x3, g, dim3(256), 0, s, views, first, n); return;
+    case GK_FLUX1: hipLaunchKernelGGL(kg_flux_lr<1>, g, dim3(256), 0, s, views, first, n); return;
+    case GK_FLUX2: hipLaunchKernelGGL(kg_flux_lr<2>, g, dim3(256), 0, s, views, first, n); return;
+    case GK_FLUX3: hipLaunchKernelGGL(kg_flux_lr<3>, g, dim3(256), 0, s, views, first, n); return;
+    case GK_EMF: hipLaunchKernelGGL(kg_emf, g, dim3(256), 0, s, views, first, n); return;
+    case GK_UPDATE: hipLaunchKernelGGL(kg_update, g, dim3(256), 0, s, views, first, n); return;
+    case GK_INVERT:
+      if (mode == INV_MIXED) hipLaunchKernelGGL(kg_invert<INV_MIXED>, g, dim3(256), 0, s, views, first, n);
+      else hipLaunchKernelGGL(kg_invert<INV_FP64>, g, dim3(256), 0, s, views, first, n);
+      return;
+    case GK_DT: hipLaunchKernelGGL(kg_dt, g, dim3(64), 0, s, views, first, n); return;
+    case GK_FXCHECK: case GK_FXRESOLVE: case GK_FXAVG: case GK_FXAPPLY:
+      launch_group_fixup(k, views, first, n, total, s);
+      return;
+    default: launch_group_weno(k, views, first, n, total, s);
+  }
 }
 
 

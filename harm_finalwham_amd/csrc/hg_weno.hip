@@ -327,17 +327,17 @@ __device__ __forceinline__ void stage_e(const double *y, double m0, double w0, d
 // stage A inputs per zone: par = (1 + u_t) u^t, upar = gam u u^t u_t + (gam-1) u
 // over the box [-2, n+1] of every direction with more than one zone
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_weno_prep(const double *__restrict__ p, const double *__restrict__ geo,
-                                                   double *__restrict__ aux, Dims d, double gam, int wide,
-                                                   unsigned long long *__restrict__ ndt_reset) {
+__device__ __forceinline__ void weno_prep_body(const double *__restrict__ p, const double *__restrict__ geo,
+                                               double *__restrict__ aux, const Dims &d, double gam, int wide,
+                                               unsigned long long *__restrict__ ndt_reset, unsigned bid) {
   // the substep's flux kernels take the minimum of their ndt into these
-  if (ndt_reset && blockIdx.x == 0 && threadIdx.x < 3)
+  if (ndt_reset && bid == 0 && threadIdx.x < 3)
     ndt_reset[threadIdx.x] = (unsigned long long)__double_as_longlong(1.e+50);
   // wide (the finite-volume flux loops): every ghost layer
   const int e1 = d.n1 > 1 ? (wide ? d.nb1 : 2) : 0, e2 = d.n2 > 1 ? (wide ? d.nb2 : 2) : 0;
   const int e3 = d.n3 > 1 ? (wide ? d.nb3 : 2) : 0;
   const int b1 = d.n1 + 2 * e1, b2 = d.n2 + 2 * e2, b3 = d.n3 + 2 * e3;
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t t = bid * (int64_t)blockDim.x + threadIdx.x;
   if (t >= (int64_t)b1 * b2 * b3) return;
   int i = -e1 + (int)(t % b1);
   int64_t r = t / b1;
@@ -366,6 +366,12 @@ __global__ void __launch_bounds__(256) k_weno_prep(const double *__restrict__ p,
   aux[z + d.spl] = gam * pr[UU] * (q.ucon[0] * q.ucov[0]) + (gam - 1.0) * pr[UU];
 }
 
+__global__ void __launch_bounds__(256) k_weno_prep(const double *__restrict__ p, const double *__restrict__ geo,
+                                                   double *__restrict__ aux, Dims d, double gam, int wide,
+                                                   unsigned long long *__restrict__ ndt_reset) {
+  weno_prep_body(p, geo, aux, d, gam, wide, ndt_reset, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // stage A along DIR (DIR 0: every direction of the grid): dP/P at every
 // zone of the box above whose two neighbours along that direction are in the
@@ -379,13 +385,13 @@ __global__ void __launch_bounds__(256) k_weno_prep(const double *__restrict__ p,
 // reads were fetched into that XCD's L2 by the blocks just before it.
 // ---------------------------------------------------------------------------
 template <int DIR>
-__global__ void __launch_bounds__(256) k_weno_dpi(const double *__restrict__ p, double *__restrict__ aux, Dims d,
-                                                  int wide) {
+__device__ __forceinline__ void weno_dpi_body(const double *__restrict__ p, double *__restrict__ aux, const Dims &d,
+                                              int wide, unsigned bid, unsigned nblk) {
   const int e1 = d.n1 > 1 ? (wide ? d.nb1 : 2) : 0, e2 = d.n2 > 1 ? (wide ? d.nb2 : 2) : 0;
   const int e3 = d.n3 > 1 ? (wide ? d.nb3 : 2) : 0;
   const int b1 = d.n1 + 2 * e1, b2 = d.n2 + 2 * e2, b3 = d.n3 + 2 * e3;
-  const unsigned xcd = blockIdx.x & 7u, q = blockIdx.x >> 3, base = gridDim.x >> 3, rem = gridDim.x & 7u;
-  const unsigned lb = xcd * base + (xcd < rem ? xcd : rem) + q;  // a bijection of [0, gridDim.x)
+  const unsigned xcd = bid & 7u, q = bid >> 3, base = nblk >> 3, rem = nblk & 7u;
+  const unsigned lb = xcd * base + (xcd < rem ? xcd : rem) + q;  // a bijection of [0, nblk)
   int64_t t = lb * (int64_t)blockDim.x + threadIdx.x;
   if (t >= (int64_t)b1 * b2 * b3) return;
   int i = -e1 + (int)(t % b1);
@@ -405,17 +411,23 @@ __global__ void __launch_bounds__(256) k_weno_dpi(const double *__restrict__ p, 
   }
 }
 
+template <int DIR>
+__global__ void __launch_bounds__(256) k_weno_dpi(const double *__restrict__ p, double *__restrict__ aux, Dims d,
+                                                  int wide) {
+  weno_dpi_body<DIR>(p, aux, d, wide, blockIdx.x, gridDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // along i: one lane per point; wavefront w covers points s0-3 .. s0+60 and
 // writes the interface pair of s0 .. s0+57
 // ---------------------------------------------------------------------------
 constexpr int XOUT = 58;
 
-__global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, const double *__restrict__ dpi,
-                                                double *__restrict__ pleft, double *__restrict__ pright, Dims d,
-                                                int nseg, int wide) {
+__device__ __forceinline__ void weno_x_body(const double *__restrict__ p, const double *__restrict__ dpi,
+                                            double *__restrict__ pleft, double *__restrict__ pright, const Dims &d,
+                                            int nseg, int wide, unsigned bid) {
   const int lane = threadIdx.x & 63;
-  const int64_t wid = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t wid = (int64_t)bid * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int n = d.n1;
   // transverse rows: -1..n, or every ghost row for the finite-volume loops
   const int s2 = d.n2 > 1 ? (wide ? d.nb2 : 1) : 0, s3 = d.n3 > 1 ? (wide ? d.nb3 : 1) : 0;
@@ -452,6 +464,12 @@ __global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, co
   }
 }
 
+__global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, const double *__restrict__ dpi,
+                                                double *__restrict__ pleft, double *__restrict__ pright, Dims d,
+                                                int nseg, int wide) {
+  weno_x_body(p, dpi, pleft, pright, d, nseg, wide, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // along j (DIR 2) or k (DIR 3): lane = (i, transverse index) flattened with i
 // fastest; each lane marches over its segment of the line: at step t it
@@ -479,9 +497,9 @@ __device__ __forceinline__ gchar *uniform_row(const void *base, int64_t off) {
 // the block, so every load and store is a uniform row address plus one
 // 32-bit per-lane byte offset (the context keeps a plane under 4 GiB).
 template <int DIR>
-__global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, const double *__restrict__ dpi,
-                                                double *__restrict__ pleft, double *__restrict__ pright, Dims d,
-                                                int nseg, int seglen, int wide, int nlb) {
+__device__ __forceinline__ void weno_m_body(const double *__restrict__ p, const double *__restrict__ dpi,
+                                            double *__restrict__ pleft, double *__restrict__ pright, const Dims &d,
+                                            int nseg, int seglen, int wide, int nlb, unsigned bid) {
   const int n = (DIR == 1) ? d.n1 : (DIR == 2) ? d.n2 : d.n3;
   // the lane's two transverse indices: a (fastest) and o, over -1..n or,
   // for the finite-volume loops, every ghost row
@@ -491,9 +509,9 @@ __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, co
   const int so = (DIR == 2) ? h3 : (DIR == 3) ? h2 : h3;
   const int ca = ((DIR == 1) ? d.n2 : d.n1) + 2 * sa;
   const int co = ((DIR == 2) ? d.n3 : (DIR == 3) ? d.n2 : d.n3) + 2 * so;
-  const unsigned slot = blockIdx.x >> 3;
+  const unsigned slot = bid >> 3;
   const int pl = (int)(slot & 7u);
-  const unsigned unit = (slot >> 3) * 8u + (blockIdx.x & 7u);
+  const unsigned unit = (slot >> 3) * 8u + (bid & 7u);
   if (unit >= (unsigned)(nlb * nseg)) return;
   const int seg = (int)(unit / (unsigned)nlb);
   const int64_t g = (int64_t)(unit % (unsigned)nlb) * blockDim.x + threadIdx.x;
@@ -582,6 +600,47 @@ __global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, co
 #undef WM_ST
 }
 
+template <int DIR>
+__global__ void __launch_bounds__(256) k_weno_m(const double *__restrict__ p, const double *__restrict__ dpi,
+                                                double *__restrict__ pleft, double *__restrict__ pright, Dims d,
+                                                int nseg, int seglen, int wide, int nlb) {
+  weno_m_body<DIR>(p, dpi, pleft, pright, d, nseg, seglen, wide, nlb, blockIdx.x);
+}
+
+// grouped launches (GridView, group_of) of the standard scheme's loops
+#define WG_PRELUDE                                     \
+  const int g_ = group_of(first, n, blockIdx.x);       \
+  const GridView &v = views[g_];                       \
+  const unsigned bid = blockIdx.x - first[g_], nblk = first[g_ + 1] - first[g_]; \
+  (void)nblk;
+
+__global__ void __launch_bounds__(256) kg_weno_prep(const GridView *__restrict__ views,
+                                                    const unsigned *__restrict__ first, int n) {
+  WG_PRELUDE
+  weno_prep_body(v.pb, v.geo, v.aux, v.d, v.gam, 0, v.ndt_bits, bid);
+}
+
+__global__ void __launch_bounds__(256) kg_weno_dpi(const GridView *__restrict__ views,
+                                                   const unsigned *__restrict__ first, int n) {
+  WG_PRELUDE
+  weno_dpi_body<0>(v.pb, v.aux, v.d, 0, bid, nblk);
+}
+
+__global__ void __launch_bounds__(256) kg_weno_x(const GridView *__restrict__ views,
+                                                 const unsigned *__restrict__ first, int n) {
+  WG_PRELUDE
+  weno_x_body(v.pb, v.aux + 2 * v.d.spl, v.pl[0], v.pr[0], v.d, v.nseg[1], 0, bid);
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) kg_weno_m(const GridView *__restrict__ views,
+                                                 const unsigned *__restrict__ first, int n) {
+  WG_PRELUDE
+  weno_m_body<DIR>(v.pb, v.aux + (1 + DIR) * v.d.spl, v.pl[DIR - 1], v.pr[DIR - 1], v.d, v.nseg[DIR],
+                   v.seglen[DIR], 0, v.nlb[DIR], bid);
+}
+#undef WG_PRELUDE
+
 void launch_weno_prep(const double *p, const double *geo, double *aux, const Dims &d, double gam, int wide,
                       bool all_dpi, unsigned long long *ndt_reset, hipStream_t s) {
   const int e1 = d.n1 > 1 ? (wide ? d.nb1 : 2) : 0, e2 = d.n2 > 1 ? (wide ? d.nb2 : 2) : 0;
@@ -592,11 +651,25 @@ void launch_weno_prep(const double *p, const double *geo, double *aux, const Dim
   if (all_dpi) hipLaunchKernelGGL(k_weno_dpi<0>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, aux, d, wide);
 }
 
+static void weno_march_shape(int dir, const Dims &d, const Tune &tu, int wide, int &nseg, int &seglen, int &nlb) {
+  const int s1 = d.n1 > 1 ? (wide ? d.nb1 : 1) : 0, s2 = d.n2 > 1 ? (wide ? d.nb2 : 1) : 0;
+  const int s3 = d.n3 > 1 ? (wide ? d.nb3 : 1) : 0;
+  const int n = (dir == 1) ? d.n1 : (dir == 2) ? d.n2 : d.n3;
+  const int64_t lanes = (dir == 1) ? (int64_t)(d.n2 + 2 * s2) * (d.n3 + 2 * s3)
+                                   : (int64_t)(d.n1 + 2 * s1) * ((dir == 2) ? d.n3 + 2 * s3 : d.n2 + 2 * s2);
+  // split long lines so that the grid holds enough wavefronts
+  const int want = (dir == 1) ? tu.weno_xmarch : (dir == 2) ? tu.weno_nseg : tu.weno_nseg3;
+  nseg = want > 0 ? want : ((n + 2 >= 96) ? 2 : 1);
+  if (nseg > n + 2) nseg = n + 2;
+  seglen = (n + 2 + nseg - 1) / nseg;
+  nseg = (n + 2 + seglen - 1) / seglen;  // every segment starts at or before point n
+  nlb = (int)((lanes + 255) / 256);
+}
+
 void launch_weno(int dir, const double *p, double *aux, double *pleft, double *pright, const Dims &d,
                  const Tune &tu, int wide, bool with_dpi, hipStream_t s) {
   // transverse halo rows (the kernels use the same widths)
-  int s1 = d.n1 > 1 ? (wide ? d.nb1 : 1) : 0, s2 = d.n2 > 1 ? (wide ? d.nb2 : 1) : 0;
-  int s3 = d.n3 > 1 ? (wide ? d.nb3 : 1) : 0;
+  const int s2 = d.n2 > 1 ? (wide ? d.nb2 : 1) : 0, s3 = d.n3 > 1 ? (wide ? d.nb3 : 1) : 0;
   // the dP/P indicators of this direction, over the box of k_weno_prep
   // (unless launch_weno_prep formed every direction's)
   if (with_dpi) {
@@ -616,18 +689,10 @@ void launch_weno(int dir, const double *p, double *aux, double *pleft, double *p
     hipLaunchKernelGGL(k_weno_x, dim3((unsigned)((nw + wpb - 1) / wpb)), dim3(64 * wpb), 0, s, p, dpi, pleft,
                        pright, d, nseg, wide);
   } else {
-    int n = (dir == 1) ? d.n1 : (dir == 2) ? d.n2 : d.n3;
-    int64_t lanes = (dir == 1) ? (int64_t)(d.n2 + 2 * s2) * (d.n3 + 2 * s3)
-                               : (int64_t)(d.n1 + 2 * s1) * ((dir == 2) ? d.n3 + 2 * s3 : d.n2 + 2 * s2);
-    // split long lines so that the grid holds enough wavefronts
-    int want = (dir == 1) ? tu.weno_xmarch : (dir == 2) ? tu.weno_nseg : tu.weno_nseg3;
-    int nseg = want > 0 ? want : ((n + 2 >= 96) ? 2 : 1);
-    if (nseg > n + 2) nseg = n + 2;
-    int seglen = (n + 2 + nseg - 1) / nseg;
-    nseg = (n + 2 + seglen - 1) / seglen;  // every segment starts at or before point n
+    int nseg, seglen, nlb;
+    weno_march_shape(dir, d, tu, wide, nseg, seglen, nlb);
     // units of (lane block, segment), padded to whole groups of 8, times
     // the 8 primitives (NPR = 8)
-    const int nlb = (int)((lanes + 255) / 256);
     const int64_t units = ((int64_t)nlb * nseg + 7) / 8 * 8;
     dim3 grid((unsigned)(units * NPR));
     if (dir == 1)
@@ -637,6 +702,48 @@ void launch_weno(int dir, const double *p, double *aux, double *pleft, double *p
     else
       hipLaunchKernelGGL(k_weno_m<3>, grid, dim3(256), 0, s, p, dpi, pleft, pright, d, nseg, seglen, wide, nlb);
   }
+}
+
+void weno_group_shapes(GridView &v, const Tune &tu) {
+  v.nseg[1] = (v.d.n1 + 2 + XOUT - 1) / XOUT;  // k_weno_x's segments
+  v.seglen[1] = v.nlb[1] = 0;
+  v.nseg[0] = v.seglen[0] = v.nlb[0] = 0;
+  for (int dir = 2; dir <= 3; dir++) weno_march_shape(dir, v.d, tu, 0, v.nseg[dir], v.seglen[dir], v.nlb[dir]);
+}
+
+// grouped: the blocks of the launchers above with the default shapes (four
+// wavefronts per block along i; the view's segments along j and k), a
+// multiple of 8 per grid for the XCD-ordered classes
+unsigned group_blocks_weno(int k, const GridView &v) {
+  const Dims &d = v.d;
+  const int e1 = d.n1 > 1 ? 2 : 0, e2 = d.n2 > 1 ? 2 : 0, e3 = d.n3 > 1 ? 2 : 0;
+  const int64_t nbox = (int64_t)(d.n1 + 2 * e1) * (d.n2 + 2 * e2) * (d.n3 + 2 * e3);
+  switch (k) {
+    case GK_PREP: return (unsigned)((nbox + 255) / 256);
+    case GK_DPI: return (unsigned)((nbox + 255) / 256 + 7) / 8 * 8;
+    case GK_WENO1: {
+      if (d.n1 <= 1) return 0u;
+      const int s2 = d.n2 > 1 ? 1 : 0, s3 = d.n3 > 1 ? 1 : 0;
+      const int64_t nw = (int64_t)v.nseg[1] * NPR * (d.n2 + 2 * s2) * (d.n3 + 2 * s3);
+      return (unsigned)((nw + 3) / 4);
+    }
+    case GK_WENO2: case GK_WENO3: {
+      const int dir = (k == GK_WENO2) ? 2 : 3;
+      if (((dir == 2) ? d.n2 : d.n3) <= 1) return 0u;
+      const int64_t units = ((int64_t)v.nlb[dir] * v.nseg[dir] + 7) / 8 * 8;
+      return (unsigned)(units * NPR);
+    }
+    default: return 0u;
+  }
+}
+
+void launch_group_weno(int k, const GridView *views, const unsigned *first, int n, unsigned total, hipStream_t s) {
+  const dim3 g(total), b(256);
+  if (k == GK_PREP) hipLaunchKernelGGL(kg_weno_prep, g, b, 0, s, views, first, n);
+  if (k == GK_DPI) hipLaunchKernelGGL(kg_weno_dpi, g, b, 0, s, views, first, n);
+  if (k == GK_WENO1) hipLaunchKernelGGL(kg_weno_x, g, b, 0, s, views, first, n);
+  if (k == GK_WENO2) hipLaunchKernelGGL(kg_weno_m<2>, g, b, 0, s, views, first, n);
+  if (k == GK_WENO3) hipLaunchKernelGGL(kg_weno_m<3>, g, b, 0, s, views, first, n);
 }
 
 }  // namespace hg

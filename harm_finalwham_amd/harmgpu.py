@@ -23,7 +23,7 @@ EXPORTS = ["hg_ctx_create", "hg_ctx_destroy", "hg_upload_geometry", "hg_mks_geom
            "hg_step", "hg_step_async", "hg_set_dt", "hg_get_dt", "hg_get_counters",
            "hg_utoprim_batch", "hg_make_c2p_batch", "hg_last_error",
            "hg_profile", "hg_profile_read", "hg_profile_class_name", "hg_c2p_fallbacks",
-           "hg_graph_create", "hg_graph_launch", "hg_graph_destroy", "hg_get_tune", "hg_set_tune", "hg_autotune",
+           "hg_graph_create", "hg_graph_create_grouped", "hg_graph_launch", "hg_graph_destroy", "hg_get_tune", "hg_set_tune", "hg_autotune",
            "hg_mlp_create", "hg_mlp_destroy", "hg_mlp_nparams", "hg_mlp_get_params", "hg_mlp_set_params",
            "hg_mlp_features", "hg_mlp_forward", "hg_mlp_gradient", "hg_mlp_train",
            "hg_utoprim_batch_learned", "hg_post_fixup", "hg_get_pflag", "hg_diagnostics", "hg_diag_record", "hg_rdump_parse",
@@ -141,6 +141,7 @@ def lib():
     L.hg_check_division.argtypes = [C.c_int64, C.c_uint64, i64p]
     L.hg_c2p_fallbacks.argtypes = [i64p]
     L.hg_graph_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.POINTER(vp)]
+    L.hg_graph_create_grouped.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.POINTER(vp)]
     L.hg_graph_launch.argtypes = [vp, vp]
     L.hg_graph_destroy.argtypes = [vp]
     L.hg_get_tune.argtypes = [vp, C.POINTER(HgTune)]
@@ -563,13 +564,17 @@ def utoprim_batch_learned(mlp, U, gcov, gcon, gdet, gcovpert, pr, pflag, iters=N
 
 
 class Graph:
-    """hg_graph: nsteps RK2 steps of several grids in one HIP graph."""
+    """hg_graph: nsteps RK2 steps of several grids in one HIP graph; grouped:
+    one launch per kernel over all the grids (hg_graph_create_grouped)
+    instead of one branch per grid (hg_graph_create)."""
 
-    def __init__(self, grids, nsteps):
+    def __init__(self, grids, nsteps, grouped=False):
+        self.h = None
         self.grids = list(grids)  # keep the contexts alive
         arr = (C.c_void_p * len(self.grids))(*[g.h for g in self.grids])
         h = C.c_void_p()
-        _check(lib().hg_graph_create(arr, len(self.grids), nsteps, C.byref(h)))
+        create = lib().hg_graph_create_grouped if grouped else lib().hg_graph_create
+        _check(create(arr, len(self.grids), nsteps, C.byref(h)))
         self.h = h
 
     def launch(self, stream=None):

@@ -411,6 +411,17 @@ int hg_set_inversion(hg_ctx *ctx, int mode, const hg_mlp *m);
    graph and must not be stepped otherwise while a launch is in flight. */
 typedef struct hg_graph hg_graph;
 int hg_graph_create(hg_ctx *const *ctxs, int n, int nsteps, hg_graph **out);
+/* The same with grouped launches, no counterpart in the reference: each
+   kernel of a substep is one launch over the blocks of every context (each
+   block reads its grid's arguments from a table on the device), so a step
+   of all the grids takes ~40 launches however many grids there are, with
+   every grid's arithmetic that of its own run, bit for bit.  For the
+   standard scheme with WENO5BND, the LAXF or HLL flux and RK2
+   (timeorder 2), every context with the same inversion mode, HG_C2P_FP64
+   or HG_C2P_MIXED, and no diagnostics recording; other contexts are
+   rejected with an error (hg_graph_create takes them).  Launch shapes are
+   the defaults, with each context's segment counts along j and k. */
+int hg_graph_create_grouped(hg_ctx *const *ctxs, int n, int nsteps, hg_graph **out);
 int hg_graph_launch(hg_graph *g, void *stream);
 int hg_graph_destroy(hg_graph *g);
 

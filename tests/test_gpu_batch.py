@@ -1,5 +1,6 @@
 """Several tori of different sizes, spins and torus radii advanced by one
-HIP graph (hg_graph_create): every grid matches its own single-grid run bit
+HIP graph (hg_graph_create, and hg_graph_create_grouped with one launch per
+kernel over all the grids): every grid matches its own single-grid run bit
 for bit, on every replay of the graph; and the launch-shape autotuner
 (hg_autotune) leaves results unchanged."""
 import math
@@ -50,6 +51,47 @@ def test_graph_matches_single_runs_on_every_replay(scheme):
         x = x * 1.0000001 + 1.0
         torch.cuda.synchronize()
     graph.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_grouped_graph_matches_single_runs_on_every_replay(mode):
+    """hg_graph_create_grouped: one launch per kernel over every grid; each
+    grid still evolves bit for bit as alone, with either inversion mode."""
+    import torch
+    from harm_finalwham_amd import harmgpu
+    print("case grouped graph of", len(SPECS), "tori,", mode, flush=True)
+    batch = [torus(*s) for s in SPECS]
+    alone = [torus(*s) for s in SPECS]
+    if mode == "mixed":
+        for g in batch + alone:
+            g.set_inversion(harmgpu.HG_C2P_MIXED)
+    graph = harmgpu.Graph(batch, nsteps=3, grouped=True)
+    x = torch.ones(1 << 20, device="cuda", dtype=torch.float64)
+    for replay in range(3):
+        graph.launch()
+        for g in alone:
+            g.step_async(3)
+        torch.cuda.synchronize()
+        for m, (gb, ga) in enumerate(zip(batch, alone)):
+            assert gb.get_dt() == ga.get_dt(), (replay, m)
+            assert np.array_equal(gb.get_prims(), ga.get_prims()), (replay, m)
+            assert np.array_equal(gb.get_pflag(), ga.get_pflag()), (replay, m)
+        x = x * 1.0000001 + 1.0  # unrelated eager work between replays
+        torch.cuda.synchronize()
+    graph.close()
+
+
+def test_grouped_graph_rejects_other_schemes():
+    from harm_finalwham_amd import harmgpu
+    a, b = torus(*SPECS[0]), torus(*SPECS[1])
+    fv = torus(*SPECS[2], scheme="fv")
+    with pytest.raises(harmgpu.HarmError, match="WENO5BND"):
+        harmgpu.Graph([a, fv], nsteps=1, grouped=True)
+    b.set_inversion(harmgpu.HG_C2P_MIXED)
+    with pytest.raises(harmgpu.HarmError, match="inversion mode"):
+        harmgpu.Graph([a, b], nsteps=1, grouped=True)
+    with pytest.raises(harmgpu.HarmError):
+        harmgpu.Graph([a, a], nsteps=1, grouped=True)
 
 
 def test_graph_rejects_duplicate_contexts():

@@ -1,5 +1,6 @@
 """48 independent 2-D tori of uneven sizes advanced by one HIP graph
-(hg_graph_create), at the batch's own sizes: {64x64, 96x128, 128x128,
+(hg_graph_create: a branch per grid; hg_graph_create_grouped: one launch
+per kernel over all the grids), at the batch's own sizes: {64x64, 96x128, 128x128,
 192x192, 256x256}, spins a ~ U[0, 0.98], pressure-maximum radii
 r_pmax ~ U[10, 14], seeded perturbations, WENO5BND + HLL + flux-CT + Noble 2D
 with RK2 (each instance is one grid's step_ch, step_ch.c:35).
@@ -46,12 +47,13 @@ def make(spec):
     return g, dt
 
 
-def test_48_uneven_tori_graph_matches_single_runs_and_reference():
+@pytest.mark.parametrize("grouped", [False, True])
+def test_48_uneven_tori_graph_matches_single_runs_and_reference(grouped):
     import torch
     from harm_finalwham_amd import harmgpu
     specs = batch_specs()
-    print("case 48 uneven tori:", {f"{a}x{b}": sum(1 for s in specs if s[0] == (a, b)) for a, b in SIZES},
-          flush=True)
+    print("case 48 uneven tori, grouped" if grouped else "case 48 uneven tori, one branch per grid",
+          {f"{a}x{b}": sum(1 for s in specs if s[0] == (a, b)) for a, b in SIZES}, flush=True)
     d = load("c5_t2d_128")
     made = [make(s) for s in specs]
     batch = [g for g, _ in made]
@@ -61,7 +63,7 @@ def test_48_uneven_tori_graph_matches_single_runs_and_reference():
     assert dt0 == d["dt_seq"][0]
     assert state_digest(gp.get_prims()) == str(d["digest_init"])
     nsteps = 5
-    graph = harmgpu.Graph(batch, nsteps=nsteps)
+    graph = harmgpu.Graph(batch, nsteps=nsteps, grouped=grouped)
     x = torch.ones(1 << 20, device="cuda", dtype=torch.float64)
     for replay in range(2):
         graph.launch()
