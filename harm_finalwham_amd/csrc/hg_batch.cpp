@@ -129,7 +129,7 @@ namespace {
 
 // the views of the RK2 midpoint substeps of one context (enqueue_step:
 // pb = p, pk; pf = pk, p; pi = p)
-void make_views(hg_ctx *c, GridView v[2]) {
+void make_views(hg_ctx *c, const Tune &tu, GridView v[2]) {
   const hg_config &f = c->cfg;
   for (int sub = 0; sub < 2; sub++) {
     GridView &w = v[sub];
@@ -170,7 +170,57 @@ void make_views(hg_ctx *c, GridView v[2]) {
     }
     w.finalstep = sub;
     w.fluxmethod = f.fluxmethod;
-    weno_group_shapes(w, c->tune);
+    weno_group_shapes(w, tu);
+  }
+}
+
+// wavefronts of the grouped marching reconstruction along dir (2 or 3) of
+// one grid with the segments of a view (lanes packed over the primitives)
+int64_t march_waves(const GridView &v, int dir) {
+  const Dims &d = v.d;
+  if (((dir == 2) ? d.n2 : d.n3) <= 1) return 0;
+  const int s1 = d.n1 > 1, s2 = d.n2 > 1, s3 = d.n3 > 1;
+  const int64_t lanes = (int64_t)(d.n1 + 2 * s1) * ((dir == 2) ? d.n3 + 2 * s3 : d.n2 + 2 * s2);
+  return (NPR * lanes + 63) / 64 * v.nseg[dir];
+}
+
+// The segments along j and k of a grouped launch: the contexts' own where
+// their tune sets them, else one count for all, the fewest that give the
+// marching launch two wavefronts per wavefront slot of the device (four
+// per SIMD).  Batches of small grids have short lines and few lanes, and
+// their launches fill the device only with more segments than a grid that
+// runs alone would take.
+void grouped_tunes(hg_ctx *const *ctxs, int n, std::vector<Tune> &tunes) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
+  const int64_t want = 2LL * cus * 4 * 4;
+  tunes.resize(n);
+  for (int i = 0; i < n; i++) tunes[i] = ctxs[i]->tune;
+  for (int dir = 2; dir <= 3; dir++) {
+    int pick = 64;
+    for (int seg = 1; seg <= 64; seg++) {
+      int64_t waves = 0;
+      for (int i = 0; i < n; i++) {
+        Tune t = ctxs[i]->tune;
+        int &req = (dir == 2) ? t.weno_nseg : t.weno_nseg3;
+        if (req == 0) req = seg;
+        GridView v;
+        memset(&v, 0, sizeof v);
+        v.d = ctxs[i]->d;
+        weno_group_shapes(v, t);
+        waves += march_waves(v, dir);
+      }
+      if (waves >= want) {
+        pick = seg;
+        break;
+      }
+    }
+    for (int i = 0; i < n; i++) {
+      int &req = (dir == 2) ? tunes[i].weno_nseg : tunes[i].weno_nseg3;
+      if (req == 0) req = pick;
+    }
   }
 }
 
@@ -184,6 +234,8 @@ const char *grouped_unsupported(const hg_ctx *c, const hg_ctx *c0) {
   if (c->c2p_mode != c0->c2p_mode) return "an inversion mode unlike the first context's";
   if (c->dg_record) return "diagnostics recording on";
   if (c->ucons) return "a carried conserved state";
+  // the packed reconstruction lanes address the 8 planes with 32-bit offsets
+  if (c->d.spl >= ((int64_t)1 << 26)) return "a primitive plane of 2^26 zones or more";
   return nullptr;
 }
 
@@ -215,10 +267,12 @@ extern "C" int hg_graph_create_grouped(hg_ctx *const *ctxs, int n, int nsteps, h
     }
   }
   // host tables: 2n views (substep-major), then per class n + 1 block offsets
+  std::vector<Tune> tunes;
+  grouped_tunes(ctxs, n, tunes);
   std::vector<GridView> views(2 * (size_t)n);
   for (int i = 0; i < n; i++) {
     GridView v[2];
-    make_views(ctxs[i], v);
+    make_views(ctxs[i], tunes[i], v);
     views[i] = v[0];
     views[n + i] = v[1];
   }

@@ -496,7 +496,12 @@ __device__ __forceinline__ gchar *uniform_row(const void *base, int64_t off) {
 // Addressing: the primitive plane and the line position t are uniform over
 // the block, so every load and store is a uniform row address plus one
 // 32-bit per-lane byte offset (the context keeps a plane under 4 GiB).
-template <int DIR>
+// PACK (the grouped launches of small grids): the lanes of a block run over
+// (primitive, a, o) with a fastest, nlb blocks of 256 lanes over all
+// NPR * ca * co of them, so that the two halo lanes of a short row do not
+// take a wavefront of their own per primitive; the primitive plane then
+// joins the 32-bit per-lane offset (8 planes under 4 GiB).
+template <int DIR, bool PACK = false>
 __device__ __forceinline__ void weno_m_body(const double *__restrict__ p, const double *__restrict__ dpi,
                                             double *__restrict__ pleft, double *__restrict__ pright, const Dims &d,
                                             int nseg, int seglen, int wide, int nlb, unsigned bid) {
@@ -510,12 +515,18 @@ __device__ __forceinline__ void weno_m_body(const double *__restrict__ p, const 
   const int ca = ((DIR == 1) ? d.n2 : d.n1) + 2 * sa;
   const int co = ((DIR == 2) ? d.n3 : (DIR == 3) ? d.n2 : d.n3) + 2 * so;
   const unsigned slot = bid >> 3;
-  const int pl = (int)(slot & 7u);
-  const unsigned unit = (slot >> 3) * 8u + (bid & 7u);
+  const unsigned unit = PACK ? bid : (slot >> 3) * 8u + (bid & 7u);
   if (unit >= (unsigned)(nlb * nseg)) return;
   const int seg = (int)(unit / (unsigned)nlb);
-  const int64_t g = (int64_t)(unit % (unsigned)nlb) * blockDim.x + threadIdx.x;
-  if (g >= (int64_t)ca * co) return;
+  int64_t g = (int64_t)(unit % (unsigned)nlb) * blockDim.x + threadIdx.x;
+  int pl = (int)(slot & 7u);
+  if (PACK) {
+    if (g >= (int64_t)NPR * ca * co) return;
+    pl = (int)(g / ((int64_t)ca * co));
+    g -= (int64_t)pl * ca * co;
+  } else if (g >= (int64_t)ca * co) {
+    return;
+  }
   const int a_ = -sa + (int)(g % ca);
   const int o = -so + (int)(g / ca);
   const int e0 = -1 + seg * seglen;
@@ -525,14 +536,19 @@ __device__ __forceinline__ void weno_m_body(const double *__restrict__ p, const 
   // lowest point read along the line (y at e0-5); row offsets count from it
   // so that the uniform part of every address is non-negative
   const int lo = e0 - 5;
-  const uint32_t zb = (uint32_t)((zl + (int64_t)lo * sd) * (int64_t)sizeof(double));
+  // per-lane byte offsets: zd into the indicator, zb into the primitive's
+  // plane (PACK: from the first plane)
+  const uint32_t zd = (uint32_t)((zl + (int64_t)lo * sd) * (int64_t)sizeof(double));
+  const uint32_t zb = PACK ? zd + (uint32_t)((int64_t)pl * d.spl * (int64_t)sizeof(double)) : zd;
   const int64_t rowb = sd * (int64_t)sizeof(double);
-  const char *yb = (const char *)(p + (int64_t)pl * d.spl);
+  const int64_t pu = PACK ? 0 : (int64_t)pl * d.spl;  // the plane when it is uniform
+  const char *yb = (const char *)(p + pu);
   const char *db = (const char *)dpi;
-  char *lb = (char *)(pleft + (int64_t)pl * d.spl), *rb = (char *)(pright + (int64_t)pl * d.spl);
-#define WM_AT(base, t) (uniform_row((base), (int64_t)((t) - lo) * rowb) + zb)
-#define WM_LD(base, t) (*(const __attribute__((address_space(1))) double *)WM_AT(base, t))
-#define WM_ST(base, t) (*(__attribute__((address_space(1))) double *)WM_AT(base, t))
+  char *lb = (char *)(pleft + pu), *rb = (char *)(pright + pu);
+#define WM_AT(base, t, off) (uniform_row((base), (int64_t)((t) - lo) * rowb) + (off))
+#define WM_LD(base, t) (*(const __attribute__((address_space(1))) double *)WM_AT(base, t, zb))
+#define WM_LDD(base, t) (*(const __attribute__((address_space(1))) double *)WM_AT(base, t, zd))
+#define WM_ST(base, t) (*(__attribute__((address_space(1))) double *)WM_AT(base, t, zb))
   // windows: yw = y(t-5..t+2), W0 = w0(t-4..t), W1/W2/M0 = (t-3..t),
   // LD = stage-D fraction (t-4..t-2); the cusp fractions of t-2..t, each
   // 0 or 1, are bits 2..0 of lofb
@@ -560,7 +576,7 @@ __device__ __forceinline__ void weno_m_body(const double *__restrict__ p, const 
   // would have to wait for them too.
   const int tlast = e1 + 3;
   double ny = WM_LD(yb, e0 - 1);
-  double ndp = WM_LD(db, e0);
+  double ndp = WM_LDD(db, e0);
   WM_ST(lb, e0) = 0.;
   WM_ST(rb, e0) = 0.;
   for (int t = e0 - 3; t <= tlast; t++) {
@@ -568,7 +584,7 @@ __device__ __forceinline__ void weno_m_body(const double *__restrict__ p, const 
     {
       const int tn = (t + 1 < tlast) ? t + 1 : tlast;
       ny = WM_LD(yb, tn + 2);
-      ndp = WM_LD(db, (tn - 3 > e0) ? tn - 3 : e0);
+      ndp = WM_LDD(db, (tn - 3 > e0) ? tn - 3 : e0);
     }
 #pragma unroll
     for (int a = 0; a < 7; a++) yw[a] = yw[a + 1];
@@ -597,6 +613,7 @@ __device__ __forceinline__ void weno_m_body(const double *__restrict__ p, const 
   }
 #undef WM_AT
 #undef WM_LD
+#undef WM_LDD
 #undef WM_ST
 }
 
@@ -636,8 +653,8 @@ template <int DIR>
 __global__ void __launch_bounds__(256) kg_weno_m(const GridView *__restrict__ views,
                                                  const unsigned *__restrict__ first, int n) {
   WG_PRELUDE
-  weno_m_body<DIR>(v.pb, v.aux + (1 + DIR) * v.d.spl, v.pl[DIR - 1], v.pr[DIR - 1], v.d, v.nseg[DIR],
-                   v.seglen[DIR], 0, v.nlb[DIR], bid);
+  weno_m_body<DIR, true>(v.pb, v.aux + (1 + DIR) * v.d.spl, v.pl[DIR - 1], v.pr[DIR - 1], v.d, v.nseg[DIR],
+                         v.seglen[DIR], 0, v.nlb[DIR], bid);
 }
 #undef WG_PRELUDE
 
@@ -708,7 +725,13 @@ void weno_group_shapes(GridView &v, const Tune &tu) {
   v.nseg[1] = (v.d.n1 + 2 + XOUT - 1) / XOUT;  // k_weno_x's segments
   v.seglen[1] = v.nlb[1] = 0;
   v.nseg[0] = v.seglen[0] = v.nlb[0] = 0;
-  for (int dir = 2; dir <= 3; dir++) weno_march_shape(dir, v.d, tu, 0, v.nseg[dir], v.seglen[dir], v.nlb[dir]);
+  for (int dir = 2; dir <= 3; dir++) {
+    weno_march_shape(dir, v.d, tu, 0, v.nseg[dir], v.seglen[dir], v.nlb[dir]);
+    // packed lanes (weno_m_body PACK): blocks of 256 over all primitives
+    const Dims &d = v.d;
+    const int64_t lanes = (int64_t)(d.n1 + 2 * (d.n1 > 1)) * ((dir == 2) ? d.n3 + 2 * (d.n3 > 1) : d.n2 + 2 * (d.n2 > 1));
+    v.nlb[dir] = (int)((NPR * lanes + 255) / 256);
+  }
 }
 
 // grouped: the blocks of the launchers above with the default shapes (four
@@ -728,10 +751,10 @@ unsigned group_blocks_weno(int k, const GridView &v) {
       return (unsigned)((nw + 3) / 4);
     }
     case GK_WENO2: case GK_WENO3: {
+      // packed lanes: one block per (lane block, segment) unit
       const int dir = (k == GK_WENO2) ? 2 : 3;
       if (((dir == 2) ? d.n2 : d.n3) <= 1) return 0u;
-      const int64_t units = ((int64_t)v.nlb[dir] * v.nseg[dir] + 7) / 8 * 8;
-      return (unsigned)(units * NPR);
+      return (unsigned)v.nlb[dir] * (unsigned)v.nseg[dir];
     }
     default: return 0u;
   }

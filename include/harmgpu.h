@@ -420,7 +420,9 @@ int hg_graph_create(hg_ctx *const *ctxs, int n, int nsteps, hg_graph **out);
    (timeorder 2), every context with the same inversion mode, HG_C2P_FP64
    or HG_C2P_MIXED, and no diagnostics recording; other contexts are
    rejected with an error (hg_graph_create takes them).  Launch shapes are
-   the defaults, with each context's segment counts along j and k. */
+   the defaults, with each context's segment counts along j and k where its
+   tune sets them; otherwise the library picks one count for all, enough
+   wavefronts to fill the device (twice its wavefront slots). */
 int hg_graph_create_grouped(hg_ctx *const *ctxs, int n, int nsteps, hg_graph **out);
 int hg_graph_launch(hg_graph *g, void *stream);
 int hg_graph_destroy(hg_graph *g);
