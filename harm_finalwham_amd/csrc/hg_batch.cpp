@@ -151,6 +151,8 @@ void make_views(hg_ctx *c, const Tune &tu, GridView v[2]) {
     }
     w.aux = c->aux;
     w.uf = c->uf;
+    w.lx = c->lx;
+    w.ly = c->ly;
     w.dt = c->dt;
     w.ndt_bits = c->ndt_bits;
     w.counters = c->counters;
@@ -230,8 +232,11 @@ const char *grouped_unsupported(const hg_ctx *c, const hg_ctx *c0) {
   if (f.doenoflux == HG_ENOFINITEVOLUME) return "the finite-volume scheme";
   if (f.timeorder != 2) return "a timeorder other than 2";
   if (f.fluxmethod != HG_FLUX_LAXF && f.fluxmethod != HG_FLUX_HLL) return "a flux other than LAXF or HLL";
-  if (c->c2p_mode != HG_C2P_FP64 && c->c2p_mode != HG_C2P_MIXED) return "an inversion mode other than FP64 or MIXED";
+  if (c->c2p_mode != HG_C2P_FP64 && c->c2p_mode != HG_C2P_MIXED && c->c2p_mode != HG_C2P_LEARNED)
+    return "an inversion mode other than FP64, MIXED or LEARNED";
   if (c->c2p_mode != c0->c2p_mode) return "an inversion mode unlike the first context's";
+  if (c->c2p_mode == HG_C2P_LEARNED && (!c->mlp || c->mlp != c0->mlp || !c->lx || !c->ly))
+    return "a learned start with a network unlike the first context's";
   if (c->dg_record) return "diagnostics recording on";
   if (c->ucons) return "a carried conserved state";
   // the packed reconstruction lanes address the 8 planes with 32-bit offsets
@@ -282,7 +287,7 @@ extern "C" int hg_graph_create_grouped(hg_ctx *const *ctxs, int n, int nsteps, h
     uint64_t acc = 0;
     for (int i = 0; i < n; i++) {
       first[(size_t)k * (n + 1) + i] = (unsigned)acc;
-      acc += group_blocks(k, views[i]);
+      acc += (k == GK_MLP) ? group_blocks_mlp(views[i]) : group_blocks(k, views[i]);
     }
     if (acc > 0x7fffffffull) {
       set_error("hg_graph_create_grouped: %llu blocks in one launch exceed the grid limit",
@@ -302,22 +307,33 @@ extern "C" int hg_graph_create_grouped(hg_ctx *const *ctxs, int n, int nsteps, h
                             hipMemcpyHostToDevice));
   const GridView *dv = (const GridView *)g->tables;
   const unsigned *dfirst = (const unsigned *)((char *)g->tables + vpad);
-  const int mode = ctxs[0]->c2p_mode == HG_C2P_MIXED ? 1 : 0;
+  const int cm = ctxs[0]->c2p_mode;
+  const int mode = cm == HG_C2P_MIXED ? 1 : cm == HG_C2P_LEARNED ? 2 : 0;
   hipStream_t cap = nullptr;
   if (!rc) HIPTRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
   if (!rc) HIPTRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
   if (!rc) {
-    auto L = [&](int k, int sub) { launch_group(k, dv + (size_t)sub * n, dfirst + (size_t)k * (n + 1), n, total[k], mode, cap); };
+    auto L = [&](int k, int sub) {
+      const GridView *vs = dv + (size_t)sub * n;
+      const unsigned *fs = dfirst + (size_t)k * (n + 1);
+      if (k == GK_MLP) launch_group_mlp(ctxs[0]->mlp, vs, fs, n, total[k], cap);
+      else launch_group(k, vs, fs, n, total[k], mode, cap);
+    };
     // one substep of step_ch.c:94-166 on every grid: fluxcalc (directions in
-    // turn, every direction's dP/P indicator in one launch), update,
-    // inversion, dt on the final substep, bound_prim, post_fixup, bound_prim
+    // turn, every direction's dP/P indicator in one launch), update, the
+    // learned start's features and network, inversion, dt on the final
+    // substep, bound_prim, post_fixup, bound_prim
     const int order[] = {GK_PREP, GK_DPI, GK_WENO1, GK_FLUX1, GK_WENO2, GK_FLUX2, GK_WENO3, GK_FLUX3,
-                         GK_EMF, GK_UPDATE, GK_INVERT, GK_DT, GK_BOUND1, GK_BOUND2, GK_BOUND3,
-                         GK_FXCHECK, GK_FXRESOLVE, GK_FXAVG, GK_FXAPPLY, GK_BOUND1, GK_BOUND2, GK_BOUND3};
+                         GK_EMF, GK_UPDATE, GK_FEATURES, GK_MLP, GK_INVERT, GK_DT, GK_BOUND1, GK_BOUND2,
+                         GK_BOUND3, GK_FXCHECK, GK_FXRESOLVE, GK_FXAVG, GK_FXAPPLY, GK_BOUND1, GK_BOUND2,
+                         GK_BOUND3};
     for (int step = 0; step < nsteps; step++)
       for (int sub = 0; sub < 2; sub++)
-        for (int k : order)
-          if (k != GK_DT || sub == 1) L(k, sub);
+        for (int k : order) {
+          if (k == GK_DT && sub == 0) continue;
+          if ((k == GK_FEATURES || k == GK_MLP) && mode != 2) continue;
+          L(k, sub);
+        }
     HIPTRY(hipGetLastError());
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(cap, &graph);

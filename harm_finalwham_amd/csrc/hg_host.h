@@ -98,6 +98,11 @@ void launch_diag_stage(hg_ctx *c, int stage, hipStream_t s);
 // inversion kernel)
 void launch_post_fixup(hg_ctx *c, double *p, int finalstep, hipStream_t s, bool reset = true);
 FxBufs fixup_bufs(hg_ctx *c);
+// grouped launches of the network's forward pass (hg_mlp.hip): every grid's
+// features lx -> predictions ly through one network
+unsigned group_blocks_mlp(const GridView &v);
+void launch_group_mlp(const hg_mlp *m, const GridView *views, const unsigned *first, int n, unsigned total,
+                      hipStream_t s);
 // batched inversion (hg_c2p_batch.hip).  The mixed path's scratch holds
 // C2P_NCTR counters (work queue, list length, list work queue) and a list of
 // n 32-bit zone indices: c2p_scratch_bytes(n) bytes.

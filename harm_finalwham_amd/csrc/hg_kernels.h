@@ -116,6 +116,7 @@ struct GridView {
   double *pf;             // its result, bounded and fixed up in place
   const double *geo, *corn, *conn, *floors;
   double *F[3], *emf[3], *pl[3], *pr[3], *aux, *uf, *dt;
+  float *lx, *ly;         // the learned start's features and predictions
   unsigned long long *ndt_bits, *counters;
   int8_t *pflag;
   FxBufs fx;
@@ -139,10 +140,13 @@ __device__ __forceinline__ int group_of(const unsigned *__restrict__ first, int 
 
 enum GroupClass {
   GK_PREP, GK_DPI, GK_WENO1, GK_WENO2, GK_WENO3, GK_FLUX1, GK_FLUX2, GK_FLUX3, GK_EMF, GK_UPDATE, GK_INVERT,
-  GK_DT, GK_BOUND1, GK_BOUND2, GK_BOUND3, GK_FXCHECK, GK_FXRESOLVE, GK_FXAVG, GK_FXAPPLY, GK_NCLASS
+  GK_DT, GK_BOUND1, GK_BOUND2, GK_BOUND3, GK_FXCHECK, GK_FXRESOLVE, GK_FXAVG, GK_FXAPPLY, GK_FEATURES, GK_MLP,
+  GK_NCLASS
 };
 // blocks of class k on one grid (0: no work there), and one grouped launch of
-// class k over n grids (total = first[n] blocks; mode: the inversion's)
+// class k over n grids (total = first[n] blocks; mode: the inversion's, 0
+// FP64, 1 mixed, 2 mixed from the learned start; GK_MLP goes through
+// launch_group_mlp, which needs the network)
 unsigned group_blocks(int k, const GridView &v);
 void launch_group(int k, const GridView *views, const unsigned *first, int n, unsigned total, int mode,
                   hipStream_t s);

@@ -1059,7 +1059,7 @@ __global__ void __launch_bounds__(256) kg_invert(const GridView *__restrict__ vi
                                                  const unsigned *__restrict__ first, int n) {
   HG_GROUP_PRELUDE
   invert_body<false, MODE>(v.uf, v.pb, v.pf, v.geo, v.floors, v.finalstep, v.d, v.gam, v.gammamax, v.counters,
-                           v.pflag, nullptr, nullptr, v.fx.count, v.err, bid);
+                           v.pflag, nullptr, MODE == INV_LEARNED ? v.ly : nullptr, v.fx.count, v.err, bid);
 }
 
 __global__ void kg_dt(const GridView *__restrict__ views, const unsigned *__restrict__ first, int n) {
@@ -1326,11 +1326,10 @@ void launch_invert(const double *ubuf, const double *pb, double *pf, const doubl
 
 // the learned start's features of the inversions k_invert will do (its zone
 // order; the guess pb with the field of ubuf, as the set-up sees it)
-__global__ void __launch_bounds__(256) k_step_features(const double *__restrict__ ubuf,
-                                                       const double *__restrict__ pb,
-                                                       const double *__restrict__ geo, Dims d, double gam,
-                                                       float *__restrict__ X) {
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void step_features_body(const double *__restrict__ ubuf, const double *__restrict__ pb,
+                                                   const double *__restrict__ geo, const Dims &d, double gam,
+                                                   float *__restrict__ X, unsigned bid) {
+  int64_t t = bid * (int64_t)blockDim.x + threadIdx.x;
   if (t >= (int64_t)d.n1 * d.n2 * d.n3) return;
   int i = (int)(t % d.n1);
   int64_t r = t / d.n1;
@@ -1357,6 +1356,20 @@ __global__ void __launch_bounds__(256) k_step_features(const double *__restrict_
   }
 #pragma unroll
   for (int q = 0; q < C2P_NFEAT; q++) X[t * C2P_NFEAT + q] = x[q];
+}
+
+__global__ void __launch_bounds__(256) k_step_features(const double *__restrict__ ubuf,
+                                                       const double *__restrict__ pb,
+                                                       const double *__restrict__ geo, Dims d, double gam,
+                                                       float *__restrict__ X) {
+  step_features_body(ubuf, pb, geo, d, gam, X, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) kg_step_features(const GridView *__restrict__ views,
+                                                        const unsigned *__restrict__ first, int n) {
+  const int g_ = group_of(first, n, blockIdx.x);
+  const GridView &v = views[g_];
+  step_features_body(v.uf, v.pb, v.geo, v.d, v.gam, v.lx, blockIdx.x - first[g_]);
 }
 
 void launch_step_features(const double *ubuf, const double *pb, const double *geo, const Dims &d, double gam,
@@ -1390,7 +1403,7 @@ unsigned group_blocks(int k, const GridView &v) {
       return nblocks((int64_t)fr.ni * fr.nj * fr.nk, 256);
     }
     case GK_EMF: return nblocks((int64_t)(d.n1 + s1) * (d.n2 + s2) * (d.n3 + s3), 256);
-    case GK_UPDATE: case GK_INVERT: return nblocks(act, 256);
+    case GK_UPDATE: case GK_INVERT: case GK_FEATURES: return nblocks(act, 256);
     case GK_DT: return 1u;
     case GK_FXCHECK: case GK_FXRESOLVE: case GK_FXAVG: case GK_FXAPPLY: return group_blocks_fixup(k, v);
     default: return group_blocks_weno(k, v);
@@ -1412,8 +1425,10 @@ void launch_group(int k, const GridView *views, const unsigned *first, int n, un
     case GK_UPDATE: hipLaunchKernelGGL(kg_update, g, dim3(256), 0, s, views, first, n); return;
     case GK_INVERT:
       if (mode == INV_MIXED) hipLaunchKernelGGL(kg_invert<INV_MIXED>, g, dim3(256), 0, s, views, first, n);
+      else if (mode == INV_LEARNED) hipLaunchKernelGGL(kg_invert<INV_LEARNED>, g, dim3(256), 0, s, views, first, n);
       else hipLaunchKernelGGL(kg_invert<INV_FP64>, g, dim3(256), 0, s, views, first, n);
       return;
+    case GK_FEATURES: hipLaunchKernelGGL(kg_step_features, g, dim3(256), 0, s, views, first, n); return;
     case GK_DT: hipLaunchKernelGGL(kg_dt, g, dim3(64), 0, s, views, first, n); return;
     case GK_FXCHECK: case GK_FXRESOLVE: case GK_FXAVG: case GK_FXAPPLY:
       launch_group_fixup(k, views, first, n, total, s);

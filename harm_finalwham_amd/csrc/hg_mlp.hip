@@ -201,8 +201,8 @@ __device__ __forceinline__ void forward16(const MlpRegs &R, const float (*xin)[N
 
 // one wavefront per block holds the network in registers and runs over
 // tiles of 16 samples (grid-stride)
-__global__ void __launch_bounds__(64) k_mlp_forward(int64_t n, const float *__restrict__ X, float *__restrict__ y,
-                                                    MlpDev m) {
+__device__ __forceinline__ void mlp_forward_body(int64_t n, const float *__restrict__ X, float *__restrict__ y,
+                                                 const MlpDev &m, unsigned bid, unsigned nblk) {
   __shared__ float xin[16][NIN];
   __shared__ float h[16][NH];
   __shared__ float yo[16];
@@ -212,7 +212,7 @@ __global__ void __launch_bounds__(64) k_mlp_forward(int64_t n, const float *__re
   // lane 8 s + k (s < 8) normalises feature k of sample s and of s + 8
   const int k = l & 7;
   const float mk = m.w[OFF_MU + k], sk = m.w[OFF_SD + k];
-  for (int64_t base = (int64_t)blockIdx.x * 16; base < n; base += (int64_t)gridDim.x * 16) {
+  for (int64_t base = (int64_t)bid * 16; base < n; base += (int64_t)nblk * 16) {
 #pragma unroll
     for (int half = 0; half < 2; half++) {
       const int s = half * 8 + (l >> 3);
@@ -224,6 +224,20 @@ __global__ void __launch_bounds__(64) k_mlp_forward(int64_t n, const float *__re
     if (l < 16 && base + l < n) y[base + l] = yo[l];
     __syncthreads();
   }
+}
+
+__global__ void __launch_bounds__(64) k_mlp_forward(int64_t n, const float *__restrict__ X, float *__restrict__ y,
+                                                    MlpDev m) {
+  mlp_forward_body(n, X, y, m, blockIdx.x, gridDim.x);
+}
+
+// grouped: every grid's step features through the one network
+__global__ void __launch_bounds__(64) kg_mlp_forward(const GridView *__restrict__ views,
+                                                     const unsigned *__restrict__ first, int ng, MlpDev m) {
+  const int g_ = group_of(first, ng, blockIdx.x);
+  const GridView &v = views[g_];
+  const int64_t n = (int64_t)v.d.n1 * v.d.n2 * v.d.n3;
+  mlp_forward_body(n, v.lx, v.ly, m, blockIdx.x - first[g_], first[g_ + 1] - first[g_]);
 }
 
 // ---------------------------------------------------------------------------
@@ -560,6 +574,16 @@ extern "C" int hg_mlp_features(int64_t n, double gam, const double *U, const dou
   MLPCHECK(hipGetLastError());
   MLPCHECK(hipStreamSynchronize(s));
   return 0;
+}
+
+unsigned hg::group_blocks_mlp(const GridView &v) {
+  return forward_blocks((int64_t)v.d.n1 * v.d.n2 * v.d.n3);
+}
+
+void hg::launch_group_mlp(const hg_mlp *m, const GridView *views, const unsigned *first, int n, unsigned total,
+                          hipStream_t s) {
+  if (total == 0) return;
+  hipLaunchKernelGGL(kg_mlp_forward, dim3(total), dim3(64), 0, s, views, first, n, dev_view(m));
 }
 
 void hg::launch_mlp_forward(const hg_mlp *m, int64_t n, const float *X, float *y, hipStream_t s) {

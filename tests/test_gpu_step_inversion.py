@@ -87,12 +87,15 @@ def test_step_inversion_mode_against_reference_inversion(mode_name):
     alt.close()
 
 
-def test_graph_with_learned_inversion_matches_single_runs():
-    """Uneven tori with the learned inversion in one graph, two replays
-    with eager work in between: every grid bit for bit as alone."""
+@pytest.mark.parametrize("grouped", [False, True])
+def test_graph_with_learned_inversion_matches_single_runs(grouped):
+    """Uneven tori with the learned inversion in one graph (one branch per
+    grid, or grouped launches over all of them), two replays with eager work
+    in between: every grid bit for bit as alone."""
     import torch
     from harm_finalwham_amd import harmgpu
-    print("case graph of 5 tori, learned inversion", flush=True)
+    print("case graph of 5 tori, learned inversion, grouped" if grouped else
+          "case graph of 5 tori, learned inversion", flush=True)
     m = trained_network()
     specs = [((64, 64), 0.2, 10.5, 1), ((96, 128), 0.6, 12.5, 2), ((128, 128), 0.9375, 12.0, 3),
              ((64, 64), 0.95, 13.5, 4), ((96, 128), 0.0, 11.0, 5)]
@@ -106,7 +109,7 @@ def test_graph_with_learned_inversion_matches_single_runs():
         made.append(pair)
     batch = [p[0] for p in made]
     alone = [p[1] for p in made]
-    graph = harmgpu.Graph(batch, nsteps=3)
+    graph = harmgpu.Graph(batch, nsteps=3, grouped=grouped)
     x = torch.ones(1 << 20, device="cuda", dtype=torch.float64)
     for replay in range(2):
         graph.launch()
