@@ -418,32 +418,46 @@ __global__ void __launch_bounds__(256) k_weno_dpi(const double *__restrict__ p, 
 }
 
 // ---------------------------------------------------------------------------
-// along i: one lane per point; wavefront w covers points s0-3 .. s0+60 and
-// writes the interface pair of s0 .. s0+57
+// along i: one lane per point.  The rows (primitive fastest, then j, then
+// k) are laid end to end, each as the n1 + 8 lanes of its points -4 .. n1+3
+// (the outputs -1 .. n1 and the three points either side that stage C and D
+// need); wavefront w takes lanes 58 w .. 58 w + 63 of that sequence and
+// writes the outputs on its lanes 3 .. 60, whose neighbours within three
+// lanes then belong to the same row and wavefront.  A short row therefore
+// shares its wavefront with the next one instead of leaving lanes idle.
 // ---------------------------------------------------------------------------
 constexpr int XOUT = 58;
 
+// wavefronts of the i-direction reconstruction (transverse halo widths s2, s3)
+__host__ __device__ inline int64_t weno_x_waves(const Dims &d, int s2, int s3) {
+  const int64_t rows = (int64_t)NPR * (d.n2 + 2 * s2) * (d.n3 + 2 * s3);
+  return (rows * (d.n1 + 8) + XOUT - 1) / XOUT;
+}
+
 __device__ __forceinline__ void weno_x_body(const double *__restrict__ p, const double *__restrict__ dpi,
                                             double *__restrict__ pleft, double *__restrict__ pright, const Dims &d,
-                                            int nseg, int wide, unsigned bid) {
+                                            int wide, unsigned bid) {
   const int lane = threadIdx.x & 63;
   const int64_t wid = (int64_t)bid * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int n = d.n1;
   // transverse rows: -1..n, or every ghost row for the finite-volume loops
   const int s2 = d.n2 > 1 ? (wide ? d.nb2 : 1) : 0, s3 = d.n3 > 1 ? (wide ? d.nb3 : 1) : 0;
   const int c1 = d.n2 + 2 * s2, c2 = d.n3 + 2 * s3;
-  if (wid >= (int64_t)nseg * NPR * c1 * c2) return;  // uniform per wavefront
-  const int seg = (int)(wid % nseg);
-  int64_t r = wid / nseg;
+  const int lr = n + 8;
+  const int64_t nrows = (int64_t)NPR * c1 * c2;
+  if (wid * XOUT >= nrows * lr) return;  // uniform per wavefront
+  const int64_t v = wid * XOUT + lane;
+  int64_t r = v / lr;
+  const int s = (int)(v - r * lr) - 4;
+  const bool live = r < nrows;
+  if (!live) r = nrows - 1;  // lanes past the last row: in-range indices, no work
   const int pl = (int)(r % NPR);
   r /= NPR;
   const int j = -s2 + (int)(r % c1), k = -s3 + (int)(r / c1);
-  const int s0 = -1 + seg * XOUT;
-  const int s = s0 - 3 + lane;
   const int64_t zl = d.idx(0, j, k);
   const double *yl = p + (int64_t)pl * d.spl + zl;
   double y[5] = {0., 0., 0., 0., 0.}, w[3] = {1., 1., 1.}, m0 = 0., lof0 = 0.;
-  if (s <= n + 3) {
+  if (live) {
 #pragma unroll
     for (int a = 0; a < 5; a++) y[a] = yl[s - 2 + a];
     stage_c<true>(y + 2, w, m0, lof0);
@@ -454,7 +468,7 @@ __device__ __forceinline__ void weno_x_body(const double *__restrict__ p, const 
   double lofd = stage_d(w[2], w0m2, w[0], w2p2, lof0);
   double lofm = __shfl_up(lofd, 1, 64);
   double lofp = __shfl_down(lofd, 1, 64);
-  if (lane >= 3 && lane < 3 + XOUT && s <= n) {
+  if (live && lane >= 3 && lane < 3 + XOUT && s >= -1 && s <= n) {
     const double dPi = dpi[zl + s];
     double ol, orr;
     stage_e<true>(y + 2, m0, w[0], w[1], w[2], lofm, lofd, lofp, dPi, ol, orr);
@@ -466,8 +480,8 @@ __device__ __forceinline__ void weno_x_body(const double *__restrict__ p, const 
 
 __global__ void __launch_bounds__(256) k_weno_x(const double *__restrict__ p, const double *__restrict__ dpi,
                                                 double *__restrict__ pleft, double *__restrict__ pright, Dims d,
-                                                int nseg, int wide) {
-  weno_x_body(p, dpi, pleft, pright, d, nseg, wide, blockIdx.x);
+                                                int wide) {
+  weno_x_body(p, dpi, pleft, pright, d, wide, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -646,7 +660,7 @@ __global__ void __launch_bounds__(256) kg_weno_dpi(const GridView *__restrict__ 
 __global__ void __launch_bounds__(256) kg_weno_x(const GridView *__restrict__ views,
                                                  const unsigned *__restrict__ first, int n) {
   WG_PRELUDE
-  weno_x_body(v.pb, v.aux + 2 * v.d.spl, v.pl[0], v.pr[0], v.d, v.nseg[1], 0, bid);
+  weno_x_body(v.pb, v.aux + 2 * v.d.spl, v.pl[0], v.pr[0], v.d, 0, bid);
 }
 
 template <int DIR>
@@ -700,11 +714,10 @@ void launch_weno(int dir, const double *p, double *aux, double *pleft, double *p
   }
   const double *dpi = aux + (int64_t)(1 + dir) * d.spl;
   if (dir == 1 && tu.weno_xmarch == 0) {
-    int nseg = (d.n1 + 2 + XOUT - 1) / XOUT;
-    int64_t nw = (int64_t)nseg * NPR * (d.n2 + 2 * s2) * (d.n3 + 2 * s3);
+    const int64_t nw = weno_x_waves(d, s2, s3);
     int wpb = tu.weno_xwaves;
     hipLaunchKernelGGL(k_weno_x, dim3((unsigned)((nw + wpb - 1) / wpb)), dim3(64 * wpb), 0, s, p, dpi, pleft,
-                       pright, d, nseg, wide);
+                       pright, d, wide);
   } else {
     int nseg, seglen, nlb;
     weno_march_shape(dir, d, tu, wide, nseg, seglen, nlb);
@@ -722,8 +735,7 @@ void launch_weno(int dir, const double *p, double *aux, double *pleft, double *p
 }
 
 void weno_group_shapes(GridView &v, const Tune &tu) {
-  v.nseg[1] = (v.d.n1 + 2 + XOUT - 1) / XOUT;  // k_weno_x's segments
-  v.seglen[1] = v.nlb[1] = 0;
+  v.nseg[1] = v.seglen[1] = v.nlb[1] = 0;
   v.nseg[0] = v.seglen[0] = v.nlb[0] = 0;
   for (int dir = 2; dir <= 3; dir++) {
     weno_march_shape(dir, v.d, tu, 0, v.nseg[dir], v.seglen[dir], v.nlb[dir]);
@@ -746,8 +758,7 @@ unsigned group_blocks_weno(int k, const GridView &v) {
     case GK_DPI: return (unsigned)((nbox + 255) / 256 + 7) / 8 * 8;
     case GK_WENO1: {
       if (d.n1 <= 1) return 0u;
-      const int s2 = d.n2 > 1 ? 1 : 0, s3 = d.n3 > 1 ? 1 : 0;
-      const int64_t nw = (int64_t)v.nseg[1] * NPR * (d.n2 + 2 * s2) * (d.n3 + 2 * s3);
+      const int64_t nw = weno_x_waves(d, d.n2 > 1 ? 1 : 0, d.n3 > 1 ? 1 : 0);
       return (unsigned)((nw + 3) / 4);
     }
     case GK_WENO2: case GK_WENO3: {
