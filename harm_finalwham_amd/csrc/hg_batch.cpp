@@ -127,17 +127,20 @@ extern "C" int hg_graph_create(hg_ctx *const *ctxs, int n, int nsteps, hg_graph 
 // ---------------------------------------------------------------------------
 namespace {
 
-// the views of the RK2 midpoint substeps of one context (enqueue_step:
-// pb = p, pk; pf = pk, p; pi = p)
-void make_views(hg_ctx *c, const Tune &tu, GridView v[2]) {
+// the views of the substeps of one step of a context (enqueue_step's pb,
+// pf and coefficients; pi = p); returns their number
+int make_views(hg_ctx *c, const Tune &tu, GridView v[4]) {
   const hg_config &f = c->cfg;
-  for (int sub = 0; sub < 2; sub++) {
+  double *pbs[4], *pfs[4];
+  RkStage rks[4];
+  const int ns = step_substeps(c, pbs, pfs, rks);
+  for (int sub = 0; sub < ns; sub++) {
     GridView &w = v[sub];
     memset(&w, 0, sizeof w);
     w.d = c->d;
     w.pi = c->p;
-    w.pb = sub ? c->pk : c->p;
-    w.pf = sub ? c->p : c->pk;
+    w.pb = pbs[sub];
+    w.pf = pfs[sub];
     w.geo = c->geo;
     w.corn = c->corn;
     w.conn = c->conn;
@@ -159,8 +162,7 @@ void make_views(hg_ctx *c, const Tune &tu, GridView v[2]) {
     w.pflag = c->pflag;
     w.fx = fixup_bufs(c);
     w.err = c->err;
-    // enqueue_step's midpoint coefficients without ucum
-    w.rk = sub ? RkStage{1.0, 0.0, 0.0, 1.0, nullptr, 0, 1, nullptr} : RkStage{0.5, 0.0, 0.0, 0.0, nullptr, 1, 0, nullptr};
+    w.rk = rks[sub];
     w.gam = f.gam;
     w.gammamax = f.gammamax;
     w.cour = f.cour;
@@ -170,10 +172,11 @@ void make_views(hg_ctx *c, const Tune &tu, GridView v[2]) {
       w.dx[k] = f.dx[k];
       w.idx[k] = 1.0 / f.dx[k];
     }
-    w.finalstep = sub;
+    w.finalstep = sub == ns - 1;
     w.fluxmethod = f.fluxmethod;
     weno_group_shapes(w, tu);
   }
+  return ns;
 }
 
 // wavefronts of the grouped marching reconstruction along dir (2 or 3) of
@@ -230,7 +233,7 @@ const char *grouped_unsupported(const hg_ctx *c, const hg_ctx *c0) {
   const hg_config &f = c->cfg;
   if (f.lim != HG_LIM_WENO5BND) return "reconstruction other than WENO5BND";
   if (f.doenoflux == HG_ENOFINITEVOLUME) return "the finite-volume scheme";
-  if (f.timeorder != 2) return "a timeorder other than 2";
+  if (f.timeorder != c0->cfg.timeorder) return "a timeorder unlike the first context's";
   if (f.fluxmethod != HG_FLUX_LAXF && f.fluxmethod != HG_FLUX_HLL) return "a flux other than LAXF or HLL";
   if (c->c2p_mode != HG_C2P_FP64 && c->c2p_mode != HG_C2P_MIXED && c->c2p_mode != HG_C2P_LEARNED)
     return "an inversion mode other than FP64, MIXED or LEARNED";
@@ -271,16 +274,22 @@ extern "C" int hg_graph_create_grouped(hg_ctx *const *ctxs, int n, int nsteps, h
       return 1;
     }
   }
-  // host tables: 2n views (substep-major), then per class n + 1 block offsets
+  // host tables: ns x n views (substep-major), then per class n + 1 block
+  // offsets
   std::vector<Tune> tunes;
   grouped_tunes(ctxs, n, tunes);
-  std::vector<GridView> views(2 * (size_t)n);
+  std::vector<GridView> views;
+  int ns = 0;
   for (int i = 0; i < n; i++) {
-    GridView v[2];
-    make_views(ctxs[i], tunes[i], v);
-    views[i] = v[0];
-    views[n + i] = v[1];
+    GridView v[4];
+    const int k = make_views(ctxs[i], tunes[i], v);
+    if (i == 0) {
+      ns = k;
+      views.resize((size_t)ns * n);
+    }
+    for (int sub = 0; sub < ns; sub++) views[(size_t)sub * n + i] = v[sub];
   }
+  const bool ucum = views[0].rk.ucum != nullptr;
   std::vector<unsigned> first((size_t)GK_NCLASS * (n + 1));
   unsigned total[GK_NCLASS];
   for (int k = 0; k < GK_NCLASS; k++) {
@@ -324,13 +333,13 @@ extern "C" int hg_graph_create_grouped(hg_ctx *const *ctxs, int n, int nsteps, h
     // learned start's features and network, inversion, dt on the final
     // substep, bound_prim, post_fixup, bound_prim
     const int order[] = {GK_PREP, GK_DPI, GK_WENO1, GK_FLUX1, GK_WENO2, GK_FLUX2, GK_WENO3, GK_FLUX3,
-                         GK_EMF, GK_UPDATE, GK_FEATURES, GK_MLP, GK_INVERT, GK_DT, GK_BOUND1, GK_BOUND2,
-                         GK_BOUND3, GK_FXCHECK, GK_FXRESOLVE, GK_FXAVG, GK_FXAPPLY, GK_BOUND1, GK_BOUND2,
-                         GK_BOUND3};
+                         GK_EMF, ucum ? GK_UPDATEU : GK_UPDATE, GK_FEATURES, GK_MLP, GK_INVERT, GK_DT, GK_BOUND1,
+                         GK_BOUND2, GK_BOUND3, GK_FXCHECK, GK_FXRESOLVE, GK_FXAVG, GK_FXAPPLY, GK_BOUND1,
+                         GK_BOUND2, GK_BOUND3};
     for (int step = 0; step < nsteps; step++)
-      for (int sub = 0; sub < 2; sub++)
+      for (int sub = 0; sub < ns; sub++)
         for (int k : order) {
-          if (k == GK_DT && sub == 0) continue;
+          if (k == GK_DT && sub != ns - 1) continue;
           if ((k == GK_FEATURES || k == GK_MLP) && mode != 2) continue;
           L(k, sub);
         }
@@ -353,7 +362,7 @@ extern "C" int hg_graph_create_grouped(hg_ctx *const *ctxs, int n, int nsteps, h
   // enqueue_step leaves it
   for (int i = 0; i < n; i++) {
     ctxs[i]->unew_in_uf = true;
-    ctxs[i]->flux_state = ctxs[i]->pk;
+    ctxs[i]->flux_state = views[(size_t)(ns - 1) * n + i].pb;
     ctxs[i]->dg_faces_valid = false;
   }
   *out = g;

@@ -572,11 +572,9 @@ void hg::ensure_conserved(hg_ctx *c, hipStream_t s) {
   c->ucons_stale = false;
 }
 
-// step_ch.c:35 step_ch_simplempi over the substeps of step_ch.c:442
-// setup_rktimestep; pi is p on every substep
-void hg::enqueue_step(hg_ctx *c, hipStream_t s) {
-  ensure_conserved(c, s);
-  c->unew_in_uf = true;
+// the substeps of step_ch.c:442 setup_rktimestep for the context's
+// TIMEORDER: each one's pb, pf and coefficients (pi is p on every substep)
+int hg::step_substeps(hg_ctx *c, double *pb[4], double *pf[4], RkStage rk[4]) {
   const int to = c->cfg.timeorder;
   if (to == 1 || to == 3) {
     // TIMEORDER 1 Euler: CUf {1, 0, 1}, Cunew {0, 0, 1}, pb = pf = p.
@@ -585,33 +583,49 @@ void hg::enqueue_step(hg_ctx *c, hipStream_t s) {
     const int n = to;
     const double cuf[3][3] = {{1.0, 0.0, 1.0}, {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
     const double cn3[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}};
-    double *pb[3] = {c->p, c->pk, c->pk2}, *pf[3] = {c->pk, c->pk2, c->p};
+    double *b3[3] = {c->p, c->pk, c->pk2}, *f3[3] = {c->pk, c->pk2, c->p};
     for (int t = 0; t < n; t++) {
       const double *cn = (n == 1) ? cn3[2] : cn3[t];
-      RkStage rk{cuf[t][2], cn[0], cn[1], cn[2], c->ucum, t == 0, t == n - 1, c->ucons, cuf[t][0], cuf[t][1]};
-      enqueue_substep(c, c->p, (n == 1) ? c->p : pb[t], (n == 1) ? c->p : pf[t], rk, t, t == n - 1, s);
+      rk[t] = RkStage{cuf[t][2], cn[0], cn[1], cn[2], c->ucum, t == 0, t == n - 1, c->ucons, cuf[t][0], cuf[t][1]};
+      pb[t] = (n == 1) ? c->p : b3[t];
+      pf[t] = (n == 1) ? c->p : f3[t];
     }
-    return;
+    return n;
   }
   if (to != 4) {
     // TIMEORDER 2 midpoint: CUf[.][2] = 0.5, 1; Cunew = {0, 0, 0}, {0, 0, 1};
     // pb = p, pk; pf = pk, p.  The standard scheme inverts Uf, which equals
     // that ucum; the finite-volume one keeps ucum itself
-    RkStage a{0.5, 0.0, 0.0, 0.0, c->ucons ? c->ucum : nullptr, 1, 0, c->ucons};
-    RkStage b{1.0, 0.0, 0.0, 1.0, c->ucons ? c->ucum : nullptr, 0, 1, c->ucons};
-    enqueue_substep(c, c->p, c->p, c->pk, a, 0, 0, s);
-    enqueue_substep(c, c->p, c->pk, c->p, b, 1, 1, s);
-    return;
+    rk[0] = RkStage{0.5, 0.0, 0.0, 0.0, c->ucons ? c->ucum : nullptr, 1, 0, c->ucons};
+    rk[1] = RkStage{1.0, 0.0, 0.0, 1.0, c->ucons ? c->ucum : nullptr, 0, 1, c->ucons};
+    pb[0] = c->p;
+    pf[0] = c->pk;
+    pb[1] = c->pk;
+    pf[1] = c->p;
+    return 2;
   }
   // TIMEORDER 4: CUf[.][2] = 1/2, 1/2, 1, 1; Cunew = {1, 1/6, 0}, {0, 1/3, 0},
   // {0, 1/3, 0}, {0, 1/6, 0}; pb = p, pk, pk2, pk; pf = pk, pk2, pk, p
   const double cuf2[4] = {0.5, 0.5, 1.0, 1.0};
   const double cn[4][3] = {{1.0, 1.0 / 6.0, 0.0}, {0.0, 1.0 / 3.0, 0.0}, {0.0, 1.0 / 3.0, 0.0}, {0.0, 1.0 / 6.0, 0.0}};
-  double *pb[4] = {c->p, c->pk, c->pk2, c->pk}, *pf[4] = {c->pk, c->pk2, c->pk, c->p};
+  double *b4[4] = {c->p, c->pk, c->pk2, c->pk}, *f4[4] = {c->pk, c->pk2, c->pk, c->p};
   for (int t = 0; t < 4; t++) {
-    RkStage rk{cuf2[t], cn[t][0], cn[t][1], cn[t][2], c->ucum, t == 0, t == 3, c->ucons};
-    enqueue_substep(c, c->p, pb[t], pf[t], rk, t, t == 3, s);
+    rk[t] = RkStage{cuf2[t], cn[t][0], cn[t][1], cn[t][2], c->ucum, t == 0, t == 3, c->ucons};
+    pb[t] = b4[t];
+    pf[t] = f4[t];
   }
+  return 4;
+}
+
+// step_ch.c:35 step_ch_simplempi over the substeps of step_ch.c:442
+// setup_rktimestep
+void hg::enqueue_step(hg_ctx *c, hipStream_t s) {
+  ensure_conserved(c, s);
+  c->unew_in_uf = true;
+  double *pb[4], *pf[4];
+  RkStage rk[4];
+  const int n = step_substeps(c, pb, pf, rk);
+  for (int t = 0; t < n; t++) enqueue_substep(c, c->p, pb[t], pf[t], rk[t], t, t == n - 1, s);
 }
 
 extern "C" int hg_set_dt(hg_ctx *c, double dt) {

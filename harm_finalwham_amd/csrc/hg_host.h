@@ -81,6 +81,9 @@ int upload_geometry_tables(hg_ctx *c, const double *gcov, const double *gcon, co
 int copy_prims(hg_ctx *c, double *dev, const double *host);
 // one step of the context on stream s (kernels only: capturable)
 void enqueue_step(hg_ctx *c, hipStream_t s);
+// the substeps of one step of the context's scheme: their number (at most 4)
+// and each one's pb, pf and coefficients (pi is p on every substep)
+int step_substeps(hg_ctx *c, double *pb[4], double *pf[4], RkStage rk[4]);
 // finite-volume scheme: derive the carried conserved state from p if p was
 // set since (kernels only)
 void ensure_conserved(hg_ctx *c, hipStream_t s);

@@ -141,6 +141,7 @@ __device__ __forceinline__ int group_of(const unsigned *__restrict__ first, int 
 enum GroupClass {
   GK_PREP, GK_DPI, GK_WENO1, GK_WENO2, GK_WENO3, GK_FLUX1, GK_FLUX2, GK_FLUX3, GK_EMF, GK_UPDATE, GK_INVERT,
   GK_DT, GK_BOUND1, GK_BOUND2, GK_BOUND3, GK_FXCHECK, GK_FXRESOLVE, GK_FXAVG, GK_FXAPPLY, GK_FEATURES, GK_MLP,
+  GK_UPDATEU,  // the update with ucum (every scheme but the midpoint one)
   GK_NCLASS
 };
 // blocks of class k on one grid (0: no work there), and one grouped launch of

@@ -1047,10 +1047,11 @@ __global__ void __launch_bounds__(256) kg_emf(const GridView *__restrict__ views
   emf_body(v.F[0], v.F[1], v.F[2], v.geo, v.corn, v.emf[0], v.emf[1], v.emf[2], v.d, bid);
 }
 
+template <bool UCUM>
 __global__ void __launch_bounds__(256) kg_update(const GridView *__restrict__ views,
                                                  const unsigned *__restrict__ first, int n) {
   HG_GROUP_PRELUDE
-  update_body<false, false>(v.pi, v.pb, v.uf, v.F[0], v.F[1], v.F[2], v.emf[0], v.emf[1], v.emf[2], v.geo, v.conn,
+  update_body<UCUM, false>(v.pi, v.pb, v.uf, v.F[0], v.F[1], v.F[2], v.emf[0], v.emf[1], v.emf[2], v.geo, v.conn,
                             v.dt, v.rk, v.d, v.gam, v.idx[1], v.idx[2], v.idx[3], v.err, bid);
 }
 
@@ -1403,7 +1404,7 @@ unsigned group_blocks(int k, const GridView &v) {
       return nblocks((int64_t)fr.ni * fr.nj * fr.nk, 256);
     }
     case GK_EMF: return nblocks((int64_t)(d.n1 + s1) * (d.n2 + s2) * (d.n3 + s3), 256);
-    case GK_UPDATE: case GK_INVERT: case GK_FEATURES: return nblocks(act, 256);
+    case GK_UPDATE: case GK_UPDATEU: case GK_INVERT: case GK_FEATURES: return nblocks(act, 256);
     case GK_DT: return 1u;
     case GK_FXCHECK: case GK_FXRESOLVE: case GK_FXAVG: case GK_FXAPPLY: return group_blocks_fixup(k, v);
     default: return group_blocks_weno(k, v);
@@ -1422,7 +1423,8 @@ void launch_group(int k, const GridView *views, const unsigned *first, int n, un
     case GK_FLUX2: hipLaunchKernelGGL(kg_flux_lr<2>, g, dim3(256), 0, s, views, first, n); return;
     case GK_FLUX3: hipLaunchKernelGGL(kg_flux_lr<3>, g, dim3(256), 0, s, views, first, n); return;
     case GK_EMF: hipLaunchKernelGGL(kg_emf, g, dim3(256), 0, s, views, first, n); return;
-    case GK_UPDATE: hipLaunchKernelGGL(kg_update, g, dim3(256), 0, s, views, first, n); return;
+    case GK_UPDATE: hipLaunchKernelGGL(kg_update<false>, g, dim3(256), 0, s, views, first, n); return;
+    case GK_UPDATEU: hipLaunchKernelGGL(kg_update<true>, g, dim3(256), 0, s, views, first, n); return;
     case GK_INVERT:
       if (mode == INV_MIXED) hipLaunchKernelGGL(kg_invert<INV_MIXED>, g, dim3(256), 0, s, views, first, n);
       else if (mode == INV_LEARNED) hipLaunchKernelGGL(kg_invert<INV_LEARNED>, g, dim3(256), 0, s, views, first, n);

@@ -416,8 +416,8 @@ int hg_graph_create(hg_ctx *const *ctxs, int n, int nsteps, hg_graph **out);
    block reads its grid's arguments from a table on the device), so a step
    of all the grids takes ~40 launches however many grids there are, with
    every grid's arithmetic that of its own run, bit for bit.  For the
-   standard scheme with WENO5BND, the LAXF or HLL flux and RK2
-   (timeorder 2), every context with the same inversion mode (HG_C2P_FP64,
+   standard scheme with WENO5BND and the LAXF or HLL flux, every context
+   with the same timeorder and the same inversion mode (HG_C2P_FP64,
    HG_C2P_MIXED, or HG_C2P_LEARNED with one network for all), and no
    diagnostics recording; other contexts are rejected with an error
    (hg_graph_create takes them).  Launch shapes are
